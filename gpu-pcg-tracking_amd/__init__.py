@@ -1,0 +1,261 @@
+"""Python host mirror of the MI355X SQP tracking-MPC library (libmpcpcg.so).
+
+A thin ctypes layer over the C-ABI declared in include/mpcpcg.h.  Device
+memory and streams come from PyTorch (ROCm); every compute call goes to the
+HIP library.  There is no CPU fallback: importing works without a GPU, but
+creating a Solver requires the built library and a visible GPU, and fails
+loudly otherwise.
+
+Names follow the reference call surface (EmreAdabag/gpu-pcg-tracking):
+  Solver.form_kkt         <- gato_form_kkt            (schur.cuh:391-524)
+  Solver.form_schur       <- form_schur               (schur.cuh:545-559)
+  Solver.pcg              <- GPU-PCG pcg<T,14,N>      (toplevel.cuh:143-157, :292)
+  Solver.compute_dz       <- compute_dz               (schur.cuh:562-566)
+  Solver.compute_merit    <- compute_merit            (merit.cuh:99-145)
+  Solver.ls_compute_merit <- ls_gato_compute_merit x8 (merit.cuh:17-94)
+  Solver.sqp_solve        <- sqpSolve                 (toplevel.cuh:24-503)
+  Solver.fk               <- grid::end_effector_positions_kernel
+  Solver.dynamics_gradient<- gato_plant::forwardDynamicsAndGradient
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libmpcpcg.so")
+
+NQ, NX, NU, NXU = 7, 14, 7, 21
+MAX_SQP_ITERS = 64
+
+# status codes / bits (include/mpcpcg.h)
+OK, ERR_ARG, ERR_HIP, ERR_NOT_READY, ERR_NONFINITE, ERR_NO_DEVICE = range(6)
+ST_ACTIVE, ST_PCG_MAXITER, ST_RHO_ABORT, ST_NONFINITE = 1, 2, 4, 8
+FLAG_LAST_KNOT_OWN_STATE = 1
+
+# every symbol the header declares (checked by the CPU test suite)
+EXPORTS = (
+    "mpc_default_config", "mpc_create", "mpc_destroy", "mpc_get_config", "mpc_set_config",
+    "mpc_status_string", "mpc_fk", "mpc_dynamics_gradient", "mpc_form_kkt", "mpc_form_schur",
+    "mpc_pcg_solve", "mpc_compute_dz", "mpc_compute_merit", "mpc_line_search_merits",
+    "mpc_sqp_solve", "mpc_batch_setup", "mpc_batch_sqp", "mpc_batch_set_timing",
+    "mpc_batch_phase_times", "mpc_synth_instance", "mpc_synth_horizons",
+)
+
+
+class MpcError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("max_knots", C.c_uint32), ("max_instances", C.c_uint32), ("max_total_knots", C.c_uint32),
+        ("dt", C.c_float), ("q_ee", C.c_float), ("qd_cost", C.c_float), ("r_cost", C.c_float),
+        ("mu", C.c_float), ("rho_factor", C.c_float), ("rho_max", C.c_float), ("rho_min", C.c_float),
+        ("rho_reset", C.c_float), ("pcg_exit_tol", C.c_float), ("pcg_max_iter", C.c_uint32),
+        ("sqp_max_iter", C.c_uint32), ("flags", C.c_uint32),
+    ]
+
+
+class SqpStats(C.Structure):
+    _fields_ = [
+        ("sqp_iters", C.c_uint32), ("not_aborted", C.c_uint32), ("sqp_us", C.c_double),
+        ("pcg_iters", C.c_uint32 * MAX_SQP_ITERS), ("pcg_maxiter", C.c_uint32 * MAX_SQP_ITERS),
+        ("linsys_us", C.c_double * MAX_SQP_ITERS), ("ls_step", C.c_int32 * MAX_SQP_ITERS),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libmpcpcg.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MpcError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, u32, f32 = C.c_void_p, C.c_uint32, C.c_float
+        sigs = {
+            "mpc_create": [C.POINTER(Config), C.POINTER(vp)],
+            "mpc_destroy": [vp],
+            "mpc_get_config": [vp, C.POINTER(Config)],
+            "mpc_set_config": [vp, C.POINTER(Config)],
+            "mpc_fk": [vp, u32, vp, u32, vp, vp],
+            "mpc_dynamics_gradient": [vp, u32, vp, vp, vp, vp],
+            "mpc_form_kkt": [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp],
+            "mpc_form_schur": [vp, u32, vp, vp, vp, vp, vp, vp, vp, f32, vp],
+            "mpc_pcg_solve": [vp, u32, vp, vp, vp, vp, u32, f32, vp, vp, vp],
+            "mpc_compute_dz": [vp, u32, vp, vp, vp, vp, vp, vp],
+            "mpc_compute_merit": [vp, u32, vp, vp, vp, vp],
+            "mpc_line_search_merits": [vp, u32, vp, vp, vp, vp, vp, vp],
+            "mpc_sqp_solve": [vp, u32, vp, vp, vp, C.POINTER(f32), f32, C.POINTER(SqpStats), vp],
+            "mpc_batch_setup": [vp, u32, C.POINTER(u32)],
+            "mpc_batch_sqp": [vp, vp, vp, vp, vp, u32, vp, vp, vp],
+            "mpc_batch_set_timing": [vp, C.c_int],
+            "mpc_batch_phase_times": [vp, C.POINTER(C.c_double)],
+        }
+        for name, argt in sigs.items():
+            fn = getattr(L, name)
+            fn.argtypes = argt
+            fn.restype = C.c_int
+        L.mpc_default_config.argtypes = [C.POINTER(Config)]
+        L.mpc_default_config.restype = None
+        L.mpc_status_string.argtypes = [C.c_int]
+        L.mpc_status_string.restype = C.c_char_p
+        L.mpc_synth_instance.argtypes = [C.c_uint64, u32, f32, vp, vp]
+        L.mpc_synth_instance.restype = None
+        L.mpc_synth_horizons.argtypes = [C.c_uint64, u32, C.POINTER(u32), u32, C.POINTER(u32)]
+        L.mpc_synth_horizons.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != OK:
+        msg = lib().mpc_status_string(rc).decode()
+        raise MpcError(f"{what} failed: {msg} ({rc})")
+
+
+def default_config(**kw) -> Config:
+    cfg = Config()
+    lib().mpc_default_config(C.byref(cfg))
+    for k, v in kw.items():
+        if not hasattr(cfg, k):
+            raise KeyError(k)
+        setattr(cfg, k, v)
+    return cfg
+
+
+# ---- synthetic workload (host side, no GPU) ---------------------------------------------------
+
+def synth_instance(seed: int, N: int, dt: float = 0.01):
+    """(x0 (14,), q_ref (N, 7)) float32 numpy arrays of one seeded instance."""
+    import numpy as np
+    x0 = np.zeros(NX, np.float32)
+    qref = np.zeros((N, NQ), np.float32)
+    lib().mpc_synth_instance(C.c_uint64(seed), N, dt, x0.ctypes.data_as(C.c_void_p),
+                             qref.ctypes.data_as(C.c_void_p))
+    return x0, qref
+
+
+def synth_horizons(seed: int, n: int, choices=(32, 64, 128, 256)):
+    import numpy as np
+    ch = (C.c_uint32 * len(choices))(*choices)
+    out = (C.c_uint32 * n)()
+    lib().mpc_synth_horizons(C.c_uint64(seed), n, ch, len(choices), out)
+    return np.frombuffer(out, dtype=np.uint32).copy()
+
+
+# ---- solver -------------------------------------------------------------------------------------
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not (t.is_cuda and t.is_contiguous()):
+        raise MpcError("expected a contiguous CUDA tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Solver:
+    """Owns an mpc_handle (workspace sized at construction)."""
+
+    def __init__(self, cfg: Config | None = None, **kw):
+        import torch
+        if not torch.cuda.is_available():
+            raise MpcError("no GPU visible: the HIP library has no CPU path")
+        self.cfg = cfg if cfg is not None else default_config(**kw)
+        h = C.c_void_p()
+        _check(lib().mpc_create(C.byref(self.cfg), C.byref(h)), "mpc_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mpc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_config(self, **kw):
+        for k, v in kw.items():
+            setattr(self.cfg, k, v)
+        _check(lib().mpc_set_config(self.h, C.byref(self.cfg)), "mpc_set_config")
+
+    # -- robot model
+    def fk(self, q, out=None):
+        import torch
+        n = q.shape[0]
+        out = out if out is not None else torch.empty((n, 6), device=q.device, dtype=torch.float32)
+        _check(lib().mpc_fk(self.h, n, _ptr(q), q.shape[1], _ptr(out), _stream()), "mpc_fk")
+        return out
+
+    def dynamics_gradient(self, qqdu):
+        import torch
+        n = qqdu.shape[0]
+        qdd = torch.empty((n, NQ), device=qqdu.device, dtype=torch.float32)
+        df = torch.empty((n, 147), device=qqdu.device, dtype=torch.float32)
+        _check(lib().mpc_dynamics_gradient(self.h, n, _ptr(qqdu), _ptr(qdd), _ptr(df), _stream()),
+               "mpc_dynamics_gradient")
+        return qdd, df
+
+    # -- stages (one instance, reference layouts)
+    def form_kkt(self, N, xu, ee_ref, xs, G, Cm, g, c):
+        _check(lib().mpc_form_kkt(self.h, N, _ptr(xu), _ptr(ee_ref), _ptr(xs), _ptr(G), _ptr(Cm), _ptr(g),
+                                  _ptr(c), _stream()), "mpc_form_kkt")
+
+    def form_schur(self, N, G, Cm, g, c, S, Pinv, gamma, rho):
+        _check(lib().mpc_form_schur(self.h, N, _ptr(G), _ptr(Cm), _ptr(g), _ptr(c), _ptr(S), _ptr(Pinv),
+                                    _ptr(gamma), float(rho), _stream()), "mpc_form_schur")
+
+    def pcg(self, N, S, Pinv, gamma, lam, max_iter, exit_tol, iters=None, flag=None):
+        _check(lib().mpc_pcg_solve(self.h, N, _ptr(S), _ptr(Pinv), _ptr(gamma), _ptr(lam), max_iter,
+                                   float(exit_tol), _ptr(iters), _ptr(flag), _stream()), "mpc_pcg_solve")
+
+    def compute_dz(self, N, Ginv, Cm, g, lam, dz):
+        _check(lib().mpc_compute_dz(self.h, N, _ptr(Ginv), _ptr(Cm), _ptr(g), _ptr(lam), _ptr(dz), _stream()),
+               "mpc_compute_dz")
+
+    def compute_merit(self, N, xu, ee_ref, out):
+        _check(lib().mpc_compute_merit(self.h, N, _ptr(xu), _ptr(ee_ref), _ptr(out), _stream()),
+               "mpc_compute_merit")
+
+    def ls_compute_merit(self, N, xs, xu, ee_ref, dz, out):
+        _check(lib().mpc_line_search_merits(self.h, N, _ptr(xs), _ptr(xu), _ptr(ee_ref), _ptr(dz), _ptr(out),
+                                            _stream()), "mpc_line_search_merits")
+
+    def sqp_solve(self, N, ee_ref, lam, xu, rho, rho_reset=1e-3):
+        """Drop-in for sqpSolve: updates xu and lam in place; returns
+        (pcg_iters, linsys_us, sqp_us, sqp_iters, not_aborted, pcg_maxiter, rho)."""
+        r = C.c_float(rho)
+        st = SqpStats()
+        _check(lib().mpc_sqp_solve(self.h, N, _ptr(ee_ref), _ptr(lam), _ptr(xu), C.byref(r), float(rho_reset),
+                                   C.byref(st), _stream()), "mpc_sqp_solve")
+        k = st.sqp_iters
+        return (list(st.pcg_iters[:k]), list(st.linsys_us[:k]), st.sqp_us, k, bool(st.not_aborted),
+                [bool(x) for x in st.pcg_maxiter[:k]], r.value, list(st.ls_step[:k]))
+
+    # -- batched
+    def batch_setup(self, horizons):
+        arr = (C.c_uint32 * len(horizons))(*[int(x) for x in horizons])
+        _check(lib().mpc_batch_setup(self.h, len(horizons), arr), "mpc_batch_setup")
+
+    def batch_sqp(self, xu, lam, ee_ref, rho, sqp_iters, status, pcg_iters=None):
+        _check(lib().mpc_batch_sqp(self.h, _ptr(xu), _ptr(lam), _ptr(ee_ref), _ptr(rho), sqp_iters,
+                                   _ptr(status), _ptr(pcg_iters), _stream()), "mpc_batch_sqp")
+
+    def batch_set_timing(self, on=True):
+        _check(lib().mpc_batch_set_timing(self.h, int(on)), "mpc_batch_set_timing")
+
+    def batch_phase_times(self):
+        out = (C.c_double * 6)()
+        _check(lib().mpc_batch_phase_times(self.h, out), "mpc_batch_phase_times")
+        return list(out)

@@ -1,0 +1,110 @@
+// common.h -- batch layout shared by the kernels and the host driver.
+//
+// A batch is a list of instances with horizons N_i packed back to back.
+// K_i = sum_{j<i} N_j is the first global knot of instance i.
+//   caller-facing:  xu / dz at 21*K_i - 7*i, lambda at 14*K_i, ee_ref at 6*K_i
+//   workspace:      per-knot strides (G 245, C 294, g 21, c 14, S / Pinv 588)
+//                   at (stride * K_i); a single instance at K = 0 is exactly
+//                   the reference layout of schur.cuh / toplevel.cuh.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpc {
+
+constexpr int NQ = 7;
+constexpr int NX = 14;
+constexpr int NU = 7;
+constexpr int NXU = 21;
+constexpr int NXX = 196;
+constexpr int G_STRIDE = 245;  // per-knot stride of G / G^-1: Q (196) + R (49)
+constexpr int C_STRIDE = 294;  // per-knot stride of C: -A (196) + -B (98)
+constexpr int S_STRIDE = 588;  // per-block-row stride of S / Pinv
+
+struct Batch {
+  const uint32_t *knot_inst;  // [total_knots] instance of each global knot (nullptr: one instance)
+  const uint32_t *inst_k0;    // [n_inst + 1] first global knot of each instance
+  uint32_t n_inst;
+  uint32_t total_knots;
+};
+
+struct KnotRef {
+  uint32_t inst, k, N, K0;
+};
+
+__device__ __forceinline__ KnotRef locate(const Batch &b, uint32_t gk) {
+  KnotRef r;
+  if (b.knot_inst == nullptr) {
+    r.inst = 0;
+    r.K0 = 0;
+    r.N = b.total_knots;
+  } else {
+    r.inst = b.knot_inst[gk];
+    r.K0 = b.inst_k0[r.inst];
+    r.N = b.inst_k0[r.inst + 1] - r.K0;
+  }
+  r.k = gk - r.K0;
+  return r;
+}
+
+__device__ __forceinline__ uint32_t inst_first(const Batch &b, uint32_t inst) {
+  return b.knot_inst == nullptr ? 0u : b.inst_k0[inst];
+}
+__device__ __forceinline__ uint32_t inst_len(const Batch &b, uint32_t inst) {
+  return b.knot_inst == nullptr ? b.total_knots : b.inst_k0[inst + 1] - b.inst_k0[inst];
+}
+
+// Numeric parameters passed by value to the kernels.
+struct Params {
+  float dt, q_ee, qd_cost, r_cost, mu;
+  float rho_factor, rho_max, rho_min, rho_reset;
+  float pcg_tol;
+  uint32_t pcg_max_iter;
+  uint32_t flags;
+};
+
+// per-instance device state of the batched SQP (struct of arrays)
+struct InstState {
+  float *rho;        // [n] regularisation (in/out)
+  float *drho;       // [n] rho multiplier (toplevel.cuh:89)
+  float *merit0;     // [n] merit of the current iterate
+  uint32_t *status;  // [n] MPC_ST_* bits
+  uint32_t *sqp_done;  // [n] SQP iterations taken
+  float *xs;         // [14 n] initial state of each solve
+};
+
+constexpr uint32_t ST_ACTIVE = 1u, ST_PCG_MAXITER = 2u, ST_RHO_ABORT = 4u, ST_NONFINITE = 8u;
+constexpr uint32_t FLAG_LAST_KNOT_OWN_STATE = 1u;
+
+// host-side launchers (implemented in the .hip files)
+hipError_t launch_fk(uint32_t n, const float *q, uint32_t q_stride, float *ee, hipStream_t st);
+hipError_t launch_dyn_grad(uint32_t n, const float *qqdu, float *qdd, float *df_du, hipStream_t st);
+hipError_t launch_form_kkt(const Batch &b, const Params &p, const float *xu, const float *ee_ref,
+                           const float *xs, const uint32_t *status, float *G, float *C, float *g,
+                           float *c, hipStream_t st);
+// mode: 0..7 line-search alphas (-2^-p, with the initial-state term); 8 the
+// current iterate (alpha 0, without it).  Writes out[(p - p0) * total + gk].
+hipError_t launch_merit(const Batch &b, const Params &p, const float *xu, const float *dz,
+                        const float *ee_ref, const float *xs, const uint32_t *status, int p0, int np,
+                        float *out, hipStream_t st);
+hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, const float *g,
+                             const float *c, const float *rho_dev, float rho, const uint32_t *status,
+                             float *S, float *Pinv, float *gamma, float *Ginv, int zero_unused,
+                             hipStream_t st);
+hipError_t launch_compute_dz(const Batch &b, const float *Ginv, const float *C, const float *g,
+                             const float *lambda, const uint32_t *status, float *dz, hipStream_t st);
+hipError_t launch_pcg(const Batch &b, uint32_t max_n, const float *S, const float *Pinv,
+                      const float *gamma, float *lambda, uint32_t max_iter, float tol,
+                      const uint32_t *status, uint32_t *iters, uint32_t *maxiter_flag,
+                      hipStream_t st);
+hipError_t launch_merit_reduce(const Batch &b, const float *knot_merits, int np, float *out,
+                               hipStream_t st);
+hipError_t launch_ls_update(const Batch &b, const Params &p, const float *knot_merits,
+                            const float *dz, float *xu, InstState s, uint32_t *pcg_iter_log,
+                            const uint32_t *pcg_iters_now, const uint32_t *pcg_flag_now,
+                            uint32_t it, uint32_t log_stride, hipStream_t st);
+hipError_t launch_axpy(uint32_t n, float alpha, const float *x, float *y, hipStream_t st);
+hipError_t launch_batch_init(const Batch &b, const float *xu, const float *knot_merits, InstState s,
+                             hipStream_t st);
+
+}  // namespace mpc

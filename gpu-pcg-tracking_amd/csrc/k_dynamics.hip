@@ -1,0 +1,455 @@
+// k_dynamics.hip -- linearisation (KKT blocks), forward kinematics,
+// dynamics gradient and merit kernels for gfx950.
+//
+// Mapping: one 16-lane group per knot, 4 groups per 64-lane workgroup (one
+// wave), so every __syncthreads() is a single-wave barrier.  Within a
+// group the lanes split the rigid-body work by direction:
+//   lanes 0..6   column j of M(q) (unit-acceleration recursion)
+//   lane  7      bias forces c(q, qd) (RNEA at qdd = 0)
+//   lanes 0..13  directional derivatives d tau / d(q_j | qd_j)
+//   lanes 0..6   Jacobian column j of the end-effector position
+// Shared per-knot quantities live in the group's LDS slice.
+//
+// Reference counterparts: gato_form_kkt (schur.cuh:391-524) with
+// gato_plant::forwardDynamicsAndGradient (rbdfiles/iiwa_eepos_plant.cuh:
+// 126-156), integratorAndGradient (integrator.cuh:159-183),
+// trackingCostGradientAndHessian(_lastblock) (rbdfiles/iiwa_eepos_plant.cuh:
+// 293-401), compute_merit / ls_gato_compute_merit (merit.cuh:17-145) and
+// grid::end_effector_positions_kernel (rbdfiles/iiwa_eepos_grid.cuh:2141).
+#include "common.h"
+#include "model.h"
+
+namespace mpc {
+
+namespace {
+
+constexpr int GROUP = 16;
+constexpr int GROUPS_PER_WG = 4;
+constexpr int WG = GROUP * GROUPS_PER_WG;
+
+struct DynLds {
+  float sn[8], cs[8];
+  float M[49];     // joint-space inertia (column-major)
+  float bias[8];   // c(q, qd)
+  float Lc[49];    // Cholesky factor of M (column-major, lower)
+  float Minv[49];  // M^-1 (column-major)
+  float qdd[8];
+  float bv[42], ba[42], bf[42], bxv[42], bxa[42];  // base RNEA at (q, qd, qdd)
+  float df[98];    // dqdd/d(q,qd) (7 x 14 column-major)
+};
+
+struct KktLds {
+  DynLds dyn;
+  float x[16], u[8], xn[16], xc[16];  // x_k, u_k, x_{k+1}, cost state
+  float ref[4];
+  float csn[8], ccs[8];                // sin / cos of the cost state
+  float zax[21], org[21];
+  float gq[8];
+  float ee[4];
+};
+
+__device__ __forceinline__ void cholesky7(const float *M, float *L) {
+  for (int j = 0; j < NQ; ++j) {
+    float d = M[j + 7 * j];
+    for (int k = 0; k < j; ++k) d -= L[j + 7 * k] * L[j + 7 * k];
+    d = sqrtf(d);
+    L[j + 7 * j] = d;
+    const float inv = 1.f / d;
+    for (int i = j + 1; i < NQ; ++i) {
+      float s = M[i + 7 * j];
+      for (int k = 0; k < j; ++k) s -= L[i + 7 * k] * L[j + 7 * k];
+      L[i + 7 * j] = s * inv;
+    }
+    for (int i = 0; i < j; ++i) L[i + 7 * j] = 0.f;
+  }
+}
+
+// x = L^-T L^-1 b for a 7x7 lower Cholesky factor
+__device__ __forceinline__ void chol_solve7(const float *L, const float *b, float *x) {
+  float y[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    float s = b[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= L[i + 7 * k] * y[k];
+    y[i] = s / L[i + 7 * i];
+  }
+#pragma unroll
+  for (int i = NQ - 1; i >= 0; --i) {
+    float s = y[i];
+#pragma unroll
+    for (int k = i + 1; k < NQ; ++k) s -= L[k + 7 * i] * x[k];
+    x[i] = s / L[i + 7 * i];
+  }
+}
+
+// Forward dynamics and its gradient for one point (q, qd, u) held by a
+// 16-lane group.  On return (after a barrier): D.qdd, D.Minv, D.df hold
+// qdd, M^-1 and [dqdd/dq, dqdd/dqd].
+__device__ __forceinline__ void group_dyn_grad(int lane, const float *q, const float *qd,
+                                               const float *u, DynLds &D) {
+  if (lane < NQ) {
+    float s, c;
+    sincosf(q[lane], &s, &c);
+    D.sn[lane] = s;
+    D.cs[lane] = c;
+  }
+  __syncthreads();
+  float s[NQ], c[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    s[i] = D.sn[i];
+    c[i] = D.cs[i];
+  }
+  if (lane < NQ) {
+    float col[NQ];
+    mass_column(lane, s, c, col);
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) D.M[r + 7 * lane] = col[r];
+  } else if (lane == NQ) {
+    float qdv[NQ], z[NQ], tau[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      qdv[i] = qd[i];
+      z[i] = 0.f;
+    }
+    rnea<false>(s, c, qdv, z, tau, nullptr, nullptr, nullptr, nullptr, nullptr);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) D.bias[i] = tau[i];
+  }
+  __syncthreads();
+  if (lane == 0) cholesky7(D.M, D.Lc);
+  __syncthreads();
+  if (lane < NQ) {
+    float L[49];
+#pragma unroll
+    for (int i = 0; i < 49; ++i) L[i] = D.Lc[i];
+    float e[NQ], x[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) e[i] = (i == lane) ? 1.f : 0.f;
+    chol_solve7(L, e, x);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) D.Minv[i + 7 * lane] = x[i];
+  }
+  __syncthreads();
+  if (lane < NQ) {
+    float acc = 0.f;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) acc += D.Minv[lane + 7 * m] * (u[m] - D.bias[m]);
+    D.qdd[lane] = acc;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    float qdv[NQ], qddv[NQ], tau[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      qdv[i] = qd[i];
+      qddv[i] = D.qdd[i];
+    }
+    rnea<true>(s, c, qdv, qddv, tau, D.bv, D.ba, D.bf, D.bxv, D.bxa);
+  }
+  __syncthreads();
+  float dtau[NQ];
+  if (lane < NX) {
+    float qdv[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) qdv[i] = qd[i];
+    rnea_dir(lane, s, c, qdv, D.bv, D.bf, D.bxv, D.bxa, dtau);
+  }
+  __syncthreads();  // every lane is done reading the base point
+  if (lane < NX) {
+    // dqdd/d(dir) = -M^-1 dtau
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) {
+      float acc = 0.f;
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) acc += D.Minv[r + 7 * m] * dtau[m];
+      D.df[r + 7 * lane] = -acc;
+    }
+  }
+  __syncthreads();
+}
+
+// --------------------------------------------------------------------------
+// forward kinematics: one thread per point
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fk(uint32_t n, const float *__restrict__ q, uint32_t q_stride,
+                                            float *__restrict__ ee) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s[NQ], c[NQ];
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) sincosf(q[(size_t)i * q_stride + j], &s[j], &c[j]);
+  float zax[21], org[21], R[9];
+  fk_frames(s, c, zax, org, R);
+  float *o = ee + 6 * (size_t)i;
+  o[0] = org[18];
+  o[1] = org[19];
+  o[2] = org[20];
+  o[3] = atan2f(R[7], R[8]);
+  o[4] = -atan2f(R[6], sqrtf(R[7] * R[7] + R[8] * R[8]));
+  o[5] = atan2f(R[3], R[0]);
+}
+
+// --------------------------------------------------------------------------
+// dynamics gradient of n points [q qd u]
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(WG) k_dyn_grad(uint32_t n, const float *__restrict__ qqdu,
+                                                 float *__restrict__ qdd_out,
+                                                 float *__restrict__ df_out) {
+  __shared__ DynLds lds[GROUPS_PER_WG];
+  __shared__ float pin[GROUPS_PER_WG][24];
+  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
+  const uint32_t pt = blockIdx.x * GROUPS_PER_WG + grp;
+  const bool valid = pt < n;
+  const uint32_t ptc = valid ? pt : n - 1;
+  for (int i = lane; i < NXU; i += GROUP) pin[grp][i] = qqdu[(size_t)ptc * NXU + i];
+  __syncthreads();
+  DynLds &D = lds[grp];
+  group_dyn_grad(lane, pin[grp], pin[grp] + NQ, pin[grp] + NX, D);
+  if (!valid) return;
+  if (lane < NQ) qdd_out[(size_t)pt * NQ + lane] = D.qdd[lane];
+  float *o = df_out + (size_t)pt * 147;
+  for (int i = lane; i < 98; i += GROUP) o[i] = D.df[i];
+  for (int i = lane; i < 49; i += GROUP) o[98 + i] = D.Minv[i];
+}
+
+// --------------------------------------------------------------------------
+// KKT blocks, one group per global knot
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(WG) k_form_kkt(Batch b, Params p, const float *__restrict__ xu,
+                                                 const float *__restrict__ ee_ref,
+                                                 const float *__restrict__ xs,
+                                                 const uint32_t *__restrict__ status,
+                                                 float *__restrict__ G, float *__restrict__ C,
+                                                 float *__restrict__ g, float *__restrict__ cvec) {
+  __shared__ KktLds lds[GROUPS_PER_WG];
+  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
+  const uint32_t gk_raw = blockIdx.x * GROUPS_PER_WG + grp;
+  bool valid = gk_raw < b.total_knots;
+  const uint32_t gk = valid ? gk_raw : b.total_knots - 1;
+  const KnotRef kr = locate(b, gk);
+  if (status && !(status[kr.inst] & ST_ACTIVE)) valid = false;
+  KktLds &L = lds[grp];
+  const float *xi = xu + (size_t)NXU * kr.K0 - (size_t)NU * kr.inst;  // instance trajectory
+  const bool dyn = kr.k + 1 < kr.N;
+  const uint32_t kc = dyn ? kr.k : ((p.flags & FLAG_LAST_KNOT_OWN_STATE) ? kr.k : kr.k - 1);
+  for (int i = lane; i < NX; i += GROUP) {
+    L.x[i] = xi[(size_t)NXU * kr.k + i];
+    L.xc[i] = xi[(size_t)NXU * kc + i];
+    L.xn[i] = dyn ? xi[(size_t)NXU * (kr.k + 1) + i] : 0.f;
+  }
+  for (int i = lane; i < NU; i += GROUP) L.u[i] = dyn ? xi[(size_t)NXU * kr.k + NX + i] : 0.f;
+  if (lane < 3) L.ref[lane] = ee_ref[(size_t)6 * (kr.K0 + kr.k) + lane];
+  __syncthreads();
+
+  // dynamics linearisation (every group runs it so barriers line up; the
+  // terminal knot's results are discarded)
+  group_dyn_grad(lane, L.x, L.x + NQ, L.u, L.dyn);
+
+  // tracking cost gradient / Gauss-Newton Hessian at the cost state
+  if (lane < NQ) {
+    float s, c;
+    sincosf(L.xc[lane], &s, &c);
+    L.csn[lane] = s;
+    L.ccs[lane] = c;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    float s[NQ], c[NQ], R[9];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      s[i] = L.csn[i];
+      c[i] = L.ccs[i];
+    }
+    fk_frames(s, c, L.zax, L.org, R);
+    L.ee[0] = L.org[18];
+    L.ee[1] = L.org[19];
+    L.ee[2] = L.org[20];
+  }
+  __syncthreads();
+  if (lane < NQ) {
+    const float d0 = L.ee[0] - L.org[3 * lane], d1 = L.ee[1] - L.org[3 * lane + 1],
+                d2 = L.ee[2] - L.org[3 * lane + 2];
+    const float z0 = L.zax[3 * lane], z1 = L.zax[3 * lane + 1], z2 = L.zax[3 * lane + 2];
+    const float j0 = z1 * d2 - z2 * d1, j1 = z2 * d0 - z0 * d2, j2 = z0 * d1 - z1 * d0;
+    const float e0 = p.q_ee * (L.ee[0] - L.ref[0]), e1 = p.q_ee * (L.ee[1] - L.ref[1]),
+                e2 = p.q_ee * (L.ee[2] - L.ref[2]);
+    L.gq[lane] = j0 * e0 + j1 * e1 + j2 * e2;
+  }
+  __syncthreads();
+  if (!valid) return;
+
+  const size_t knot = (size_t)kr.K0 + kr.k;
+  float *Gk = G + (size_t)G_STRIDE * knot;
+  for (int i = lane; i < NXX; i += GROUP) {
+    const int r = i % NX, col = i / NX;
+    Gk[i] = (r < NQ && col < NQ) ? L.gq[r] * L.gq[col] : (r == col ? p.qd_cost : 0.f);
+  }
+  float *gk_ = g + (size_t)NXU * knot;
+  if (lane < NX) gk_[lane] = lane < NQ ? L.gq[lane] : p.qd_cost * L.xc[lane];
+  if (kr.k == 0 && lane < NX) cvec[(size_t)NX * kr.K0 + lane] = L.x[lane] - xs[(size_t)NX * kr.inst + lane];
+  if (!dyn) return;
+  for (int i = lane; i < 49; i += GROUP) Gk[NXX + i] = (i % NU == i / NU) ? p.r_cost : 0.f;
+  if (lane < NU) gk_[NX + lane] = p.r_cost * L.u[lane];
+  float *Ck = C + (size_t)C_STRIDE * knot;
+  const float dt = p.dt;
+  for (int i = lane; i < NXX; i += GROUP) {
+    const int r = i % NX, col = i / NX;
+    float v = (r == col) ? 1.f : 0.f;
+    if (r < NQ && col == r + NQ) v += dt;
+    if (r >= NQ) v += dt * L.dyn.df[(r - NQ) + NQ * col];
+    Ck[i] = -v;
+  }
+  for (int i = lane; i < NX * NU; i += GROUP) {
+    const int r = i % NX, col = i / NX;
+    Ck[NXX + i] = r >= NQ ? -dt * L.dyn.Minv[(r - NQ) + NQ * col] : 0.f;
+  }
+  if (lane < NX) {
+    float v;
+    if (lane < NQ) v = L.xn[lane] - (L.x[lane] + dt * L.x[NQ + lane]);
+    else v = L.xn[lane] - (L.x[lane] + dt * L.dyn.qdd[lane - NQ]);
+    cvec[(size_t)NX * (knot + 1) + lane] = v;
+  }
+}
+
+// --------------------------------------------------------------------------
+// merit terms, one group per (alpha, global knot)
+// --------------------------------------------------------------------------
+struct MeritLds {
+  float z[40];  // trial x_k, u_k, x_{k+1}
+  float z0[16]; // trial x_0 (terminal slot)
+  float ref[4];
+  float sn[8], cs[8];
+  float M[49];
+  float bias[8];
+  float ee[4];
+};
+
+__global__ void __launch_bounds__(WG) k_merit(Batch b, Params p, const float *__restrict__ xu,
+                                              const float *__restrict__ dz,
+                                              const float *__restrict__ ee_ref,
+                                              const float *__restrict__ xs,
+                                              const uint32_t *__restrict__ status, int p0, int np,
+                                              float *__restrict__ out) {
+  __shared__ MeritLds lds[GROUPS_PER_WG];
+  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
+  const uint32_t T = b.total_knots;
+  const uint32_t items = T * (uint32_t)np;
+  const uint32_t item_raw = blockIdx.x * GROUPS_PER_WG + grp;
+  const bool valid = item_raw < items;
+  const uint32_t item = valid ? item_raw : items - 1;
+  const uint32_t slot = item / T;  // 0 .. np-1
+  const uint32_t gk = item % T;
+  const int pidx = p0 + (int)slot;
+  const KnotRef kr = locate(b, gk);
+  bool active = !(status && !(status[kr.inst] & ST_ACTIVE));
+  const float alpha = (pidx >= 8) ? 0.f : -1.f / (float)(1u << pidx);
+  const bool with_x0 = pidx < 8;
+  MeritLds &L = lds[grp];
+  const float *xi = xu + (size_t)NXU * kr.K0 - (size_t)NU * kr.inst;
+  const float *di = dz ? dz + (size_t)NXU * kr.K0 - (size_t)NU * kr.inst : nullptr;
+  const bool dyn = kr.k + 1 < kr.N;
+  const int len = dyn ? NXU + NX : NX;
+  for (int i = lane; i < len; i += GROUP) {
+    const size_t o = (size_t)NXU * kr.k + i;
+    L.z[i] = di ? xi[o] + alpha * di[o] : xi[o];
+  }
+  if (!dyn && with_x0)
+    for (int i = lane; i < NX; i += GROUP) L.z0[i] = di ? xi[i] + alpha * di[i] : xi[i];
+  if (lane < 3) L.ref[lane] = ee_ref[(size_t)6 * (kr.K0 + kr.k) + lane];
+  if (lane < NQ) {
+    float s, c;
+    sincosf(L.z[lane], &s, &c);  // z written by this lane above (lane < 16 covers i = lane)
+    L.sn[lane] = s;
+    L.cs[lane] = c;
+  }
+  __syncthreads();
+  float s[NQ], c[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    s[i] = L.sn[i];
+    c[i] = L.cs[i];
+  }
+  if (dyn && lane < NQ) {
+    float col[NQ];
+    mass_column(lane, s, c, col);
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) L.M[r + 7 * lane] = col[r];
+  } else if (dyn && lane == NQ) {
+    float qdv[NQ], zz[NQ], tau[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      qdv[i] = L.z[NQ + i];
+      zz[i] = 0.f;
+    }
+    rnea<false>(s, c, qdv, zz, tau, nullptr, nullptr, nullptr, nullptr, nullptr);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) L.bias[i] = tau[i];
+  } else if (lane == NQ + 1) {
+    fk_pos(s, c, L.ee);
+  }
+  __syncthreads();
+  if (lane == 0 && active && valid) {
+    // cost: 0.5 QD |qd|^2 + 0.5 R |u|^2 + 0.5 w |ee - ref|^2
+    float J = 0.f;
+    for (int i = 0; i < NQ; ++i) J += 0.5f * (p.qd_cost * L.z[NQ + i] * L.z[NQ + i]);
+    if (dyn)
+      for (int i = 0; i < NU; ++i) J += 0.5f * (p.r_cost * L.z[NX + i] * L.z[NX + i]);
+    for (int i = 0; i < 3; ++i) {
+      const float e = L.ee[i] - L.ref[i];
+      J += 0.5f * p.q_ee * e * e;
+    }
+    float ck = 0.f;
+    if (dyn) {
+      float Lc[49], rhs[NQ], qdd[NQ];
+      cholesky7(L.M, Lc);
+      for (int i = 0; i < NQ; ++i) rhs[i] = L.z[NX + i] - L.bias[i];
+      chol_solve7(Lc, rhs, qdd);
+      for (int i = 0; i < NQ; ++i) {
+        ck += fabsf(L.z[NXU + i] - (L.z[i] + p.dt * L.z[NQ + i]));
+        ck += fabsf(L.z[NXU + NQ + i] - (L.z[NQ + i] + p.dt * qdd[i]));
+      }
+    } else if (with_x0) {
+      const float *x0s = xs + (size_t)NX * kr.inst;
+      for (int i = 0; i < NX; ++i) ck += fabsf(L.z0[i] - x0s[i]);
+    }
+    out[(size_t)slot * T + gk] = J + p.mu * ck;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_fk(uint32_t n, const float *q, uint32_t q_stride, float *ee, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fk, dim3((n + 255) / 256), dim3(256), 0, st, n, q, q_stride, ee);
+  return hipGetLastError();
+}
+
+hipError_t launch_dyn_grad(uint32_t n, const float *qqdu, float *qdd, float *df_du, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dyn_grad, dim3((n + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG), 0, st, n,
+                     qqdu, qdd, df_du);
+  return hipGetLastError();
+}
+
+hipError_t launch_form_kkt(const Batch &b, const Params &p, const float *xu, const float *ee_ref,
+                           const float *xs, const uint32_t *status, float *G, float *C, float *g,
+                           float *c, hipStream_t st) {
+  if (b.total_knots == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_form_kkt, dim3((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG),
+                     0, st, b, p, xu, ee_ref, xs, status, G, C, g, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_merit(const Batch &b, const Params &p, const float *xu, const float *dz,
+                        const float *ee_ref, const float *xs, const uint32_t *status, int p0, int np,
+                        float *out, hipStream_t st) {
+  if (b.total_knots == 0 || np <= 0) return hipSuccess;
+  const uint32_t items = b.total_knots * (uint32_t)np;
+  hipLaunchKernelGGL(k_merit, dim3((items + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG), 0, st, b, p,
+                     xu, dz, ee_ref, xs, status, p0, np, out);
+  return hipGetLastError();
+}
+
+}  // namespace mpc

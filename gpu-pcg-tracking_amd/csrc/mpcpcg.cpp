@@ -1,0 +1,494 @@
+// mpcpcg.cpp -- host side of the C-ABI (include/mpcpcg.h): handle and
+// workspace ownership, stage entry points, the plain single-instance SQP
+// driver (drop-in for sqpSolve, toplevel.cuh:24-503) and the batched,
+// fully device-side SQP driver.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.h"
+#include "mpcpcg.h"
+
+using namespace mpc;
+
+#define HIP_TRY(expr)                          \
+  do {                                         \
+    hipError_t e_ = (expr);                    \
+    if (e_ != hipSuccess) return MPC_ERR_HIP;  \
+  } while (0)
+
+struct mpc_handle {
+  mpc_config cfg{};
+  int device = 0;
+  // workspace, sized for cfg.max_total_knots (T) and cfg.max_instances (B)
+  float *G = nullptr, *Ginv = nullptr, *C = nullptr, *g = nullptr, *c = nullptr;
+  float *S = nullptr, *P = nullptr, *gamma = nullptr, *dz = nullptr;
+  float *knot_merits = nullptr;  // [9 T]
+  float *merits = nullptr;       // [9 B]
+  float *xs = nullptr;           // [14 B]
+  float *drho = nullptr, *merit0 = nullptr, *rho1 = nullptr;
+  uint32_t *status_ws = nullptr, *sqp_done = nullptr, *pcg_iters = nullptr, *pcg_flag = nullptr;
+  uint32_t *knot_inst = nullptr, *inst_k0 = nullptr;
+  // pinned staging for the host round trips of the plain driver
+  float *h_merits = nullptr;
+  uint32_t *h_pcg = nullptr;
+  float *h_rho = nullptr;
+  // batch layout
+  std::vector<uint32_t> horizons, k0;
+  uint32_t n_inst = 0, total = 0, max_n = 0;
+  bool batch_ready = false;
+  // timing
+  bool timing = false;
+  hipEvent_t ev[8]{};
+  double phase_us[6]{};
+};
+
+static Params make_params(const mpc_config &c) {
+  Params p;
+  p.dt = c.dt;
+  p.q_ee = c.q_ee;
+  p.qd_cost = c.qd_cost;
+  p.r_cost = c.r_cost;
+  p.mu = c.mu;
+  p.rho_factor = c.rho_factor;
+  p.rho_max = c.rho_max;
+  p.rho_min = c.rho_min;
+  p.rho_reset = c.rho_reset;
+  p.pcg_tol = c.pcg_exit_tol;
+  p.pcg_max_iter = c.pcg_max_iter;
+  p.flags = c.flags;
+  return p;
+}
+
+static Batch single(uint32_t N) { return Batch{nullptr, nullptr, 1u, N}; }
+
+static Batch batch_of(const mpc_handle *h) {
+  return Batch{h->knot_inst, h->inst_k0, h->n_inst, h->total};
+}
+
+static bool valid_numbers(const mpc_config &c) {
+  return c.dt > 0 && std::isfinite(c.dt) && c.q_ee >= 0 && c.qd_cost >= 0 && c.r_cost >= 0 &&
+         c.mu >= 0 && c.rho_factor > 1 && c.rho_max > 0 && c.rho_min > 0 && c.pcg_exit_tol >= 0 &&
+         c.sqp_max_iter <= MPC_MAX_SQP_ITERS;
+}
+
+extern "C" {
+
+void mpc_default_config(mpc_config *cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->max_knots = 256;
+  cfg->max_instances = 1;
+  cfg->max_total_knots = 256;
+  cfg->dt = 0.01f;
+  cfg->q_ee = 1.f;
+  cfg->qd_cost = 1e-2f;
+  cfg->r_cost = 1e-4f;
+  cfg->mu = 10.f;
+  cfg->rho_factor = 1.2f;
+  cfg->rho_max = 10.f;
+  cfg->rho_min = 1e-3f;
+  cfg->rho_reset = 1e-3f;
+  cfg->pcg_exit_tol = 1e-5f;
+  cfg->pcg_max_iter = 173;
+  cfg->sqp_max_iter = 3;
+  cfg->flags = 0;
+}
+
+const char *mpc_status_string(int s) {
+  switch (s) {
+    case MPC_OK: return "ok";
+    case MPC_ERR_ARG: return "bad argument";
+    case MPC_ERR_HIP: return "HIP runtime error";
+    case MPC_ERR_NOT_READY: return "batch layout not set up";
+    case MPC_ERR_NONFINITE: return "non-finite value";
+    case MPC_ERR_NO_DEVICE: return "no usable GPU";
+    default: return "unknown status";
+  }
+}
+
+static void free_all(mpc_handle *h) {
+  void *ptrs[] = {h->G, h->Ginv, h->C, h->g, h->c, h->S, h->P, h->gamma, h->dz, h->knot_merits,
+                  h->merits, h->xs, h->drho, h->merit0, h->rho1, h->status_ws, h->sqp_done,
+                  h->pcg_iters, h->pcg_flag, h->knot_inst, h->inst_k0};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h->h_merits) (void)hipHostFree(h->h_merits);
+  if (h->h_pcg) (void)hipHostFree(h->h_pcg);
+  if (h->h_rho) (void)hipHostFree(h->h_rho);
+  for (auto &e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+int mpc_create(const mpc_config *cfg, mpc_handle **out) {
+  if (!cfg || !out) return MPC_ERR_ARG;
+  *out = nullptr;
+  if (cfg->max_knots < 2 || cfg->max_knots > 512 || cfg->max_instances < 1 ||
+      cfg->max_total_knots < cfg->max_knots || !valid_numbers(*cfg))
+    return MPC_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MPC_ERR_NO_DEVICE;
+  mpc_handle *h = new (std::nothrow) mpc_handle();
+  if (!h) return MPC_ERR_ARG;
+  h->cfg = *cfg;
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return MPC_ERR_HIP;
+  }
+  const size_t T = cfg->max_total_knots, B = cfg->max_instances;
+  auto alloc = [&](void **p, size_t bytes) { return hipMalloc(p, bytes) == hipSuccess; };
+  bool ok = alloc((void **)&h->G, sizeof(float) * G_STRIDE * T) &&
+            alloc((void **)&h->Ginv, sizeof(float) * G_STRIDE * T) &&
+            alloc((void **)&h->C, sizeof(float) * C_STRIDE * T) &&
+            alloc((void **)&h->g, sizeof(float) * NXU * T) &&
+            alloc((void **)&h->c, sizeof(float) * NX * T) &&
+            alloc((void **)&h->S, sizeof(float) * S_STRIDE * T) &&
+            alloc((void **)&h->P, sizeof(float) * S_STRIDE * T) &&
+            alloc((void **)&h->gamma, sizeof(float) * NX * T) &&
+            alloc((void **)&h->dz, sizeof(float) * NXU * T) &&
+            alloc((void **)&h->knot_merits, sizeof(float) * 9 * T) &&
+            alloc((void **)&h->merits, sizeof(float) * 9 * B) &&
+            alloc((void **)&h->xs, sizeof(float) * NX * B) &&
+            alloc((void **)&h->drho, sizeof(float) * B) &&
+            alloc((void **)&h->merit0, sizeof(float) * B) &&
+            alloc((void **)&h->rho1, sizeof(float) * B) &&
+            alloc((void **)&h->status_ws, sizeof(uint32_t) * B) &&
+            alloc((void **)&h->sqp_done, sizeof(uint32_t) * B) &&
+            alloc((void **)&h->pcg_iters, sizeof(uint32_t) * B) &&
+            alloc((void **)&h->pcg_flag, sizeof(uint32_t) * B) &&
+            alloc((void **)&h->knot_inst, sizeof(uint32_t) * T) &&
+            alloc((void **)&h->inst_k0, sizeof(uint32_t) * (B + 1)) &&
+            hipHostMalloc((void **)&h->h_merits, sizeof(float) * 16) == hipSuccess &&
+            hipHostMalloc((void **)&h->h_pcg, sizeof(uint32_t) * 4) == hipSuccess &&
+            hipHostMalloc((void **)&h->h_rho, sizeof(float) * 4) == hipSuccess;
+  for (auto &e : h->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  if (!ok) {
+    free_all(h);
+    delete h;
+    return MPC_ERR_HIP;
+  }
+  *out = h;
+  return MPC_OK;
+}
+
+int mpc_destroy(mpc_handle *h) {
+  if (!h) return MPC_ERR_ARG;
+  (void)hipDeviceSynchronize();
+  free_all(h);
+  delete h;
+  return MPC_OK;
+}
+
+int mpc_get_config(const mpc_handle *h, mpc_config *cfg) {
+  if (!h || !cfg) return MPC_ERR_ARG;
+  *cfg = h->cfg;
+  return MPC_OK;
+}
+
+int mpc_set_config(mpc_handle *h, const mpc_config *cfg) {
+  if (!h || !cfg || !valid_numbers(*cfg)) return MPC_ERR_ARG;
+  mpc_config c = *cfg;
+  c.max_knots = h->cfg.max_knots;
+  c.max_instances = h->cfg.max_instances;
+  c.max_total_knots = h->cfg.max_total_knots;
+  h->cfg = c;
+  return MPC_OK;
+}
+
+// ---- robot model ------------------------------------------------------------
+
+int mpc_fk(mpc_handle *h, uint32_t n, const float *d_q, uint32_t q_stride, float *d_ee,
+           mpc_stream_t stream) {
+  if (!h || (n && (!d_q || !d_ee)) || q_stride < NQ) return MPC_ERR_ARG;
+  HIP_TRY(launch_fk(n, d_q, q_stride, d_ee, (hipStream_t)stream));
+  return MPC_OK;
+}
+
+int mpc_dynamics_gradient(mpc_handle *h, uint32_t n, const float *d_qqdu, float *d_qdd,
+                          float *d_df_du, mpc_stream_t stream) {
+  if (!h || (n && (!d_qqdu || !d_qdd || !d_df_du))) return MPC_ERR_ARG;
+  HIP_TRY(launch_dyn_grad(n, d_qqdu, d_qdd, d_df_du, (hipStream_t)stream));
+  return MPC_OK;
+}
+
+// ---- stages -----------------------------------------------------------------
+
+static bool n_ok(const mpc_handle *h, uint32_t N) {
+  return N >= 2 && N <= h->cfg.max_knots && N <= h->cfg.max_total_knots;
+}
+
+int mpc_form_kkt(mpc_handle *h, uint32_t N, const float *d_xu, const float *d_ee_ref,
+                 const float *d_xs, float *d_G, float *d_C, float *d_g, float *d_c,
+                 mpc_stream_t stream) {
+  if (!h || !n_ok(h, N) || !d_xu || !d_ee_ref || !d_xs || !d_G || !d_C || !d_g || !d_c)
+    return MPC_ERR_ARG;
+  HIP_TRY(launch_form_kkt(single(N), make_params(h->cfg), d_xu, d_ee_ref, d_xs, nullptr, d_G, d_C,
+                          d_g, d_c, (hipStream_t)stream));
+  return MPC_OK;
+}
+
+int mpc_form_schur(mpc_handle *h, uint32_t N, float *d_G, const float *d_C, const float *d_g,
+                   const float *d_c, float *d_S, float *d_Pinv, float *d_gamma, float rho,
+                   mpc_stream_t stream) {
+  if (!h || !n_ok(h, N) || !d_G || !d_C || !d_g || !d_c || !d_S || !d_Pinv || !d_gamma ||
+      !(rho >= 0))
+    return MPC_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  HIP_TRY(launch_form_schur(single(N), d_G, d_C, d_g, d_c, nullptr, rho, nullptr, d_S, d_Pinv, d_gamma,
+                            h->Ginv, 1, st));
+  HIP_TRY(hipMemcpyAsync(d_G, h->Ginv, sizeof(float) * (G_STRIDE * (size_t)N - 49),
+                         hipMemcpyDeviceToDevice, st));
+  return MPC_OK;
+}
+
+int mpc_pcg_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_Pinv,
+                  const float *d_gamma, float *d_lambda, uint32_t max_iter, float exit_tol,
+                  uint32_t *d_iters, uint32_t *d_max_iter_exit, mpc_stream_t stream) {
+  if (!h || !n_ok(h, N) || !d_S || !d_Pinv || !d_gamma || !d_lambda || !(exit_tol >= 0))
+    return MPC_ERR_ARG;
+  HIP_TRY(launch_pcg(single(N), N, d_S, d_Pinv, d_gamma, d_lambda, max_iter, exit_tol, nullptr,
+                     d_iters, d_max_iter_exit, (hipStream_t)stream));
+  return MPC_OK;
+}
+
+int mpc_compute_dz(mpc_handle *h, uint32_t N, const float *d_Ginv, const float *d_C,
+                   const float *d_g, const float *d_lambda, float *d_dz, mpc_stream_t stream) {
+  if (!h || !n_ok(h, N) || !d_Ginv || !d_C || !d_g || !d_lambda || !d_dz) return MPC_ERR_ARG;
+  HIP_TRY(launch_compute_dz(single(N), d_Ginv, d_C, d_g, d_lambda, nullptr, d_dz, (hipStream_t)stream));
+  return MPC_OK;
+}
+
+int mpc_compute_merit(mpc_handle *h, uint32_t N, const float *d_xu, const float *d_ee_ref,
+                      float *d_merit_out, mpc_stream_t stream) {
+  if (!h || !n_ok(h, N) || !d_xu || !d_ee_ref || !d_merit_out) return MPC_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const Batch b = single(N);
+  HIP_TRY(launch_merit(b, make_params(h->cfg), d_xu, nullptr, d_ee_ref, h->xs, nullptr, 8, 1,
+                       h->knot_merits, st));
+  HIP_TRY(launch_merit_reduce(b, h->knot_merits, 1, d_merit_out, st));
+  return MPC_OK;
+}
+
+int mpc_line_search_merits(mpc_handle *h, uint32_t N, const float *d_xs, const float *d_xu,
+                           const float *d_ee_ref, const float *d_dz, float *d_merits_out,
+                           mpc_stream_t stream) {
+  if (!h || !n_ok(h, N) || !d_xs || !d_xu || !d_ee_ref || !d_dz || !d_merits_out) return MPC_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const Batch b = single(N);
+  HIP_TRY(launch_merit(b, make_params(h->cfg), d_xu, d_dz, d_ee_ref, d_xs, nullptr, 0, 8,
+                       h->knot_merits, st));
+  HIP_TRY(launch_merit_reduce(b, h->knot_merits, 8, d_merits_out, st));
+  return MPC_OK;
+}
+
+// ---- plain SQP (one instance, host decisions as in the reference) -----------
+
+int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lambda, float *d_xu,
+                  float *rho, float rho_reset, mpc_sqp_stats *stats, mpc_stream_t stream) {
+  if (!h || !n_ok(h, N) || !d_ee_ref || !d_lambda || !d_xu || !rho || !(*rho > 0)) return MPC_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const mpc_config &cfg = h->cfg;
+  const Params prm = make_params(cfg);
+  const Batch b = single(N);
+  mpc_sqp_stats local;
+  mpc_sqp_stats &s = stats ? *stats : local;
+  std::memset(&s, 0, sizeof(s));
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint32_t len = NXU * N - NU;
+
+  HIP_TRY(hipMemcpyAsync(h->xs, d_xu, sizeof(float) * NX, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(launch_merit(b, prm, d_xu, nullptr, d_ee_ref, h->xs, nullptr, 8, 1, h->knot_merits, st));
+  HIP_TRY(launch_merit_reduce(b, h->knot_merits, 1, h->merits, st));
+  HIP_TRY(hipMemcpyAsync(h->h_merits, h->merits, sizeof(float), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  float merit0 = h->h_merits[0];
+  float drho = 1.f;
+  const float rf = cfg.rho_factor;
+  uint32_t sqp_iter = 0;
+  bool not_aborted = true;
+  for (uint32_t it = 0; it < cfg.sqp_max_iter; ++it) {
+    HIP_TRY(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, nullptr, h->G, h->C, h->g, h->c, st));
+    HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, h->c, nullptr, *rho, nullptr, h->S, h->P, h->gamma,
+                              h->Ginv, 0, st));
+    HIP_TRY(hipEventRecord(h->ev[0], st));
+    HIP_TRY(launch_pcg(b, N, h->S, h->P, h->gamma, d_lambda, cfg.pcg_max_iter, cfg.pcg_exit_tol,
+                       nullptr, h->pcg_iters, h->pcg_flag, st));
+    HIP_TRY(hipEventRecord(h->ev[1], st));
+    HIP_TRY(hipMemcpyAsync(h->h_pcg, h->pcg_iters, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(h->h_pcg + 1, h->pcg_flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, nullptr, h->dz, st));
+    HIP_TRY(launch_merit(b, prm, d_xu, h->dz, d_ee_ref, h->xs, nullptr, 0, 8, h->knot_merits, st));
+    HIP_TRY(launch_merit_reduce(b, h->knot_merits, 8, h->merits, st));
+    HIP_TRY(hipMemcpyAsync(h->h_merits, h->merits, sizeof(float) * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+    s.pcg_iters[it] = h->h_pcg[0];
+    s.pcg_maxiter[it] = h->h_pcg[1];
+    s.linsys_us[it] = 1e3 * ms;
+    int step = -1;
+    float minm = merit0;
+    for (int i = 0; i < 8; ++i)
+      if (h->h_merits[i] < minm) {
+        minm = h->h_merits[i];
+        step = i;
+      }
+    s.ls_step[it] = step;
+    if (step < 0) {
+      drho = std::fmax(drho * rf, rf);
+      *rho = std::fmax(*rho * drho, cfg.rho_min);
+      sqp_iter++;
+      if (*rho > cfg.rho_max) {
+        not_aborted = false;
+        *rho = rho_reset;
+        break;
+      }
+      continue;
+    }
+    const float alpha = -1.f / (float)(1u << step);
+    drho = std::fmin(drho / rf, 1.f / rf);
+    *rho = std::fmax(*rho * drho, cfg.rho_min);
+    HIP_TRY(launch_axpy(len, alpha, h->dz, d_xu, st));
+    sqp_iter++;
+    merit0 = minm;
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  s.sqp_iters = sqp_iter;
+  s.not_aborted = not_aborted ? 1u : 0u;
+  s.sqp_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  return MPC_OK;
+}
+
+// ---- batched SQP ------------------------------------------------------------
+
+int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
+  if (!h || !horizons || n_inst == 0 || n_inst > h->cfg.max_instances) return MPC_ERR_ARG;
+  std::vector<uint32_t> k0(n_inst + 1, 0);
+  uint32_t max_n = 0;
+  for (uint32_t i = 0; i < n_inst; ++i) {
+    if (horizons[i] < 2 || horizons[i] > h->cfg.max_knots) return MPC_ERR_ARG;
+    k0[i + 1] = k0[i] + horizons[i];
+    if (k0[i + 1] > h->cfg.max_total_knots) return MPC_ERR_ARG;
+    max_n = horizons[i] > max_n ? horizons[i] : max_n;
+  }
+  const uint32_t T = k0[n_inst];
+  std::vector<uint32_t> ki(T);
+  for (uint32_t i = 0; i < n_inst; ++i)
+    for (uint32_t k = k0[i]; k < k0[i + 1]; ++k) ki[k] = i;
+  HIP_TRY(hipMemcpy(h->knot_inst, ki.data(), sizeof(uint32_t) * T, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->inst_k0, k0.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice));
+  h->horizons.assign(horizons, horizons + n_inst);
+  h->k0 = k0;
+  h->n_inst = n_inst;
+  h->total = T;
+  h->max_n = max_n;
+  h->batch_ready = true;
+  return MPC_OK;
+}
+
+int mpc_batch_set_timing(mpc_handle *h, int enable) {
+  if (!h) return MPC_ERR_ARG;
+  h->timing = enable != 0;
+  return MPC_OK;
+}
+
+int mpc_batch_phase_times(mpc_handle *h, double *us6) {
+  if (!h || !us6) return MPC_ERR_ARG;
+  for (int i = 0; i < 6; ++i) us6[i] = h->phase_us[i];
+  return MPC_OK;
+}
+
+int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee_ref, float *d_rho,
+                  uint32_t sqp_iters, uint32_t *d_status, uint32_t *d_pcg_iters, mpc_stream_t stream) {
+  if (!h || !d_xu || !d_lambda || !d_ee_ref || !d_rho || !d_status) return MPC_ERR_ARG;
+  if (!h->batch_ready) return MPC_ERR_NOT_READY;
+  if (sqp_iters > MPC_MAX_SQP_ITERS) return MPC_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const Batch b = batch_of(h);
+  const Params prm = make_params(h->cfg);
+  InstState s{d_rho, h->drho, h->merit0, d_status, h->sqp_done, h->xs};
+  const bool tm = h->timing;
+  for (double &v : h->phase_us) v = 0;
+
+  HIP_TRY(launch_merit(b, prm, d_xu, nullptr, d_ee_ref, h->xs, nullptr, 8, 1, h->knot_merits, st));
+  HIP_TRY(launch_batch_init(b, d_xu, h->knot_merits, s, st));
+  for (uint32_t it = 0; it < sqp_iters; ++it) {
+    if (tm) HIP_TRY(hipEventRecord(h->ev[0], st));
+    HIP_TRY(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, d_status, h->G, h->C, h->g, h->c, st));
+    if (tm) HIP_TRY(hipEventRecord(h->ev[1], st));
+    HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma,
+                              h->Ginv, 0, st));
+    if (tm) HIP_TRY(hipEventRecord(h->ev[2], st));
+    HIP_TRY(launch_pcg(b, h->max_n, h->S, h->P, h->gamma, d_lambda, h->cfg.pcg_max_iter,
+                       h->cfg.pcg_exit_tol, d_status, h->pcg_iters, h->pcg_flag, st));
+    if (tm) HIP_TRY(hipEventRecord(h->ev[3], st));
+    HIP_TRY(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
+    if (tm) HIP_TRY(hipEventRecord(h->ev[4], st));
+    HIP_TRY(launch_merit(b, prm, d_xu, h->dz, d_ee_ref, h->xs, d_status, 0, 8, h->knot_merits, st));
+    if (tm) HIP_TRY(hipEventRecord(h->ev[5], st));
+    HIP_TRY(launch_ls_update(b, prm, h->knot_merits, h->dz, d_xu, s, d_pcg_iters, h->pcg_iters,
+                             h->pcg_flag, it, sqp_iters, st));
+    if (tm) {
+      HIP_TRY(hipEventRecord(h->ev[6], st));
+      HIP_TRY(hipEventSynchronize(h->ev[6]));
+      for (int ph = 0; ph < 6; ++ph) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, h->ev[ph], h->ev[ph + 1]));
+        h->phase_us[ph] += 1e3 * ms;
+      }
+    }
+  }
+  return MPC_OK;
+}
+
+// ---- synthetic workload -------------------------------------------------------
+
+static inline uint64_t splitmix64(uint64_t &state) {
+  uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static inline double uniform01(uint64_t &state) { return (double)(splitmix64(state) >> 11) * 0x1.0p-53; }
+static inline double normal01(uint64_t &state) {
+  const double u1 = 1.0 - uniform01(state), u2 = uniform01(state);
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+}
+
+void mpc_synth_instance(uint64_t seed, uint32_t N, float dt, float *h_x0, float *h_qref) {
+  uint64_t st = seed;
+  double q0[NQ], qd0[NQ], f[NQ], ph[NQ];
+  for (int j = 0; j < NQ; ++j) q0[j] = -0.5 + uniform01(st);
+  for (int j = 0; j < NQ; ++j) qd0[j] = 0.1 * normal01(st);
+  for (int j = 0; j < NQ; ++j) f[j] = 0.2 + 0.8 * uniform01(st);
+  for (int j = 0; j < NQ; ++j) ph[j] = 2.0 * M_PI * uniform01(st);
+  if (h_x0)
+    for (int j = 0; j < NQ; ++j) {
+      h_x0[j] = (float)q0[j];
+      h_x0[NQ + j] = (float)qd0[j];
+    }
+  if (h_qref)
+    for (uint32_t k = 0; k < N; ++k)
+      for (int j = 0; j < NQ; ++j)
+        h_qref[(size_t)NQ * k + j] =
+            (float)(q0[j] + 0.3 * std::sin(2.0 * M_PI * f[j] * (double)k * (double)dt + ph[j]));
+}
+
+void mpc_synth_horizons(uint64_t seed, uint32_t n_inst, const uint32_t *choices, uint32_t n_choices,
+                        uint32_t *horizons) {
+  if (!choices || !horizons || n_choices == 0) return;
+  uint64_t st = seed;
+  for (uint32_t i = 0; i < n_inst; ++i) {
+    uint32_t idx = (uint32_t)(uniform01(st) * n_choices);
+    if (idx >= n_choices) idx = n_choices - 1;
+    horizons[i] = choices[idx];
+  }
+}
+
+}  // extern "C"

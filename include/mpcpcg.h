@@ -1,0 +1,211 @@
+/*
+ * mpcpcg.h -- C-ABI of the MI355X-native SQP tracking-MPC inner loop
+ * (KUKA iiwa 7-DOF end-effector tracking; nx = 14, nu = 7, fp32).
+ *
+ * Every entry point below replaces one piece of the reference call surface
+ * of EmreAdabag/gpu-pcg-tracking (file:line cited per function).  Buffers
+ * use the reference's layouts, all fp32, column-major 14x14 / 14x7 / 7x7
+ * blocks, so fixtures and buffers are interchangeable:
+ *
+ *   xu      21N-7   [x_0 u_0 x_1 u_1 ... x_{N-1}]           (toplevel.cuh:48)
+ *   ee_ref  6N      [xyz rpy] per knot; only xyz enters the cost
+ *   lambda  14N     Schur multipliers
+ *   G       245N-49 per knot [Q_k 196 | R_k 49], last knot Q only
+ *   C       294(N-1) per knot [-A_k 196 | -B_k 98]
+ *   g       21N-7   per knot [q_k 14 | r_k 7], last knot q only
+ *   c       14N     c_0 = x_0 - x_s, c_{k+1} = x_{k+1} - (x_k + dt f(x_k,u_k))
+ *   S, Pinv 588N    block row b = [S(b,b-1) | S(b,b) | S(b,b+1)], 196 each
+ *   gamma   14N
+ *   dz      21N-7
+ *
+ * All device pointers are hipMalloc'd device memory owned by the caller.
+ * All entry points are stream-ordered on the given stream (NULL = default
+ * stream); none of them synchronises the device.  A handle is used by one
+ * host thread at a time.  Errors are returned as MPC_* status codes; no
+ * entry point exits the process.
+ */
+#ifndef MPCPCG_H
+#define MPCPCG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *mpc_stream_t; /* == hipStream_t */
+
+enum {
+  MPC_OK = 0,
+  MPC_ERR_ARG = 1,         /* bad argument (null pointer, N < 2, size beyond capacity) */
+  MPC_ERR_HIP = 2,         /* a HIP runtime call failed */
+  MPC_ERR_NOT_READY = 3,   /* batch layout not set up */
+  MPC_ERR_NONFINITE = 4,   /* non-finite value detected */
+  MPC_ERR_NO_DEVICE = 5    /* no usable GPU */
+};
+
+/* per-instance status bits (batched solve) */
+enum {
+  MPC_ST_ACTIVE = 1,       /* still iterating */
+  MPC_ST_PCG_MAXITER = 2,  /* some PCG solve hit max_iter */
+  MPC_ST_RHO_ABORT = 4,    /* line search failed with rho > rho_max (toplevel.cuh:396-407) */
+  MPC_ST_NONFINITE = 8     /* a merit or step was not finite */
+};
+
+/* Runtime configuration.  Reference equivalents: settings.cuh and the -D
+ * flags of perform_experiment.sh:42-59; pcg_config (toplevel.cuh:573-576). */
+typedef struct mpc_config {
+  uint32_t max_knots;        /* largest N of one instance */
+  uint32_t max_instances;    /* batch capacity */
+  uint32_t max_total_knots;  /* capacity of sum N_i over a batch */
+  float dt;                  /* integrator step (runme.cu:65) */
+  float q_ee;                /* end-effector position weight (reference: 1) */
+  float qd_cost;             /* QD_COST */
+  float r_cost;              /* R_COST */
+  float mu;                  /* merit penalty, 10 (toplevel.cuh:54) */
+  float rho_factor;          /* RHO_FACTOR */
+  float rho_max;             /* RHO_MAX */
+  float rho_min;             /* RHO_MIN (settings.cuh:50) */
+  float rho_reset;           /* rho after an abort in batched solves (track passes 1e-3, toplevel.cuh:579) */
+  float pcg_exit_tol;        /* pcg_config.pcg_exit_tol */
+  uint32_t pcg_max_iter;     /* pcg_config.pcg_max_iter */
+  uint32_t sqp_max_iter;     /* SQP_MAX_ITER */
+  uint32_t flags;            /* MPC_FLAG_* */
+} mpc_config;
+
+/* The terminal knot's cost uses x_{N-1} instead of the reference's x_{N-2}
+ * (rbdfiles/iiwa_eepos_plant.cuh:399 passes the knot N-2 state). */
+#define MPC_FLAG_LAST_KNOT_OWN_STATE 1u
+
+typedef struct mpc_handle mpc_handle;
+
+/* Statistics of one plain SQP solve; mirrors the tuple returned by sqpSolve
+ * (toplevel.cuh:502): per-iteration PCG iterations and max-iter flags,
+ * per-iteration linear-system time (PCG kernel, device events), SQP time,
+ * SQP iterations, and whether the solve ended without a rho abort. */
+#define MPC_MAX_SQP_ITERS 64
+typedef struct mpc_sqp_stats {
+  uint32_t sqp_iters;
+  uint32_t not_aborted;
+  double sqp_us;
+  uint32_t pcg_iters[MPC_MAX_SQP_ITERS];
+  uint32_t pcg_maxiter[MPC_MAX_SQP_ITERS];
+  double linsys_us[MPC_MAX_SQP_ITERS];
+  int32_t ls_step[MPC_MAX_SQP_ITERS];   /* accepted alpha index, -1 = failure */
+} mpc_sqp_stats;
+
+/* Fill *cfg with the defaults of the published workload: dt = 0.01,
+ * Q_ee = 1, QD = 1e-2, R = 1e-4, mu = 10, rho_factor 1.2, rho_max 10,
+ * rho_min 1e-3, rho_reset 1e-3, tol 1e-5, max_iter 173, 3 SQP iterations. */
+void mpc_default_config(mpc_config *cfg);
+
+/* Create / destroy a handle; all workspace for max_knots / max_instances /
+ * max_total_knots is allocated here, none inside the solves.  Replaces the
+ * per-call allocations of sqpSolve (toplevel.cuh:67-215) and
+ * gato_plant::initializeDynamicsConstMem (rbdfiles/iiwa_eepos_plant.cuh:66-74). */
+int mpc_create(const mpc_config *cfg, mpc_handle **out);
+int mpc_destroy(mpc_handle *h);
+int mpc_get_config(const mpc_handle *h, mpc_config *cfg);
+/* Update the numeric parameters (not the capacities) of a handle. */
+int mpc_set_config(mpc_handle *h, const mpc_config *cfg);
+const char *mpc_status_string(int status);
+
+/* ---- robot model (GRiD-level) -------------------------------------------- */
+
+/* End-effector xyz + rpy of n joint vectors (stride q_stride floats).
+ * grid::end_effector_positions_kernel (rbdfiles/iiwa_eepos_grid.cuh:2141-2174). */
+int mpc_fk(mpc_handle *h, uint32_t n, const float *d_q, uint32_t q_stride, float *d_ee,
+           mpc_stream_t stream);
+
+/* qdd (7 per point) and df_du (7x21 per point: dqdd/dq, dqdd/dqd, dqdd/du)
+ * for n points [q qd u] (21 floats each).
+ * gato_plant::forwardDynamicsAndGradient (rbdfiles/iiwa_eepos_plant.cuh:126-156). */
+int mpc_dynamics_gradient(mpc_handle *h, uint32_t n, const float *d_qqdu, float *d_qdd,
+                          float *d_df_du, mpc_stream_t stream);
+
+/* ---- stage entry points (one instance, reference layouts) ---------------- */
+
+/* KKT blocks G, C, g, c from the trajectory.  gato_form_kkt (schur.cuh:391-524). */
+int mpc_form_kkt(mpc_handle *h, uint32_t N, const float *d_xu, const float *d_ee_ref,
+                 const float *d_xs, float *d_G, float *d_C, float *d_g, float *d_c,
+                 mpc_stream_t stream);
+
+/* Schur complement S, block-Jacobi Pinv (off-diagonal blocks written as 0),
+ * gamma, and G overwritten with the regularised inverses (Q_k+rho I)^-1,
+ * (R_k+rho I)^-1 as the reference does.  form_schur (schur.cuh:545-559). */
+int mpc_form_schur(mpc_handle *h, uint32_t N, float *d_G, const float *d_C, const float *d_g,
+                   const float *d_c, float *d_S, float *d_Pinv, float *d_gamma, float rho,
+                   mpc_stream_t stream);
+
+/* PCG on S lambda = gamma, warm-started from lambda.  Exit when
+ * |r^T Pinv r| < exit_tol (also tested before the first iteration); writes
+ * the iteration count and a max-iter flag (1 = max_iter reached).
+ * GPU-PCG pcg<T,14,N> (call site toplevel.cuh:143-157, :292-294). */
+int mpc_pcg_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_Pinv,
+                  const float *d_gamma, float *d_lambda, uint32_t max_iter, float exit_tol,
+                  uint32_t *d_iters, uint32_t *d_max_iter_exit, mpc_stream_t stream);
+
+/* dz from G^-1, C, g, lambda.  compute_dz (schur.cuh:562-566). */
+int mpc_compute_dz(mpc_handle *h, uint32_t N, const float *d_Ginv, const float *d_C,
+                   const float *d_g, const float *d_lambda, float *d_dz, mpc_stream_t stream);
+
+/* Merit of xu (no initial-state term).  compute_merit (merit.cuh:99-145). */
+int mpc_compute_merit(mpc_handle *h, uint32_t N, const float *d_xu, const float *d_ee_ref,
+                      float *d_merit_out, mpc_stream_t stream);
+
+/* The 8 line-search merits of xu + alpha_p dz, alpha_p = -2^-p, including
+ * mu |x_0 + alpha_p dz_0 - x_s|_1.  ls_gato_compute_merit x 8
+ * (merit.cuh:17-94, toplevel.cuh:357-374). */
+int mpc_line_search_merits(mpc_handle *h, uint32_t N, const float *d_xs, const float *d_xu,
+                           const float *d_ee_ref, const float *d_dz, float *d_merits_out,
+                           mpc_stream_t stream);
+
+/* ---- full solves ----------------------------------------------------------- */
+
+/* One SQP solve of one instance; drop-in for sqpSolve (toplevel.cuh:24-503).
+ * d_xu (in/out; x_0 is the current state), d_lambda (in/out warm start),
+ * *rho (in/out).  Host-synchronising once per SQP iteration, like the
+ * reference, to report per-iteration statistics. */
+int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lambda,
+                  float *d_xu, float *rho, float rho_reset, mpc_sqp_stats *stats,
+                  mpc_stream_t stream);
+
+/* Batched solves of many instances with uneven horizons (no reference
+ * counterpart: the reference solves one instance per call).  Instance i has
+ * horizon horizons[i]; its buffers are packed back to back in the
+ * reference per-instance layouts: xu at 21*K_i - 7*i, lambda at 14*K_i,
+ * ee_ref at 6*K_i, with K_i = sum_{j<i} horizons[j]. */
+int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons);
+
+/* Run sqp_iters SQP iterations of every instance, entirely stream-ordered
+ * on the device (line-search acceptance and rho rule on the device).
+ * d_rho (n_inst floats, in/out), d_status (n_inst uint32, out: MPC_ST_*),
+ * d_pcg_iters (n_inst x sqp_iters uint32, out, may be NULL). */
+int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee_ref, float *d_rho,
+                  uint32_t sqp_iters, uint32_t *d_status, uint32_t *d_pcg_iters,
+                  mpc_stream_t stream);
+
+/* Device-event timing of the phases of the last mpc_batch_sqp call made
+ * with timing enabled (mpc_batch_set_timing(h, 1)): microseconds spent in
+ * linearisation, Schur formation, PCG, dz, line search, update. */
+int mpc_batch_set_timing(mpc_handle *h, int enable);
+int mpc_batch_phase_times(mpc_handle *h, double *us6);
+
+/* ---- synthetic workload (seeded, reproducible) ----------------------------- */
+
+/* Joint-space data of one synthetic instance: splitmix64 stream seeded with
+ * seed; q0 ~ U(-0.5,0.5)^7, qd0 ~ N(0,0.1^2)^7, f ~ U(0.2,1)^7 Hz,
+ * phi ~ U(0,2 pi)^7; q_ref(k) = q0 + 0.3 sin(2 pi f k dt + phi), k < N.
+ * Writes x0 (14) and q_ref (7N).  Host-only, no GPU needed. */
+void mpc_synth_instance(uint64_t seed, uint32_t N, float dt, float *h_x0, float *h_qref);
+
+/* Horizons drawn uniformly from choices[0..n_choices) with a splitmix64
+ * stream seeded with seed.  Host-only. */
+void mpc_synth_horizons(uint64_t seed, uint32_t n_inst, const uint32_t *choices,
+                        uint32_t n_choices, uint32_t *horizons);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPCPCG_H */
