@@ -1,0 +1,228 @@
+"""Benchmark of the batched SQP tracking-MPC inner loop on one MI355X.
+
+Workload (the batched configuration): 1024 independent KUKA iiwa
+end-effector tracking instances, horizons drawn uniformly from
+{32, 64, 128, 256}, dt = 0.01 s, Q_ee = 1, QD = 1e-2, R = 1e-4, fp32,
+block-Jacobi PCG with exit tolerance 1e-5, 3 SQP iterations per solve.
+Synthetic seeded inputs (splitmix64): q0 ~ U(-0.5, 0.5), qd0 ~ N(0, 0.1^2),
+reference = FK of joint sinusoids (amplitude 0.3 rad, f ~ U(0.2, 1) Hz,
+random phase); start trajectory x_k = x0, u_k = 0, lambda = 0, rho = 1e-3.
+
+One step = one batched SQP solve (3 SQP iterations, every phase on the
+device) over all instances, starting from the same initial iterate; each
+SQP iteration of an instance is one KKT solve (linearisation, Schur +
+block-Jacobi, PCG, dz, 8-alpha line search, update).  value = KKT
+solves / s over the whole step.  Also reported: the plain
+one-instance-per-launch solve (reference call pattern) on a sample, the
+per-phase split, the roofline of the dominant kernel and a CPU baseline
+(the sparse LDL^T solve of the Schur system, fp32, one core).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+PHASES = ("linearize", "schur", "pcg", "dz", "merit", "update")
+# algorithmic bytes of one PCG iteration per knot (fp32): S diagonal upper
+# triangle 105 + one off-diagonal block 196 + Jacobi upper triangle 105
+# floats, plus 12 x 14 vector floats
+PCG_BYTES_PER_KNOT_ITER = 4 * (105 + 196 + 105 + 12 * 14)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--instances", type=int, default=1024)
+    ap.add_argument("--sqp-iters", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def build_workload(pkg, solver, torch, n, seed, dt):
+    horizons = pkg.synth_horizons(seed, n).astype(np.int64)
+    xus, qrefs, offs = [], [], []
+    for i, N in enumerate(horizons):
+        x0, qref = pkg.synth_instance(seed * 7919 + i, int(N), dt)
+        xu = np.zeros(21 * N - 7, np.float32)
+        for k in range(N):
+            xu[21 * k:21 * k + 14] = x0
+        xus.append(xu)
+        qrefs.append(qref)
+    q_all = torch.from_numpy(np.concatenate(qrefs)).cuda()
+    ee = solver.fk(q_all).reshape(-1).contiguous()  # device FK of the reference path
+    xu = torch.from_numpy(np.concatenate(xus)).cuda()
+    return horizons, xu, ee
+
+
+def cpu_baseline(orc, helpers, horizons, seconds, seed):
+    """fp32 sparse LDL^T factor + solve of the Schur system (oracle, one
+    core) on a sample of the workload's instances, weighted by the
+    horizon mix.  Returns KKT solves / s and the sample description."""
+    cfg = orc.make_cfg()
+    per_n = {}
+    for N in sorted(set(int(h) for h in horizons)):
+        x0, ee, xu = helpers.make_instance(orc, seed + N, N)
+        G, Cm, g, c = orc.form_kkt(cfg, N, xu.astype(np.float64), ee.astype(np.float64),
+                                   xu[:14].astype(np.float64), dtype=np.float32)
+        S, P, gam, _ = orc.form_schur(N, G, Cm, g, c, 1e-3, dtype=np.float32)
+        per_n[N] = (S, gam)
+    budget = seconds / len(per_n)
+    t_per = {}
+    for N, (S, gam) in per_n.items():
+        reps, el = 1, 0.0
+        while el < budget:
+            el, _ = orc.ldl_bench(N, S, gam, reps)
+            if el < budget:
+                reps = max(reps * 2, int(reps * budget / max(el, 1e-6)))
+        t_per[N] = el / reps
+    mean_t = float(np.mean([t_per[int(h)] for h in horizons]))
+    desc = "; ".join(f"N={N}: {t_per[N] * 1e6:.0f} us" for N in sorted(t_per))
+    return 1.0 / mean_t, desc, t_per
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: every instance is independent, so N GPUs would "
+                                   "split the instance list (see DESIGN.md); only --gpus 1 is measured"}))
+        sys.exit(2)
+    import torch
+    import helpers
+    pkg = helpers.load_pkg()
+
+    dt = 0.01
+    n = args.instances
+    cfg = pkg.default_config(max_knots=256, max_instances=n, max_total_knots=256 * n, sqp_max_iter=args.sqp_iters)
+    solver = pkg.Solver(cfg)
+    horizons, xu0, ee = build_workload(pkg, solver, torch, n, args.seed, dt)
+    T = int(horizons.sum())
+    solver.batch_setup(horizons.tolist())
+    xu = torch.empty_like(xu0)
+    lam = torch.zeros(14 * T, device="cuda")
+    rho = torch.empty(n, device="cuda")
+    status = torch.zeros(n, dtype=torch.int32, device="cuda")
+    pcg_log = torch.zeros(n * args.sqp_iters, dtype=torch.int32, device="cuda")
+    K = args.sqp_iters
+
+    def step():
+        xu.copy_(xu0)
+        lam.zero_()
+        rho.fill_(1e-3)
+        pcg_log.fill_(-1)
+        solver.batch_sqp(xu, lam, ee, rho, K, status, pcg_log)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_wall = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    ms_total = e0.elapsed_time(e1)
+    ms_step = ms_total / args.steps
+    log = pcg_log.cpu().numpy().reshape(n, K)
+    st = status.cpu().numpy()
+    executed = log >= 0
+    kkt_per_step = int(executed.sum())
+    value = kkt_per_step / (ms_step * 1e-3)
+
+    # per-phase split and PCG roofline (timed run: events between phases)
+    solver.batch_set_timing(True)
+    step()
+    torch.cuda.synchronize()
+    ph = solver.batch_phase_times()
+    solver.batch_set_timing(False)
+    log_t = pcg_log.cpu().numpy().reshape(n, K)
+    knot_iters = float(sum(int(horizons[i]) * int(log_t[i, j]) for i in range(n) for j in range(K) if log_t[i, j] >= 0))
+    pcg_us_per_launch = ph[2] / K
+    pcg_bytes_per_launch = PCG_BYTES_PER_KNOT_ITER * knot_iters / K
+    achieved = pcg_bytes_per_launch / (pcg_us_per_launch * 1e-6) / 1e9
+    phases = {name: round(v / K, 2) for name, v in zip(PHASES, ph)}
+    kkt_phase_us = (ph[1] + ph[2] + ph[3]) / K
+
+    # plain: one instance per launch sequence, host decisions (sqpSolve pattern)
+    rng = np.random.default_rng(args.seed)
+    sample = rng.choice(n, size=min(args.plain_sample, n), replace=False)
+    offs_xu = np.concatenate([[0], np.cumsum(21 * horizons - 7)])
+    offs_k = np.concatenate([[0], np.cumsum(horizons)])
+    plain_iters, plain_s = 0, 0.0
+    for i in sample:
+        N = int(horizons[i])
+        pxu = xu0[offs_xu[i]:offs_xu[i + 1]].clone()
+        pee = ee[6 * offs_k[i]:6 * offs_k[i + 1]].clone()
+        plam = torch.zeros(14 * N, device="cuda")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = solver.sqp_solve(N, pee, plam, pxu, 1e-3)
+        torch.cuda.synchronize()
+        plain_s += time.perf_counter() - t
+        plain_iters += out[3]
+    plain_value = plain_iters / plain_s
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle as orc
+        cval, desc, _ = cpu_baseline(orc, helpers, horizons, args.cpu_seconds, args.seed)
+        cpu = {"value": round(cval, 1), "unit": "KKT solves/s", "cores": 1, "kind": "port",
+               "sample": f"fp32 sparse LDL^T factor+solve of the Schur system (oracle, reference CSC pattern), "
+                         f"one instance per horizon at its initial iterate, weighted by the workload's horizon mix; "
+                         f"{desc}; ~{args.cpu_seconds:.0f} s of CPU work"}
+
+    its = log[executed]
+    out = {
+        "metric": "batched KKT solves/sec & SQP iters/sec vs plain at tol 1e-5; adjoint/fwd cost",
+        "value": round(value, 1),
+        "unit": "KKT solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded splitmix64 start states and sinusoid EE references)",
+        "config": {"workload": "batched SQP: 1024 KUKA iiwa EE-tracking instances, N ~ U{32,64,128,256}, "
+                               "3 SQP iterations, block-Jacobi PCG tol 1e-5",
+                   "instances": n, "total_knots": T, "sqp_iters": K, "pcg_exit_tol": cfg.pcg_exit_tol,
+                   "pcg_max_iter": cfg.pcg_max_iter},
+        "sqp_iters_per_sec": round(value, 1),
+        "kkt_phase_solves_per_sec": round(kkt_per_step / (kkt_phase_us * 1e-6), 1),
+        "plain_kkt_solves_per_sec": round(plain_value, 1),
+        "speedup_vs_plain": round(value / plain_value, 2),
+        "phase_us_per_sqp_iter": phases,
+        "pcg_iters": {"mean": round(float(its.mean()), 1), "max": int(its.max()), "min": int(its.min()),
+                      "hit_max_iter": int((st & 2 != 0).sum())},
+        "status": {"aborted": int((st & 4 != 0).sum()), "nonfinite": int((st & 8 != 0).sum())},
+        "roofline": {"kernel": "k_pcg", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(pcg_bytes_per_launch),
+                     "avg_launch_us": round(pcg_us_per_launch, 2)},
+        "cpu_baseline": cpu,
+        "wall_s_timed": round(t_wall, 3),
+    }
+    print(json.dumps(out))
+    solver.close()
+
+
+if __name__ == "__main__":
+    main()

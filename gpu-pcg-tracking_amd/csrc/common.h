@@ -93,10 +93,22 @@ hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, con
                              hipStream_t st);
 hipError_t launch_compute_dz(const Batch &b, const float *Ginv, const float *C, const float *g,
                              const float *lambda, const uint32_t *status, float *dz, hipStream_t st);
-hipError_t launch_pcg(const Batch &b, uint32_t max_n, const float *S, const float *Pinv,
-                      const float *gamma, float *lambda, uint32_t max_iter, float tol,
-                      const uint32_t *status, uint32_t *iters, uint32_t *maxiter_flag,
-                      hipStream_t st);
+// PCG streaming S from memory every iteration (any N <= 512); inst_list
+// (n_list entries) restricts the launch to some instances of the batch.
+hipError_t launch_pcg_stream(const Batch &b, uint32_t max_n, const float *S, const float *Pinv,
+                             const float *gamma, float *lambda, uint32_t max_iter, float tol,
+                             const uint32_t *status, const uint32_t *inst_list, uint32_t n_list,
+                             uint32_t *iters, uint32_t *flag, hipStream_t st);
+// PCG with S resident on chip (N <= pcg_res_max_knots()); one workgroup per
+// bin of instances.  items[bin_off[w] .. bin_off[w+1]) = {inst, first group,
+// groups, 0}; items == nullptr: a single instance (0) in one bin.
+hipError_t launch_pcg_res(const Batch &b, const uint4 *items, const uint32_t *bin_off, uint32_t n_bins,
+                          const float *S, const float *Pinv, const float *gamma, float *lambda,
+                          uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters,
+                          uint32_t *flag, hipStream_t st);
+uint32_t pcg_res_groups(uint32_t N);
+uint32_t pcg_res_max_knots();
+uint32_t pcg_res_max_items();
 hipError_t launch_merit_reduce(const Batch &b, const float *knot_merits, int np, float *out,
                                hipStream_t st);
 hipError_t launch_ls_update(const Batch &b, const Params &p, const float *knot_merits,

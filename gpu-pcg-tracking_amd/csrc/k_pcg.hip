@@ -57,12 +57,13 @@ __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restric
                                                  const float *__restrict__ gamma,
                                                  float *__restrict__ lambda, uint32_t max_iter,
                                                  float tol, const uint32_t *__restrict__ status,
+                                                 const uint32_t *__restrict__ inst_list,
                                                  uint32_t *__restrict__ iters_out,
                                                  uint32_t *__restrict__ flag_out) {
   constexpr int NT = NG * 16;
   constexpr int NW = NT / 64;
   extern __shared__ float smem[];
-  const uint32_t inst = blockIdx.x;
+  const uint32_t inst = inst_list ? inst_list[blockIdx.x] : blockIdx.x;
   if (status && !(status[inst] & ST_ACTIVE)) return;  // uniform over the workgroup
   const uint32_t K0 = inst_first(b, inst);
   const uint32_t N = inst_len(b, inst);
@@ -180,27 +181,32 @@ __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restric
 template <int NG, int R>
 hipError_t launch_pcg_t(const Batch &b, uint32_t max_n, const float *S, const float *Pinv,
                         const float *gamma, float *lambda, uint32_t max_iter, float tol,
-                        const uint32_t *status, uint32_t *iters, uint32_t *flag, hipStream_t st) {
+                        const uint32_t *status, const uint32_t *inst_list, uint32_t n_list,
+                        uint32_t *iters, uint32_t *flag, hipStream_t st) {
   const size_t smem = sizeof(float) * (2 * NX * (size_t)max_n + 2 * (NG * 16 / 64));
-  hipLaunchKernelGGL((k_pcg<NG, R>), dim3(b.n_inst), dim3(NG * 16), smem, st, b, S, Pinv, gamma, lambda,
-                     max_iter, tol, status, iters, flag);
+  const uint32_t grid = inst_list ? n_list : b.n_inst;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_pcg<NG, R>), dim3(grid), dim3(NG * 16), smem, st, b, S, Pinv, gamma, lambda,
+                     max_iter, tol, status, inst_list, iters, flag);
   return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_pcg(const Batch &b, uint32_t max_n, const float *S, const float *Pinv,
-                      const float *gamma, float *lambda, uint32_t max_iter, float tol,
-                      const uint32_t *status, uint32_t *iters, uint32_t *flag, hipStream_t st) {
+hipError_t launch_pcg_stream(const Batch &b, uint32_t max_n, const float *S, const float *Pinv,
+                             const float *gamma, float *lambda, uint32_t max_iter, float tol,
+                             const uint32_t *status, const uint32_t *inst_list, uint32_t n_list,
+                             uint32_t *iters, uint32_t *flag, hipStream_t st) {
   if (b.n_inst == 0) return hipSuccess;
+#define MPC_PCG_ARGS b, max_n, S, Pinv, gamma, lambda, max_iter, tol, status, inst_list, n_list, iters, flag, st
   // 16 groups (256 lanes) up to N = 256, then 32 / 64 groups; R = rows per group.
-  if (max_n <= 16) return launch_pcg_t<16, 1>(b, max_n, S, Pinv, gamma, lambda, max_iter, tol, status, iters, flag, st);
-  if (max_n <= 32) return launch_pcg_t<16, 2>(b, max_n, S, Pinv, gamma, lambda, max_iter, tol, status, iters, flag, st);
-  if (max_n <= 64) return launch_pcg_t<16, 4>(b, max_n, S, Pinv, gamma, lambda, max_iter, tol, status, iters, flag, st);
-  if (max_n <= 128) return launch_pcg_t<16, 8>(b, max_n, S, Pinv, gamma, lambda, max_iter, tol, status, iters, flag, st);
-  if (max_n <= 256) return launch_pcg_t<16, 16>(b, max_n, S, Pinv, gamma, lambda, max_iter, tol, status, iters, flag, st);
-  if (max_n <= 512) return launch_pcg_t<32, 16>(b, max_n, S, Pinv, gamma, lambda, max_iter, tol, status, iters, flag, st);
-  if (max_n <= 1024) return launch_pcg_t<64, 16>(b, max_n, S, Pinv, gamma, lambda, max_iter, tol, status, iters, flag, st);
+  if (max_n <= 16) return launch_pcg_t<16, 1>(MPC_PCG_ARGS);
+  if (max_n <= 32) return launch_pcg_t<16, 2>(MPC_PCG_ARGS);
+  if (max_n <= 64) return launch_pcg_t<16, 4>(MPC_PCG_ARGS);
+  if (max_n <= 128) return launch_pcg_t<16, 8>(MPC_PCG_ARGS);
+  if (max_n <= 256) return launch_pcg_t<16, 16>(MPC_PCG_ARGS);
+  if (max_n <= 512) return launch_pcg_t<32, 16>(MPC_PCG_ARGS);
+#undef MPC_PCG_ARGS
   return hipErrorInvalidValue;
 }
 

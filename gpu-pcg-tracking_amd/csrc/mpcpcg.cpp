@@ -4,6 +4,7 @@
 // fully device-side SQP driver.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -35,6 +36,11 @@ struct mpc_handle {
   float *drho = nullptr, *merit0 = nullptr, *rho1 = nullptr;
   uint32_t *status_ws = nullptr, *sqp_done = nullptr, *pcg_iters = nullptr, *pcg_flag = nullptr;
   uint32_t *knot_inst = nullptr, *inst_k0 = nullptr;
+  // PCG dispatch of a batch: bins of small instances (resident kernel) and
+  // the list of long instances (streaming kernel)
+  uint4 *bin_items = nullptr;
+  uint32_t *bin_off = nullptr, *big_list = nullptr;
+  uint32_t n_bins = 0, n_big = 0, max_big_n = 0;
   // pinned staging for the host round trips of the plain driver
   float *h_merits = nullptr;
   uint32_t *h_pcg = nullptr;
@@ -67,6 +73,16 @@ static Params make_params(const mpc_config &c) {
 }
 
 static Batch single(uint32_t N) { return Batch{nullptr, nullptr, 1u, N}; }
+
+// PCG of one instance: resident kernel when its Schur system fits on chip
+static hipError_t pcg_single(uint32_t N, const float *S, const float *P, const float *gamma, float *lambda,
+                             uint32_t max_iter, float tol, uint32_t *iters, uint32_t *flag, hipStream_t st) {
+  if (N <= pcg_res_max_knots())
+    return launch_pcg_res(single(N), nullptr, nullptr, 1, S, P, gamma, lambda, max_iter, tol, nullptr, iters,
+                          flag, st);
+  return launch_pcg_stream(single(N), N, S, P, gamma, lambda, max_iter, tol, nullptr, nullptr, 0, iters, flag,
+                           st);
+}
 
 static Batch batch_of(const mpc_handle *h) {
   return Batch{h->knot_inst, h->inst_k0, h->n_inst, h->total};
@@ -116,7 +132,8 @@ const char *mpc_status_string(int s) {
 static void free_all(mpc_handle *h) {
   void *ptrs[] = {h->G, h->Ginv, h->C, h->g, h->c, h->S, h->P, h->gamma, h->dz, h->knot_merits,
                   h->merits, h->xs, h->drho, h->merit0, h->rho1, h->status_ws, h->sqp_done,
-                  h->pcg_iters, h->pcg_flag, h->knot_inst, h->inst_k0};
+                  h->pcg_iters, h->pcg_flag, h->knot_inst, h->inst_k0, h->bin_items,
+                  h->bin_off, h->big_list};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (h->h_merits) (void)hipHostFree(h->h_merits);
@@ -164,6 +181,9 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
             alloc((void **)&h->pcg_flag, sizeof(uint32_t) * B) &&
             alloc((void **)&h->knot_inst, sizeof(uint32_t) * T) &&
             alloc((void **)&h->inst_k0, sizeof(uint32_t) * (B + 1)) &&
+            alloc((void **)&h->bin_items, sizeof(uint4) * B) &&
+            alloc((void **)&h->bin_off, sizeof(uint32_t) * (B + 1)) &&
+            alloc((void **)&h->big_list, sizeof(uint32_t) * B) &&
             hipHostMalloc((void **)&h->h_merits, sizeof(float) * 16) == hipSuccess &&
             hipHostMalloc((void **)&h->h_pcg, sizeof(uint32_t) * 4) == hipSuccess &&
             hipHostMalloc((void **)&h->h_rho, sizeof(float) * 4) == hipSuccess;
@@ -252,8 +272,8 @@ int mpc_pcg_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_Pi
                   uint32_t *d_iters, uint32_t *d_max_iter_exit, mpc_stream_t stream) {
   if (!h || !n_ok(h, N) || !d_S || !d_Pinv || !d_gamma || !d_lambda || !(exit_tol >= 0))
     return MPC_ERR_ARG;
-  HIP_TRY(launch_pcg(single(N), N, d_S, d_Pinv, d_gamma, d_lambda, max_iter, exit_tol, nullptr,
-                     d_iters, d_max_iter_exit, (hipStream_t)stream));
+  HIP_TRY(pcg_single(N, d_S, d_Pinv, d_gamma, d_lambda, max_iter, exit_tol, d_iters, d_max_iter_exit,
+                     (hipStream_t)stream));
   return MPC_OK;
 }
 
@@ -317,8 +337,8 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
     HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, h->c, nullptr, *rho, nullptr, h->S, h->P, h->gamma,
                               h->Ginv, 0, st));
     HIP_TRY(hipEventRecord(h->ev[0], st));
-    HIP_TRY(launch_pcg(b, N, h->S, h->P, h->gamma, d_lambda, cfg.pcg_max_iter, cfg.pcg_exit_tol,
-                       nullptr, h->pcg_iters, h->pcg_flag, st));
+    HIP_TRY(pcg_single(N, h->S, h->P, h->gamma, d_lambda, cfg.pcg_max_iter, cfg.pcg_exit_tol, h->pcg_iters,
+                       h->pcg_flag, st));
     HIP_TRY(hipEventRecord(h->ev[1], st));
     HIP_TRY(hipMemcpyAsync(h->h_pcg, h->pcg_iters, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(h->h_pcg + 1, h->pcg_flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -383,6 +403,58 @@ int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
     for (uint32_t k = k0[i]; k < k0[i + 1]; ++k) ki[k] = i;
   HIP_TRY(hipMemcpy(h->knot_inst, ki.data(), sizeof(uint32_t) * T, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(h->inst_k0, k0.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice));
+  // Bins for the resident PCG: instances with N <= 128 packed first-fit
+  // decreasing into workgroups of 64 groups (whole waves per instance, at
+  // most pcg_res_max_items() instances per bin); longer instances go to the
+  // streaming kernel.
+  const uint32_t cap_groups = pcg_res_max_knots() / 2, max_items = pcg_res_max_items();
+  std::vector<uint32_t> order;
+  std::vector<uint32_t> big;
+  uint32_t max_big = 0;
+  for (uint32_t i = 0; i < n_inst; ++i) {
+    if (horizons[i] <= pcg_res_max_knots()) order.push_back(i);
+    else {
+      big.push_back(i);
+      max_big = horizons[i] > max_big ? horizons[i] : max_big;
+    }
+  }
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return horizons[a] > horizons[c]; });
+  struct BinB { uint32_t used = 0; std::vector<uint32_t> inst; };
+  std::vector<BinB> bins;
+  for (uint32_t i : order) {
+    const uint32_t need = pcg_res_groups(horizons[i]);
+    BinB *dst = nullptr;
+    for (auto &bb : bins)
+      if (bb.used + need <= cap_groups && bb.inst.size() < max_items) {
+        dst = &bb;
+        break;
+      }
+    if (!dst) {
+      bins.emplace_back();
+      dst = &bins.back();
+    }
+    dst->inst.push_back(i);
+    dst->used += need;
+  }
+  std::vector<uint4> items;
+  std::vector<uint32_t> boff(1, 0);
+  for (auto &bb : bins) {
+    uint32_t g0 = 0;
+    for (uint32_t i : bb.inst) {
+      const uint32_t ng = pcg_res_groups(horizons[i]);
+      items.push_back(make_uint4(i, g0, ng, 0u));
+      g0 += ng;
+    }
+    boff.push_back((uint32_t)items.size());
+  }
+  if (!items.empty())
+    HIP_TRY(hipMemcpy(h->bin_items, items.data(), sizeof(uint4) * items.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->bin_off, boff.data(), sizeof(uint32_t) * boff.size(), hipMemcpyHostToDevice));
+  if (!big.empty())
+    HIP_TRY(hipMemcpy(h->big_list, big.data(), sizeof(uint32_t) * big.size(), hipMemcpyHostToDevice));
+  h->n_bins = (uint32_t)bins.size();
+  h->n_big = (uint32_t)big.size();
+  h->max_big_n = max_big;
   h->horizons.assign(horizons, horizons + n_inst);
   h->k0 = k0;
   h->n_inst = n_inst;
@@ -425,8 +497,11 @@ int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee
     HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma,
                               h->Ginv, 0, st));
     if (tm) HIP_TRY(hipEventRecord(h->ev[2], st));
-    HIP_TRY(launch_pcg(b, h->max_n, h->S, h->P, h->gamma, d_lambda, h->cfg.pcg_max_iter,
-                       h->cfg.pcg_exit_tol, d_status, h->pcg_iters, h->pcg_flag, st));
+    HIP_TRY(launch_pcg_stream(b, h->max_big_n, h->S, h->P, h->gamma, d_lambda, h->cfg.pcg_max_iter,
+                              h->cfg.pcg_exit_tol, d_status, h->big_list, h->n_big, h->pcg_iters, h->pcg_flag,
+                              st));
+    HIP_TRY(launch_pcg_res(b, h->bin_items, h->bin_off, h->n_bins, h->S, h->P, h->gamma, d_lambda,
+                           h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, d_status, h->pcg_iters, h->pcg_flag, st));
     if (tm) HIP_TRY(hipEventRecord(h->ev[3], st));
     HIP_TRY(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
     if (tm) HIP_TRY(hipEventRecord(h->ev[4], st));

@@ -943,3 +943,19 @@ int F(orc_sqp_solve)(const orc_cfg *cfg, int N, const real *ee_ref, real *lambda
   free(G); free(Ginv); free(C); free(g); free(c); free(S); free(P); free(gam); free(dz);
   return rc;
 }
+
+/* ------------------------------------------------------------------------
+ * Timing helper for the CPU baseline: reps x (CSC value fill + LDL^T factor
+ * + solve) of one Schur system, as in the reference's host linear-system
+ * step (qdldl_helper.cuh:62-180).  Returns elapsed seconds.
+ * ---------------------------------------------------------------------- */
+#include <time.h>
+double F(orc_ldl_bench)(int N, const real *S, const real *gamma, int reps, real *lambda) {
+  void *w = F(orc_ldl_create)(N);
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (int r = 0; r < reps; ++r) F(orc_ldl_schur_solve)(w, N, S, gamma, lambda);
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  F(orc_ldl_destroy)(w);
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}

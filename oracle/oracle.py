@@ -346,3 +346,13 @@ def dense_C(N, Cm):
         M[14 * (k + 1):14 * (k + 2), 21 * k + 14:21 * k + 21] = mB
         M[14 * (k + 1):14 * (k + 2), 21 * (k + 1):21 * (k + 1) + 14] = np.eye(14)
     return M
+
+
+def ldl_bench(N, S, gamma, reps, dtype=np.float32):
+    """Seconds for reps x (fill + factor + solve) of S lambda = gamma in C."""
+    S, gamma = _arr(S, dtype), _arr(gamma, dtype)
+    lam = np.zeros(14 * N, dtype)
+    f = _fn("orc_ldl_bench", dtype)
+    f.restype = C.c_double
+    t = f(N, _ptr(S), _ptr(gamma), int(reps), _ptr(lam))
+    return float(t), lam
