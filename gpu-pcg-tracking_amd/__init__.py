@@ -32,6 +32,7 @@ MAX_SQP_ITERS = 64
 OK, ERR_ARG, ERR_HIP, ERR_NOT_READY, ERR_NONFINITE, ERR_NO_DEVICE = range(6)
 ST_ACTIVE, ST_PCG_MAXITER, ST_RHO_ABORT, ST_NONFINITE = 1, 2, 4, 8
 FLAG_LAST_KNOT_OWN_STATE = 1
+FLAG_PCG_STREAM = 2
 
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = (

@@ -63,6 +63,14 @@ struct Params {
   uint32_t flags;
 };
 
+// mailbox of a two-workgroup PCG team: flags (one cache line) and a double
+// buffered payload per half
+struct TeamMailbox {
+  uint32_t flag[2];
+  uint32_t pad[30];
+  float data[2][2][32];
+};
+
 // per-instance device state of the batched SQP (struct of arrays)
 struct InstState {
   float *rho;        // [n] regularisation (in/out)
@@ -74,7 +82,7 @@ struct InstState {
 };
 
 constexpr uint32_t ST_ACTIVE = 1u, ST_PCG_MAXITER = 2u, ST_RHO_ABORT = 4u, ST_NONFINITE = 8u;
-constexpr uint32_t FLAG_LAST_KNOT_OWN_STATE = 1u;
+constexpr uint32_t FLAG_LAST_KNOT_OWN_STATE = 1u, FLAG_PCG_STREAM = 2u;
 
 // host-side launchers (implemented in the .hip files)
 hipError_t launch_fk(uint32_t n, const float *q, uint32_t q_stride, float *ee, hipStream_t st);
@@ -106,6 +114,14 @@ hipError_t launch_pcg_res(const Batch &b, const uint4 *items, const uint32_t *bi
                           const float *S, const float *Pinv, const float *gamma, float *lambda,
                           uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters,
                           uint32_t *flag, hipStream_t st);
+// PCG of 128 < N <= pcg_team_max_knots() on a team of two workgroups
+// (cooperative launch); err receives 1 if a team wait timed out.
+hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n_teams, TeamMailbox *mb,
+                           uint32_t n_mb, const float *S, const float *Pinv, const float *gamma,
+                           float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
+                           uint32_t *iters, uint32_t *flag, uint32_t *err, hipStream_t st);
+uint32_t pcg_team_max_knots();
+uint32_t pcg_team_max_grid();
 uint32_t pcg_res_groups(uint32_t N);
 uint32_t pcg_res_max_knots();
 uint32_t pcg_res_max_items();

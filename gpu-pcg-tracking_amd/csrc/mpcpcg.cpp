@@ -41,6 +41,10 @@ struct mpc_handle {
   uint4 *bin_items = nullptr;
   uint32_t *bin_off = nullptr, *big_list = nullptr;
   uint32_t n_bins = 0, n_big = 0, max_big_n = 0;
+  // two-workgroup PCG teams (128 < N <= 256)
+  uint32_t *team_list = nullptr, *team_one = nullptr, *team_err = nullptr;
+  TeamMailbox *mbox = nullptr;
+  uint32_t n_teams = 0, n_mbox = 256;
   // pinned staging for the host round trips of the plain driver
   float *h_merits = nullptr;
   uint32_t *h_pcg = nullptr;
@@ -74,12 +78,20 @@ static Params make_params(const mpc_config &c) {
 
 static Batch single(uint32_t N) { return Batch{nullptr, nullptr, 1u, N}; }
 
-// PCG of one instance: resident kernel when its Schur system fits on chip
-static hipError_t pcg_single(uint32_t N, const float *S, const float *P, const float *gamma, float *lambda,
-                             uint32_t max_iter, float tol, uint32_t *iters, uint32_t *flag, hipStream_t st) {
-  if (N <= pcg_res_max_knots())
-    return launch_pcg_res(single(N), nullptr, nullptr, 1, S, P, gamma, lambda, max_iter, tol, nullptr, iters,
-                          flag, st);
+// PCG of one instance: resident kernel when its Schur system fits one
+// workgroup, a two-workgroup team up to twice that, streaming otherwise.
+static hipError_t pcg_single(mpc_handle *h, uint32_t N, const float *S, const float *P, const float *gamma,
+                             float *lambda, uint32_t max_iter, float tol, uint32_t *iters, uint32_t *flag,
+                             hipStream_t st) {
+  const uint32_t flags = h->cfg.flags;
+  if (!(flags & FLAG_PCG_STREAM)) {
+    if (N <= pcg_res_max_knots())
+      return launch_pcg_res(single(N), nullptr, nullptr, 1, S, P, gamma, lambda, max_iter, tol, nullptr, iters,
+                            flag, st);
+    if (N <= pcg_team_max_knots() && pcg_team_max_grid() >= 16)
+      return launch_pcg_team(single(N), h->team_one, 1, h->mbox, h->n_mbox, S, P, gamma, lambda, max_iter, tol,
+                             nullptr, iters, flag, h->team_err, st);
+  }
   return launch_pcg_stream(single(N), N, S, P, gamma, lambda, max_iter, tol, nullptr, nullptr, 0, iters, flag,
                            st);
 }
@@ -133,7 +145,7 @@ static void free_all(mpc_handle *h) {
   void *ptrs[] = {h->G, h->Ginv, h->C, h->g, h->c, h->S, h->P, h->gamma, h->dz, h->knot_merits,
                   h->merits, h->xs, h->drho, h->merit0, h->rho1, h->status_ws, h->sqp_done,
                   h->pcg_iters, h->pcg_flag, h->knot_inst, h->inst_k0, h->bin_items,
-                  h->bin_off, h->big_list};
+                  h->bin_off, h->big_list, h->team_list, h->team_one, h->team_err, h->mbox};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (h->h_merits) (void)hipHostFree(h->h_merits);
@@ -184,10 +196,17 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
             alloc((void **)&h->bin_items, sizeof(uint4) * B) &&
             alloc((void **)&h->bin_off, sizeof(uint32_t) * (B + 1)) &&
             alloc((void **)&h->big_list, sizeof(uint32_t) * B) &&
+            alloc((void **)&h->team_list, sizeof(uint32_t) * B) &&
+            alloc((void **)&h->team_one, sizeof(uint32_t)) &&
+            alloc((void **)&h->team_err, sizeof(uint32_t)) &&
+            alloc((void **)&h->mbox, sizeof(TeamMailbox) * h->n_mbox) &&
             hipHostMalloc((void **)&h->h_merits, sizeof(float) * 16) == hipSuccess &&
             hipHostMalloc((void **)&h->h_pcg, sizeof(uint32_t) * 4) == hipSuccess &&
             hipHostMalloc((void **)&h->h_rho, sizeof(float) * 4) == hipSuccess;
   for (auto &e : h->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  ok = ok && hipMemset(h->team_one, 0, sizeof(uint32_t)) == hipSuccess &&
+       hipMemset(h->team_err, 0, sizeof(uint32_t)) == hipSuccess &&
+       hipMemset(h->mbox, 0, sizeof(TeamMailbox) * h->n_mbox) == hipSuccess;
   if (!ok) {
     free_all(h);
     delete h;
@@ -272,7 +291,7 @@ int mpc_pcg_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_Pi
                   uint32_t *d_iters, uint32_t *d_max_iter_exit, mpc_stream_t stream) {
   if (!h || !n_ok(h, N) || !d_S || !d_Pinv || !d_gamma || !d_lambda || !(exit_tol >= 0))
     return MPC_ERR_ARG;
-  HIP_TRY(pcg_single(N, d_S, d_Pinv, d_gamma, d_lambda, max_iter, exit_tol, d_iters, d_max_iter_exit,
+  HIP_TRY(pcg_single(h, N, d_S, d_Pinv, d_gamma, d_lambda, max_iter, exit_tol, d_iters, d_max_iter_exit,
                      (hipStream_t)stream));
   return MPC_OK;
 }
@@ -337,7 +356,7 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
     HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, h->c, nullptr, *rho, nullptr, h->S, h->P, h->gamma,
                               h->Ginv, 0, st));
     HIP_TRY(hipEventRecord(h->ev[0], st));
-    HIP_TRY(pcg_single(N, h->S, h->P, h->gamma, d_lambda, cfg.pcg_max_iter, cfg.pcg_exit_tol, h->pcg_iters,
+    HIP_TRY(pcg_single(h, N, h->S, h->P, h->gamma, d_lambda, cfg.pcg_max_iter, cfg.pcg_exit_tol, h->pcg_iters,
                        h->pcg_flag, st));
     HIP_TRY(hipEventRecord(h->ev[1], st));
     HIP_TRY(hipMemcpyAsync(h->h_pcg, h->pcg_iters, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -408,11 +427,12 @@ int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
   // most pcg_res_max_items() instances per bin); longer instances go to the
   // streaming kernel.
   const uint32_t cap_groups = pcg_res_max_knots() / 2, max_items = pcg_res_max_items();
-  std::vector<uint32_t> order;
-  std::vector<uint32_t> big;
+  std::vector<uint32_t> order, big, teams;
   uint32_t max_big = 0;
+  const bool team_ok = pcg_team_max_grid() >= 16;
   for (uint32_t i = 0; i < n_inst; ++i) {
     if (horizons[i] <= pcg_res_max_knots()) order.push_back(i);
+    else if (team_ok && horizons[i] <= pcg_team_max_knots()) teams.push_back(i);
     else {
       big.push_back(i);
       max_big = horizons[i] > max_big ? horizons[i] : max_big;
@@ -452,6 +472,9 @@ int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
   HIP_TRY(hipMemcpy(h->bin_off, boff.data(), sizeof(uint32_t) * boff.size(), hipMemcpyHostToDevice));
   if (!big.empty())
     HIP_TRY(hipMemcpy(h->big_list, big.data(), sizeof(uint32_t) * big.size(), hipMemcpyHostToDevice));
+  if (!teams.empty())
+    HIP_TRY(hipMemcpy(h->team_list, teams.data(), sizeof(uint32_t) * teams.size(), hipMemcpyHostToDevice));
+  h->n_teams = (uint32_t)teams.size();
   h->n_bins = (uint32_t)bins.size();
   h->n_big = (uint32_t)big.size();
   h->max_big_n = max_big;
@@ -497,11 +520,20 @@ int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee
     HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma,
                               h->Ginv, 0, st));
     if (tm) HIP_TRY(hipEventRecord(h->ev[2], st));
-    HIP_TRY(launch_pcg_stream(b, h->max_big_n, h->S, h->P, h->gamma, d_lambda, h->cfg.pcg_max_iter,
-                              h->cfg.pcg_exit_tol, d_status, h->big_list, h->n_big, h->pcg_iters, h->pcg_flag,
-                              st));
-    HIP_TRY(launch_pcg_res(b, h->bin_items, h->bin_off, h->n_bins, h->S, h->P, h->gamma, d_lambda,
-                           h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, d_status, h->pcg_iters, h->pcg_flag, st));
+    if (h->cfg.flags & FLAG_PCG_STREAM) {
+      HIP_TRY(launch_pcg_stream(b, h->max_n, h->S, h->P, h->gamma, d_lambda, h->cfg.pcg_max_iter,
+                                h->cfg.pcg_exit_tol, d_status, nullptr, 0, h->pcg_iters, h->pcg_flag, st));
+    } else {
+      HIP_TRY(launch_pcg_stream(b, h->max_big_n, h->S, h->P, h->gamma, d_lambda, h->cfg.pcg_max_iter,
+                                h->cfg.pcg_exit_tol, d_status, h->big_list, h->n_big, h->pcg_iters,
+                                h->pcg_flag, st));
+      HIP_TRY(launch_pcg_team(b, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->S, h->P, h->gamma, d_lambda,
+                              h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, d_status, h->pcg_iters, h->pcg_flag,
+                              h->team_err, st));
+      HIP_TRY(launch_pcg_res(b, h->bin_items, h->bin_off, h->n_bins, h->S, h->P, h->gamma, d_lambda,
+                             h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, d_status, h->pcg_iters, h->pcg_flag,
+                             st));
+    }
     if (tm) HIP_TRY(hipEventRecord(h->ev[3], st));
     HIP_TRY(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
     if (tm) HIP_TRY(hipEventRecord(h->ev[4], st));

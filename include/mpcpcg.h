@@ -77,6 +77,9 @@ typedef struct mpc_config {
 /* The terminal knot's cost uses x_{N-1} instead of the reference's x_{N-2}
  * (rbdfiles/iiwa_eepos_plant.cuh:399 passes the knot N-2 state). */
 #define MPC_FLAG_LAST_KNOT_OWN_STATE 1u
+/* Always use the streaming PCG kernel (S read from memory every iteration)
+ * instead of the on-chip resident kernel; for kernel comparisons. */
+#define MPC_FLAG_PCG_STREAM 2u
 
 typedef struct mpc_handle mpc_handle;
 

@@ -107,23 +107,34 @@ def _gpu_pcg(solver, N, S32, P32, gam32, max_iter, tol):
     return host(lam), int(it.item()), int(fl.item())
 
 
+def _fp32_floor(orc, N, S32, P32, gam32, tol, ref):
+    """Rounding floor of an fp32 PCG for this system: the deviation of the
+    fp32 oracle twin (same algorithm, sequential sums) from ref.  Any fp32
+    implementation, the reference's own included, scatters at this level."""
+    lt, _, _ = orc.pcg(N, S32, P32, gam32, np.zeros(14 * N), 5000, tol, dtype=np.float32)
+    return helpers.rel_err(lt, ref)
+
+
 # Same exit rule on both sides (|r^T Pinv r| < 1e-5): fp32 GPU vs the fp64
-# oracle PCG, relative solution difference <= 1e-4 (up to ~850 iterations
-# from the initial iterate at N = 256).
+# oracle PCG.  Bound: max(2e-4, 4 x the fp32 rounding floor); measured values
+# are printed (typically 1e-6 .. 1e-4, up to ~1e-3 after ~450 iterations).
 @pytest.mark.parametrize("N", [2, 16, 32, 64, 128, 256])
 def test_pcg_same_rule_parity(solver, orc, N):
     S32, P32, gam32 = _natural_schur(orc, 30 + N, N)
     lam_g, it_g, fl_g = _gpu_pcg(solver, N, S32, P32, gam32, 5000, 1e-5)
     lam_o, it_o, fl_o = orc.pcg(N, S32, P32, gam32, np.zeros(14 * N), 5000, 1e-5)
     lam_d = np.linalg.solve(orc.dense_S(N, S32), gam32.astype(np.float64))
-    print(f"N={N} iters gpu={it_g} oracle={it_o} rel(gpu,oracle)={helpers.rel_err(lam_g, lam_o):.2e} "
+    floor = _fp32_floor(orc, N, S32, P32, gam32, 1e-5, lam_o)
+    err = helpers.rel_err(lam_g, lam_o)
+    print(f"N={N} iters gpu={it_g} oracle={it_o} rel(gpu,oracle)={err:.2e} fp32-floor={floor:.2e} "
           f"rel(gpu,direct)={helpers.rel_err(lam_g, lam_d):.2e} rel(oracle,direct)={helpers.rel_err(lam_o, lam_d):.2e}")
     assert fl_g == 0 and fl_o == 0
     assert abs(it_g - it_o) <= max(1, it_o // 50)
-    assert helpers.rel_err(lam_g, lam_o) < 1e-4
+    assert err < max(2e-4, 4 * floor)
 
 
-# Convergence to the direct solve: run to |r^T Pinv r| < 1e-9.
+# Convergence to the direct solve: run to |r^T Pinv r| < 1e-9; bound as above
+# with the floor measured against the direct solution.
 @pytest.mark.parametrize("N", [2, 16, 32, 64, 128, 256])
 def test_pcg_converges_to_direct_solve(solver, orc, N):
     S32, P32, gam32 = _natural_schur(orc, 30 + N, N)
@@ -131,9 +142,30 @@ def test_pcg_converges_to_direct_solve(solver, orc, N):
     ldl = orc.LDL(N)
     lam_d = ldl.solve(S32.astype(np.float64), gam32.astype(np.float64))
     err = helpers.rel_err(lam_g, lam_d)
-    print(f"N={N} iters={it_g} rel(gpu,direct)={err:.2e}")
+    floor = _fp32_floor(orc, N, S32, P32, gam32, 1e-9, lam_d)
+    print(f"N={N} iters={it_g} rel(gpu,direct)={err:.2e} fp32-floor={floor:.2e}")
     assert fl_g == 0
-    assert err < 1e-4
+    assert err < max(2e-4, 4 * floor)
+
+
+# The resident (on-chip) and streaming PCG kernels take the same iterations
+# and agree to the fp32 floor on several seeds.
+@pytest.mark.parametrize("N", [16, 64, 128])
+def test_pcg_kernels_agree(solver, orc, pkg, N):
+    for seed in range(3):
+        S32, P32, gam32 = _natural_schur(orc, 500 + 11 * seed + N, N)
+        lam_o, it_o, _ = orc.pcg(N, S32, P32, gam32, np.zeros(14 * N), 5000, 1e-5)
+        floor = _fp32_floor(orc, N, S32, P32, gam32, 1e-5, lam_o)
+        out = []
+        for flag in (0, pkg.FLAG_PCG_STREAM):
+            solver.set_config(flags=flag)
+            try:
+                out.append(_gpu_pcg(solver, N, S32, P32, gam32, 5000, 1e-5))
+            finally:
+                solver.set_config(flags=0)
+        for lam_g, it_g, fl_g in out:
+            assert fl_g == 0 and it_g == it_o
+            assert helpers.rel_err(lam_g, lam_o) < max(2e-4, 4 * floor)
 
 
 # Ill-conditioned stress: perturbed trajectories with large controls need
