@@ -63,12 +63,10 @@ struct Params {
   uint32_t flags;
 };
 
-// mailbox of a two-workgroup PCG team: flags (one cache line) and a double
-// buffered payload per half
+// mailbox of a two-workgroup PCG team: per half and epoch parity, 16
+// self-validating words {epoch (high 32 bits), float bits (low 32 bits)}
 struct TeamMailbox {
-  uint32_t flag[2];
-  uint32_t pad[30];
-  float data[2][2][32];
+  unsigned long long w[2][2][16];
 };
 
 // per-instance device state of the batched SQP (struct of arrays)

@@ -14,8 +14,9 @@
 //
 // The launch is cooperative (all workgroups co-resident); the two halves of
 // a team sit 8 workgroup ids apart so they land on the same XCD and share
-// its L2.  Every wait is bounded: after ~2^22 polls a half gives up, sets
-// the error word and leaves its instance unsolved.
+// its L2.  Mailbox words carry their epoch, so no memory fences are needed.
+// Every wait is bounded: after ~2^22 polls a half gives up, sets the error
+// word and leaves its instance unsolved.
 #include "common.h"
 #include "model.h"
 
@@ -62,31 +63,34 @@ __device__ __forceinline__ float wg_sum(float v, float *red) {
 }
 
 // Send sh.send[0..n) to the partner, receive its payload into sh.recv.
+// Every mailbox word carries its epoch, so relaxed (fence-free) atomics
+// suffice: a word is taken only once its tag shows the current epoch.
 // Returns false (uniformly) if the partner did not answer in time.
 __device__ bool team_exchange(TeamMailbox *mb, int half, uint32_t epoch, TeamShared &sh, int n,
                               uint32_t *err) {
   __syncthreads();
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
-    float *mine = mb->data[epoch & 1u][half];
-    const float *theirs = mb->data[epoch & 1u][1 - half];
-    if (lane < n) __hip_atomic_store(mine + lane, sh.send[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    uint32_t good = 1;
-    if (lane == 0) {
-      __hip_atomic_store(&mb->flag[half], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t spins = 0;
-      while (__hip_atomic_load(&mb->flag[1 - half], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > SPIN_LIMIT) {
-          good = 0;
-          break;
-        }
+    unsigned long long *mine = mb->w[epoch & 1u][half];
+    const unsigned long long *theirs = mb->w[epoch & 1u][1 - half];
+    const unsigned long long tag = (unsigned long long)epoch << 32;
+    if (lane < n)
+      __hip_atomic_store(mine + lane, tag | __float_as_uint(sh.send[lane]), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    bool got = lane >= n;
+    unsigned long long v = 0;
+    uint32_t spins = 0;
+    while (!__all(got)) {
+      if (!got) {
+        v = __hip_atomic_load(theirs + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        got = (uint32_t)(v >> 32) == epoch;
       }
-      sh.ok = good;
+      if (++spins > SPIN_LIMIT) break;
+      if (!__all(got)) __builtin_amdgcn_s_sleep(1);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (lane < n) sh.recv[lane] = __hip_atomic_load(theirs + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const bool all_got = __all(got);
+    if (lane < n && got) sh.recv[lane] = __uint_as_float((uint32_t)v);
+    if (lane == 0) sh.ok = all_got ? 1u : 0u;
   }
   __syncthreads();
   const bool ok = sh.ok != 0;
@@ -285,10 +289,7 @@ __global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__rest
 
 __global__ void k_reset_mailboxes(TeamMailbox *mb, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    mb[i].flag[0] = 0u;
-    mb[i].flag[1] = 0u;
-  }
+  if (i < n * 64u) (&mb[0].w[0][0][0])[i] = 0ull;
 }
 
 }  // namespace
@@ -325,7 +326,7 @@ hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n
   uint32_t grid = 16u * ((n_teams + 7u) / 8u);
   if (grid > cap) grid = cap;
   if (grid / 2 > n_mb) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_reset_mailboxes, dim3((grid / 2 + 255) / 256), dim3(256), 0, st, mb, grid / 2);
+  hipLaunchKernelGGL(k_reset_mailboxes, dim3((grid / 2 * 64 + 255) / 256), dim3(256), 0, st, mb, grid / 2);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   Batch bb = b;
