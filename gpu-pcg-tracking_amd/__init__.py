@@ -257,6 +257,6 @@ class Solver:
         _check(lib().mpc_batch_set_timing(self.h, int(on)), "mpc_batch_set_timing")
 
     def batch_phase_times(self):
-        out = (C.c_double * 6)()
+        out = (C.c_double * 7)()
         _check(lib().mpc_batch_phase_times(self.h, out), "mpc_batch_phase_times")
         return list(out)

@@ -55,8 +55,8 @@ struct mpc_handle {
   bool batch_ready = false;
   // timing
   bool timing = false;
-  hipEvent_t ev[8]{};
-  double phase_us[6]{};
+  hipEvent_t ev[9]{};
+  double phase_us[7]{};
 };
 
 static Params make_params(const mpc_config &c) {
@@ -493,9 +493,9 @@ int mpc_batch_set_timing(mpc_handle *h, int enable) {
   return MPC_OK;
 }
 
-int mpc_batch_phase_times(mpc_handle *h, double *us6) {
-  if (!h || !us6) return MPC_ERR_ARG;
-  for (int i = 0; i < 6; ++i) us6[i] = h->phase_us[i];
+int mpc_batch_phase_times(mpc_handle *h, double *us7) {
+  if (!h || !us7) return MPC_ERR_ARG;
+  for (int i = 0; i < 7; ++i) us7[i] = h->phase_us[i];
   return MPC_OK;
 }
 
@@ -523,28 +523,32 @@ int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee
     if (h->cfg.flags & FLAG_PCG_STREAM) {
       HIP_TRY(launch_pcg_stream(b, h->max_n, h->S, h->P, h->gamma, d_lambda, h->cfg.pcg_max_iter,
                                 h->cfg.pcg_exit_tol, d_status, nullptr, 0, h->pcg_iters, h->pcg_flag, st));
+      if (tm) HIP_TRY(hipEventRecord(h->ev[3], st));
     } else {
-      HIP_TRY(launch_pcg_stream(b, h->max_big_n, h->S, h->P, h->gamma, d_lambda, h->cfg.pcg_max_iter,
-                                h->cfg.pcg_exit_tol, d_status, h->big_list, h->n_big, h->pcg_iters,
-                                h->pcg_flag, st));
+      // long horizons first: two-workgroup teams, then any beyond their
+      // reach on the streaming kernel, then the bins of the resident kernel
       HIP_TRY(launch_pcg_team(b, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->S, h->P, h->gamma, d_lambda,
                               h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, d_status, h->pcg_iters, h->pcg_flag,
                               h->team_err, st));
+      if (tm) HIP_TRY(hipEventRecord(h->ev[3], st));
+      HIP_TRY(launch_pcg_stream(b, h->max_big_n, h->S, h->P, h->gamma, d_lambda, h->cfg.pcg_max_iter,
+                                h->cfg.pcg_exit_tol, d_status, h->big_list, h->n_big, h->pcg_iters,
+                                h->pcg_flag, st));
       HIP_TRY(launch_pcg_res(b, h->bin_items, h->bin_off, h->n_bins, h->S, h->P, h->gamma, d_lambda,
                              h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, d_status, h->pcg_iters, h->pcg_flag,
                              st));
     }
-    if (tm) HIP_TRY(hipEventRecord(h->ev[3], st));
-    HIP_TRY(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
     if (tm) HIP_TRY(hipEventRecord(h->ev[4], st));
-    HIP_TRY(launch_merit(b, prm, d_xu, h->dz, d_ee_ref, h->xs, d_status, 0, 8, h->knot_merits, st));
+    HIP_TRY(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
     if (tm) HIP_TRY(hipEventRecord(h->ev[5], st));
+    HIP_TRY(launch_merit(b, prm, d_xu, h->dz, d_ee_ref, h->xs, d_status, 0, 8, h->knot_merits, st));
+    if (tm) HIP_TRY(hipEventRecord(h->ev[6], st));
     HIP_TRY(launch_ls_update(b, prm, h->knot_merits, h->dz, d_xu, s, d_pcg_iters, h->pcg_iters,
                              h->pcg_flag, it, sqp_iters, st));
     if (tm) {
-      HIP_TRY(hipEventRecord(h->ev[6], st));
-      HIP_TRY(hipEventSynchronize(h->ev[6]));
-      for (int ph = 0; ph < 6; ++ph) {
+      HIP_TRY(hipEventRecord(h->ev[7], st));
+      HIP_TRY(hipEventSynchronize(h->ev[7]));
+      for (int ph = 0; ph < 7; ++ph) {
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, h->ev[ph], h->ev[ph + 1]));
         h->phase_us[ph] += 1e3 * ms;

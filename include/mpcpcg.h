@@ -190,10 +190,11 @@ int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee
                   mpc_stream_t stream);
 
 /* Device-event timing of the phases of the last mpc_batch_sqp call made
- * with timing enabled (mpc_batch_set_timing(h, 1)): microseconds spent in
- * linearisation, Schur formation, PCG, dz, line search, update. */
+ * with timing enabled (mpc_batch_set_timing(h, 1)), summed over its SQP
+ * iterations, in microseconds: linearisation, Schur formation, PCG on
+ * two-workgroup teams, the rest of PCG, dz, line-search merits, update. */
 int mpc_batch_set_timing(mpc_handle *h, int enable);
-int mpc_batch_phase_times(mpc_handle *h, double *us6);
+int mpc_batch_phase_times(mpc_handle *h, double *us7);
 
 /* ---- synthetic workload (seeded, reproducible) ----------------------------- */
 

@@ -78,12 +78,3 @@ def test_batch_matches_plain_and_is_deterministic(solver, orc, pkg, horizons):
         off += L
         loff += 14 * N
     assert (host(st).astype(np.int64) & ~np.int64(3)).max() == 0
-
-
-def test_synth_generator_matches_spec(pkg):
-    for seed, N in ((0, 5), (12345, 33)):
-        x0, q = pkg.synth_instance(seed, N)
-        sx0, sq = helpers.synth_instance_spec(seed, N)
-        assert np.allclose(x0, sx0, atol=1e-6) and np.allclose(q, sq, atol=1e-6)
-    h = pkg.synth_horizons(7, 100)
-    assert np.array_equal(h, helpers.synth_horizons_spec(7, 100))
