@@ -121,6 +121,7 @@ hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n
 uint32_t pcg_team_max_knots();
 uint32_t pcg_team_max_grid();
 uint32_t pcg_res_groups(uint32_t N);
+uint32_t pcg_res_bin_groups();  // 16-lane groups per workgroup
 uint32_t pcg_res_max_knots();
 uint32_t pcg_res_max_items();
 hipError_t launch_merit_reduce(const Batch &b, const float *knot_merits, int np, float *out,

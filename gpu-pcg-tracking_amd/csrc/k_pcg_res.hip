@@ -2,61 +2,51 @@
 //
 // Same algorithm and exit rule as k_pcg.hip (GPU-PCG pcg<T,14,N>, call
 // site toplevel.cuh:143-157), but every instance's matrix is read from HBM
-// once per solve instead of once per iteration:
-//   - off-diagonal blocks S(b,b-1) in LDS (196 floats per knot), read as
-//     rows for S(b,b-1) p_{b-1} and as columns for S(b+1,b)^T p_{b+1};
-//   - row i of the diagonal block S(b,b) and of Pinv(b,b) in the registers
-//     of lane i (2 x 14 floats per knot and lane).
-// A 1024-lane workgroup holds up to 128 knots (R = 2 knots per 16-lane
-// group, 64 groups).  Several instances share a workgroup ("bin"): each
-// instance owns a whole number of waves, so its dot products are
-// wave64 butterflies plus a fixed-order sum over its own waves, and its
-// arithmetic does not depend on which bin it is packed into.  Instances in
-// a bin iterate in lockstep; a converged instance idles until the bin is
-// done.  Per iteration: 3 workgroup barriers (p exchange, p.Sp, r.z); the
-// residual exchange for the block-diagonal preconditioner stays inside a
-// 16-lane group (one wave).
+// once per solve, and vectors never leave registers except for one
+// neighbour exchange per iteration (block-row arithmetic: pcg_onchip.h).
+//
+// A 512-lane workgroup holds up to 128 knots: 32 groups of 16 lanes, each
+// serving RK = 4 consecutive knots (4 lg + t of its instance, t = 0..3).
+// Several instances share a workgroup ("bin"): each owns whole waves, so
+// its dot products are a wave64 reduction plus a fixed-order sum over its
+// own waves, and its arithmetic does not depend on which bin it is packed
+// into.  Instances of a bin iterate in lockstep; a converged instance
+// skips its arithmetic until the bin is done.
+//
+// Per iteration, two workgroup barriers:
+//   y = S p (registers + L from LDS); p.y -> B1 -> alpha; lambda, r;
+//   z = Pinv r; stage z of the group's first and last knot; r.z -> B2 ->
+//   beta; p = z + beta p for the own knots and, from the staged z, for the
+//   copies of the neighbouring groups' elements p_{4lg-1}[i], p_{4lg+4}[i]
+//   (the same fused multiply-add as their owners, so every copy stays
+//   bitwise equal to its owner).
 #include "common.h"
-#include "model.h"
+#include "pcg_onchip.h"
 
 namespace mpc {
 
 namespace {
 
-constexpr int NT = 1024;
-constexpr int NGRP = NT / 16;  // 64 groups
-constexpr int NWAV = NT / 64;  // 16 waves
-constexpr int RK = 2;          // knots per group
+using namespace onchip;
+
+constexpr int NT = 512;
+constexpr int NGRP = NT / 16;  // 32 groups
+constexpr int NWAV = NT / 64;  // 8 waves
+constexpr int RK = 4;          // knots per group
 constexpr int KNOTS = NGRP * RK;
 constexpr int MAXI = 8;        // instances per bin
 
 struct ResShared {
-  float L[KNOTS * NXX];  // S(b,b-1), column-major per knot slot
-  float pv[KNOTS * NX];
-  float rv[KNOTS * NX];
+  float L[(KNOTS + 1) * L_SLOT + L_GUARD];  // S(b,b-1) per knot slot (+ one zero slot), pre-rotated rows
+  float zb[(KNOTS + 1) * 16];         // staged vector element per knot slot and lane
   float red[2][NWAV];
-  uint32_t act[MAXI];
+  uint32_t act[NWAV];
 };
+static_assert(((KNOTS + 1) * L_SLOT + L_GUARD) % 4 == 0, "L and zb are cleared as one float4 range");
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// sum of v over the waves [w0, w0 + nw) of the workgroup, fixed order
-__device__ __forceinline__ float seg_sum(float v, float *red, int w0, int nw) {
-  v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
+__device__ __forceinline__ float seg_sum(const float *red, uint32_t w0, uint32_t nw) {
   float s = 0.f;
-  for (int w = 0; w < nw; ++w) s += red[w0 + w];
+  for (uint32_t v = 0; v < nw; ++v) s += red[w0 + v];
   return s;
 }
 
@@ -71,7 +61,13 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
                                                 uint32_t *__restrict__ flag_out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   ResShared &sh = *reinterpret_cast<ResShared *>(smem_raw);
-  const int tid = threadIdx.x, g = tid >> 4, i = tid & 15;
+  const uint32_t tid = threadIdx.x, g = tid >> 4, i = tid & 15u, w = tid >> 6;
+  const uint32_t d = ror_step();
+  {
+    float4 *z4 = reinterpret_cast<float4 *>(sh.L);
+    constexpr uint32_t n4 = ((KNOTS + 1) * L_SLOT + L_GUARD + (KNOTS + 1) * 16) / 4;
+    for (uint32_t idx = tid; idx < n4; idx += NT) z4[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 
   // my instance in this bin (items: {inst, first group, groups, -}); a null
   // item table means one instance (0) starting at group 0
@@ -82,11 +78,10 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
   }
   int slot = -1;
   uint32_t inst = 0, g0 = 0, ng = NGRP;
-  // a lone instance uses the same group count as inside a bin
-  const uint32_t ng1 = ((inst_len(b, 0) + RK - 1) / RK + 3u) & ~3u;
+  const uint32_t ng1 = ((inst_len(b, 0) + RK - 1) / RK + 3u) & ~3u;  // lone instance: as in a bin
   for (uint32_t j = 0; j < ni; ++j) {
     const uint4 it = items ? items[i0 + j] : make_uint4(0u, 0u, ng1, 0u);
-    if ((uint32_t)g >= it.y && (uint32_t)g < it.y + it.z) {
+    if (g >= it.y && g < it.y + it.z) {
       slot = (int)j;
       inst = it.x;
       g0 = it.y;
@@ -97,140 +92,112 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
   const uint32_t N = mine ? inst_len(b, inst) : 0u;
   const uint32_t K0 = mine ? inst_first(b, inst) : 0u;
   const bool live = mine && !(status && !(status[inst] & ST_ACTIVE));
-  const int lg = (int)(g - g0);
-  const int w0 = (int)(g0 >> 2), nw = (int)(ng >> 2);
-  const uint32_t lbase = g0 * RK;  // first knot slot of the instance
-  const bool lane_ok = live && i < NX;
+  const uint32_t lg = g - g0, w0 = g0 >> 2, nw = ng >> 2, lbase = g0 * RK;
+  const bool lane_ok = i < (uint32_t)NX;
+  __syncthreads();  // LDS cleared
 
-  // ---- load: L blocks into LDS, D / Pinv rows into registers
-  if (live) {
-    const uint32_t nthr = ng * 16, t0 = (uint32_t)(tid - (int)g0 * 16);
-    for (uint32_t idx = t0; idx < N * NXX; idx += nthr) {
-      const uint32_t bk = idx / NXX, e = idx - bk * NXX;
-      sh.L[(lbase + bk) * NXX + e] = S[(size_t)S_STRIDE * (K0 + bk) + e];
-    }
-  }
-  float D[RK][NX], P[RK][NX];
+  // ---- load: D, Pinv rows into registers, S(b,b-1) into LDS
+  const uint32_t b0 = lg * RK, s0 = lbase + b0;  // first knot of the group and its slot
+  RowRegs R[RK];
   float lam[RK], r[RK], p[RK], y[RK], z[RK];
-  bool has[RK];
 #pragma unroll
   for (int t = 0; t < RK; ++t) {
-    const uint32_t bk = (uint32_t)lg + ng * t;
-    has[t] = lane_ok && bk < N;
-    const float *Sd = S + (size_t)S_STRIDE * (K0 + bk) + NXX;
-    const float *Pd = Pinv + (size_t)S_STRIDE * (K0 + bk) + NXX;
-#pragma unroll
-    for (int j = 0; j < NX; ++j) {
-      D[t][j] = has[t] ? Sd[i + NX * j] : 0.f;
-      P[t][j] = has[t] ? Pd[i + NX * j] : 0.f;
-    }
-    lam[t] = has[t] ? lambda[(size_t)NX * (K0 + bk) + i] : 0.f;
-    if (has[t]) sh.pv[(lbase + bk) * NX + i] = lam[t];
+    const uint32_t bk = b0 + (uint32_t)t;
+    const bool has = live && bk < N;
+    load_row(R[t], sh.L + (s0 + t) * L_SLOT, has, i, d, S, Pinv, K0, bk);
+    lam[t] = has && lane_ok ? lambda[(size_t)NX * (K0 + bk) + i] : 0.f;
   }
-  __syncthreads();
-
-  // y = S x with x staged in sh.pv
-  auto smv = [&]() {
-#pragma unroll
-    for (int t = 0; t < RK; ++t) {
-      float acc = 0.f;
-      if (has[t]) {
-        const uint32_t bk = (uint32_t)lg + ng * t, ks = lbase + bk;
-        const float *x0 = sh.pv + ks * NX;
-#pragma unroll
-        for (int j = 0; j < NX; ++j) acc += D[t][j] * x0[j];
-        if (bk > 0) {
-          const float *Lb = sh.L + ks * NXX;
-          const float *xm = x0 - NX;
-#pragma unroll
-          for (int j = 0; j < NX; ++j) acc += Lb[i + NX * j] * xm[j];
-        }
-        if (bk + 1 < N) {
-          const float *Ln = sh.L + (ks + 1) * NXX;
-          const float *xp = x0 + NX;
-#pragma unroll
-          for (int j = 0; j < NX; ++j) acc += Ln[NX * i + j] * xp[j];
-        }
-      }
-      y[t] = acc;
-    }
+  // first / last element of the group to LDS; neighbouring groups' copies back
+  const bool in_m = lg > 0, in_p = b0 + RK < N;
+  float *zb0 = sh.zb + s0 * 16 + i;
+  auto stage = [&](const float (&v)[RK]) {
+    zb0[0] = v[0];
+    zb0[(RK - 1) * 16] = v[RK - 1];
   };
-  // z = Pinv r (r staged in sh.rv by this group), returns r.z partial
-  auto precond = [&]() {
-    float part = 0.f;
-#pragma unroll
-    for (int t = 0; t < RK; ++t) {
-      float acc = 0.f;
-      if (has[t]) {
-        const uint32_t ks = lbase + (uint32_t)lg + ng * t;
-        const float *rr = sh.rv + ks * NX;
-#pragma unroll
-        for (int j = 0; j < NX; ++j) acc += P[t][j] * rr[j];
-        part += r[t] * acc;
-      }
-      z[t] = acc;
-    }
-    return part;
+  auto neighbours = [&](float &vm, float &vp) {
+    vm = in_m ? zb0[-16] : 0.f;
+    vp = in_p ? zb0[RK * 16] : 0.f;
   };
-  auto stage_r = [&]() {
+  // y = S x for the group's knots; xm / xp: elements of knots 4lg-1, 4lg+4
+  auto smv = [&](const float (&x)[RK], float xm, float xp) {
 #pragma unroll
     for (int t = 0; t < RK; ++t)
-      if (has[t]) sh.rv[(lbase + (uint32_t)lg + ng * t) * NX + i] = r[t];
-    wave_sync();
+      y[t] = row_smv(R[t], sh.L + (s0 + t) * L_SLOT, i, x[t], t > 0 ? x[t - 1] : xm, t < RK - 1 ? x[t + 1] : xp);
   };
+  stage(lam);
+  __syncthreads();
+  float pm, pp;
+  neighbours(pm, pp);
 
   // ---- prologue: r = gamma - S lambda, z = Pinv r, p = z, eta = r.z
-  smv();
+  smv(lam, pm, pp);
+  float part = 0.f;
 #pragma unroll
   for (int t = 0; t < RK; ++t) {
-    const uint32_t bk = (uint32_t)lg + ng * t;
-    r[t] = has[t] ? gamma[(size_t)NX * (K0 + bk) + i] - y[t] : 0.f;
+    const uint32_t bk = b0 + (uint32_t)t;
+    r[t] = live && bk < N && lane_ok ? gamma[(size_t)NX * (K0 + bk) + i] - y[t] : 0.f;
+    z[t] = rot_mv(R[t].P, r[t]);
+    part += r[t] * z[t];
   }
-  stage_r();
-  float eta = seg_sum(precond(), sh.red[0], w0, nw);
+  part = wave_sum(part);
+  __syncthreads();  // every neighbour lambda read before zb is reused
+  if ((tid & 63u) == 0) sh.red[1][w] = part;
+  stage(z);
+  __syncthreads();
+  float eta = seg_sum(sh.red[1], w0, nw);
+  neighbours(pm, pp);
 #pragma unroll
-  for (int t = 0; t < RK; ++t) {
-    p[t] = z[t];
-    if (has[t]) sh.pv[(lbase + (uint32_t)lg + ng * t) * NX + i] = p[t];
-  }
+  for (int t = 0; t < RK; ++t) p[t] = z[t];
   uint32_t it = 0;
   bool conv = !live || fabsf(eta) < tol;
 
   for (;;) {
-    if (mine && lg == 0 && i == 0) sh.act[slot] = (!conv && it < max_iter) ? 1u : 0u;
-    __syncthreads();  // p (and the flags) visible to the workgroup
-    uint32_t any = 0;
-    for (uint32_t j = 0; j < ni; ++j) any |= sh.act[j];
-    if (!any) break;
-    const bool run = live && !conv && it < max_iter;
-    smv();
-    float part = 0.f;
-#pragma unroll
-    for (int t = 0; t < RK; ++t) part += p[t] * y[t];
-    const float pAp = seg_sum(run ? part : 0.f, sh.red[1], w0, nw);
+    const bool run = live && !conv && it < max_iter;  // uniform over the instance's waves
+    float pyp = 0.f;
     if (run) {
-      const float alpha = eta / pAp;
+      smv(p, pm, pp);
+#pragma unroll
+      for (int t = 0; t < RK; ++t) pyp += p[t] * y[t];
+      pyp = wave_sum(pyp);
+    }
+    if ((tid & 63u) == 0) {
+      sh.red[0][w] = pyp;
+      sh.act[w] = run ? 1u : 0u;
+    }
+    __syncthreads();
+    uint32_t any = 0;
+#pragma unroll
+    for (int v = 0; v < NWAV; ++v) any |= sh.act[v];
+    if (!any) break;
+    float rz = 0.f;
+    if (run) {
+      const float alpha = eta / seg_sum(sh.red[0], w0, nw);
 #pragma unroll
       for (int t = 0; t < RK; ++t) {
         lam[t] += alpha * p[t];
-        r[t] -= alpha * y[t];
+        r[t] = lane_ok ? r[t] - alpha * y[t] : 0.f;
+        z[t] = rot_mv(R[t].P, r[t]);
+        rz += r[t] * z[t];
       }
+      stage(z);
+      rz = wave_sum(rz);
     }
-    stage_r();
-    const float rz = precond();
-    const float eta_new = seg_sum(run ? rz : 0.f, sh.red[0], w0, nw);
+    if ((tid & 63u) == 0) sh.red[1][w] = rz;
+    __syncthreads();
     if (run) {
+      const float eta_new = seg_sum(sh.red[1], w0, nw);
       ++it;
       if (fabsf(eta_new) < tol) {
         conv = true;
       } else {
         const float beta = eta_new / eta;
         eta = eta_new;
+        float zm, zp;
+        neighbours(zm, zp);
 #pragma unroll
-        for (int t = 0; t < RK; ++t) {
-          p[t] = z[t] + beta * p[t];
-          if (has[t]) sh.pv[(lbase + (uint32_t)lg + ng * t) * NX + i] = p[t];
-        }
+        for (int t = 0; t < RK; ++t) p[t] = fmaf(beta, p[t], z[t]);
+        pm = fmaf(beta, pm, zm);
+        pp = fmaf(beta, pp, zp);
       }
     }
   }
@@ -238,7 +205,7 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
   if (!live) return;
 #pragma unroll
   for (int t = 0; t < RK; ++t)
-    if (has[t]) lambda[(size_t)NX * (K0 + (uint32_t)lg + ng * t) + i] = lam[t];
+    if (b0 + t < N && lane_ok) lambda[(size_t)NX * (K0 + b0 + t) + i] = lam[t];
   if (lg == 0 && i == 0) {
     if (iters_out) iters_out[inst] = it;
     if (flag_out) flag_out[inst] = conv ? 0u : 1u;
@@ -251,6 +218,7 @@ uint32_t pcg_res_groups(uint32_t N) {
   const uint32_t groups = (N + RK - 1) / RK;
   return (groups + 3u) & ~3u;  // whole waves
 }
+uint32_t pcg_res_bin_groups() { return NGRP; }
 uint32_t pcg_res_max_knots() { return KNOTS; }
 uint32_t pcg_res_max_items() { return MAXI; }
 

@@ -423,10 +423,10 @@ int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
   HIP_TRY(hipMemcpy(h->knot_inst, ki.data(), sizeof(uint32_t) * T, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(h->inst_k0, k0.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice));
   // Bins for the resident PCG: instances with N <= 128 packed first-fit
-  // decreasing into workgroups of 64 groups (whole waves per instance, at
+  // decreasing into workgroups of pcg_res_bin_groups() 16-lane groups (whole waves per instance, at
   // most pcg_res_max_items() instances per bin); longer instances go to the
   // streaming kernel.
-  const uint32_t cap_groups = pcg_res_max_knots() / 2, max_items = pcg_res_max_items();
+  const uint32_t cap_groups = pcg_res_bin_groups(), max_items = pcg_res_max_items();
   std::vector<uint32_t> order, big, teams;
   uint32_t max_big = 0;
   const bool team_ok = pcg_team_max_grid() >= 16;
