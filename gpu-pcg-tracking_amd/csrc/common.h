@@ -63,10 +63,12 @@ struct Params {
   uint32_t flags;
 };
 
-// mailbox of a two-workgroup PCG team: per half and epoch parity, 16
-// self-validating words {epoch (high 32 bits), float bits (low 32 bits)}
+// mailbox of a two-workgroup PCG team: per epoch parity and half, 32
+// self-validating words {epoch (high 32 bits), float bits (low 32 bits)}:
+// words 0..7 the partial sums of the half's 8 waves, 8..21 the vector
+// element of the half's row next to the split
 struct TeamMailbox {
-  unsigned long long w[2][2][16];
+  unsigned long long w[2][2][32];
 };
 
 // per-instance device state of the batched SQP (struct of arrays)

@@ -119,11 +119,7 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
     vp = in_p ? zb0[RK * 16] : 0.f;
   };
   // y = S x for the group's knots; xm / xp: elements of knots 4lg-1, 4lg+4
-  auto smv = [&](const float (&x)[RK], float xm, float xp) {
-#pragma unroll
-    for (int t = 0; t < RK; ++t)
-      y[t] = row_smv(R[t], sh.L + (s0 + t) * L_SLOT, i, x[t], t > 0 ? x[t - 1] : xm, t < RK - 1 ? x[t + 1] : xp);
-  };
+  auto smv = [&](const float (&x)[RK], float xm, float xp) { group_smv<RK>(R, sh.L + s0 * L_SLOT, i, x, xm, xp, y); };
   stage(lam);
   __syncthreads();
   float pm, pp;
@@ -136,9 +132,10 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
   for (int t = 0; t < RK; ++t) {
     const uint32_t bk = b0 + (uint32_t)t;
     r[t] = live && bk < N && lane_ok ? gamma[(size_t)NX * (K0 + bk) + i] - y[t] : 0.f;
-    z[t] = rot_mv(R[t].P, r[t]);
-    part += r[t] * z[t];
   }
+  group_precond<RK>(R, r, z);
+#pragma unroll
+  for (int t = 0; t < RK; ++t) part += r[t] * z[t];
   part = wave_sum(part);
   __syncthreads();  // every neighbour lambda read before zb is reused
   if ((tid & 63u) == 0) sh.red[1][w] = part;
@@ -176,15 +173,18 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
       for (int t = 0; t < RK; ++t) {
         lam[t] += alpha * p[t];
         r[t] = lane_ok ? r[t] - alpha * y[t] : 0.f;
-        z[t] = rot_mv(R[t].P, r[t]);
-        rz += r[t] * z[t];
       }
+      group_precond<RK>(R, r, z);
+#pragma unroll
+      for (int t = 0; t < RK; ++t) rz += r[t] * z[t];
       stage(z);
       rz = wave_sum(rz);
     }
     if ((tid & 63u) == 0) sh.red[1][w] = rz;
     __syncthreads();
     if (run) {
+      float zm, zp;
+      neighbours(zm, zp);
       const float eta_new = seg_sum(sh.red[1], w0, nw);
       ++it;
       if (fabsf(eta_new) < tol) {
@@ -192,8 +192,6 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
       } else {
         const float beta = eta_new / eta;
         eta = eta_new;
-        float zm, zp;
-        neighbours(zm, zp);
 #pragma unroll
         for (int t = 0; t < RK; ++t) p[t] = fmaf(beta, p[t], z[t]);
         pm = fmaf(beta, pm, zm);

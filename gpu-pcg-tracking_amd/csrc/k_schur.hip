@@ -1,11 +1,15 @@
 // k_schur.hip -- Schur complement, block-Jacobi preconditioner and dz
 // recovery for gfx950.
 //
-// Mapping: one 16-lane group per block row, lane j owns column j of every
-// 14x14 block (registers), 4 groups per 64-lane workgroup.  Block inverses
-// are Gauss-Jordan eliminations on [M | I] with the pivot column broadcast
-// through the group's LDS slice (the regularised blocks are symmetric
-// positive definite, so no pivoting is needed).
+// Two kernels, one 16-lane group per knot / block row, lane j owning
+// column j of every 14x14 block (registers), 4 groups per 64-lane workgroup:
+//   k_invert_g:   G~^-1 of every knot (Q~_k^-1, R~_k^-1) into the G^-1 buffer;
+//   k_form_schur: per block row, the products, theta, its inverse and the
+//                 outputs, reading Q~^-1 / R~^-1 from G^-1.
+// Block inverses are Gauss-Jordan eliminations on [M | I]; the pivot column
+// is broadcast within the group by ds_swizzle (no LDS round trip, no
+// barrier).  The regularised blocks are symmetric positive definite, so no
+// pivoting is needed.
 //
 // Math (reference: oldschur.cuh:518-963, signs as there; C stores -A, -B):
 //   Q~_k = Q_k + rho I, R~_k = R_k + rho I
@@ -29,61 +33,107 @@ constexpr int GROUP = 16;
 constexpr int GROUPS_PER_WG = 4;
 constexpr int WG = GROUP * GROUPS_PER_WG;
 
-struct SchurLds {
-  float A[NXX];    // A_k (column-major)
-  float B[98];     // B_k
-  float Qi[NXX];   // Q~_k^-1
-  float Q1i[NXX];  // Q~_{k+1}^-1
-  float Ri[49];    // R~_k^-1
-  float AQ[NXX];   // A Q~_k^-1
-  float BR[98];    // B R~_k^-1
-  float th[NXX];   // theta
-  float piv[16];
-  float qk[16], q1[16], rk[8], cb[16];
-};
+// value of lane P of the calling lane's 16-lane group (ds_swizzle bitmask
+// mode: lane' = (lane & 0x10) | P within each 32-lane half)
+template <int P>
+__device__ __forceinline__ float bcast16(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x10 | (P << 5)));
+}
 
-// In-place Gauss-Jordan inverse of an n x n column-major LDS matrix
-// mat (n <= 14) held as columns by lanes 0..n-1.  The inverse column j is
-// returned in x (registers of lane j) and written back to out (may alias mat).
+// Materialise values at this point of the program: keeps the compiler from
+// sinking their computation to a later (conditional) use, which would keep
+// all of its inputs alive in registers.
 template <int n>
-__device__ __forceinline__ void gj_inverse(int lane, const float *mat, float *piv, float *x,
-                                           float *out, float add_diag) {
-  float a[n];
-  const int j = lane < n ? lane : n - 1;
+__device__ __forceinline__ void pin(float (&v)[n]) {
+#pragma unroll
+  for (int r = 0; r < n; ++r) asm volatile("" : "+v"(v[r]));
+}
+
+template <int n, int p>
+__device__ __forceinline__ void gj_step(float (&a)[NX], float (&x)[NX]) {
+  float pc[n];
+#pragma unroll
+  for (int r = 0; r < n; ++r) pc[r] = bcast16<p>(a[r]);
+  const float inv = 1.f / pc[p];
+  const float ap = a[p] * inv, xp = x[p] * inv;
 #pragma unroll
   for (int r = 0; r < n; ++r) {
-    a[r] = mat[r + n * j] + ((r == j) ? add_diag : 0.f);
-    x[r] = (r == j) ? 1.f : 0.f;
+    if (r == p) continue;
+    a[r] -= pc[r] * ap;
+    x[r] -= pc[r] * xp;
   }
-#pragma unroll
-  for (int p = 0; p < n; ++p) {
-    __syncthreads();
-    if (lane == p) {
-#pragma unroll
-      for (int r = 0; r < n; ++r) piv[r] = a[r];
-    }
-    __syncthreads();
-    float pc[n];
-#pragma unroll
-    for (int r = 0; r < n; ++r) pc[r] = piv[r];
-    const float inv = 1.f / pc[p];
-    const float ap = a[p] * inv, xp = x[p] * inv;
-#pragma unroll
-    for (int r = 0; r < n; ++r) {
-      if (r == p) continue;
-      a[r] -= pc[r] * ap;
-      x[r] -= pc[r] * xp;
-    }
-    a[p] = ap;
-    x[p] = xp;
-  }
-  __syncthreads();
-  if (lane < n) {
-#pragma unroll
-    for (int r = 0; r < n; ++r) out[r + n * lane] = x[r];
-  }
-  __syncthreads();
+  a[p] = ap;
+  x[p] = xp;
+  // the x updates would otherwise sink to their use after the last step and
+  // keep every step's pivot column alive
+  pin(a);
+  pin(x);
 }
+
+template <int n, int p = 0>
+__device__ __forceinline__ void gj_steps(float (&a)[NX], float (&x)[NX]) {
+  if constexpr (p < n) {
+    gj_step<n, p>(a, x);
+    gj_steps<n, p + 1>(a, x);
+  }
+}
+
+// Gauss-Jordan inverse of an n x n matrix (n <= 14) held by columns: lane
+// j < n holds column j of M in a[0..n) (lanes >= n hold a copy of column
+// n-1, which no lane reads); returns column j of M^-1 in x.  Every lane of
+// the group must be active.
+template <int n>
+__device__ __forceinline__ void gj_inverse(int lane, float (&a)[NX], float (&x)[NX]) {
+  const int j = lane < n ? lane : n - 1;
+#pragma unroll
+  for (int r = 0; r < n; ++r) x[r] = (r == j) ? 1.f : 0.f;
+  gj_steps<n>(a, x);
+}
+
+// G~_k^-1 = [ (Q_k + rho I)^-1 | (R_k + rho I)^-1 ] of every knot (the
+// last knot of an instance has no R block)
+__global__ void __launch_bounds__(WG) k_invert_g(Batch b, const float *__restrict__ G,
+                                                 const float *__restrict__ rho_dev, float rho_s,
+                                                 const uint32_t *__restrict__ status, float *__restrict__ Ginv) {
+  const int lane = threadIdx.x % GROUP;
+  const uint32_t gk_raw = blockIdx.x * GROUPS_PER_WG + threadIdx.x / GROUP;
+  bool valid = gk_raw < b.total_knots;
+  const uint32_t gk = valid ? gk_raw : b.total_knots - 1;
+  const KnotRef kr = locate(b, gk);
+  if (status && !(status[kr.inst] & ST_ACTIVE)) valid = false;
+  const float rho = rho_dev ? rho_dev[kr.inst] : rho_s;
+  const float *Gk = G + (size_t)G_STRIDE * gk;
+  float *Gi = Ginv + (size_t)G_STRIDE * gk;
+  float a[NX], x[NX];
+  {
+    const int j = lane < NX ? lane : NX - 1;
+#pragma unroll
+    for (int r = 0; r < NX; ++r) a[r] = Gk[r + NX * j] + ((r == j) ? rho : 0.f);
+    gj_inverse<NX>(lane, a, x);
+    if (valid && lane < NX) {
+#pragma unroll
+      for (int r = 0; r < NX; ++r) Gi[r + NX * lane] = x[r];
+    }
+  }
+  if (kr.k + 1 < kr.N) {  // uniform over the group
+    const int j = lane < NU ? lane : NU - 1;
+#pragma unroll
+    for (int r = 0; r < NU; ++r) a[r] = Gk[NXX + r + NU * j] + ((r == j) ? rho : 0.f);
+    gj_inverse<NU>(lane, a, x);
+    if (valid && lane < NU) {
+#pragma unroll
+      for (int r = 0; r < NU; ++r) Gi[NXX + r + NU * lane] = x[r];
+    }
+  }
+}
+
+struct SchurLds {
+  float A[NXX];   // A_k (column-major)
+  float B[98];    // B_k
+  float AQ[NXX];  // A Q~_k^-1
+  float BR[98];   // B R~_k^-1
+  float qk[16], q1[16], rk[8], cb[16];
+};
 
 __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restrict__ G,
                                                    const float *__restrict__ C,
@@ -92,7 +142,7 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
                                                    const float *__restrict__ rho_dev, float rho_s,
                                                    const uint32_t *__restrict__ status,
                                                    float *__restrict__ S, float *__restrict__ Pinv,
-                                                   float *__restrict__ gamma, float *__restrict__ Ginv,
+                                                   float *__restrict__ gamma, const float *__restrict__ Ginv,
                                                    int zero_unused) {
   __shared__ SchurLds lds[GROUPS_PER_WG];
   const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
@@ -107,22 +157,16 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
   const size_t row = (size_t)kr.K0 + bk;  // global block row
   float *Srow = S + (size_t)S_STRIDE * row;
   float *Prow = Pinv + (size_t)S_STRIDE * row;
-  float x[NX];
-  // Every group runs the same barrier sequence; block row 0 stages its own
-  // knot in the 'previous knot' slots and only keeps Q~_0^-1.
+  // Every group runs the same sequence; block row 0 stages its own knot in
+  // the 'previous knot' slots and only keeps Q~_0^-1.
   const bool row0 = bk == 0;
-  // block row b >= 1: k = b - 1
-  const size_t kprev = row0 ? row : row - 1;
-  const float *Gk = G + (size_t)G_STRIDE * kprev;
-  const float *Gk1 = G + (size_t)G_STRIDE * row;
+  const size_t kprev = row0 ? row : row - 1;  // block row b >= 1: k = b - 1
   const float *Ck = C + (size_t)C_STRIDE * kprev;
-  for (int i = lane; i < NXX; i += GROUP) {
-    L.A[i] = -Ck[i];
-    L.th[i] = Gk[i];    // staging for Q_k
-    L.AQ[i] = Gk1[i];   // staging for Q_{k+1}
-  }
+  const float *Qi = Ginv + (size_t)G_STRIDE * kprev;  // Q~_k^-1
+  const float *Ri = Qi + NXX;                         // R~_k^-1
+  const float *Q1i = Ginv + (size_t)G_STRIDE * row;   // Q~_b^-1
+  for (int i = lane; i < NXX; i += GROUP) L.A[i] = -Ck[i];
   for (int i = lane; i < 98; i += GROUP) L.B[i] = -Ck[NXX + i];
-  for (int i = lane; i < 49; i += GROUP) L.BR[i] = Gk[NXX + i];  // staging for R_k
   if (lane < NX) {
     L.qk[lane] = g[(size_t)NXU * kprev + lane];
     L.q1[lane] = g[(size_t)NXU * row + lane];
@@ -130,17 +174,13 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
   }
   if (lane < NU) L.rk[lane] = g[(size_t)NXU * kprev + NX + lane];
   __syncthreads();
-  gj_inverse<NX>(lane, L.th, L.piv, x, L.Qi, rho);
-  gj_inverse<NX>(lane, L.AQ, L.piv, x, L.Q1i, rho);
-  float xr[NU];
-  gj_inverse<NU>(lane, L.BR, L.piv, xr, L.Ri, rho);
 
   // AQ = A Q~_k^-1 (column j), BR = B R~_k^-1 (column j < 7)
+  const int j = lane < NX ? lane : NX - 1;
   {
-    const int j = lane < NX ? lane : NX - 1;
     float qi[NX];
 #pragma unroll
-    for (int l = 0; l < NX; ++l) qi[l] = L.Qi[l + NX * j];
+    for (int l = 0; l < NX; ++l) qi[l] = Qi[l + NX * j];
     float acc[NX];
 #pragma unroll
     for (int r = 0; r < NX; ++r) acc[r] = 0.f;
@@ -148,63 +188,59 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
     for (int l = 0; l < NX; ++l)
 #pragma unroll
       for (int r = 0; r < NX; ++r) acc[r] += L.A[r + NX * l] * qi[l];
-    float bacc[NX];
+    pin(acc);
+    if (lane < NX) {
+#pragma unroll
+      for (int r = 0; r < NX; ++r) L.AQ[r + NX * lane] = acc[r];
+    }
+  }
+  {
     const int ju = lane < NU ? lane : NU - 1;
     float ri[NU];
 #pragma unroll
-    for (int l = 0; l < NU; ++l) ri[l] = L.Ri[l + NU * ju];
+    for (int l = 0; l < NU; ++l) ri[l] = row0 ? 0.f : Ri[l + NU * ju];
+    float bacc[NX];
 #pragma unroll
     for (int r = 0; r < NX; ++r) bacc[r] = 0.f;
 #pragma unroll
     for (int l = 0; l < NU; ++l)
 #pragma unroll
       for (int r = 0; r < NX; ++r) bacc[r] += L.B[r + NX * l] * ri[l];
-    __syncthreads();
-    if (lane < NX) {
-#pragma unroll
-      for (int r = 0; r < NX; ++r) L.AQ[r + NX * lane] = acc[r];
-    }
+    pin(bacc);
     if (lane < NU) {
 #pragma unroll
       for (int r = 0; r < NX; ++r) L.BR[r + NX * lane] = bacc[r];
     }
-    __syncthreads();
   }
-  // theta(:, j) = sum_l AQ(:, l) A(j, l) + sum_l BR(:, l) B(j, l) + Q~_{k+1}^-1(:, j)
-  {
-    const int j = lane < NX ? lane : NX - 1;
-    float acc[NX];
+  __syncthreads();
+  // theta(:, j) = sum_l AQ(:, l) A(j, l) + sum_l BR(:, l) B(j, l) + Q~_b^-1(:, j)
+  float th[NX];
 #pragma unroll
-    for (int r = 0; r < NX; ++r) acc[r] = L.Q1i[r + NX * j];
+  for (int r = 0; r < NX; ++r) th[r] = Q1i[r + NX * j];
 #pragma unroll
-    for (int l = 0; l < NX; ++l) {
-      const float ajl = L.A[j + NX * l];
+  for (int l = 0; l < NX; ++l) {
+    const float ajl = L.A[j + NX * l];
 #pragma unroll
-      for (int r = 0; r < NX; ++r) acc[r] += L.AQ[r + NX * l] * ajl;
-    }
-#pragma unroll
-    for (int l = 0; l < NU; ++l) {
-      const float bjl = L.B[j + NX * l];
-#pragma unroll
-      for (int r = 0; r < NX; ++r) acc[r] += L.BR[r + NX * l] * bjl;
-    }
-    if (lane < NX) {
-#pragma unroll
-      for (int r = 0; r < NX; ++r) L.th[r + NX * lane] = acc[r];
-    }
-    __syncthreads();
+    for (int r = 0; r < NX; ++r) th[r] += L.AQ[r + NX * l] * ajl;
   }
+#pragma unroll
+  for (int l = 0; l < NU; ++l) {
+    const float bjl = L.B[j + NX * l];
+#pragma unroll
+    for (int r = 0; r < NX; ++r) th[r] += L.BR[r + NX * l] * bjl;
+  }
+  pin(th);
   if (valid && row0) {
     // S(0,0) = -Q~_0^-1, gamma_0 = -Q~_0^-1 q_0, Pinv(0,0) = -Q~_0
     const float *Q0 = G + (size_t)G_STRIDE * row;
     for (int i = lane; i < NXX; i += GROUP) {
-      Srow[NXX + i] = -L.Qi[i];
+      Srow[NXX + i] = -Qi[i];
       Prow[NXX + i] = -(Q0[i] + ((i % NX == i / NX) ? rho : 0.f));
     }
     if (lane < NX) {
       float acc = 0.f;
 #pragma unroll
-      for (int l = 0; l < NX; ++l) acc += L.Qi[lane + NX * l] * L.qk[l];
+      for (int l = 0; l < NX; ++l) acc += Qi[lane + NX * l] * L.qk[l];
       gamma[(size_t)NX * row + lane] = -acc;
     }
     if (zero_unused)
@@ -214,11 +250,11 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
         Prow[2 * NXX + i] = 0.f;
       }
   }
-  // stores that do not need theta^-1
   if (valid && !row0) {
-    for (int i = lane; i < NXX; i += GROUP) {
-      Srow[i] = L.AQ[i];
-      Srow[NXX + i] = -L.th[i];
+    for (int i = lane; i < NXX; i += GROUP) Srow[i] = L.AQ[i];
+    if (lane < NX) {
+#pragma unroll
+      for (int r = 0; r < NX; ++r) Srow[NXX + r + NX * lane] = -th[r];
     }
     // right block of the previous row: S(b-1, b) = AQ^T
     float *Sprev = S + (size_t)S_STRIDE * kprev + 2 * NXX;
@@ -230,33 +266,28 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
       float a1 = 0.f, a2 = 0.f, a3 = 0.f;
 #pragma unroll
       for (int l = 0; l < NX; ++l) {
-        a1 += L.Q1i[lane + NX * l] * L.q1[l];
+        a1 += Q1i[lane + NX * l] * L.q1[l];
         a2 += L.AQ[lane + NX * l] * L.qk[l];
       }
 #pragma unroll
       for (int l = 0; l < NU; ++l) a3 += L.BR[lane + NX * l] * L.rk[l];
       gamma[(size_t)NX * row + lane] = L.cb[lane] - (a1 - a2 - a3);
     }
-    float *Gi = Ginv + (size_t)G_STRIDE * kprev;
-    for (int i = lane; i < NXX; i += GROUP) Gi[i] = L.Qi[i];
-    for (int i = lane; i < 49; i += GROUP) Gi[NXX + i] = L.Ri[i];
-    if (bk + 1 == kr.N) {
-      float *Gl = Ginv + (size_t)G_STRIDE * row;
-      for (int i = lane; i < NXX; i += GROUP) Gl[i] = L.Q1i[i];
-      if (zero_unused)
+    if (zero_unused) {
+      if (bk + 1 == kr.N)
         for (int i = lane; i < NXX; i += GROUP) {
           Srow[2 * NXX + i] = 0.f;
           Prow[2 * NXX + i] = 0.f;
         }
-    }
-    if (zero_unused)
       for (int i = lane; i < NXX; i += GROUP) {
         Prow[i] = 0.f;
         if (bk + 1 < kr.N) Prow[2 * NXX + i] = 0.f;
       }
+    }
   }
   // Pinv(b,b) = -theta^-1
-  gj_inverse<NX>(lane, L.th, L.piv, x, L.AQ, 0.f);
+  float x[NX];
+  gj_inverse<NX>(lane, th, x);
   if (valid && !row0 && lane < NX) {
 #pragma unroll
     for (int r = 0; r < NX; ++r) Prow[NXX + r + NX * lane] = -x[r];
@@ -328,8 +359,12 @@ hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, con
                              float *S, float *Pinv, float *gamma, float *Ginv, int zero_unused,
                              hipStream_t st) {
   if (b.total_knots == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_form_schur, dim3((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG),
-                     0, st, b, G, C, g, c, rho_dev, rho, status, S, Pinv, gamma, Ginv, zero_unused);
+  const dim3 grid((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG);
+  hipLaunchKernelGGL(k_invert_g, grid, dim3(WG), 0, st, b, G, rho_dev, rho, status, Ginv);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_form_schur, grid, dim3(WG), 0, st, b, G, C, g, c, rho_dev, rho, status, S, Pinv, gamma,
+                     (const float *)Ginv, zero_unused);
   return hipGetLastError();
 }
 

@@ -41,22 +41,36 @@ __device__ __forceinline__ uint32_t ror_step() {
   return (uint32_t)(s - i) & 15u;
 }
 
-// acc + sum_{n=1..15} M[n] * ror_n(x).  A leading s_nop 4 covers the wait
-// states required between a VALU (or EXEC) write and a DPP read of x.
-__device__ __forceinline__ float rot_fma15(float acc, float x, const float (&M)[16]) {
-#define MPC_RF(n, op) "v_fmac_f32_dpp %0, %1, %" #op " row_ror:" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
-  asm("s_nop 4\n" MPC_RF(1, 2) MPC_RF(2, 3) MPC_RF(3, 4) MPC_RF(4, 5) MPC_RF(5, 6) MPC_RF(6, 7) MPC_RF(7, 8)
-          MPC_RF(8, 9) MPC_RF(9, 10) MPC_RF(10, 11) MPC_RF(11, 12) MPC_RF(12, 13) MPC_RF(13, 14)
-              MPC_RF(14, 15) MPC_RF(15, 16)
-      : "+v"(acc)
-      : "v"(x), "v"(M[1]), "v"(M[2]), "v"(M[3]), "v"(M[4]), "v"(M[5]), "v"(M[6]), "v"(M[7]), "v"(M[8]),
-        "v"(M[9]), "v"(M[10]), "v"(M[11]), "v"(M[12]), "v"(M[13]), "v"(M[14]), "v"(M[15]));
-#undef MPC_RF
-  return acc;
+// Two interleaved accumulations (independent dependency chains):
+//   a += sum_{n=1..15} Ma[n] * ror_n(xa),  b += sum_{n=1..15} Mb[n] * ror_n(xb).
+// A leading s_nop 4 covers the wait states required between a VALU (or
+// EXEC) write and a DPP read of xa / xb.
+__device__ __forceinline__ void rot_fma15x2(float &a, float xa, const float (&Ma)[16], float &b, float xb,
+                                            const float (&Mb)[16]) {
+#define MPC_RF2(n, oa, ob)                                                                        \
+  "v_fmac_f32_dpp %0, %2, %" #oa " row_ror:" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+  "v_fmac_f32_dpp %1, %3, %" #ob " row_ror:" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+  asm("s_nop 4\n" MPC_RF2(1, 4, 11) MPC_RF2(2, 5, 12) MPC_RF2(3, 6, 13) MPC_RF2(4, 7, 14) MPC_RF2(5, 8, 15)
+          MPC_RF2(6, 9, 16) MPC_RF2(7, 10, 17)
+      : "+v"(a), "+v"(b)
+      : "v"(xa), "v"(xb), "v"(Ma[1]), "v"(Ma[2]), "v"(Ma[3]), "v"(Ma[4]), "v"(Ma[5]), "v"(Ma[6]), "v"(Ma[7]),
+        "v"(Mb[1]), "v"(Mb[2]), "v"(Mb[3]), "v"(Mb[4]), "v"(Mb[5]), "v"(Mb[6]), "v"(Mb[7]));
+  asm("s_nop 4\n" MPC_RF2(8, 4, 12) MPC_RF2(9, 5, 13) MPC_RF2(10, 6, 14) MPC_RF2(11, 7, 15) MPC_RF2(12, 8, 16)
+          MPC_RF2(13, 9, 17) MPC_RF2(14, 10, 18) MPC_RF2(15, 11, 19)
+      : "+v"(a), "+v"(b)
+      : "v"(xa), "v"(xb), "v"(Ma[8]), "v"(Ma[9]), "v"(Ma[10]), "v"(Ma[11]), "v"(Ma[12]), "v"(Ma[13]),
+        "v"(Ma[14]), "v"(Ma[15]), "v"(Mb[8]), "v"(Mb[9]), "v"(Mb[10]), "v"(Mb[11]), "v"(Mb[12]), "v"(Mb[13]),
+        "v"(Mb[14]), "v"(Mb[15]));
+#undef MPC_RF2
 }
 
-// (M x)_i for a rotated row M of a full 16-lane row
-__device__ __forceinline__ float rot_mv(const float (&M)[16], float x) { return rot_fma15(M[0] * x, x, M); }
+// (Ma xa)_i and (Mb xb)_i for rotated rows Ma, Mb of full 16-lane rows
+__device__ __forceinline__ void rot_mv2(const float (&Ma)[16], float xa, float &ya, const float (&Mb)[16], float xb,
+                                        float &yb) {
+  ya = Ma[0] * xa;
+  yb = Mb[0] * xb;
+  rot_fma15x2(ya, xa, Ma, yb, xb, Mb);
+}
 
 // wave64 sum, identical in every lane, in a fixed order: quads, eights and
 // rows by DPP, then (row0 + row1) + (row2 + row3)
@@ -114,41 +128,58 @@ __device__ __forceinline__ void read_lrow(float (&Lr)[16], const float *Ls, uint
 
 // (L^T x)_i from the rotated rows Lr of L held by the lanes of the row.
 // The products are formed first so that no DPP read directly follows the
-// VALU write of its source.
+// VALU write of its source; two partial sums (odd / even n) shorten the
+// dependency chain.
 __device__ __forceinline__ float rot_mtv(const float (&Lr)[16], float x) {
   float m[16];
 #pragma unroll
   for (int n = 0; n < 16; ++n) m[n] = Lr[(16 - n) & 15] * x;
-  float acc = m[0];
-  acc += dpp_f<0x121>(m[1]);
-  acc += dpp_f<0x122>(m[2]);
-  acc += dpp_f<0x123>(m[3]);
-  acc += dpp_f<0x124>(m[4]);
-  acc += dpp_f<0x125>(m[5]);
-  acc += dpp_f<0x126>(m[6]);
-  acc += dpp_f<0x127>(m[7]);
-  acc += dpp_f<0x128>(m[8]);
-  acc += dpp_f<0x129>(m[9]);
-  acc += dpp_f<0x12A>(m[10]);
-  acc += dpp_f<0x12B>(m[11]);
-  acc += dpp_f<0x12C>(m[12]);
-  acc += dpp_f<0x12D>(m[13]);
-  acc += dpp_f<0x12E>(m[14]);
-  acc += dpp_f<0x12F>(m[15]);
-  return acc;
+  float a = m[0], b = dpp_f<0x121>(m[1]);
+  a += dpp_f<0x122>(m[2]);
+  b += dpp_f<0x123>(m[3]);
+  a += dpp_f<0x124>(m[4]);
+  b += dpp_f<0x125>(m[5]);
+  a += dpp_f<0x126>(m[6]);
+  b += dpp_f<0x127>(m[7]);
+  a += dpp_f<0x128>(m[8]);
+  b += dpp_f<0x129>(m[9]);
+  a += dpp_f<0x12A>(m[10]);
+  b += dpp_f<0x12B>(m[11]);
+  a += dpp_f<0x12C>(m[12]);
+  b += dpp_f<0x12D>(m[13]);
+  a += dpp_f<0x12E>(m[14]);
+  b += dpp_f<0x12F>(m[15]);
+  return a + b;
 }
 
-// y_i = (D x + L_b xm + L_{b+1}^T xp)_i; Ls is the LDS slot of knot b, the
-// next slot holds L_{b+1} (finite, and multiplied by xp = 0 when b is last)
-__device__ __forceinline__ float row_smv(const RowRegs &R, const float *Ls, uint32_t i, float x, float xm,
-                                         float xp) {
-  float Lr[16], Ln[16];
-  read_lrow(Lr, Ls, i);
-  read_lrow(Ln, Ls + L_SLOT, i);
-  const float a = rot_mv(R.D, x);
-  const float bm = rot_mv(Lr, xm);
-  const float bp = rot_mtv(Ln, xp);
-  return (a + bm) + bp;
+// y = S x for the RK consecutive knots of a group: Ls0 is the LDS slot of
+// its first knot (slot t + 1 holds L of the next knot, finite and multiplied
+// by xp = 0 after the last knot); xm / xp are the elements of the knots just
+// before / after the group.  Each slot's rows are read once.
+template <int RK>
+__device__ __forceinline__ void group_smv(const RowRegs (&R)[RK], const float *Ls0, uint32_t i, const float (&x)[RK],
+                                          float xm, float xp, float (&y)[RK]) {
+  float La[16], Lb[16];
+  read_lrow(La, Ls0, i);
+#pragma unroll
+  for (int t = 0; t < RK; ++t) {
+    float(&Lc)[16] = (t & 1) ? Lb : La;  // L_b rows of knot t
+    float(&Ln)[16] = (t & 1) ? La : Lb;  // rows of knot t + 1
+    read_lrow(Ln, Ls0 + (t + 1) * L_SLOT, i);
+    const float xl = t > 0 ? x[t - 1] : xm, xr = t < RK - 1 ? x[t + 1] : xp;
+    float a, bm;
+    rot_mv2(R[t].D, x[t], a, Lc, xl, bm);
+    const float bp = rot_mtv(Ln, xr);
+    y[t] = (a + bm) + bp;
+  }
+}
+
+// z = Pinv r for the RK knots of a group, two interleaved chains at a time
+template <int RK>
+__device__ __forceinline__ void group_precond(const RowRegs (&R)[RK], const float (&r)[RK], float (&z)[RK]) {
+  static_assert(RK % 2 == 0, "knots are taken in pairs");
+#pragma unroll
+  for (int t = 0; t < RK; t += 2) rot_mv2(R[t].P, r[t], z[t], R[t + 1].P, r[t + 1], z[t + 1]);
 }
 
 }  // namespace onchip
