@@ -133,19 +133,32 @@ def test_pcg_same_rule_parity(solver, orc, N):
     assert err < max(2e-4, 4 * floor)
 
 
-# Convergence to the direct solve: run to |r^T Pinv r| < 1e-9; bound as above
-# with the floor measured against the direct solution.
-@pytest.mark.parametrize("N", [2, 16, 32, 64, 128, 256])
+# Convergence to the direct solve: run to |r^T Pinv r| < 1e-9 (hundreds to
+# ~1100 iterations).  At that depth every fp32 PCG scatters by rounding
+# alone: on these systems the GPU kernels and the fp32 oracle twin each land
+# 2e-5 .. 5e-4 from the direct solution, in no fixed order.  So the checks
+# are (a) the true residual |gamma - S lambda| / |gamma| of the GPU solution,
+# evaluated in fp64, is within 2x the twin's -- an error in the arithmetic
+# shows up there first -- and (b) the error stays below 1e-3.
+@pytest.mark.parametrize("N", [2, 16, 32, 64, 128, 200, 256])
 def test_pcg_converges_to_direct_solve(solver, orc, N):
     S32, P32, gam32 = _natural_schur(orc, 30 + N, N)
     lam_g, it_g, fl_g = _gpu_pcg(solver, N, S32, P32, gam32, 5000, 1e-9)
     ldl = orc.LDL(N)
     lam_d = ldl.solve(S32.astype(np.float64), gam32.astype(np.float64))
-    err = helpers.rel_err(lam_g, lam_d)
-    floor = _fp32_floor(orc, N, S32, P32, gam32, 1e-9, lam_d)
-    print(f"N={N} iters={it_g} rel(gpu,direct)={err:.2e} fp32-floor={floor:.2e}")
+    lam_t, _, _ = orc.pcg(N, S32, P32, gam32, np.zeros(14 * N), 5000, 1e-9, dtype=np.float32)
+    Sd = orc.dense_S(N, S32.astype(np.float64))
+    g64 = gam32.astype(np.float64)
+
+    def res(lam):
+        return np.linalg.norm(g64 - Sd @ lam) / np.linalg.norm(g64)
+
+    err, err_t = helpers.rel_err(lam_g, lam_d), helpers.rel_err(lam_t, lam_d)
+    print(f"N={N} iters={it_g} rel(gpu,direct)={err:.2e} residual={res(lam_g):.2e} | "
+          f"fp32 twin: rel={err_t:.2e} residual={res(lam_t):.2e}")
     assert fl_g == 0
-    assert err < max(2e-4, 4 * floor)
+    assert res(lam_g) <= 2 * res(lam_t) + 1e-7
+    assert err < 1e-3
 
 
 # The resident (on-chip) and streaming PCG kernels take the same iterations
