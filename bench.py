@@ -16,6 +16,13 @@ solves / s over the whole step.  Also reported: the plain
 one-instance-per-launch solve (reference call pattern) on a sample, the
 per-phase split, the roofline of the dominant kernel and a CPU baseline
 (the sparse LDL^T solve of the Schur system, fp32, one core).
+
+Also reported ("differentiation"): the adjoint / forward cost of the
+differentiable QP layer on 256 instances at N = 64 (same cost and inputs):
+forward = the QP layer at the final SQP iterate plus the tracking loss and
+its gradient, adjoint = recomputed linearisation, adjoint KKT solve and the
+parameter gradients (21 weights + all reference points); both with two
+steps of iterative refinement.
 """
 from __future__ import annotations
 
@@ -60,6 +67,7 @@ def parse():
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-diff", action="store_true")
     return ap.parse_args()
 
 
@@ -77,6 +85,73 @@ def build_workload(pkg, solver, torch, n, seed, dt):
     ee = solver.fk(q_all).reshape(-1).contiguous()  # device FK of the reference path
     xu = torch.from_numpy(np.concatenate(xus)).cuda()
     return horizons, xu, ee
+
+
+def differentiation(pkg, torch, seed, reps=10, n=256, N=64, sqp_iters=3):
+    """Adjoint / forward cost of the QP layer on n instances at horizon N.
+    The PCG cap is 1000 (the layer needs converged solves; the SQP setting's
+    173 cuts some of them short)."""
+    dt = 0.01
+    cfg = pkg.default_config(max_knots=N, max_instances=n, max_total_knots=n * N, pcg_max_iter=1000)
+    solver = pkg.Solver(cfg)
+    xus, qrefs = [], []
+    for i in range(n):
+        x0, qref = pkg.synth_instance(seed * 7919 + 100000 + i, N, dt)
+        xu = np.zeros(21 * N - 7, np.float32)
+        for k in range(N):
+            xu[21 * k:21 * k + 14] = x0
+        xus.append(xu)
+        qrefs.append(qref)
+    ee = solver.fk(torch.from_numpy(np.concatenate(qrefs)).cuda()).reshape(-1).contiguous()
+    xu = torch.from_numpy(np.concatenate(xus)).cuda()
+    solver.batch_setup([N] * n)
+    T = n * N
+    lam, rho = torch.zeros(14 * T, device="cuda"), torch.full((n,), 1e-3, device="cuda")
+    status = torch.zeros(n, dtype=torch.int32, device="cuda")
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    solver.batch_sqp(xu, lam, ee, rho, sqp_iters, status)
+    e[1].record()
+    torch.cuda.synchronize()
+    sqp_ms = e[0].elapsed_time(e[1])
+    lam0 = lam.clone()
+    dz, xstar = torch.empty_like(xu), torch.empty_like(xu)
+    loss = torch.empty(n, device="cuda")
+    dLdx, dLdref = torch.empty_like(xu), torch.empty_like(ee)
+    gw = torch.empty(21 * n, device="cuda")
+    gr = torch.empty_like(ee)
+
+    def forward():
+        lam.copy_(lam0)
+        solver.qp_layer(xu, ee, rho, lam, dz, xstar)
+        solver.qp_loss(xstar, ee, loss, dLdx, dLdref)
+
+    def adjoint():
+        gr.copy_(dLdref)
+        solver.qp_adjoint(xu, ee, rho, dz, dLdx, gw, gr)
+
+    for _ in range(2):
+        forward()
+        adjoint()
+    torch.cuda.synchronize()
+    e[0].record()
+    for _ in range(reps):
+        forward()
+    e[1].record()
+    for _ in range(reps):
+        adjoint()
+    e[2].record()
+    torch.cuda.synchronize()
+    fwd_ms, adj_ms = e[0].elapsed_time(e[1]) / reps, e[1].elapsed_time(e[2]) / reps
+    finite = bool(torch.isfinite(gw).all().item() and torch.isfinite(gr).all().item())
+    solver.close()
+    return {"workload": f"{n} instances x N={N}, Q_ee=1, QD=1e-2, R=1e-4, {sqp_iters} SQP iterations, then the "
+                        f"QP layer (PCG tol 1e-5, 2 refinement steps) and its adjoint; 21 weight + {3 * N} "
+                        f"reference-point gradients per instance",
+            "sqp_forward_ms": round(sqp_ms, 3), "layer_forward_ms": round(fwd_ms, 3),
+            "adjoint_ms": round(adj_ms, 3), "adjoint_over_layer_forward": round(adj_ms / fwd_ms, 3),
+            "adjoint_over_sqp_plus_layer": round(adj_ms / (sqp_ms + fwd_ms), 3),
+            "gradients_per_sec": round(n / (adj_ms * 1e-3), 1), "gradients_finite": finite}
 
 
 def cpu_baseline(orc, helpers, horizons, seconds, seed):
@@ -206,6 +281,8 @@ def main():
                          f"one instance per horizon at its initial iterate, weighted by the workload's horizon mix; "
                          f"{desc}; ~{args.cpu_seconds:.0f} s of CPU work"}
 
+    diff = None if args.no_diff else differentiation(pkg, torch, args.seed)
+
     its = log[executed]
     out = {
         "metric": "batched KKT solves/sec & SQP iters/sec vs plain at tol 1e-5; adjoint/fwd cost",
@@ -244,6 +321,7 @@ def main():
                      "fp32": {"achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": round(tflops / FP32_PEAK_TFLOPS, 4)}},
         "cpu_baseline": cpu,
+        "differentiation": diff,
         "wall_s_timed": round(t_wall, 3),
     }
     print(json.dumps(out))

@@ -40,8 +40,16 @@ EXPORTS = (
     "mpc_status_string", "mpc_fk", "mpc_dynamics_gradient", "mpc_form_kkt", "mpc_form_schur",
     "mpc_pcg_solve", "mpc_compute_dz", "mpc_compute_merit", "mpc_line_search_merits",
     "mpc_sqp_solve", "mpc_batch_setup", "mpc_batch_sqp", "mpc_batch_set_timing",
-    "mpc_batch_phase_times", "mpc_synth_instance", "mpc_synth_horizons",
+    "mpc_batch_phase_times", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
+    "mpc_batch_qp_loss", "mpc_batch_qp_adjoint", "mpc_synth_instance", "mpc_synth_horizons",
 )
+
+
+# iterative-refinement steps of the QP layer and adjoint solves: the fp32
+# Schur / PCG solve leaves relative errors of 1e-3..1e-2 on ill-conditioned
+# instances (rho = 1e-3, control weights near 1e-4); each step with an fp64
+# KKT residual removes most of it
+REFINE_STEPS = 2
 
 
 class MpcError(RuntimeError):
@@ -95,6 +103,11 @@ def lib():
             "mpc_batch_sqp": [vp, vp, vp, vp, vp, u32, vp, vp, vp],
             "mpc_batch_set_timing": [vp, C.c_int],
             "mpc_batch_phase_times": [vp, C.POINTER(C.c_double)],
+            "mpc_set_cost_weights": [vp, vp, vp],
+            "mpc_get_cost_weights": [vp, vp, vp],
+            "mpc_batch_qp_layer": [vp, vp, vp, vp, vp, f32, u32, f32, vp, vp, vp],
+            "mpc_batch_qp_loss": [vp, vp, vp, vp, vp, vp, vp],
+            "mpc_batch_qp_adjoint": [vp, vp, vp, vp, vp, vp, f32, u32, f32, vp, vp, vp],
         }
         for name, argt in sigs.items():
             fn = getattr(L, name)
@@ -255,6 +268,36 @@ class Solver:
 
     def batch_set_timing(self, on=True):
         _check(lib().mpc_batch_set_timing(self.h, int(on)), "mpc_batch_set_timing")
+
+    def set_cost_weights(self, wx, wu):
+        import numpy as np
+        wx = np.ascontiguousarray(np.asarray(wx, np.float32))
+        wu = np.ascontiguousarray(np.asarray(wu, np.float32))
+        if wx.shape != (14,) or wu.shape != (7,):
+            raise MpcError("weights: wx has 14 entries, wu has 7")
+        _check(lib().mpc_set_cost_weights(self.h, wx.ctypes.data, wu.ctypes.data), "mpc_set_cost_weights")
+
+    def cost_weights(self):
+        import numpy as np
+        wx, wu = np.zeros(14, np.float32), np.zeros(7, np.float32)
+        _check(lib().mpc_get_cost_weights(self.h, wx.ctypes.data, wu.ctypes.data), "mpc_get_cost_weights")
+        return wx, wu
+
+    def qp_layer(self, xu, ee_ref, rho, lam, dz, xstar, pcg_tol=0.0, refine=REFINE_STEPS, refine_tol=0.0):
+        _check(lib().mpc_batch_qp_layer(self.h, _ptr(xu), _ptr(ee_ref), _ptr(rho), _ptr(lam), float(pcg_tol),
+                                        int(refine), float(refine_tol), _ptr(dz), _ptr(xstar), _stream()),
+               "mpc_batch_qp_layer")
+
+    def qp_loss(self, xu, ee_ref, loss, dLdx=None, dLdref=None):
+        _check(lib().mpc_batch_qp_loss(self.h, _ptr(xu), _ptr(ee_ref), _ptr(loss), _ptr(dLdx), _ptr(dLdref),
+                                       _stream()), "mpc_batch_qp_loss")
+
+    def qp_adjoint(self, xu, ee_ref, rho, dz, a, grad_w, grad_ref, pcg_tol=0.0, refine=REFINE_STEPS,
+                   refine_tol=0.0):
+        _check(lib().mpc_batch_qp_adjoint(self.h, _ptr(xu), _ptr(ee_ref), _ptr(rho), _ptr(dz), _ptr(a),
+                                          float(pcg_tol), int(refine), float(refine_tol), _ptr(grad_w),
+                                          _ptr(grad_ref), _stream()),
+               "mpc_batch_qp_adjoint")
 
     def batch_phase_times(self):
         out = (C.c_double * 7)()

@@ -8,6 +8,7 @@
 //                   the reference layout of schur.cuh / toplevel.cuh.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <float.h>
 #include <stdint.h>
 
 namespace mpc {
@@ -56,7 +57,9 @@ __device__ __forceinline__ uint32_t inst_len(const Batch &b, uint32_t inst) {
 
 // Numeric parameters passed by value to the kernels.
 struct Params {
-  float dt, q_ee, qd_cost, r_cost, mu;
+  float dt, q_ee, mu;
+  float wx[14];  // per-dimension state weights (reference: 0 x 7, QD x 7)
+  float wu[7];   // per-dimension control weights (reference: R x 7)
   float rho_factor, rho_max, rho_min, rho_reset;
   float pcg_tol;
   uint32_t pcg_max_iter;
@@ -133,6 +136,24 @@ hipError_t launch_ls_update(const Batch &b, const Params &p, const float *knot_m
                             const uint32_t *pcg_iters_now, const uint32_t *pcg_flag_now,
                             uint32_t it, uint32_t log_stride, hipStream_t st);
 hipError_t launch_axpy(uint32_t n, float alpha, const float *x, float *y, hipStream_t st);
+// differentiable QP layer (k_adjoint.hip)
+hipError_t launch_qp_loss(const Batch &b, const float *xu, const float *ee_ref, float *knot_loss, float *dLdx,
+                          float *dLdref, hipStream_t st);
+hipError_t launch_pack_neg(const Batch &b, const float *a, float *gz, hipStream_t st);
+hipError_t launch_adj_params(const Batch &b, const Params &p, const float *xu, const float *ee_ref,
+                             const float *dz, const float *v, float *knot_w, float *gref, hipStream_t st);
+hipError_t launch_reduce_knots(const Batch &b, const float *vals, uint32_t S, float *out, hipStream_t st);
+hipError_t launch_sub(uint32_t n, const float *a, const float *bb, float *o, hipStream_t st);
+hipError_t launch_fill(uint32_t n, float v, float *o, hipStream_t st);
+// iterative refinement of the layer's KKT solves (k_refine.hip)
+hipError_t launch_kkt_residual(const Batch &b, const float *G, const float *C, const float *g, const float *c,
+                               const float *rho_dev, const float *dz, const float *lam, float *rz, float *rc,
+                               hipStream_t st);
+hipError_t launch_rhs_normalise(const Batch &b, float *rz, float *rc, float *scale, hipStream_t st);
+hipError_t launch_form_gamma(const Batch &b, const float *Ginv, const float *C, const float *g, const float *c,
+                             float *gamma, hipStream_t st);
+hipError_t launch_refine_update(const Batch &b, const float *scale, const float *ddz, const float *dlam, float *dz,
+                                float *lam, hipStream_t st);
 hipError_t launch_batch_init(const Batch &b, const float *xu, const float *knot_merits, InstState s,
                              hipStream_t st);
 

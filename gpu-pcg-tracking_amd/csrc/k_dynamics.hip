@@ -284,14 +284,15 @@ __global__ void __launch_bounds__(WG) k_form_kkt(Batch b, Params p, const float 
   float *Gk = G + (size_t)G_STRIDE * knot;
   for (int i = lane; i < NXX; i += GROUP) {
     const int r = i % NX, col = i / NX;
-    Gk[i] = (r < NQ && col < NQ) ? L.gq[r] * L.gq[col] : (r == col ? p.qd_cost : 0.f);
+    const float w = r == col ? p.wx[r] : 0.f;
+    Gk[i] = (r < NQ && col < NQ) ? L.gq[r] * L.gq[col] + w : w;
   }
   float *gk_ = g + (size_t)NXU * knot;
-  if (lane < NX) gk_[lane] = lane < NQ ? L.gq[lane] : p.qd_cost * L.xc[lane];
+  if (lane < NX) gk_[lane] = (lane < NQ ? L.gq[lane] : 0.f) + p.wx[lane] * L.xc[lane];
   if (kr.k == 0 && lane < NX) cvec[(size_t)NX * kr.K0 + lane] = L.x[lane] - xs[(size_t)NX * kr.inst + lane];
   if (!dyn) return;
-  for (int i = lane; i < 49; i += GROUP) Gk[NXX + i] = (i % NU == i / NU) ? p.r_cost : 0.f;
-  if (lane < NU) gk_[NX + lane] = p.r_cost * L.u[lane];
+  for (int i = lane; i < 49; i += GROUP) Gk[NXX + i] = (i % NU == i / NU) ? p.wu[i % NU] : 0.f;
+  if (lane < NU) gk_[NX + lane] = p.wu[lane] * L.u[lane];
   float *Ck = C + (size_t)C_STRIDE * knot;
   const float dt = p.dt;
   for (int i = lane; i < NXX; i += GROUP) {
@@ -391,11 +392,11 @@ __global__ void __launch_bounds__(WG) k_merit(Batch b, Params p, const float *__
   }
   __syncthreads();
   if (lane == 0 && active && valid) {
-    // cost: 0.5 QD |qd|^2 + 0.5 R |u|^2 + 0.5 w |ee - ref|^2
+    // cost: 0.5 sum wx x^2 + 0.5 sum wu u^2 + 0.5 w |ee - ref|^2 (reference: wx = (0, QD), wu = R)
     float J = 0.f;
-    for (int i = 0; i < NQ; ++i) J += 0.5f * (p.qd_cost * L.z[NQ + i] * L.z[NQ + i]);
+    for (int i = 0; i < NX; ++i) J += 0.5f * (p.wx[i] * L.z[i] * L.z[i]);
     if (dyn)
-      for (int i = 0; i < NU; ++i) J += 0.5f * (p.r_cost * L.z[NX + i] * L.z[NX + i]);
+      for (int i = 0; i < NU; ++i) J += 0.5f * (p.wu[i] * L.z[NX + i] * L.z[NX + i]);
     for (int i = 0; i < 3; ++i) {
       const float e = L.ee[i] - L.ref[i];
       J += 0.5f * p.q_ee * e * e;

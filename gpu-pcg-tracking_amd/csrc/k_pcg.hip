@@ -132,7 +132,9 @@ __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restric
   }
   __syncthreads();
   uint32_t it = 0;
-  bool conv = fabsf(eta) < tol;
+  // tol < 0: exit relative to the first residual, |eta| < -tol |eta_0|
+  const float tol_i = tol < 0.f ? fmaxf(-tol * fabsf(eta), FLT_MIN) : tol;
+  bool conv = fabsf(eta) < tol_i;
   int slot = 1;
   while (!conv && it < max_iter) {
     smv(y);
@@ -153,7 +155,7 @@ __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restric
     const float eta_new = block_sum<NW>(precond(z), red + NW * slot);
     slot ^= 1;
     ++it;
-    if (fabsf(eta_new) < tol) {
+    if (fabsf(eta_new) < tol_i) {
       conv = true;
       break;
     }

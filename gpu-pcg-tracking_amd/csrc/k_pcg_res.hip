@@ -146,7 +146,9 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
 #pragma unroll
   for (int t = 0; t < RK; ++t) p[t] = z[t];
   uint32_t it = 0;
-  bool conv = !live || fabsf(eta) < tol;
+  // tol < 0: exit relative to the first residual, |eta| < -tol |eta_0|
+  const float tol_i = tol < 0.f ? fmaxf(-tol * fabsf(eta), FLT_MIN) : tol;
+  bool conv = !live || fabsf(eta) < tol_i;
 
   for (;;) {
     const bool run = live && !conv && it < max_iter;  // uniform over the instance's waves
@@ -187,7 +189,7 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
       neighbours(zm, zp);
       const float eta_new = seg_sum(sh.red[1], w0, nw);
       ++it;
-      if (fabsf(eta_new) < tol) {
+      if (fabsf(eta_new) < tol_i) {
         conv = true;
       } else {
         const float beta = eta_new / eta;

@@ -233,7 +233,9 @@ __global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__rest
 #pragma unroll
   for (int t = 0; t < RK; ++t) p[t] = z[t];
   uint32_t it = 0;
-  bool conv = fabsf(eta) < tol;
+  // tol < 0: exit relative to the first residual, |eta| < -tol |eta_0|
+  const float tol_i = tol < 0.f ? fmaxf(-tol * fabsf(eta), FLT_MIN) : tol;
+  bool conv = fabsf(eta) < tol_i;
 
   while (!conv && it < max_iter) {
     smv(p, pm, pp);
@@ -272,7 +274,7 @@ __global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__rest
     if (far_m && lane_ok) zm = sh.recv[i];
     if (far_p && lane_ok) zp = sh.recv[i];
     ++it;
-    if (fabsf(eta_new) < tol) {
+    if (fabsf(eta_new) < tol_i) {
       conv = true;
       break;
     }

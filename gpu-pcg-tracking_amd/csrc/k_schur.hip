@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
   if (lane < NX) {
     L.qk[lane] = g[(size_t)NXU * kprev + lane];
     L.q1[lane] = g[(size_t)NXU * row + lane];
-    L.cb[lane] = row0 ? 0.f : cvec[(size_t)NX * row + lane];
+    L.cb[lane] = (row0 || !cvec) ? 0.f : cvec[(size_t)NX * row + lane];  // no c: c = 0
   }
   if (lane < NU) L.rk[lane] = g[(size_t)NXU * kprev + NX + lane];
   __syncthreads();

@@ -45,6 +45,12 @@ struct mpc_handle {
   uint32_t *team_list = nullptr, *team_one = nullptr, *team_err = nullptr;
   TeamMailbox *mbox = nullptr;
   uint32_t n_teams = 0, n_mbox = 256;
+  // differentiable QP layer workspace: adjoint multipliers (14 T), per-knot
+  // gradient terms (21 T) and per-knot loss terms (T)
+  float *adj_mu = nullptr, *adj_knot = nullptr, *loss_knot = nullptr;
+  // iterative refinement: residual right-hand side (21 T + 14 T), correction
+  // (21 T + 14 T) and per-instance scale (B)
+  float *ref_g = nullptr, *ref_c = nullptr, *ref_dz = nullptr, *ref_lam = nullptr, *ref_scale = nullptr;
   // pinned staging for the host round trips of the plain driver
   float *h_merits = nullptr;
   uint32_t *h_pcg = nullptr;
@@ -53,18 +59,27 @@ struct mpc_handle {
   std::vector<uint32_t> horizons, k0;
   uint32_t n_inst = 0, total = 0, max_n = 0;
   bool batch_ready = false;
+  // per-dimension cost weights (mpc_set_cost_weights; derived from cfg by default)
+  float wx[NX]{}, wu[NU]{};
   // timing
   bool timing = false;
   hipEvent_t ev[9]{};
   double phase_us[7]{};
 };
 
-static Params make_params(const mpc_config &c) {
+// the reference's cost as per-dimension weights: wx = (0 x 7, QD x 7), wu = R x 7
+static void weights_from_config(mpc_handle *h) {
+  for (int j = 0; j < NX; ++j) h->wx[j] = j < NQ ? 0.f : h->cfg.qd_cost;
+  for (int j = 0; j < NU; ++j) h->wu[j] = h->cfg.r_cost;
+}
+
+static Params make_params(const mpc_handle *h) {
+  const mpc_config &c = h->cfg;
   Params p;
   p.dt = c.dt;
   p.q_ee = c.q_ee;
-  p.qd_cost = c.qd_cost;
-  p.r_cost = c.r_cost;
+  for (int j = 0; j < NX; ++j) p.wx[j] = h->wx[j];
+  for (int j = 0; j < NU; ++j) p.wu[j] = h->wu[j];
   p.mu = c.mu;
   p.rho_factor = c.rho_factor;
   p.rho_max = c.rho_max;
@@ -99,6 +114,9 @@ static hipError_t pcg_single(mpc_handle *h, uint32_t N, const float *S, const fl
 static Batch batch_of(const mpc_handle *h) {
   return Batch{h->knot_inst, h->inst_k0, h->n_inst, h->total};
 }
+
+static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_status, uint32_t max_iter, float tol,
+                            hipEvent_t mid, hipStream_t st);
 
 static bool valid_numbers(const mpc_config &c) {
   return c.dt > 0 && std::isfinite(c.dt) && c.q_ee >= 0 && c.qd_cost >= 0 && c.r_cost >= 0 &&
@@ -145,7 +163,9 @@ static void free_all(mpc_handle *h) {
   void *ptrs[] = {h->G, h->Ginv, h->C, h->g, h->c, h->S, h->P, h->gamma, h->dz, h->knot_merits,
                   h->merits, h->xs, h->drho, h->merit0, h->rho1, h->status_ws, h->sqp_done,
                   h->pcg_iters, h->pcg_flag, h->knot_inst, h->inst_k0, h->bin_items,
-                  h->bin_off, h->big_list, h->team_list, h->team_one, h->team_err, h->mbox};
+                  h->bin_off, h->big_list, h->team_list, h->team_one, h->team_err, h->mbox,
+                  h->adj_mu, h->adj_knot, h->loss_knot, h->ref_g, h->ref_c, h->ref_dz, h->ref_lam,
+                  h->ref_scale};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (h->h_merits) (void)hipHostFree(h->h_merits);
@@ -166,6 +186,7 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
   mpc_handle *h = new (std::nothrow) mpc_handle();
   if (!h) return MPC_ERR_ARG;
   h->cfg = *cfg;
+  weights_from_config(h);
   if (hipGetDevice(&h->device) != hipSuccess) {
     delete h;
     return MPC_ERR_HIP;
@@ -200,6 +221,14 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
             alloc((void **)&h->team_one, sizeof(uint32_t)) &&
             alloc((void **)&h->team_err, sizeof(uint32_t)) &&
             alloc((void **)&h->mbox, sizeof(TeamMailbox) * h->n_mbox) &&
+            alloc((void **)&h->adj_mu, sizeof(float) * NX * T) &&
+            alloc((void **)&h->adj_knot, sizeof(float) * NXU * T) &&
+            alloc((void **)&h->loss_knot, sizeof(float) * T) &&
+            alloc((void **)&h->ref_g, sizeof(float) * NXU * T) &&
+            alloc((void **)&h->ref_c, sizeof(float) * NX * T) &&
+            alloc((void **)&h->ref_dz, sizeof(float) * NXU * T) &&
+            alloc((void **)&h->ref_lam, sizeof(float) * NX * T) &&
+            alloc((void **)&h->ref_scale, sizeof(float) * B) &&
             hipHostMalloc((void **)&h->h_merits, sizeof(float) * 16) == hipSuccess &&
             hipHostMalloc((void **)&h->h_pcg, sizeof(uint32_t) * 4) == hipSuccess &&
             hipHostMalloc((void **)&h->h_rho, sizeof(float) * 4) == hipSuccess;
@@ -237,6 +266,25 @@ int mpc_set_config(mpc_handle *h, const mpc_config *cfg) {
   c.max_instances = h->cfg.max_instances;
   c.max_total_knots = h->cfg.max_total_knots;
   h->cfg = c;
+  weights_from_config(h);
+  return MPC_OK;
+}
+
+int mpc_set_cost_weights(mpc_handle *h, const float *wx14, const float *wu7) {
+  if (!h || !wx14 || !wu7) return MPC_ERR_ARG;
+  for (int j = 0; j < NX; ++j)
+    if (!(wx14[j] >= 0.f) || !std::isfinite(wx14[j])) return MPC_ERR_ARG;
+  for (int j = 0; j < NU; ++j)
+    if (!(wu7[j] >= 0.f) || !std::isfinite(wu7[j])) return MPC_ERR_ARG;
+  std::memcpy(h->wx, wx14, sizeof(h->wx));
+  std::memcpy(h->wu, wu7, sizeof(h->wu));
+  return MPC_OK;
+}
+
+int mpc_get_cost_weights(mpc_handle *h, float *wx14, float *wu7) {
+  if (!h || !wx14 || !wu7) return MPC_ERR_ARG;
+  std::memcpy(wx14, h->wx, sizeof(h->wx));
+  std::memcpy(wu7, h->wu, sizeof(h->wu));
   return MPC_OK;
 }
 
@@ -267,7 +315,7 @@ int mpc_form_kkt(mpc_handle *h, uint32_t N, const float *d_xu, const float *d_ee
                  mpc_stream_t stream) {
   if (!h || !n_ok(h, N) || !d_xu || !d_ee_ref || !d_xs || !d_G || !d_C || !d_g || !d_c)
     return MPC_ERR_ARG;
-  HIP_TRY(launch_form_kkt(single(N), make_params(h->cfg), d_xu, d_ee_ref, d_xs, nullptr, d_G, d_C,
+  HIP_TRY(launch_form_kkt(single(N), make_params(h), d_xu, d_ee_ref, d_xs, nullptr, d_G, d_C,
                           d_g, d_c, (hipStream_t)stream));
   return MPC_OK;
 }
@@ -308,7 +356,7 @@ int mpc_compute_merit(mpc_handle *h, uint32_t N, const float *d_xu, const float 
   if (!h || !n_ok(h, N) || !d_xu || !d_ee_ref || !d_merit_out) return MPC_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const Batch b = single(N);
-  HIP_TRY(launch_merit(b, make_params(h->cfg), d_xu, nullptr, d_ee_ref, h->xs, nullptr, 8, 1,
+  HIP_TRY(launch_merit(b, make_params(h), d_xu, nullptr, d_ee_ref, h->xs, nullptr, 8, 1,
                        h->knot_merits, st));
   HIP_TRY(launch_merit_reduce(b, h->knot_merits, 1, d_merit_out, st));
   return MPC_OK;
@@ -320,7 +368,7 @@ int mpc_line_search_merits(mpc_handle *h, uint32_t N, const float *d_xs, const f
   if (!h || !n_ok(h, N) || !d_xs || !d_xu || !d_ee_ref || !d_dz || !d_merits_out) return MPC_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const Batch b = single(N);
-  HIP_TRY(launch_merit(b, make_params(h->cfg), d_xu, d_dz, d_ee_ref, d_xs, nullptr, 0, 8,
+  HIP_TRY(launch_merit(b, make_params(h), d_xu, d_dz, d_ee_ref, d_xs, nullptr, 0, 8,
                        h->knot_merits, st));
   HIP_TRY(launch_merit_reduce(b, h->knot_merits, 8, d_merits_out, st));
   return MPC_OK;
@@ -333,7 +381,7 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
   if (!h || !n_ok(h, N) || !d_ee_ref || !d_lambda || !d_xu || !rho || !(*rho > 0)) return MPC_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const mpc_config &cfg = h->cfg;
-  const Params prm = make_params(cfg);
+  const Params prm = make_params(h);
   const Batch b = single(N);
   mpc_sqp_stats local;
   mpc_sqp_stats &s = stats ? *stats : local;
@@ -487,6 +535,85 @@ int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
   return MPC_OK;
 }
 
+// ---- differentiable QP layer --------------------------------------------------
+
+// refine_steps steps of iterative refinement of K [dz; lam] = [g; c]
+// (constraint row 0: dx_0 = 0; c == nullptr: c = 0) on the workspace's G, C,
+// G~^-1, S and Pinv; dz in the xu layout, lam 14 per knot, both in place.
+static hipError_t refine_kkt(mpc_handle *h, const float *d_rho, const float *g, const float *c, float *dz,
+                             float *lam, uint32_t steps, float rel_tol, hipStream_t st) {
+  const Batch b = batch_of(h);
+  const float tol = -(rel_tol > 0.f ? rel_tol : 1e-6f);  // negative: relative PCG exit
+  for (uint32_t s = 0; s < steps; ++s) {
+    hipError_t e = launch_kkt_residual(b, h->G, h->C, g, c, d_rho, dz, lam, h->ref_g, h->ref_c, st);
+    if (e == hipSuccess) e = launch_rhs_normalise(b, h->ref_g, h->ref_c, h->ref_scale, st);
+    if (e == hipSuccess) e = launch_form_gamma(b, h->Ginv, h->C, h->ref_g, h->ref_c, h->gamma, st);
+    if (e == hipSuccess) e = launch_fill(NX * h->total, 0.f, h->ref_lam, st);
+    if (e == hipSuccess) e = pcg_batch(h, h->ref_lam, nullptr, h->cfg.pcg_max_iter, tol, nullptr, st);
+    if (e == hipSuccess) e = launch_compute_dz(b, h->Ginv, h->C, h->ref_g, h->ref_lam, nullptr, h->ref_dz, st);
+    if (e == hipSuccess) e = launch_refine_update(b, h->ref_scale, h->ref_dz, h->ref_lam, dz, lam, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+int mpc_batch_qp_layer(mpc_handle *h, const float *d_xu, const float *d_ee_ref, const float *d_rho,
+                       float *d_lambda, float pcg_tol, uint32_t refine_steps, float refine_tol, float *d_dz,
+                       float *d_xstar, mpc_stream_t stream) {
+  if (!h || !d_xu || !d_ee_ref || !d_rho || !d_lambda || !d_dz || !d_xstar || !(refine_tol == refine_tol))
+    return MPC_ERR_ARG;
+  if (!h->batch_ready) return MPC_ERR_NOT_READY;
+  hipStream_t st = (hipStream_t)stream;
+  const Batch b = batch_of(h);
+  const Params prm = make_params(h);
+  const float tol = pcg_tol > 0.f ? pcg_tol : h->cfg.pcg_exit_tol;
+  // c_0 (x_0 - x_s) is not used by the Schur step, so x_s is left as it is
+  HIP_TRY(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, nullptr, h->G, h->C, h->g, h->c, st));
+  HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, nullptr, h->S, h->P, h->gamma, h->Ginv, 0, st));
+  HIP_TRY(pcg_batch(h, d_lambda, nullptr, h->cfg.pcg_max_iter, tol, nullptr, st));
+  HIP_TRY(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, nullptr, d_dz, st));
+  HIP_TRY(refine_kkt(h, d_rho, h->g, h->c, d_dz, d_lambda, refine_steps, refine_tol, st));
+  HIP_TRY(launch_sub(NXU * h->total - NU * h->n_inst, d_xu, d_dz, d_xstar, st));
+  return MPC_OK;
+}
+
+int mpc_batch_qp_loss(mpc_handle *h, const float *d_xu, const float *d_ee_ref, float *d_loss, float *d_dLdx,
+                      float *d_dLdref, mpc_stream_t stream) {
+  if (!h || !d_xu || !d_ee_ref || !d_loss) return MPC_ERR_ARG;
+  if (!h->batch_ready) return MPC_ERR_NOT_READY;
+  hipStream_t st = (hipStream_t)stream;
+  const Batch b = batch_of(h);
+  HIP_TRY(launch_qp_loss(b, d_xu, d_ee_ref, h->loss_knot, d_dLdx, d_dLdref, st));
+  HIP_TRY(launch_reduce_knots(b, h->loss_knot, 1, d_loss, st));
+  return MPC_OK;
+}
+
+int mpc_batch_qp_adjoint(mpc_handle *h, const float *d_xu, const float *d_ee_ref, const float *d_rho,
+                         const float *d_dz, const float *d_a, float pcg_tol, uint32_t refine_steps,
+                         float refine_tol, float *d_grad_w, float *d_grad_ref, mpc_stream_t stream) {
+  if (!h || !d_xu || !d_ee_ref || !d_rho || !d_dz || !d_a || !d_grad_w || !d_grad_ref ||
+      !(refine_tol == refine_tol))
+    return MPC_ERR_ARG;
+  if (!h->batch_ready) return MPC_ERR_NOT_READY;
+  hipStream_t st = (hipStream_t)stream;
+  const Batch b = batch_of(h);
+  const Params prm = make_params(h);
+  const float tol = pcg_tol > 0.f ? pcg_tol : h->cfg.pcg_exit_tol;
+  // recompute the linearisation at xu (nothing of the forward pass is kept
+  // but dz), then the Schur system with g := -a and c := 0
+  HIP_TRY(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, nullptr, h->G, h->C, h->g, h->c, st));
+  HIP_TRY(launch_pack_neg(b, d_a, h->g, st));
+  HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, nullptr, d_rho, 0.f, nullptr, h->S, h->P, h->gamma, h->Ginv, 0,
+                            st));
+  HIP_TRY(launch_fill(NX * h->total, 0.f, h->adj_mu, st));
+  HIP_TRY(pcg_batch(h, h->adj_mu, nullptr, h->cfg.pcg_max_iter, tol, nullptr, st));
+  HIP_TRY(launch_compute_dz(b, h->Ginv, h->C, h->g, h->adj_mu, nullptr, h->dz, st));  // v_z
+  HIP_TRY(refine_kkt(h, d_rho, h->g, nullptr, h->dz, h->adj_mu, refine_steps, refine_tol, st));
+  HIP_TRY(launch_adj_params(b, prm, d_xu, d_ee_ref, d_dz, h->dz, h->adj_knot, d_grad_ref, st));
+  HIP_TRY(launch_reduce_knots(b, h->adj_knot, NXU, d_grad_w, st));
+  return MPC_OK;
+}
+
 int mpc_batch_set_timing(mpc_handle *h, int enable) {
   if (!h) return MPC_ERR_ARG;
   h->timing = enable != 0;
@@ -499,6 +626,33 @@ int mpc_batch_phase_times(mpc_handle *h, double *us7) {
   return MPC_OK;
 }
 
+// PCG of every instance of the batch on S, P, gamma of the workspace: long
+// horizons first on two-workgroup teams, then any beyond their reach on the
+// streaming kernel, then the bins of the resident kernel (or everything
+// streaming with FLAG_PCG_STREAM).  mid (optional) is recorded after the
+// team launches.
+static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_status, uint32_t max_iter, float tol,
+                            hipEvent_t mid, hipStream_t st) {
+  const Batch b = batch_of(h);
+  hipError_t e;
+  if (h->cfg.flags & FLAG_PCG_STREAM) {
+    e = launch_pcg_stream(b, h->max_n, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, nullptr, 0,
+                          h->pcg_iters, h->pcg_flag, st);
+    if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
+    return e;
+  }
+  e = launch_pcg_team(b, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->S, h->P, h->gamma, d_lambda, max_iter,
+                      tol, d_status, h->pcg_iters, h->pcg_flag, h->team_err, st);
+  if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
+  if (e == hipSuccess)
+    e = launch_pcg_stream(b, h->max_big_n, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, h->big_list,
+                          h->n_big, h->pcg_iters, h->pcg_flag, st);
+  if (e == hipSuccess)
+    e = launch_pcg_res(b, h->bin_items, h->bin_off, h->n_bins, h->S, h->P, h->gamma, d_lambda, max_iter, tol,
+                       d_status, h->pcg_iters, h->pcg_flag, st);
+  return e;
+}
+
 int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee_ref, float *d_rho,
                   uint32_t sqp_iters, uint32_t *d_status, uint32_t *d_pcg_iters, mpc_stream_t stream) {
   if (!h || !d_xu || !d_lambda || !d_ee_ref || !d_rho || !d_status) return MPC_ERR_ARG;
@@ -506,7 +660,7 @@ int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee
   if (sqp_iters > MPC_MAX_SQP_ITERS) return MPC_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const Batch b = batch_of(h);
-  const Params prm = make_params(h->cfg);
+  const Params prm = make_params(h);
   InstState s{d_rho, h->drho, h->merit0, d_status, h->sqp_done, h->xs};
   const bool tm = h->timing;
   for (double &v : h->phase_us) v = 0;
@@ -520,24 +674,7 @@ int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee
     HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma,
                               h->Ginv, 0, st));
     if (tm) HIP_TRY(hipEventRecord(h->ev[2], st));
-    if (h->cfg.flags & FLAG_PCG_STREAM) {
-      HIP_TRY(launch_pcg_stream(b, h->max_n, h->S, h->P, h->gamma, d_lambda, h->cfg.pcg_max_iter,
-                                h->cfg.pcg_exit_tol, d_status, nullptr, 0, h->pcg_iters, h->pcg_flag, st));
-      if (tm) HIP_TRY(hipEventRecord(h->ev[3], st));
-    } else {
-      // long horizons first: two-workgroup teams, then any beyond their
-      // reach on the streaming kernel, then the bins of the resident kernel
-      HIP_TRY(launch_pcg_team(b, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->S, h->P, h->gamma, d_lambda,
-                              h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, d_status, h->pcg_iters, h->pcg_flag,
-                              h->team_err, st));
-      if (tm) HIP_TRY(hipEventRecord(h->ev[3], st));
-      HIP_TRY(launch_pcg_stream(b, h->max_big_n, h->S, h->P, h->gamma, d_lambda, h->cfg.pcg_max_iter,
-                                h->cfg.pcg_exit_tol, d_status, h->big_list, h->n_big, h->pcg_iters,
-                                h->pcg_flag, st));
-      HIP_TRY(launch_pcg_res(b, h->bin_items, h->bin_off, h->n_bins, h->S, h->P, h->gamma, d_lambda,
-                             h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, d_status, h->pcg_iters, h->pcg_flag,
-                             st));
-    }
+    HIP_TRY(pcg_batch(h, d_lambda, d_status, h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, tm ? h->ev[3] : nullptr, st));
     if (tm) HIP_TRY(hipEventRecord(h->ev[4], st));
     HIP_TRY(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
     if (tm) HIP_TRY(hipEventRecord(h->ev[5], st));

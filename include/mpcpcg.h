@@ -196,6 +196,47 @@ int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee
 int mpc_batch_set_timing(mpc_handle *h, int enable);
 int mpc_batch_phase_times(mpc_handle *h, double *us7);
 
+/* ---- cost weights and the differentiable QP layer --------------------------
+ * Neither has a counterpart in the reference (its QD_COST / R_COST are
+ * scalar compile-time macros, settings.cuh, perform_experiment.sh:44; it has
+ * no differentiation).  Stage cost with per-dimension weights:
+ *   0.5 sum_j wx_j x_j^2 + 0.5 sum_j wu_j u_j^2 + 0.5 q_ee |ee(q) - ref|^2;
+ * mpc_create and mpc_set_config set the reference's cost, wx = (0 x 7,
+ * qd_cost x 7) and wu = (r_cost x 7).  Weights must be finite and >= 0. */
+int mpc_set_cost_weights(mpc_handle *h, const float *wx14, const float *wu7);
+int mpc_get_cost_weights(mpc_handle *h, float *wx14, float *wu7);
+
+/* QP layer at the iterate d_xu of every instance of the batch layout: the
+ * KKT system of the SQP step (KKT blocks, Schur complement with the
+ * instance's rho, PCG from the warm start d_lambda with exit tolerance
+ * pcg_tol, <= 0: the configured one, and the dz recovery) and its
+ * full-step solution d_xstar = d_xu - d_dz (both in the xu layout).
+ * refine_steps > 0 adds that many steps of iterative refinement: the KKT
+ * residual is evaluated in fp64 from the fp32 blocks and solved again with
+ * the same Schur complement and preconditioner, each correction's PCG
+ * exiting at refine_tol (<= 0: 1e-6) relative to its first r^T Pinv r.
+ * d_lambda returns the refined multipliers. */
+int mpc_batch_qp_layer(mpc_handle *h, const float *d_xu, const float *d_ee_ref, const float *d_rho,
+                       float *d_lambda, float pcg_tol, uint32_t refine_steps, float refine_tol, float *d_dz,
+                       float *d_xstar, mpc_stream_t stream);
+
+/* Tracking loss L_i = sum_k |ee(q_k) - ref_k|^2 (xyz) of each instance of the
+ * trajectory d_xu (d_loss, n_inst floats), with dL/dxu (xu layout) and the
+ * direct dL/dref (6 floats per knot, xyz used); the gradients may be NULL. */
+int mpc_batch_qp_loss(mpc_handle *h, const float *d_xu, const float *d_ee_ref, float *d_loss, float *d_dLdx,
+                      float *d_dLdref, mpc_stream_t stream);
+
+/* Adjoint of mpc_batch_qp_layer at its solution d_dz, for d_a = dL/dx*
+ * (xu layout): recomputes the linearisation at d_xu, solves
+ * K v = [-a; 0] with the same Schur complement, preconditioner and PCG
+ * (exit tolerance pcg_tol, <= 0: the configured one; refine_steps and
+ * refine_tol as for the layer), and writes d_grad_w (21 floats per
+ * instance: dL/dwx then dL/dwu) and ADDS the implicit part of dL/dref to
+ * d_grad_ref (6 floats per knot). */
+int mpc_batch_qp_adjoint(mpc_handle *h, const float *d_xu, const float *d_ee_ref, const float *d_rho,
+                         const float *d_dz, const float *d_a, float pcg_tol, uint32_t refine_steps,
+                         float refine_tol, float *d_grad_w, float *d_grad_ref, mpc_stream_t stream);
+
 /* ---- synthetic workload (seeded, reproducible) ----------------------------- */
 
 /* Joint-space data of one synthetic instance: splitmix64 stream seeded with

@@ -32,6 +32,8 @@
  *   - dz recovery: oldschur.cuh:966-1105.  Pinned by the dense KKT solve.
  *   - merit / line search / rho rule: merit.cuh:17-145, toplevel.cuh:234-446.
  *     Pinned by brute-force numpy evaluation from the pinned FK / FD.
+ *   - differentiable QP layer and its adjoint (growth direction of the
+ *     build; the reference has none): pinned by fp64 central differences.
  *   - sparse LDL^T (QDLDL-equivalent, published up-looking algorithm) with the
  *     reference's CSC pattern: utils.cuh:54-118, qdldl_helper.cuh:62-180,
  *     toplevel.cuh:160-215.  Pinned by L D L^T = S reconstruction and
@@ -427,10 +429,17 @@ typedef struct {
   int pcg_max_iter;
   int sqp_max_iter;
   int last_knot_own_state; /* 0: reference quirk (knot N-1 cost uses x_{N-2}) */
+  /* per-dimension quadratic weights 0.5 sum_j wx_j x_j^2 + 0.5 sum_j wu_j u_j^2;
+   * the reference's cost is wx = (0 x 7, QD x 7), wu = (R x 7) */
+  double wx[14];
+  double wu[7];
 } orc_cfg;
 
-/* Cost gradient/Hessian at one knot (rbdfiles/iiwa_eepos_plant.cuh:293-378):
- * q_k = [Jq^T e ; QD qd], Q_k = [q_q q_q^T, 0; 0, QD I], r = R u, R_k = R I. */
+/* Cost gradient/Hessian at one knot (rbdfiles/iiwa_eepos_plant.cuh:293-378),
+ * with per-dimension weights (reference: wx = (0, QD), wu = R):
+ *   q_k = [Jq^T e + wx_q x_q ; wx_qd qd], Q_k = [g_e g_e^T + diag(wx_q), 0; 0, diag(wx_qd)]
+ *   (g_e = Jq^T e, the rank-1 outer product of the end-effector gradient as in
+ *   the reference), r = wu u, R_k = diag(wu). */
 static void cost_grad_hess(const orc_cfg *cfg, const real *x, const real *u, const real *ref,
                            real *Q, real *qv, real *R, real *rv) {
   real ee[6], J[21];
@@ -438,19 +447,19 @@ static void cost_grad_hess(const orc_cfg *cfg, const real *x, const real *u, con
   F(orc_fk_grad)(x, J);
   real w = (real)cfg->q_ee;
   real e[3] = {w * (ee[0] - ref[0]), w * (ee[1] - ref[1]), w * (ee[2] - ref[2])};
-  for (int j = 0; j < NQ; ++j) qv[j] = J[3 * j] * e[0] + J[3 * j + 1] * e[1] + J[3 * j + 2] * e[2];
-  for (int j = 0; j < NQ; ++j) qv[NQ + j] = (real)cfg->qd_cost * x[NQ + j];
+  real ge[NQ];
+  for (int j = 0; j < NQ; ++j) ge[j] = J[3 * j] * e[0] + J[3 * j + 1] * e[1] + J[3 * j + 2] * e[2];
+  for (int j = 0; j < NX; ++j) qv[j] = (j < NQ ? ge[j] : 0) + (real)cfg->wx[j] * x[j];
   for (int c = 0; c < NX; ++c)
     for (int r = 0; r < NX; ++r) {
-      real val;
-      if (r < NQ && c < NQ) val = qv[r] * qv[c];
-      else val = (r == c) ? (real)cfg->qd_cost : 0;
+      real val = (r == c) ? (real)cfg->wx[r] : 0;
+      if (r < NQ && c < NQ) val += ge[r] * ge[c];
       Q[r + NX * c] = val;
     }
   if (R) {
-    for (int j = 0; j < NU; ++j) rv[j] = (real)cfg->r_cost * u[j];
+    for (int j = 0; j < NU; ++j) rv[j] = (real)cfg->wu[j] * u[j];
     for (int c = 0; c < NU; ++c)
-      for (int r = 0; r < NU; ++r) R[r + NU * c] = (r == c) ? (real)cfg->r_cost : 0;
+      for (int r = 0; r < NU; ++r) R[r + NU * c] = (r == c) ? (real)cfg->wu[r] : 0;
   }
 }
 
@@ -663,7 +672,8 @@ void F(orc_compute_dz)(int N, const real *Ginv, const real *C, const real *g, co
 
 /* Merit (merit.cuh:17-145 with rbdfiles/iiwa_eepos_plant.cuh:239-290 and
  * integrator.cuh:186-219) of the trajectory xu + alpha dz:
- *   sum_k [0.5 QD |qd_k|^2 + 0.5 R |u_k|^2 (k<N-1) + 0.5 w |ee(q_k) - ref_k|^2]
+ *   sum_k [0.5 sum_j wx_j x_kj^2 + 0.5 sum_j wu_j u_kj^2 (k<N-1) + 0.5 w |ee(q_k) - ref_k|^2]
+ *   (reference: wx = (0, QD), wu = R)
  *   + mu sum_{k<N-1} |x_{k+1} - x_k - dt f(x_k, u_k)|_1
  *   + (with_x0) mu |x_0 - x_s|_1
  * dz may be NULL (alpha ignored).  knot_out (N) receives per-knot terms. */
@@ -677,9 +687,9 @@ real F(orc_merit)(const orc_cfg *cfg, int N, const real *xu, const real *dz, rea
   for (int k = 0; k < N; ++k) {
     const real *x = z + NXU * k;
     real J = 0;
-    for (int i = 0; i < NQ; ++i) J += (real)0.5 * ((real)cfg->qd_cost * x[NQ + i] * x[NQ + i]);
+    for (int i = 0; i < NX; ++i) J += (real)0.5 * ((real)cfg->wx[i] * x[i] * x[i]);
     if (k < N - 1)
-      for (int i = 0; i < NU; ++i) J += (real)0.5 * ((real)cfg->r_cost * x[NX + i] * x[NX + i]);
+      for (int i = 0; i < NU; ++i) J += (real)0.5 * ((real)cfg->wu[i] * x[NX + i] * x[NX + i]);
     real ee[6];
     F(orc_fk)(x, ee);
     for (int i = 0; i < 3; ++i) {
@@ -941,6 +951,123 @@ int F(orc_sqp_solve)(const orc_cfg *cfg, int N, const real *ee_ref, real *lambda
   if (not_aborted_out) *not_aborted_out = not_aborted;
   if (ldl) F(orc_ldl_destroy)(ldl);
   free(G); free(Ginv); free(C); free(g); free(c); free(S); free(P); free(gam); free(dz);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------
+ * Differentiable QP layer at a fixed iterate (the final SQP iterate xu):
+ * K w = b with K = [G~ C^T; C 0] (G~ = G + rho I), w = [dz; lambda],
+ * b = [g; c] formed at xu as above, and the full-step solution
+ * x* = xu - dz.  Loss L = sum_k |ee(q*_k) - ref_k|^2 (xyz).  Adjoint:
+ * v = K^-1 [-dL/dx*; 0], obtained through the same Schur complement (gamma
+ * of g := -dL/dx*, c := 0, then the dz recovery), and for any cost
+ * parameter theta (the weights wx, wu and the reference points; the
+ * dynamics rows C, c do not depend on them)
+ *     dL/dtheta = v_z^T (dg/dtheta - dG/dtheta dz)  (+ the direct ref term).
+ * This is the restatement the GPU adjoint is checked against; it is itself
+ * pinned by fp64 central differences of orc_qp_layer / orc_qp_loss.
+ * ---------------------------------------------------------------------- */
+
+/* x* = xu - dz of the QP at xu, solved with the direct LDL^T of S; dz and
+ * lambda (may be NULL) receive the solution.  Returns 0 or a negative error. */
+int F(orc_qp_layer)(const orc_cfg *cfg, int N, const real *xu, const real *ee_ref, const real *xs, real rho,
+                    real *xstar, real *dz_out, real *lambda_out) {
+  int len = NXU * N - NU, rc = 0;
+  real *G = (real *)malloc(sizeof(real) * 245 * N), *Ginv = (real *)malloc(sizeof(real) * 245 * N);
+  real *C = (real *)malloc(sizeof(real) * 294 * N), *g = (real *)malloc(sizeof(real) * NXU * N);
+  real *c = (real *)malloc(sizeof(real) * NX * N), *S = (real *)malloc(sizeof(real) * 588 * N);
+  real *P = (real *)malloc(sizeof(real) * 588 * N), *gam = (real *)malloc(sizeof(real) * NX * N);
+  real *dz = (real *)malloc(sizeof(real) * NXU * N), *lam = (real *)malloc(sizeof(real) * NX * N);
+  void *ldl = F(orc_ldl_create)(N);
+  F(orc_form_kkt)(cfg, N, xu, ee_ref, xs, G, C, g, c);
+  if (F(orc_form_schur)(N, G, C, g, c, rho, S, P, gam, Ginv)) rc = -1;
+  if (!rc && F(orc_ldl_schur_solve)(ldl, N, S, gam, lam)) rc = -2;
+  if (!rc) {
+    F(orc_compute_dz)(N, Ginv, C, g, lam, dz);
+    for (int i = 0; i < len; ++i) xstar[i] = xu[i] - dz[i];
+    if (dz_out) memcpy(dz_out, dz, sizeof(real) * len);
+    if (lambda_out) memcpy(lambda_out, lam, sizeof(real) * NX * N);
+  }
+  F(orc_ldl_destroy)(ldl);
+  free(G); free(Ginv); free(C); free(g); free(c); free(S); free(P); free(gam); free(dz); free(lam);
+  return rc;
+}
+
+/* L = sum_k |ee(q_k) - ref_k|^2 of the trajectory xu (xyz, every knot at its
+ * own state); dLdx (21N-7) and dLdref (6N, xyz entries) receive the partial
+ * derivatives (either may be NULL). */
+real F(orc_qp_loss)(int N, const real *xu, const real *ee_ref, real *dLdx, real *dLdref) {
+  real L = 0;
+  if (dLdx) memset(dLdx, 0, sizeof(real) * (NXU * N - NU));
+  if (dLdref) memset(dLdref, 0, sizeof(real) * 6 * N);
+  for (int k = 0; k < N; ++k) {
+    const real *x = xu + NXU * k;
+    real ee[6], J[21];
+    F(orc_fk)(x, ee);
+    F(orc_fk_grad)(x, J);
+    for (int a = 0; a < 3; ++a) {
+      const real e = ee[a] - ee_ref[6 * k + a];
+      L += e * e;
+      if (dLdref) dLdref[6 * k + a] = -2 * e;
+      if (dLdx)
+        for (int j = 0; j < NQ; ++j) dLdx[NXU * k + j] += 2 * e * J[3 * j + a];
+    }
+  }
+  return L;
+}
+
+/* Adjoint of orc_qp_layer at its solution dz: given a = dL/dx* (21N-7),
+ * writes the weight gradient gw (14 wx then 7 wu) and ADDS the implicit
+ * part of dL/dref to gref (6N, xyz entries).  Returns 0 or a negative error. */
+int F(orc_qp_adjoint)(const orc_cfg *cfg, int N, const real *xu, const real *ee_ref, const real *xs, real rho,
+                      const real *dz, const real *a, real *gw, real *gref) {
+  int len = NXU * N - NU, rc = 0;
+  real *G = (real *)malloc(sizeof(real) * 245 * N), *Ginv = (real *)malloc(sizeof(real) * 245 * N);
+  real *C = (real *)malloc(sizeof(real) * 294 * N), *g = (real *)malloc(sizeof(real) * NXU * N);
+  real *c = (real *)malloc(sizeof(real) * NX * N), *S = (real *)malloc(sizeof(real) * 588 * N);
+  real *P = (real *)malloc(sizeof(real) * 588 * N), *gam = (real *)malloc(sizeof(real) * NX * N);
+  real *az = (real *)malloc(sizeof(real) * NXU * N), *mu = (real *)malloc(sizeof(real) * NX * N);
+  real *v = (real *)malloc(sizeof(real) * NXU * N), *c0 = (real *)calloc(NX * N, sizeof(real));
+  void *ldl = F(orc_ldl_create)(N);
+  F(orc_form_kkt)(cfg, N, xu, ee_ref, xs, G, C, g, c);
+  for (int i = 0; i < len; ++i) az[i] = -a[i];
+  if (F(orc_form_schur)(N, G, C, az, c0, rho, S, P, gam, Ginv)) rc = -1;
+  if (!rc && F(orc_ldl_schur_solve)(ldl, N, S, gam, mu)) rc = -2;
+  if (!rc) {
+    F(orc_compute_dz)(N, Ginv, C, az, mu, v);
+    for (int j = 0; j < 21; ++j) gw[j] = 0;
+    for (int k = 0; k < N; ++k) {
+      const real *xc = (k < N - 1 || cfg->last_knot_own_state) ? xu + NXU * k : xu + NXU * (k - 1);
+      const real *vx = v + NXU * k, *dzx = dz + NXU * k;
+      for (int j = 0; j < NX; ++j) gw[j] += vx[j] * (xc[j] - dzx[j]);
+      if (k < N - 1)
+        for (int j = 0; j < NU; ++j) gw[NX + j] += vx[NX + j] * (xu[NXU * k + NX + j] - dzx[NX + j]);
+      real ee[6], J[21], ge[NQ];
+      F(orc_fk)(xc, ee);
+      F(orc_fk_grad)(xc, J);
+      const real w = (real)cfg->q_ee;
+      for (int j = 0; j < NQ; ++j) {
+        ge[j] = 0;
+        for (int b = 0; b < 3; ++b) ge[j] += J[3 * j + b] * w * (ee[b] - ee_ref[6 * k + b]);
+      }
+      real vge = 0, gedz = 0;
+      for (int j = 0; j < NQ; ++j) {
+        vge += vx[j] * ge[j];
+        gedz += ge[j] * dzx[j];
+      }
+      for (int b = 0; b < 3; ++b) {
+        real vd = 0, ddz = 0;  /* d ge / d ref_b = -w J[b, :] */
+        for (int j = 0; j < NQ; ++j) {
+          const real dge = -w * J[3 * j + b];
+          vd += vx[j] * dge;
+          ddz += dge * dzx[j];
+        }
+        gref[6 * k + b] += vd - vd * gedz - vge * ddz;
+      }
+    }
+  }
+  F(orc_ldl_destroy)(ldl);
+  free(G); free(Ginv); free(C); free(g); free(c); free(S); free(P); free(gam); free(az); free(mu); free(v); free(c0);
   return rc;
 }
 

@@ -54,13 +54,24 @@ class Cfg(C.Structure):
         ("pcg_max_iter", C.c_int),
         ("sqp_max_iter", C.c_int),
         ("last_knot_own_state", C.c_int),
+        ("wx", C.c_double * 14),
+        ("wu", C.c_double * 7),
     ]
 
 
+def default_weights(qd_cost=1e-2, r_cost=1e-4):
+    """Per-dimension weights of the reference's cost: wx = (0 x 7, QD x 7), wu = R x 7."""
+    return np.array([0.0] * 7 + [qd_cost] * 7), np.array([r_cost] * 7)
+
+
 def make_cfg(dt=0.01, q_ee=1.0, qd_cost=1e-2, r_cost=1e-4, mu=10.0, rho_factor=1.2, rho_max=10.0,
-             rho_min=1e-3, pcg_tol=1e-5, pcg_max_iter=173, sqp_max_iter=3, last_knot_own_state=0):
+             rho_min=1e-3, pcg_tol=1e-5, pcg_max_iter=173, sqp_max_iter=3, last_knot_own_state=0,
+             wx=None, wu=None):
+    dwx, dwu = default_weights(qd_cost, r_cost)
+    wx = dwx if wx is None else np.asarray(wx, np.float64)
+    wu = dwu if wu is None else np.asarray(wu, np.float64)
     return Cfg(dt, q_ee, qd_cost, r_cost, mu, rho_factor, rho_max, rho_min, pcg_tol, pcg_max_iter,
-               sqp_max_iter, last_knot_own_state)
+               sqp_max_iter, last_knot_own_state, (C.c_double * 14)(*wx), (C.c_double * 7)(*wu))
 
 
 def _sfx(dtype):
@@ -356,3 +367,42 @@ def ldl_bench(N, S, gamma, reps, dtype=np.float32):
     f.restype = C.c_double
     t = f(N, _ptr(S), _ptr(gamma), int(reps), _ptr(lam))
     return float(t), lam
+
+
+# ---- differentiable QP layer (fp64 restatement of the GPU adjoint) --------
+
+def _real(x, dtype):
+    return (C.c_double if np.dtype(dtype) == np.float64 else C.c_float)(x)
+
+
+def qp_layer(cfg, N, xu, ee_ref, xs, rho, dtype=np.float64):
+    """x* = xu - dz of the QP formed at xu (direct LDL^T solve); returns (x*, dz, lambda)."""
+    xu, ee_ref, xs = (_arr(a, dtype) for a in (xu, ee_ref, xs))
+    L = 21 * N - 7
+    xs_, dz, lam = np.zeros(L, dtype), np.zeros(L, dtype), np.zeros(14 * N, dtype)
+    rc = _fn("orc_qp_layer", dtype)(C.byref(cfg), N, _ptr(xu), _ptr(ee_ref), _ptr(xs), _real(rho, dtype),
+                                   _ptr(xs_), _ptr(dz), _ptr(lam))
+    if rc:
+        raise RuntimeError(f"orc_qp_layer failed ({rc})")
+    return xs_, dz, lam
+
+
+def qp_loss(N, xu, ee_ref, dtype=np.float64):
+    """L = sum_k |ee(q_k) - ref_k|^2 with dL/dxu (21N-7) and dL/dref (6N)."""
+    xu, ee_ref = _arr(xu, dtype), _arr(ee_ref, dtype)
+    dx, dr = np.zeros(21 * N - 7, dtype), np.zeros(6 * N, dtype)
+    fn = _fn("orc_qp_loss", dtype)
+    fn.restype = C.c_double if np.dtype(dtype) == np.float64 else C.c_float
+    L = fn(N, _ptr(xu), _ptr(ee_ref), _ptr(dx), _ptr(dr))
+    return float(L), dx, dr
+
+
+def qp_adjoint(cfg, N, xu, ee_ref, xs, rho, dz, a, dtype=np.float64):
+    """Adjoint of qp_layer: weight gradient (14 wx + 7 wu) and the implicit part of dL/dref (6N)."""
+    xu, ee_ref, xs, dz, a = (_arr(v, dtype) for v in (xu, ee_ref, xs, dz, a))
+    gw, gref = np.zeros(21, dtype), np.zeros(6 * N, dtype)
+    rc = _fn("orc_qp_adjoint", dtype)(C.byref(cfg), N, _ptr(xu), _ptr(ee_ref), _ptr(xs), _real(rho, dtype),
+                                     _ptr(dz), _ptr(a), _ptr(gw), _ptr(gref))
+    if rc:
+        raise RuntimeError(f"orc_qp_adjoint failed ({rc})")
+    return gw, gref
