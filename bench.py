@@ -242,10 +242,17 @@ def main():
     team = horizons >= TEAM_MIN_N
     units = float(sum(int(horizons[i]) * int(log_t[i, j]) for i in range(n) for j in range(K)
                       if team[i] and log_t[i, j] >= 0)) / K
-    t_launch = ph[2] / K
-    achieved = PCG_BYTES_PER_KNOT_ITER * units / (t_launch * 1e-6) / 1e9
-    tflops = PCG_FLOPS_PER_KNOT_ITER * units / (t_launch * 1e-6) / 1e12
-    lds_gbs = PCG_LDS_BYTES_PER_KNOT_ITER * units / (t_launch * 1e-6) / 1e9
+    # the team PCG of one SQP iteration is `launches` cooperative launches
+    # (at most one workgroup per CU, two per team); per launch: the phase's
+    # knot-iterations and time divided by that count
+    layout = solver.batch_pcg_layout()
+    launches = max(layout["team_launches"], 1)
+    t_phase = ph[2] / K
+    t_launch = t_phase / launches
+    units_launch = units / launches
+    achieved = PCG_BYTES_PER_KNOT_ITER * units / (t_phase * 1e-6) / 1e9
+    tflops = PCG_FLOPS_PER_KNOT_ITER * units / (t_phase * 1e-6) / 1e12
+    lds_gbs = PCG_LDS_BYTES_PER_KNOT_ITER * units / (t_phase * 1e-6) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
@@ -311,8 +318,9 @@ def main():
         "status": {"aborted": int((st & 4 != 0).sum()), "nonfinite": int((st & 8 != 0).sum())},
         "roofline": {"kernel": "k_pcg_team", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "algorithmic_bytes_per_launch": int(PCG_BYTES_PER_KNOT_ITER * units),
-                     "knot_iterations_per_launch": int(units), "avg_launch_us": round(t_launch, 2),
+                     "traffic": traffic, "algorithmic_bytes_per_launch": int(PCG_BYTES_PER_KNOT_ITER * units_launch),
+                     "knot_iterations_per_launch": int(units_launch), "avg_launch_us": round(t_launch, 2),
+                     "launches_per_sqp_iter": launches, "pcg_layout": layout,
                      "note": "algorithmic bytes = 2296 B per knot and PCG iteration (S, Pinv and vectors); "
                              "the kernel keeps S and Pinv on chip, so they come from LDS and registers and "
                              "the HBM traffic (PMC, per launch) is the one-time load",

@@ -40,7 +40,7 @@ EXPORTS = (
     "mpc_status_string", "mpc_fk", "mpc_dynamics_gradient", "mpc_form_kkt", "mpc_form_schur",
     "mpc_pcg_solve", "mpc_compute_dz", "mpc_compute_merit", "mpc_line_search_merits",
     "mpc_sqp_solve", "mpc_batch_setup", "mpc_batch_sqp", "mpc_batch_set_timing",
-    "mpc_batch_phase_times", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
+    "mpc_batch_phase_times", "mpc_batch_pcg_layout", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
     "mpc_batch_qp_loss", "mpc_batch_qp_adjoint", "mpc_synth_instance", "mpc_synth_horizons",
 )
 
@@ -103,6 +103,7 @@ def lib():
             "mpc_batch_sqp": [vp, vp, vp, vp, vp, u32, vp, vp, vp],
             "mpc_batch_set_timing": [vp, C.c_int],
             "mpc_batch_phase_times": [vp, C.POINTER(C.c_double)],
+            "mpc_batch_pcg_layout": [vp, C.POINTER(C.c_uint32)],
             "mpc_set_cost_weights": [vp, vp, vp],
             "mpc_get_cost_weights": [vp, vp, vp],
             "mpc_batch_qp_layer": [vp, vp, vp, vp, vp, f32, u32, f32, vp, vp, vp],
@@ -298,6 +299,12 @@ class Solver:
                                           float(pcg_tol), int(refine), float(refine_tol), _ptr(grad_w),
                                           _ptr(grad_ref), _stream()),
                "mpc_batch_qp_adjoint")
+
+    def batch_pcg_layout(self):
+        """{bins, teams, streamed, team_launches} of the current batch layout."""
+        out = (C.c_uint32 * 4)()
+        _check(lib().mpc_batch_pcg_layout(self.h, out), "mpc_batch_pcg_layout")
+        return dict(zip(("bins", "teams", "streamed", "team_launches"), list(out)))
 
     def batch_phase_times(self):
         out = (C.c_double * 7)()

@@ -614,6 +614,17 @@ int mpc_batch_qp_adjoint(mpc_handle *h, const float *d_xu, const float *d_ee_ref
   return MPC_OK;
 }
 
+int mpc_batch_pcg_layout(const mpc_handle *h, uint32_t *out4) {
+  if (!h || !out4) return MPC_ERR_ARG;
+  if (!h->batch_ready) return MPC_ERR_NOT_READY;
+  const uint32_t per_launch = pcg_team_max_grid() / 2;
+  out4[0] = h->n_bins;
+  out4[1] = h->n_teams;
+  out4[2] = h->n_big;
+  out4[3] = (h->cfg.flags & FLAG_PCG_STREAM) || per_launch == 0 ? 0u : (h->n_teams + per_launch - 1) / per_launch;
+  return MPC_OK;
+}
+
 int mpc_batch_set_timing(mpc_handle *h, int enable) {
   if (!h) return MPC_ERR_ARG;
   h->timing = enable != 0;

@@ -194,6 +194,10 @@ int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee
  * iterations, in microseconds: linearisation, Schur formation, PCG on
  * two-workgroup teams, the rest of PCG, dz, line-search merits, update. */
 int mpc_batch_set_timing(mpc_handle *h, int enable);
+/* PCG packing of the batch layout: out4 = {resident-kernel bins, two-
+ * workgroup teams, instances on the streaming kernel, cooperative team
+ * launches per PCG phase}. */
+int mpc_batch_pcg_layout(const mpc_handle *h, uint32_t *out4);
 int mpc_batch_phase_times(mpc_handle *h, double *us7);
 
 /* ---- cost weights and the differentiable QP layer --------------------------
