@@ -42,6 +42,7 @@ EXPORTS = (
     "mpc_sqp_solve", "mpc_batch_setup", "mpc_batch_sqp", "mpc_batch_set_timing",
     "mpc_batch_phase_times", "mpc_batch_pcg_layout", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
     "mpc_batch_qp_loss", "mpc_batch_qp_adjoint", "mpc_synth_instance", "mpc_synth_horizons",
+    "mpc_inverse_dynamics", "mpc_default_closed_loop", "mpc_closed_loop", "mpc_synth_track",
 )
 
 
@@ -71,6 +72,21 @@ class SqpStats(C.Structure):
         ("sqp_iters", C.c_uint32), ("not_aborted", C.c_uint32), ("sqp_us", C.c_double),
         ("pcg_iters", C.c_uint32 * MAX_SQP_ITERS), ("pcg_maxiter", C.c_uint32 * MAX_SQP_ITERS),
         ("linsys_us", C.c_double * MAX_SQP_ITERS), ("ls_step", C.c_int32 * MAX_SQP_ITERS),
+    ]
+
+
+class ClosedLoopConfig(C.Structure):
+    _fields_ = [
+        ("steps", C.c_uint32), ("sqp_iters", C.c_uint32), ("substeps", C.c_uint32), ("sim_dt", C.c_float),
+        ("noise_sigma", C.c_float), ("use_graph", C.c_uint32), ("noise_seed", C.c_uint64),
+        ("timing", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
+class ClosedLoopStats(C.Structure):
+    _fields_ = [
+        ("total_us", C.c_double), ("solve_us", C.c_double), ("linsys_us", C.c_double), ("plant_us", C.c_double),
+        ("graph", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
 
@@ -109,6 +125,9 @@ def lib():
             "mpc_batch_qp_layer": [vp, vp, vp, vp, vp, f32, u32, f32, vp, vp, vp],
             "mpc_batch_qp_loss": [vp, vp, vp, vp, vp, vp, vp],
             "mpc_batch_qp_adjoint": [vp, vp, vp, vp, vp, vp, f32, u32, f32, vp, vp, vp],
+            "mpc_inverse_dynamics": [vp, u32, vp, vp, vp],
+            "mpc_closed_loop": [vp, C.POINTER(ClosedLoopConfig), u32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                C.POINTER(ClosedLoopStats), vp],
         }
         for name, argt in sigs.items():
             fn = getattr(L, name)
@@ -122,6 +141,10 @@ def lib():
         L.mpc_synth_instance.restype = None
         L.mpc_synth_horizons.argtypes = [C.c_uint64, u32, C.POINTER(u32), u32, C.POINTER(u32)]
         L.mpc_synth_horizons.restype = None
+        L.mpc_synth_track.argtypes = [C.c_uint64, u32, f32, vp, vp]
+        L.mpc_synth_track.restype = None
+        L.mpc_default_closed_loop.argtypes = [C.POINTER(ClosedLoopConfig)]
+        L.mpc_default_closed_loop.restype = None
         _lib = L
     return _lib
 
@@ -152,6 +175,25 @@ def synth_instance(seed: int, N: int, dt: float = 0.01):
     lib().mpc_synth_instance(C.c_uint64(seed), N, dt, x0.ctypes.data_as(C.c_void_p),
                              qref.ctypes.data_as(C.c_void_p))
     return x0, qref
+
+
+def synth_track(seed: int, K: int, dt: float = 0.01):
+    """(x0 (14,), [q, qd, qdd] (K, 21)) float32: the seeded joint path and its derivatives."""
+    import numpy as np
+    x0 = np.zeros(NX, np.float32)
+    out = np.zeros((K, 21), np.float32)
+    lib().mpc_synth_track(C.c_uint64(seed), K, dt, x0.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p))
+    return x0, out
+
+
+def default_closed_loop(**kw) -> ClosedLoopConfig:
+    cl = ClosedLoopConfig()
+    lib().mpc_default_closed_loop(C.byref(cl))
+    for k, v in kw.items():
+        if not hasattr(cl, k):
+            raise KeyError(k)
+        setattr(cl, k, v)
+    return cl
 
 
 def synth_horizons(seed: int, n: int, choices=(32, 64, 128, 256)):
@@ -299,6 +341,46 @@ class Solver:
                                           float(pcg_tol), int(refine), float(refine_tol), _ptr(grad_w),
                                           _ptr(grad_ref), _stream()),
                "mpc_batch_qp_adjoint")
+
+    def inverse_dynamics(self, qqdqdd, out=None):
+        """tau (n, 7) of n points [q qd qdd] (n, 21) on the device."""
+        import torch
+        n = qqdqdd.shape[0]
+        out = torch.empty(n, NQ, device=qqdqdd.device) if out is None else out
+        _check(lib().mpc_inverse_dynamics(self.h, n, _ptr(qqdqdd), _ptr(out), _stream()), "mpc_inverse_dynamics")
+        return out
+
+    def track_inputs(self, seed, steps, N, dt=0.01):
+        """Device inputs of a closed-loop run on the seeded path: x0 (14), the
+        reference window path ee (6 (steps + N)), and the precomputed trajectory
+        (q_ref, qd_ref and the inverse-dynamics torques, xu layout over
+        steps + N knots)."""
+        import torch
+        K = steps + N
+        x0, qqdqdd = synth_track(seed, K, dt)
+        pts = torch.from_numpy(qqdqdd).cuda()
+        ee = self.fk(pts[:, :NQ].contiguous()).reshape(-1).contiguous()
+        tau = self.inverse_dynamics(pts)
+        traj = torch.zeros(21 * K, device="cuda")
+        tv = traj.view(K, 21)
+        tv[:, :NX] = pts[:, :NX]
+        tv[:, NX:] = tau
+        return torch.from_numpy(x0).cuda(), ee, traj[:21 * K - 7].contiguous()
+
+    def closed_loop(self, N, x0, ee_path, xu, lam, cl=None, traj=None, states=None, ee_err=None, pcg_iters=None):
+        """Closed-loop tracking on the device (see mpc_closed_loop); returns
+        (states (steps+1, 14), ee_err (steps, 2), pcg_iters (steps, K) or None, stats dict)."""
+        import torch
+        cl = default_closed_loop() if cl is None else cl
+        steps, K = cl.steps, cl.sqp_iters
+        states = torch.empty(steps + 1, NX, device="cuda") if states is None else states
+        ee_err = torch.empty(steps, 2, device="cuda") if ee_err is None else ee_err
+        st = ClosedLoopStats()
+        _check(lib().mpc_closed_loop(self.h, C.byref(cl), N, _ptr(x0), _ptr(ee_path), _ptr(traj), _ptr(xu), _ptr(lam),
+                                     _ptr(states), _ptr(ee_err), _ptr(pcg_iters), C.byref(st), _stream()),
+               "mpc_closed_loop")
+        stats = {k: getattr(st, k) for k, _ in ClosedLoopStats._fields_ if k != "reserved"}
+        return states, ee_err, pcg_iters, stats
 
     def batch_pcg_layout(self):
         """{bins, teams, streamed, team_launches} of the current batch layout."""

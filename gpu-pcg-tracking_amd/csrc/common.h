@@ -84,12 +84,20 @@ struct InstState {
   float *xs;         // [14 n] initial state of each solve
 };
 
+// one control step of the closed-loop harness (k_closed_loop.hip)
+struct ClStep {
+  uint32_t N, substeps, sqp_iters;
+  float sim_dt, sigma;
+  uint64_t seed;
+};
+
 constexpr uint32_t ST_ACTIVE = 1u, ST_PCG_MAXITER = 2u, ST_RHO_ABORT = 4u, ST_NONFINITE = 8u;
 constexpr uint32_t FLAG_LAST_KNOT_OWN_STATE = 1u, FLAG_PCG_STREAM = 2u;
 
 // host-side launchers (implemented in the .hip files)
 hipError_t launch_fk(uint32_t n, const float *q, uint32_t q_stride, float *ee, hipStream_t st);
 hipError_t launch_dyn_grad(uint32_t n, const float *qqdu, float *qdd, float *df_du, hipStream_t st);
+hipError_t launch_inverse_dynamics(uint32_t n, const float *qqdqdd, float *tau, hipStream_t st);
 hipError_t launch_form_kkt(const Batch &b, const Params &p, const float *xu, const float *ee_ref,
                            const float *xs, const uint32_t *status, float *G, float *C, float *g,
                            float *c, hipStream_t st);
@@ -145,6 +153,12 @@ hipError_t launch_adj_params(const Batch &b, const Params &p, const float *xu, c
 hipError_t launch_reduce_knots(const Batch &b, const float *vals, uint32_t S, float *out, hipStream_t st);
 hipError_t launch_sub(uint32_t n, const float *a, const float *bb, float *o, hipStream_t st);
 hipError_t launch_fill(uint32_t n, float v, float *o, hipStream_t st);
+// closed-loop harness (k_closed_loop.hip)
+hipError_t launch_cl_init(uint32_t N, const float *x0, const float *path, float *win, float *xu, float *xp,
+                          uint32_t *ctr, float *states, float *rho, float rho0, hipStream_t st);
+hipError_t launch_cl_step(const ClStep &p, const float *path, const float *traj, float *win, float *xu, float *lam,
+                          float *xp, uint32_t *ctr, float *states, float *err, const uint32_t *pcg_now,
+                          uint32_t *pcg_log, hipStream_t st);
 // iterative refinement of the layer's KKT solves (k_refine.hip)
 hipError_t launch_kkt_residual(const Batch &b, const float *G, const float *C, const float *g, const float *c,
                                const float *rho_dev, const float *dz, const float *lam, float *rz, float *rc,

@@ -48,41 +48,6 @@ struct KktLds {
   float ee[4];
 };
 
-__device__ __forceinline__ void cholesky7(const float *M, float *L) {
-  for (int j = 0; j < NQ; ++j) {
-    float d = M[j + 7 * j];
-    for (int k = 0; k < j; ++k) d -= L[j + 7 * k] * L[j + 7 * k];
-    d = sqrtf(d);
-    L[j + 7 * j] = d;
-    const float inv = 1.f / d;
-    for (int i = j + 1; i < NQ; ++i) {
-      float s = M[i + 7 * j];
-      for (int k = 0; k < j; ++k) s -= L[i + 7 * k] * L[j + 7 * k];
-      L[i + 7 * j] = s * inv;
-    }
-    for (int i = 0; i < j; ++i) L[i + 7 * j] = 0.f;
-  }
-}
-
-// x = L^-T L^-1 b for a 7x7 lower Cholesky factor
-__device__ __forceinline__ void chol_solve7(const float *L, const float *b, float *x) {
-  float y[NQ];
-#pragma unroll
-  for (int i = 0; i < NQ; ++i) {
-    float s = b[i];
-#pragma unroll
-    for (int k = 0; k < i; ++k) s -= L[i + 7 * k] * y[k];
-    y[i] = s / L[i + 7 * i];
-  }
-#pragma unroll
-  for (int i = NQ - 1; i >= 0; --i) {
-    float s = y[i];
-#pragma unroll
-    for (int k = i + 1; k < NQ; ++k) s -= L[k + 7 * i] * x[k];
-    x[i] = s / L[i + 7 * i];
-  }
-}
-
 // Forward dynamics and its gradient for one point (q, qd, u) held by a
 // 16-lane group.  On return (after a barrier): D.qdd, D.Minv, D.df hold
 // qdd, M^-1 and [dqdd/dq, dqdd/dqd].
@@ -212,6 +177,28 @@ __global__ void __launch_bounds__(WG) k_dyn_grad(uint32_t n, const float *__rest
   float *o = df_out + (size_t)pt * 147;
   for (int i = lane; i < 98; i += GROUP) o[i] = D.df[i];
   for (int i = lane; i < 49; i += GROUP) o[98 + i] = D.Minv[i];
+}
+
+// --------------------------------------------------------------------------
+// inverse dynamics tau = M(q) qdd + c(q, qd) of n points [q qd qdd] (stride
+// 21), one thread per point (grid::inverse_dynamics_kernel,
+// rbdfiles/iiwa_eepos_grid.cuh, with the plant's gravity 0)
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_inverse_dynamics(uint32_t n, const float *__restrict__ qqdqdd,
+                                                          float *__restrict__ tau) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float *p = qqdqdd + (size_t)NXU * i;
+  float s[NQ], c[NQ], qd[NQ], qdd[NQ], t[NQ];
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) {
+    sincosf(p[j], &s[j], &c[j]);
+    qd[j] = p[NQ + j];
+    qdd[j] = p[NX + j];
+  }
+  rnea<false>(s, c, qd, qdd, t, nullptr, nullptr, nullptr, nullptr, nullptr);
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) tau[(size_t)NQ * i + j] = t[j];
 }
 
 // --------------------------------------------------------------------------
@@ -424,6 +411,12 @@ __global__ void __launch_bounds__(WG) k_merit(Batch b, Params p, const float *__
 hipError_t launch_fk(uint32_t n, const float *q, uint32_t q_stride, float *ee, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fk, dim3((n + 255) / 256), dim3(256), 0, st, n, q, q_stride, ee);
+  return hipGetLastError();
+}
+
+hipError_t launch_inverse_dynamics(uint32_t n, const float *qqdqdd, float *tau, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_inverse_dynamics, dim3((n + 255) / 256), dim3(256), 0, st, n, qqdqdd, tau);
   return hipGetLastError();
 }
 

@@ -380,4 +380,41 @@ __device__ __forceinline__ void fk_pos(const float *s, const float *c, float *p)
   p[2] = pw[2];
 }
 
+// Lower Cholesky factor L of a 7x7 symmetric positive definite M (both
+// column-major), one lane.
+__device__ __forceinline__ void cholesky7(const float *M, float *L) {
+  for (int j = 0; j < NQ; ++j) {
+    float d = M[j + 7 * j];
+    for (int k = 0; k < j; ++k) d -= L[j + 7 * k] * L[j + 7 * k];
+    d = sqrtf(d);
+    L[j + 7 * j] = d;
+    const float inv = 1.f / d;
+    for (int i = j + 1; i < NQ; ++i) {
+      float s = M[i + 7 * j];
+      for (int k = 0; k < j; ++k) s -= L[i + 7 * k] * L[j + 7 * k];
+      L[i + 7 * j] = s * inv;
+    }
+    for (int i = 0; i < j; ++i) L[i + 7 * j] = 0.f;
+  }
+}
+
+// x = L^-T L^-1 b for a 7x7 lower Cholesky factor
+__device__ __forceinline__ void chol_solve7(const float *L, const float *b, float *x) {
+  float y[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    float s = b[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= L[i + 7 * k] * y[k];
+    y[i] = s / L[i + 7 * i];
+  }
+#pragma unroll
+  for (int i = NQ - 1; i >= 0; --i) {
+    float s = y[i];
+#pragma unroll
+    for (int k = i + 1; k < NQ; ++k) s -= L[k + 7 * i] * x[k];
+    x[i] = s / L[i + 7 * i];
+  }
+}
+
 }  // namespace mpc

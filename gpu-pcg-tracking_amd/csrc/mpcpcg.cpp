@@ -51,6 +51,11 @@ struct mpc_handle {
   // iterative refinement: residual right-hand side (21 T + 14 T), correction
   // (21 T + 14 T) and per-instance scale (B)
   float *ref_g = nullptr, *ref_c = nullptr, *ref_dz = nullptr, *ref_lam = nullptr, *ref_scale = nullptr;
+  // closed loop: reference window (6 max_knots), plant state (16), rho,
+  // step counter, SQP status, PCG log of one step; a stream for capture
+  float *cl_win = nullptr, *cl_xp = nullptr, *cl_rho = nullptr;
+  uint32_t *cl_ctr = nullptr, *cl_status = nullptr, *cl_pcg = nullptr;
+  hipStream_t cl_stream = nullptr;
   // pinned staging for the host round trips of the plain driver
   float *h_merits = nullptr;
   uint32_t *h_pcg = nullptr;
@@ -63,7 +68,7 @@ struct mpc_handle {
   float wx[NX]{}, wu[NU]{};
   // timing
   bool timing = false;
-  hipEvent_t ev[9]{};
+  hipEvent_t ev[11]{};  // 0..8: batched SQP phases; 9, 10: closed-loop total
   double phase_us[7]{};
 };
 
@@ -165,7 +170,7 @@ static void free_all(mpc_handle *h) {
                   h->pcg_iters, h->pcg_flag, h->knot_inst, h->inst_k0, h->bin_items,
                   h->bin_off, h->big_list, h->team_list, h->team_one, h->team_err, h->mbox,
                   h->adj_mu, h->adj_knot, h->loss_knot, h->ref_g, h->ref_c, h->ref_dz, h->ref_lam,
-                  h->ref_scale};
+                  h->ref_scale, h->cl_win, h->cl_xp, h->cl_rho, h->cl_ctr, h->cl_status, h->cl_pcg};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (h->h_merits) (void)hipHostFree(h->h_merits);
@@ -173,6 +178,7 @@ static void free_all(mpc_handle *h) {
   if (h->h_rho) (void)hipHostFree(h->h_rho);
   for (auto &e : h->ev)
     if (e) (void)hipEventDestroy(e);
+  if (h->cl_stream) (void)hipStreamDestroy(h->cl_stream);
 }
 
 int mpc_create(const mpc_config *cfg, mpc_handle **out) {
@@ -229,6 +235,12 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
             alloc((void **)&h->ref_dz, sizeof(float) * NXU * T) &&
             alloc((void **)&h->ref_lam, sizeof(float) * NX * T) &&
             alloc((void **)&h->ref_scale, sizeof(float) * B) &&
+            alloc((void **)&h->cl_win, sizeof(float) * 6 * cfg->max_knots) &&
+            alloc((void **)&h->cl_xp, sizeof(float) * 16) && alloc((void **)&h->cl_rho, sizeof(float) * 4) &&
+            alloc((void **)&h->cl_ctr, sizeof(uint32_t) * 4) &&
+            alloc((void **)&h->cl_status, sizeof(uint32_t) * 4) &&
+            alloc((void **)&h->cl_pcg, sizeof(uint32_t) * MPC_MAX_SQP_ITERS) &&
+            hipStreamCreate(&h->cl_stream) == hipSuccess &&
             hipHostMalloc((void **)&h->h_merits, sizeof(float) * 16) == hipSuccess &&
             hipHostMalloc((void **)&h->h_pcg, sizeof(uint32_t) * 4) == hipSuccess &&
             hipHostMalloc((void **)&h->h_rho, sizeof(float) * 4) == hipSuccess;
@@ -294,6 +306,12 @@ int mpc_fk(mpc_handle *h, uint32_t n, const float *d_q, uint32_t q_stride, float
            mpc_stream_t stream) {
   if (!h || (n && (!d_q || !d_ee)) || q_stride < NQ) return MPC_ERR_ARG;
   HIP_TRY(launch_fk(n, d_q, q_stride, d_ee, (hipStream_t)stream));
+  return MPC_OK;
+}
+
+int mpc_inverse_dynamics(mpc_handle *h, uint32_t n, const float *d_qqdqdd, float *d_tau, mpc_stream_t stream) {
+  if (!h || (n && (!d_qqdqdd || !d_tau))) return MPC_ERR_ARG;
+  HIP_TRY(launch_inverse_dynamics(n, d_qqdqdd, d_tau, (hipStream_t)stream));
   return MPC_OK;
 }
 
@@ -664,44 +682,155 @@ static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_st
   return e;
 }
 
+// sqp_iters batched SQP iterations, stream-ordered on the device; with
+// h->timing, per-phase event times are added to h->phase_us (one
+// synchronisation per iteration).
+static hipError_t batch_sqp_run(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee_ref, float *d_rho,
+                                uint32_t sqp_iters, uint32_t *d_status, uint32_t *d_pcg_iters, hipStream_t st) {
+  const Batch b = batch_of(h);
+  const Params prm = make_params(h);
+  InstState s{d_rho, h->drho, h->merit0, d_status, h->sqp_done, h->xs};
+  const bool tm = h->timing;
+  hipError_t e;
+#define MPC_RUN(expr)           \
+  do {                          \
+    e = (expr);                 \
+    if (e != hipSuccess) return e; \
+  } while (0)
+  MPC_RUN(launch_merit(b, prm, d_xu, nullptr, d_ee_ref, h->xs, nullptr, 8, 1, h->knot_merits, st));
+  MPC_RUN(launch_batch_init(b, d_xu, h->knot_merits, s, st));
+  for (uint32_t it = 0; it < sqp_iters; ++it) {
+    if (tm) MPC_RUN(hipEventRecord(h->ev[0], st));
+    MPC_RUN(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, d_status, h->G, h->C, h->g, h->c, st));
+    if (tm) MPC_RUN(hipEventRecord(h->ev[1], st));
+    MPC_RUN(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma, h->Ginv, 0,
+                              st));
+    if (tm) MPC_RUN(hipEventRecord(h->ev[2], st));
+    MPC_RUN(pcg_batch(h, d_lambda, d_status, h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, tm ? h->ev[3] : nullptr, st));
+    if (tm) MPC_RUN(hipEventRecord(h->ev[4], st));
+    MPC_RUN(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
+    if (tm) MPC_RUN(hipEventRecord(h->ev[5], st));
+    MPC_RUN(launch_merit(b, prm, d_xu, h->dz, d_ee_ref, h->xs, d_status, 0, 8, h->knot_merits, st));
+    if (tm) MPC_RUN(hipEventRecord(h->ev[6], st));
+    MPC_RUN(launch_ls_update(b, prm, h->knot_merits, h->dz, d_xu, s, d_pcg_iters, h->pcg_iters, h->pcg_flag, it,
+                             sqp_iters, st));
+    if (tm) {
+      MPC_RUN(hipEventRecord(h->ev[7], st));
+      MPC_RUN(hipEventSynchronize(h->ev[7]));
+      for (int ph = 0; ph < 7; ++ph) {
+        float ms = 0.f;
+        MPC_RUN(hipEventElapsedTime(&ms, h->ev[ph], h->ev[ph + 1]));
+        h->phase_us[ph] += 1e3 * ms;
+      }
+    }
+  }
+#undef MPC_RUN
+  return hipSuccess;
+}
+
 int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee_ref, float *d_rho,
                   uint32_t sqp_iters, uint32_t *d_status, uint32_t *d_pcg_iters, mpc_stream_t stream) {
   if (!h || !d_xu || !d_lambda || !d_ee_ref || !d_rho || !d_status) return MPC_ERR_ARG;
   if (!h->batch_ready) return MPC_ERR_NOT_READY;
   if (sqp_iters > MPC_MAX_SQP_ITERS) return MPC_ERR_ARG;
-  hipStream_t st = (hipStream_t)stream;
-  const Batch b = batch_of(h);
-  const Params prm = make_params(h);
-  InstState s{d_rho, h->drho, h->merit0, d_status, h->sqp_done, h->xs};
-  const bool tm = h->timing;
   for (double &v : h->phase_us) v = 0;
+  HIP_TRY(batch_sqp_run(h, d_xu, d_lambda, d_ee_ref, d_rho, sqp_iters, d_status, d_pcg_iters,
+                        (hipStream_t)stream));
+  return MPC_OK;
+}
 
-  HIP_TRY(launch_merit(b, prm, d_xu, nullptr, d_ee_ref, h->xs, nullptr, 8, 1, h->knot_merits, st));
-  HIP_TRY(launch_batch_init(b, d_xu, h->knot_merits, s, st));
-  for (uint32_t it = 0; it < sqp_iters; ++it) {
-    if (tm) HIP_TRY(hipEventRecord(h->ev[0], st));
-    HIP_TRY(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, d_status, h->G, h->C, h->g, h->c, st));
-    if (tm) HIP_TRY(hipEventRecord(h->ev[1], st));
-    HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma,
-                              h->Ginv, 0, st));
-    if (tm) HIP_TRY(hipEventRecord(h->ev[2], st));
-    HIP_TRY(pcg_batch(h, d_lambda, d_status, h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, tm ? h->ev[3] : nullptr, st));
-    if (tm) HIP_TRY(hipEventRecord(h->ev[4], st));
-    HIP_TRY(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
-    if (tm) HIP_TRY(hipEventRecord(h->ev[5], st));
-    HIP_TRY(launch_merit(b, prm, d_xu, h->dz, d_ee_ref, h->xs, d_status, 0, 8, h->knot_merits, st));
-    if (tm) HIP_TRY(hipEventRecord(h->ev[6], st));
-    HIP_TRY(launch_ls_update(b, prm, h->knot_merits, h->dz, d_xu, s, d_pcg_iters, h->pcg_iters,
-                             h->pcg_flag, it, sqp_iters, st));
-    if (tm) {
-      HIP_TRY(hipEventRecord(h->ev[7], st));
-      HIP_TRY(hipEventSynchronize(h->ev[7]));
-      for (int ph = 0; ph < 7; ++ph) {
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, h->ev[ph], h->ev[ph + 1]));
-        h->phase_us[ph] += 1e3 * ms;
-      }
+// ---- closed loop ----------------------------------------------------------------
+
+void mpc_default_closed_loop(mpc_closed_loop_config *cl) {
+  if (!cl) return;
+  std::memset(cl, 0, sizeof(*cl));
+  cl->steps = 2000;
+  cl->sqp_iters = 3;
+  cl->substeps = 50;
+  cl->sim_dt = 2e-4f;
+  cl->noise_sigma = 1e-3f;
+  cl->noise_seed = 12345;
+  cl->use_graph = 1;
+}
+
+int mpc_closed_loop(mpc_handle *h, const mpc_closed_loop_config *cl, uint32_t N, const float *d_x0,
+                    const float *d_ee_path, const float *d_xu_traj, float *d_xu, float *d_lambda, float *d_states,
+                    float *d_ee_err, uint32_t *d_pcg_iters, mpc_closed_loop_stats *stats, mpc_stream_t stream) {
+  if (!h || !cl || !d_x0 || !d_ee_path || !d_xu || !d_lambda || !d_states || !d_ee_err) return MPC_ERR_ARG;
+  if (!n_ok(h, N) || cl->steps < 1 || cl->sqp_iters < 1 || cl->sqp_iters > MPC_MAX_SQP_ITERS ||
+      cl->substeps < 1 || !(cl->sim_dt > 0.f) || !std::isfinite(cl->sim_dt) || !(cl->noise_sigma >= 0.f) ||
+      !std::isfinite(cl->noise_sigma) || (cl->use_graph && cl->timing))
+    return MPC_ERR_ARG;
+  const uint32_t horizon[1] = {N};
+  int rc = mpc_batch_setup(h, 1, horizon);
+  if (rc != MPC_OK) return rc;
+  if (cl->use_graph && h->n_teams) return MPC_ERR_ARG;  // cooperative launches are not captured
+  hipStream_t user = (hipStream_t)stream;
+  hipStream_t st = cl->use_graph ? h->cl_stream : user;
+  const ClStep step{N, cl->substeps, cl->sqp_iters, cl->sim_dt, cl->noise_sigma, cl->noise_seed};
+  double solve_us = 0, linsys_us = 0, plant_us = 0;
+  const bool tm = cl->timing != 0;
+  const bool saved_timing = h->timing;
+  h->timing = tm;
+  auto one_step = [&]() -> hipError_t {
+    if (tm)
+      for (double &v : h->phase_us) v = 0;
+    hipError_t e = batch_sqp_run(h, d_xu, d_lambda, h->cl_win, h->cl_rho, cl->sqp_iters, h->cl_status, h->cl_pcg,
+                                 st);
+    if (e == hipSuccess && tm) e = hipEventRecord(h->ev[0], st);
+    if (e == hipSuccess)
+      e = launch_cl_step(step, d_ee_path, d_xu_traj, h->cl_win, d_xu, d_lambda, h->cl_xp, h->cl_ctr, d_states,
+                         d_ee_err, h->cl_pcg, d_pcg_iters, st);
+    if (e == hipSuccess && tm) {
+      e = hipEventRecord(h->ev[1], st);
+      if (e == hipSuccess) e = hipEventSynchronize(h->ev[1]);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+      plant_us += 1e3 * ms;
+      for (int ph = 0; ph < 7; ++ph) solve_us += h->phase_us[ph];
+      linsys_us += h->phase_us[1] + h->phase_us[2] + h->phase_us[3] + h->phase_us[4];
     }
+    return e;
+  };
+  hipError_t e = hipSuccess;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t t0 = h->ev[9], t1 = h->ev[10];
+  if (cl->use_graph) {
+    // order the private stream after the caller's work
+    e = hipEventRecord(t0, user);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, t0, 0);
+  }
+  if (e == hipSuccess)
+    e = launch_cl_init(N, d_x0, d_ee_path, h->cl_win, d_xu, h->cl_xp, h->cl_ctr, d_states, h->cl_rho,
+                       h->cfg.rho_reset, st);
+  if (e == hipSuccess && cl->use_graph) {
+    e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      const hipError_t ec = one_step();
+      e = hipStreamEndCapture(st, &graph);
+      if (ec != hipSuccess) e = ec;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  }
+  if (e == hipSuccess) e = hipEventRecord(t0, st);
+  for (uint32_t k = 0; e == hipSuccess && k < cl->steps; ++k) e = cl->use_graph ? hipGraphLaunch(exec, st) : one_step();
+  if (e == hipSuccess) e = hipEventRecord(t1, st);
+  if (e == hipSuccess && cl->use_graph) e = hipStreamWaitEvent(user, t1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(t1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  h->timing = saved_timing;
+  if (e != hipSuccess) return MPC_ERR_HIP;
+  if (stats) {
+    stats->total_us = 1e3 * ms;
+    stats->solve_us = solve_us;
+    stats->linsys_us = linsys_us;
+    stats->plant_us = plant_us;
+    stats->graph = cl->use_graph ? 1u : 0u;
+    stats->reserved = 0;
   }
   return MPC_OK;
 }
@@ -737,6 +866,29 @@ void mpc_synth_instance(uint64_t seed, uint32_t N, float dt, float *h_x0, float 
       for (int j = 0; j < NQ; ++j)
         h_qref[(size_t)NQ * k + j] =
             (float)(q0[j] + 0.3 * std::sin(2.0 * M_PI * f[j] * (double)k * (double)dt + ph[j]));
+}
+
+void mpc_synth_track(uint64_t seed, uint32_t K, float dt, float *h_x0, float *h_qqdqdd) {
+  uint64_t st = seed;
+  double q0[NQ], qd0[NQ], f[NQ], ph[NQ];
+  for (int j = 0; j < NQ; ++j) q0[j] = -0.5 + uniform01(st);
+  for (int j = 0; j < NQ; ++j) qd0[j] = 0.1 * normal01(st);
+  for (int j = 0; j < NQ; ++j) f[j] = 0.2 + 0.8 * uniform01(st);
+  for (int j = 0; j < NQ; ++j) ph[j] = 2.0 * M_PI * uniform01(st);
+  if (h_x0)
+    for (int j = 0; j < NQ; ++j) {
+      h_x0[j] = (float)q0[j];
+      h_x0[NQ + j] = (float)qd0[j];
+    }
+  if (h_qqdqdd)
+    for (uint32_t k = 0; k < K; ++k)
+      for (int j = 0; j < NQ; ++j) {
+        const double w = 2.0 * M_PI * f[j], a = w * (double)k * (double)dt + ph[j];
+        float *o = h_qqdqdd + (size_t)NXU * k;
+        o[j] = (float)(q0[j] + 0.3 * std::sin(a));
+        o[NQ + j] = (float)(0.3 * w * std::cos(a));
+        o[NX + j] = (float)(-0.3 * w * w * std::sin(a));
+      }
 }
 
 void mpc_synth_horizons(uint64_t seed, uint32_t n_inst, const uint32_t *choices, uint32_t n_choices,

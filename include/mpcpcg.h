@@ -121,6 +121,12 @@ const char *mpc_status_string(int status);
 int mpc_fk(mpc_handle *h, uint32_t n, const float *d_q, uint32_t q_stride, float *d_ee,
            mpc_stream_t stream);
 
+/* tau = M(q) qdd + c(q, qd) (7 per point) for n points [q qd qdd] (21 floats
+ * each), gravity 0 as the plant.  grid::inverse_dynamics_kernel
+ * (rbdfiles/iiwa_eepos_grid.cuh, called with gato_plant::GRAVITY = 0,
+ * rbdfiles/iiwa_eepos_plant.cuh:51). */
+int mpc_inverse_dynamics(mpc_handle *h, uint32_t n, const float *d_qqdqdd, float *d_tau, mpc_stream_t stream);
+
 /* qdd (7 per point) and df_du (7x21 per point: dqdd/dq, dqdd/dqd, dqdd/du)
  * for n points [q qd u] (21 floats each).
  * gato_plant::forwardDynamicsAndGradient (rbdfiles/iiwa_eepos_plant.cuh:126-156). */
@@ -200,6 +206,52 @@ int mpc_batch_set_timing(mpc_handle *h, int enable);
 int mpc_batch_pcg_layout(const mpc_handle *h, uint32_t *out4);
 int mpc_batch_phase_times(mpc_handle *h, double *us7);
 
+/* ---- closed-loop tracking ----------------------------------------------------
+ * Receding-horizon tracking of one instance entirely on the device: per
+ * control step the batched SQP (sqp_iters iterations from the warm start),
+ * then `substeps` explicit Euler plant steps of sim_dt under the plan's
+ * first control, Gaussian state noise (sigma, seeded splitmix64 +
+ * Box-Muller), the end-effector error against the reference and a
+ * one-knot shift of the plan, the multipliers and the reference window.
+ * Replaces the tracking loop of track() (toplevel.cuh:505-775, plant
+ * integrator.cuh:283-398), with one knot per control step (substeps *
+ * sim_dt should equal dt).  Sets the batch layout to one instance of
+ * horizon N.  Buffers (device): d_x0 (14), d_ee_path (6 (steps + N)),
+ * d_xu_traj (precomputed trajectory in the xu layout over steps + N knots,
+ * may be NULL: the shift fills the new horizon tail from it, else with a
+ * zero control and the kept last state), d_xu (21N-7, in: initial plan,
+ * out: last plan), d_lambda (14N, in/out), d_states (14 (steps + 1), out:
+ * plant states), d_ee_err (2 steps, out: Euclidean and L1 norm of the xyz
+ * error per step), d_pcg_iters (steps x sqp_iters, out, may be NULL).  use_graph captures
+ * one control step in a HIP graph and replays it (N <= 128 only, where the
+ * PCG has no cooperative launch); timing adds per-phase HIP event timing
+ * (one synchronisation per step, no graph). */
+typedef struct mpc_closed_loop_config {
+  uint32_t steps;
+  uint32_t sqp_iters;
+  uint32_t substeps;
+  float sim_dt;
+  float noise_sigma;
+  uint32_t use_graph;
+  uint64_t noise_seed;
+  uint32_t timing;
+  uint32_t reserved;
+} mpc_closed_loop_config;
+
+typedef struct mpc_closed_loop_stats {
+  double total_us;   /* device time of the whole loop (HIP events) */
+  double solve_us;   /* timing = 1: SQP phases, summed over the steps */
+  double linsys_us;  /* timing = 1: Schur + PCG + dz phases, summed */
+  double plant_us;   /* timing = 1: plant / shift kernel, summed */
+  uint32_t graph;    /* 1 when the loop ran as graph replays */
+  uint32_t reserved;
+} mpc_closed_loop_stats;
+
+void mpc_default_closed_loop(mpc_closed_loop_config *cl);
+int mpc_closed_loop(mpc_handle *h, const mpc_closed_loop_config *cl, uint32_t N, const float *d_x0,
+                    const float *d_ee_path, const float *d_xu_traj, float *d_xu, float *d_lambda, float *d_states,
+                    float *d_ee_err, uint32_t *d_pcg_iters, mpc_closed_loop_stats *stats, mpc_stream_t stream);
+
 /* ---- cost weights and the differentiable QP layer --------------------------
  * Neither has a counterpart in the reference (its QD_COST / R_COST are
  * scalar compile-time macros, settings.cuh, perform_experiment.sh:44; it has
@@ -248,6 +300,11 @@ int mpc_batch_qp_adjoint(mpc_handle *h, const float *d_xu, const float *d_ee_ref
  * phi ~ U(0,2 pi)^7; q_ref(k) = q0 + 0.3 sin(2 pi f k dt + phi), k < N.
  * Writes x0 (14) and q_ref (7N).  Host-only, no GPU needed. */
 void mpc_synth_instance(uint64_t seed, uint32_t N, float dt, float *h_x0, float *h_qref);
+
+/* The joint path of mpc_synth_instance with its derivatives: per knot
+ * k < K, [q_ref, qd_ref, qdd_ref] (21 floats) at t = k dt; x0 as there.
+ * Host-only. */
+void mpc_synth_track(uint64_t seed, uint32_t K, float dt, float *h_x0, float *h_qqdqdd);
 
 /* Horizons drawn uniformly from choices[0..n_choices) with a splitmix64
  * stream seeded with seed.  Host-only. */

@@ -34,6 +34,10 @@
  *     Pinned by brute-force numpy evaluation from the pinned FK / FD.
  *   - differentiable QP layer and its adjoint (growth direction of the
  *     build; the reference has none): pinned by fp64 central differences.
+ *   - closed-loop harness: toplevel.cuh:505-775, integrator.cuh:283-398,
+ *     restated with one knot shift per control step.  Pinned by the noise
+ *     statistics, the Euler plant against orc_fd, the shift invariants and
+ *     convergence to a stationary reachable target.
  *   - sparse LDL^T (QDLDL-equivalent, published up-looking algorithm) with the
  *     reference's CSC pattern: utils.cuh:54-118, qdldl_helper.cuh:62-180,
  *     toplevel.cuh:160-215.  Pinned by L D L^T = S reconstruction and
@@ -1068,6 +1072,93 @@ int F(orc_qp_adjoint)(const orc_cfg *cfg, int N, const real *xu, const real *ee_
   }
   F(orc_ldl_destroy)(ldl);
   free(G); free(Ginv); free(C); free(g); free(c); free(S); free(P); free(gam); free(az); free(mu); free(v); free(c0);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------
+ * Closed-loop tracking harness: track() (toplevel.cuh:505-775) with the
+ * plant of integrator.cuh:283-398, restated with one knot shift per
+ * control step.  Control step s:
+ *   SQP (cfg->sqp_max_iter iterations) from the warm start xu, lambda, rho
+ *   on the reference window path[s .. s+N-1];
+ *   plant: `substeps` explicit Euler steps of sim_dt (q += h qd,
+ *   qd += h qdd(q, qd, u_0)) under the plan's first control;
+ *   noise: x += sigma n, pair i (components 2i, 2i+1) of step s from
+ *   splitmix64 seeded with seed ^ 0xD1B54A32D192ED03 (8 s + i + 1), two
+ *   draws u1 in (0, 1], u2 in [0, 1), Box-Muller r cos / r sin;
+ *   error: |ee(q_{s+1}) - path[s+1]| (xyz, Euclidean);
+ *   shift: x_k <- x_{k+1}, u_k <- u_{k+1} (k < N-2), lambda_k <- lambda_{k+1},
+ *   x_0 <- plant state; the new tail (u_{N-2}, x_{N-1}) is the precomputed
+ *   trajectory's (u_{s+N-1}, x_{s+N}) when traj (xu layout over steps + N
+ *   knots) is given, else u_{N-2} <- 0 and x_{N-1} is kept.
+ * states: 14 (steps + 1); ee_err: 2 per step (Euclidean and L1 norm of the
+ * xyz error); pcg_iters: steps x sqp_max_iter (may be NULL).  Returns 0 or
+ * the first SQP error.
+ * ---------------------------------------------------------------------- */
+static uint64_t cl_sm64(uint64_t *s) {
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+void F(orc_cl_noise)(uint64_t seed, uint32_t step, real *n14) {
+  for (uint32_t i = 0; i < 7; ++i) {
+    uint64_t s = seed ^ (0xD1B54A32D192ED03ull * (8ull * step + i + 1ull));
+    const double u1 = (double)((cl_sm64(&s) >> 11) + 1ull) * 0x1.0p-53;
+    const double u2 = (double)(cl_sm64(&s) >> 11) * 0x1.0p-53;
+    const double r = sqrt(-2.0 * log(u1)), t = 6.283185307179586 * u2;
+    n14[2 * i] = (real)(float)(r * cos(t));
+    n14[2 * i + 1] = (real)(float)(r * sin(t));
+  }
+}
+
+int F(orc_closed_loop)(const orc_cfg *cfg, int N, int steps, int substeps, real sim_dt, real sigma,
+                       uint64_t seed, real rho_reset, const real *x0, const real *path, const real *traj,
+                       real *xu, real *lambda, real *rho, real *states, real *ee_err, int *pcg_iters) {
+  const int K = cfg->sqp_max_iter;
+  real x[NX];
+  memcpy(x, x0, sizeof x);
+  memcpy(states, x0, sizeof x);
+  memcpy(xu, x0, sizeof x);
+  int *pit = (int *)calloc((size_t)(K > 0 ? K : 1), sizeof(int));
+  int rc = 0;
+  for (int s = 0; s < steps && !rc; ++s) {
+    int si, na;
+    rc = F(orc_sqp_solve)(cfg, N, path + 6 * s, lambda, xu, rho, rho_reset, 0, &si, pit, NULL, NULL, &na);
+    if (pcg_iters)
+      for (int j = 0; j < K; ++j) pcg_iters[K * s + j] = j < si ? pit[j] : 0;
+    const real *u = xu + NX;
+    for (int k = 0; k < substeps; ++k) {
+      real qdd[NQ];
+      F(orc_fd)(x, x + NQ, u, qdd);
+      for (int i = 0; i < NQ; ++i) {
+        const real q = x[i], qd = x[NQ + i];
+        x[i] = q + sim_dt * qd;
+        x[NQ + i] = qd + sim_dt * qdd[i];
+      }
+    }
+    if (sigma > 0) {
+      real n[NX];
+      F(orc_cl_noise)(seed, (uint32_t)s, n);
+      for (int i = 0; i < NX; ++i) x[i] += sigma * n[i];
+    }
+    memcpy(states + NX * (s + 1), x, sizeof x);
+    real ee[6];
+    F(orc_fk)(x, ee);
+    const real *r = path + 6 * (s + 1);
+    const real d0 = ee[0] - r[0], d1 = ee[1] - r[1], d2 = ee[2] - r[2];
+    ee_err[2 * s] = (real)sqrt((double)(d0 * d0 + d1 * d1 + d2 * d2));
+    ee_err[2 * s + 1] = (real)(fabs((double)d0) + fabs((double)d1) + fabs((double)d2));
+    memmove(xu, xu + NXU, sizeof(real) * (size_t)(NXU * N - NU - NXU));
+    if (traj)
+      memcpy(xu + NXU * (N - 2) + NX, traj + (size_t)NXU * (s + N - 1) + NX, sizeof(real) * NXU);
+    else
+      for (int i = 0; i < NU; ++i) xu[NXU * (N - 2) + NX + i] = 0;
+    memmove(lambda, lambda + NX, sizeof(real) * (size_t)(NX * (N - 1)));
+    memcpy(xu, x, sizeof x);
+  }
+  free(pit);
   return rc;
 }
 

@@ -406,3 +406,37 @@ def qp_adjoint(cfg, N, xu, ee_ref, xs, rho, dz, a, dtype=np.float64):
     if rc:
         raise RuntimeError(f"orc_qp_adjoint failed ({rc})")
     return gw, gref
+
+
+# -- closed loop -----------------------------------------------------------------------------------
+
+def cl_noise(seed, step, dtype=np.float64):
+    """The 14 standard normals of control step `step` (splitmix64 + Box-Muller)."""
+    out = np.zeros(14, dtype)
+    _fn("orc_cl_noise", dtype)(C.c_uint64(seed), C.c_uint32(step), _ptr(out))
+    return out
+
+
+def closed_loop(cfg, N, steps, x0, path, xu, lam, rho=1e-3, substeps=50, sim_dt=2e-4, sigma=1e-3, seed=12345,
+                rho_reset=1e-3, traj=None, dtype=np.float64):
+    """Closed-loop tracking (cfg.sqp_max_iter SQP iterations per control step).
+    Returns dict(states (steps+1, 14), ee_err (steps, 2: Euclidean, L1), pcg_iters (steps, K), xu, lam, rho)."""
+    x0, path = _arr(x0, dtype), _arr(path, dtype)
+    traj = None if traj is None else _arr(traj, dtype)
+    xu, lam = _arr(xu, dtype).copy(), _arr(lam, dtype).copy()
+    K = max(cfg.sqp_max_iter, 1)
+    states = np.zeros((steps + 1) * 14, dtype)
+    err = np.zeros(2 * steps, dtype)
+    pit = np.zeros(steps * K, np.int32)
+    rt = C.c_double if np.dtype(dtype) == np.float64 else C.c_float
+    rho_c = rt(rho)
+    fn = _fn("orc_closed_loop", dtype)
+    fn.restype = C.c_int
+    rc = fn(C.byref(cfg), N, steps, substeps, rt(sim_dt), rt(sigma), C.c_uint64(seed), rt(rho_reset), _ptr(x0),
+            _ptr(path), _ptr(traj) if traj is not None else None, _ptr(xu), _ptr(lam), C.byref(rho_c),
+            _ptr(states), _ptr(err), _ptr(pit))
+    if rc:
+        raise RuntimeError(f"orc_closed_loop failed ({rc})")
+    return dict(states=states.reshape(steps + 1, 14), ee_err=err.reshape(steps, 2), pcg_iters=pit.reshape(steps, K), xu=xu, lam=lam,
+                rho=float(rho_c.value))
+
