@@ -17,6 +17,13 @@ one-instance-per-launch solve (reference call pattern) on a sample, the
 per-phase split, the roofline of the dominant kernel and a CPU baseline
 (the sparse LDL^T solve of the Schur system, fp32, one core).
 
+Also reported ("closed_loop"): the closed-loop tracking workload, one
+instance at N = 128 for 2000 control steps (PCG tolerance 5e-6, state
+noise 1e-3, plant = 50 Euler substeps of 2e-4 s per step), run on the
+device with one captured control step replayed as a HIP graph: time per
+control step, RMS / mean-L1 end-effector error, and from a separate
+event-timed run the per-SQP-iteration linear-solve time.
+
 Also reported ("differentiation"): the adjoint / forward cost of the
 differentiable QP layer on 256 instances at N = 64 (same cost and inputs):
 forward = the QP layer at the final SQP iterate plus the tracking loss and
@@ -68,6 +75,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--no-closed-loop", action="store_true")
     return ap.parse_args()
 
 
@@ -152,6 +160,36 @@ def differentiation(pkg, torch, seed, reps=10, n=256, N=64, sqp_iters=3):
             "adjoint_ms": round(adj_ms, 3), "adjoint_over_layer_forward": round(adj_ms / fwd_ms, 3),
             "adjoint_over_sqp_plus_layer": round(adj_ms / (sqp_ms + fwd_ms), 3),
             "gradients_per_sec": round(n / (adj_ms * 1e-3), 1), "gradients_finite": finite}
+
+
+def closed_loop(pkg, torch, seed, N=128, steps=2000, timed_steps=200, K=3):
+    """The closed-loop workload; see the module docstring."""
+    solver = pkg.Solver(pkg.default_config(max_knots=N, max_instances=1, max_total_knots=N, pcg_exit_tol=5e-6))
+    x0, ee, traj = solver.track_inputs(seed, steps, N)
+
+    def run(n, graph, timing):
+        xu = traj[:21 * N - 7].clone()
+        lam = torch.zeros(14 * N, device="cuda")
+        pcg = torch.zeros(n * K, dtype=torch.int32, device="cuda")
+        cl = pkg.default_closed_loop(steps=n, sqp_iters=K, use_graph=graph, timing=timing)
+        _, err, _, st = solver.closed_loop(N, x0, ee, xu, lam, cl, traj, pcg_iters=pcg)
+        return err.cpu().numpy(), pcg.cpu().numpy(), st
+
+    run(20, 1, 0)  # warm-up (first launches, graph instantiation paths)
+    err, pcg, st = run(steps, 1, 0)
+    _, pcg_t, st_t = run(timed_steps, 0, 1)
+    solver.close()
+    n_it = timed_steps * K
+    return {"workload": f"1 instance, N={N}, {steps} control steps, {K} SQP iterations per step, PCG tol 5e-6, "
+                        f"state noise 1e-3, plant 50 x 2e-4 s Euler substeps, precomputed-trajectory warm start",
+            "us_per_control_step": round(st["total_us"] / steps, 1), "graph": bool(st["graph"]),
+            "rms_ee_error_m": round(float(np.sqrt(np.mean(err[:, 0] ** 2))), 4),
+            "mean_l1_ee_error_m": round(float(err[:, 1].mean()), 4),
+            "pcg_iters_mean": round(float(pcg.mean()), 1), "pcg_hit_cap_frac": round(float((pcg >= 173).mean()), 3),
+            "timed_run": {"steps": timed_steps, "linsys_us_per_sqp_iter": round(st_t["linsys_us"] / n_it, 1),
+                          "sqp_us_per_sqp_iter": round(st_t["solve_us"] / n_it, 1),
+                          "plant_us_per_step": round(st_t["plant_us"] / timed_steps, 1)},
+            "reference_pcg_linsys_us": {"N128_tol1e-5_rtx4090": 448.79}}
 
 
 def cpu_baseline(orc, helpers, horizons, seconds, seed):
@@ -289,6 +327,7 @@ def main():
                          f"{desc}; ~{args.cpu_seconds:.0f} s of CPU work"}
 
     diff = None if args.no_diff else differentiation(pkg, torch, args.seed)
+    closed = None if args.no_closed_loop else closed_loop(pkg, torch, args.seed)
 
     its = log[executed]
     out = {
@@ -330,6 +369,7 @@ def main():
                               "frac": round(tflops / FP32_PEAK_TFLOPS, 4)}},
         "cpu_baseline": cpu,
         "differentiation": diff,
+        "closed_loop": closed,
         "wall_s_timed": round(t_wall, 3),
     }
     print(json.dumps(out))

@@ -33,6 +33,7 @@ OK, ERR_ARG, ERR_HIP, ERR_NOT_READY, ERR_NONFINITE, ERR_NO_DEVICE = range(6)
 ST_ACTIVE, ST_PCG_MAXITER, ST_RHO_ABORT, ST_NONFINITE = 1, 2, 4, 8
 FLAG_LAST_KNOT_OWN_STATE = 1
 FLAG_PCG_STREAM = 2
+FLAG_DIRECT_SOLVE = 4
 
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = (
@@ -42,7 +43,7 @@ EXPORTS = (
     "mpc_sqp_solve", "mpc_batch_setup", "mpc_batch_sqp", "mpc_batch_set_timing",
     "mpc_batch_phase_times", "mpc_batch_pcg_layout", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
     "mpc_batch_qp_loss", "mpc_batch_qp_adjoint", "mpc_synth_instance", "mpc_synth_horizons",
-    "mpc_inverse_dynamics", "mpc_default_closed_loop", "mpc_closed_loop", "mpc_synth_track",
+    "mpc_inverse_dynamics", "mpc_default_closed_loop", "mpc_closed_loop", "mpc_synth_track", "mpc_direct_solve",
 )
 
 
@@ -126,6 +127,7 @@ def lib():
             "mpc_batch_qp_loss": [vp, vp, vp, vp, vp, vp, vp],
             "mpc_batch_qp_adjoint": [vp, vp, vp, vp, vp, vp, f32, u32, f32, vp, vp, vp],
             "mpc_inverse_dynamics": [vp, u32, vp, vp, vp],
+            "mpc_direct_solve": [vp, u32, vp, vp, vp, u32, vp, vp],
             "mpc_closed_loop": [vp, C.POINTER(ClosedLoopConfig), u32, vp, vp, vp, vp, vp, vp, vp, vp,
                                 C.POINTER(ClosedLoopStats), vp],
         }
@@ -276,6 +278,14 @@ class Solver:
     def pcg(self, N, S, Pinv, gamma, lam, max_iter, exit_tol, iters=None, flag=None):
         _check(lib().mpc_pcg_solve(self.h, N, _ptr(S), _ptr(Pinv), _ptr(gamma), _ptr(lam), max_iter,
                                    float(exit_tol), _ptr(iters), _ptr(flag), _stream()), "mpc_pcg_solve")
+
+    def direct_solve(self, N, S, gamma, lam, refine=2, work=None):
+        """lambda = S^-1 gamma by block LDL^T on the GPU with `refine` steps of
+        iterative refinement (work: 588 N floats)."""
+        import torch
+        work = torch.empty(588 * N, device="cuda") if work is None else work
+        _check(lib().mpc_direct_solve(self.h, N, _ptr(S), _ptr(gamma), _ptr(lam), int(refine), _ptr(work), _stream()),
+               "mpc_direct_solve")
 
     def compute_dz(self, N, Ginv, Cm, g, lam, dz):
         _check(lib().mpc_compute_dz(self.h, N, _ptr(Ginv), _ptr(Cm), _ptr(g), _ptr(lam), _ptr(dz), _stream()),

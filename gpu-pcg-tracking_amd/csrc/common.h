@@ -92,7 +92,7 @@ struct ClStep {
 };
 
 constexpr uint32_t ST_ACTIVE = 1u, ST_PCG_MAXITER = 2u, ST_RHO_ABORT = 4u, ST_NONFINITE = 8u;
-constexpr uint32_t FLAG_LAST_KNOT_OWN_STATE = 1u, FLAG_PCG_STREAM = 2u;
+constexpr uint32_t FLAG_LAST_KNOT_OWN_STATE = 1u, FLAG_PCG_STREAM = 2u, FLAG_DIRECT_SOLVE = 4u;
 
 // host-side launchers (implemented in the .hip files)
 hipError_t launch_fk(uint32_t n, const float *q, uint32_t q_stride, float *ee, hipStream_t st);
@@ -153,6 +153,13 @@ hipError_t launch_adj_params(const Batch &b, const Params &p, const float *xu, c
 hipError_t launch_reduce_knots(const Batch &b, const float *vals, uint32_t S, float *out, hipStream_t st);
 hipError_t launch_sub(uint32_t n, const float *a, const float *bb, float *o, hipStream_t st);
 hipError_t launch_fill(uint32_t n, float v, float *o, hipStream_t st);
+// direct block-tridiagonal solve of the Schur system (k_direct.hip); order:
+// instance indices longest first (NULL: instance = list index); work: the
+// factors E_b, D_b^-1 and a vector per knot in each knot's S_STRIDE slot;
+// refine: iterative-refinement steps with an fp64 residual
+hipError_t launch_direct_solve(const Batch &b, const uint32_t *order, uint32_t n_list, const float *S,
+                               const float *gamma, const uint32_t *status, float *work, float *lambda,
+                               uint32_t refine, uint32_t *iters, uint32_t *flag, hipStream_t st);
 // closed-loop harness (k_closed_loop.hip)
 hipError_t launch_cl_init(uint32_t N, const float *x0, const float *path, float *win, float *xu, float *xp,
                           uint32_t *ctr, float *states, float *rho, float rho0, hipStream_t st);

@@ -23,6 +23,7 @@
 // dz (oldschur.cuh:966-1105):
 //   dx_k = Q~_k^-1 (q_k - lambda_k + A_k^T lambda_{k+1}),  du_k = R~_k^-1 (r_k + B_k^T lambda_{k+1})
 #include "common.h"
+#include "gj16.h"
 #include "model.h"
 
 namespace mpc {
@@ -32,63 +33,6 @@ namespace {
 constexpr int GROUP = 16;
 constexpr int GROUPS_PER_WG = 4;
 constexpr int WG = GROUP * GROUPS_PER_WG;
-
-// value of lane P of the calling lane's 16-lane group (ds_swizzle bitmask
-// mode: lane' = (lane & 0x10) | P within each 32-lane half)
-template <int P>
-__device__ __forceinline__ float bcast16(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x10 | (P << 5)));
-}
-
-// Materialise values at this point of the program: keeps the compiler from
-// sinking their computation to a later (conditional) use, which would keep
-// all of its inputs alive in registers.
-template <int n>
-__device__ __forceinline__ void pin(float (&v)[n]) {
-#pragma unroll
-  for (int r = 0; r < n; ++r) asm volatile("" : "+v"(v[r]));
-}
-
-template <int n, int p>
-__device__ __forceinline__ void gj_step(float (&a)[NX], float (&x)[NX]) {
-  float pc[n];
-#pragma unroll
-  for (int r = 0; r < n; ++r) pc[r] = bcast16<p>(a[r]);
-  const float inv = 1.f / pc[p];
-  const float ap = a[p] * inv, xp = x[p] * inv;
-#pragma unroll
-  for (int r = 0; r < n; ++r) {
-    if (r == p) continue;
-    a[r] -= pc[r] * ap;
-    x[r] -= pc[r] * xp;
-  }
-  a[p] = ap;
-  x[p] = xp;
-  // the x updates would otherwise sink to their use after the last step and
-  // keep every step's pivot column alive
-  pin(a);
-  pin(x);
-}
-
-template <int n, int p = 0>
-__device__ __forceinline__ void gj_steps(float (&a)[NX], float (&x)[NX]) {
-  if constexpr (p < n) {
-    gj_step<n, p>(a, x);
-    gj_steps<n, p + 1>(a, x);
-  }
-}
-
-// Gauss-Jordan inverse of an n x n matrix (n <= 14) held by columns: lane
-// j < n holds column j of M in a[0..n) (lanes >= n hold a copy of column
-// n-1, which no lane reads); returns column j of M^-1 in x.  Every lane of
-// the group must be active.
-template <int n>
-__device__ __forceinline__ void gj_inverse(int lane, float (&a)[NX], float (&x)[NX]) {
-  const int j = lane < n ? lane : n - 1;
-#pragma unroll
-  for (int r = 0; r < n; ++r) x[r] = (r == j) ? 1.f : 0.f;
-  gj_steps<n>(a, x);
-}
 
 // G~_k^-1 = [ (Q_k + rho I)^-1 | (R_k + rho I)^-1 ] of every knot (the
 // last knot of an instance has no R block)

@@ -43,6 +43,7 @@ struct mpc_handle {
   uint32_t n_bins = 0, n_big = 0, max_big_n = 0;
   // two-workgroup PCG teams (128 < N <= 256)
   uint32_t *team_list = nullptr, *team_one = nullptr, *team_err = nullptr;
+  uint32_t *dir_order = nullptr;  // direct solve: instances longest first
   TeamMailbox *mbox = nullptr;
   uint32_t n_teams = 0, n_mbox = 256;
   // differentiable QP layer workspace: adjoint multipliers (14 T), per-knot
@@ -54,6 +55,7 @@ struct mpc_handle {
   // closed loop: reference window (6 max_knots), plant state (16), rho,
   // step counter, SQP status, PCG log of one step; a stream for capture
   float *cl_win = nullptr, *cl_xp = nullptr, *cl_rho = nullptr;
+  float *rho_one = nullptr;  // the plain driver's rho on the device (direct mode)
   uint32_t *cl_ctr = nullptr, *cl_status = nullptr, *cl_pcg = nullptr;
   hipStream_t cl_stream = nullptr;
   // pinned staging for the host round trips of the plain driver
@@ -98,12 +100,21 @@ static Params make_params(const mpc_handle *h) {
 
 static Batch single(uint32_t N) { return Batch{nullptr, nullptr, 1u, N}; }
 
+// iterative-refinement steps of the direct Schur solve, and of the KKT
+// solve (fp64 residual of the whole step) after it, in the SQP drivers: the
+// dz recovery G~^-1 (g - C^T lambda) cancels, so even a lambda at fp32
+// storage precision leaves dz off by up to 1/rho times that
+constexpr uint32_t kDirectRefine = 2, kDirectKktRefine = 2;
+
 // PCG of one instance: resident kernel when its Schur system fits one
 // workgroup, a two-workgroup team up to twice that, streaming otherwise.
 static hipError_t pcg_single(mpc_handle *h, uint32_t N, const float *S, const float *P, const float *gamma,
                              float *lambda, uint32_t max_iter, float tol, uint32_t *iters, uint32_t *flag,
                              hipStream_t st) {
   const uint32_t flags = h->cfg.flags;
+  if (flags & FLAG_DIRECT_SOLVE)
+    return launch_direct_solve(single(N), nullptr, 1, S, gamma, nullptr, const_cast<float *>(P), lambda,
+                               kDirectRefine, iters, flag, st);
   if (!(flags & FLAG_PCG_STREAM)) {
     if (N <= pcg_res_max_knots())
       return launch_pcg_res(single(N), nullptr, nullptr, 1, S, P, gamma, lambda, max_iter, tol, nullptr, iters,
@@ -122,6 +133,38 @@ static Batch batch_of(const mpc_handle *h) {
 
 static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_status, uint32_t max_iter, float tol,
                             hipEvent_t mid, hipStream_t st);
+
+// `steps` steps of iterative refinement of K [dz; lam] = [g; c] (constraint
+// row 0: dx_0 = 0; c == nullptr: c = 0) for the instances of layout b
+// (total knots T) on the workspace's G, C, G~^-1, S and Pinv, with
+// solve(lam, tol) the Schur solver of the layout; dz in the xu layout, lam
+// 14 per knot, both in place.
+template <class Solve>
+static hipError_t refine_kkt_with(mpc_handle *h, const Batch &b, uint32_t T, const float *d_rho, const float *g,
+                                  const float *c, float *dz, float *lam, uint32_t steps, float rel_tol, Solve solve,
+                                  hipStream_t st) {
+  const float tol = -(rel_tol > 0.f ? rel_tol : 1e-6f);  // negative: relative PCG exit
+  for (uint32_t s = 0; s < steps; ++s) {
+    hipError_t e = launch_kkt_residual(b, h->G, h->C, g, c, d_rho, dz, lam, h->ref_g, h->ref_c, st);
+    if (e == hipSuccess) e = launch_rhs_normalise(b, h->ref_g, h->ref_c, h->ref_scale, st);
+    if (e == hipSuccess) e = launch_form_gamma(b, h->Ginv, h->C, h->ref_g, h->ref_c, h->gamma, st);
+    if (e == hipSuccess) e = launch_fill(NX * T, 0.f, h->ref_lam, st);
+    if (e == hipSuccess) e = solve(h->ref_lam, tol);
+    if (e == hipSuccess) e = launch_compute_dz(b, h->Ginv, h->C, h->ref_g, h->ref_lam, nullptr, h->ref_dz, st);
+    if (e == hipSuccess) e = launch_refine_update(b, h->ref_scale, h->ref_dz, h->ref_lam, dz, lam, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// the same for the batch layout
+static hipError_t refine_kkt(mpc_handle *h, const float *d_rho, const float *g, const float *c, float *dz,
+                             float *lam, uint32_t steps, float rel_tol, hipStream_t st) {
+  return refine_kkt_with(h, batch_of(h), h->total, d_rho, g, c, dz, lam, steps, rel_tol,
+                         [&](float *l, float tol) { return pcg_batch(h, l, nullptr, h->cfg.pcg_max_iter, tol, nullptr, st); },
+                         st);
+}
+
 
 static bool valid_numbers(const mpc_config &c) {
   return c.dt > 0 && std::isfinite(c.dt) && c.q_ee >= 0 && c.qd_cost >= 0 && c.r_cost >= 0 &&
@@ -168,9 +211,10 @@ static void free_all(mpc_handle *h) {
   void *ptrs[] = {h->G, h->Ginv, h->C, h->g, h->c, h->S, h->P, h->gamma, h->dz, h->knot_merits,
                   h->merits, h->xs, h->drho, h->merit0, h->rho1, h->status_ws, h->sqp_done,
                   h->pcg_iters, h->pcg_flag, h->knot_inst, h->inst_k0, h->bin_items,
-                  h->bin_off, h->big_list, h->team_list, h->team_one, h->team_err, h->mbox,
+                  h->bin_off, h->big_list, h->team_list, h->team_one, h->team_err, h->mbox, h->dir_order,
                   h->adj_mu, h->adj_knot, h->loss_knot, h->ref_g, h->ref_c, h->ref_dz, h->ref_lam,
-                  h->ref_scale, h->cl_win, h->cl_xp, h->cl_rho, h->cl_ctr, h->cl_status, h->cl_pcg};
+                  h->ref_scale, h->cl_win, h->cl_xp, h->cl_rho, h->cl_ctr, h->cl_status, h->cl_pcg,
+                  h->rho_one};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (h->h_merits) (void)hipHostFree(h->h_merits);
@@ -224,6 +268,7 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
             alloc((void **)&h->bin_off, sizeof(uint32_t) * (B + 1)) &&
             alloc((void **)&h->big_list, sizeof(uint32_t) * B) &&
             alloc((void **)&h->team_list, sizeof(uint32_t) * B) &&
+            alloc((void **)&h->dir_order, sizeof(uint32_t) * B) &&
             alloc((void **)&h->team_one, sizeof(uint32_t)) &&
             alloc((void **)&h->team_err, sizeof(uint32_t)) &&
             alloc((void **)&h->mbox, sizeof(TeamMailbox) * h->n_mbox) &&
@@ -237,6 +282,7 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
             alloc((void **)&h->ref_scale, sizeof(float) * B) &&
             alloc((void **)&h->cl_win, sizeof(float) * 6 * cfg->max_knots) &&
             alloc((void **)&h->cl_xp, sizeof(float) * 16) && alloc((void **)&h->cl_rho, sizeof(float) * 4) &&
+            alloc((void **)&h->rho_one, sizeof(float) * 4) &&
             alloc((void **)&h->cl_ctr, sizeof(uint32_t) * 4) &&
             alloc((void **)&h->cl_status, sizeof(uint32_t) * 4) &&
             alloc((void **)&h->cl_pcg, sizeof(uint32_t) * MPC_MAX_SQP_ITERS) &&
@@ -362,6 +408,14 @@ int mpc_pcg_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_Pi
   return MPC_OK;
 }
 
+int mpc_direct_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_gamma, float *d_lambda,
+                     uint32_t refine_steps, float *d_work, mpc_stream_t stream) {
+  if (!h || !n_ok(h, N) || !d_S || !d_gamma || !d_lambda || !d_work || refine_steps > 16) return MPC_ERR_ARG;
+  HIP_TRY(launch_direct_solve(single(N), nullptr, 1, d_S, d_gamma, nullptr, d_work, d_lambda, refine_steps, nullptr,
+                              nullptr, (hipStream_t)stream));
+  return MPC_OK;
+}
+
 int mpc_compute_dz(mpc_handle *h, uint32_t N, const float *d_Ginv, const float *d_C,
                    const float *d_g, const float *d_lambda, float *d_dz, mpc_stream_t stream) {
   if (!h || !n_ok(h, N) || !d_Ginv || !d_C || !d_g || !d_lambda || !d_dz) return MPC_ERR_ARG;
@@ -428,6 +482,16 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
     HIP_TRY(hipMemcpyAsync(h->h_pcg, h->pcg_iters, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(h->h_pcg + 1, h->pcg_flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, nullptr, h->dz, st));
+    if (cfg.flags & FLAG_DIRECT_SOLVE) {
+      h->h_rho[3] = *rho;
+      HIP_TRY(hipMemcpyAsync(h->rho_one, h->h_rho + 3, sizeof(float), hipMemcpyHostToDevice, st));
+      HIP_TRY(refine_kkt_with(h, b, N, h->rho_one, h->g, h->c, h->dz, d_lambda, kDirectKktRefine, 0.f,
+                              [&](float *l, float tol) {
+                                return pcg_single(h, N, h->S, h->P, h->gamma, l, cfg.pcg_max_iter, tol, nullptr,
+                                                  nullptr, st);
+                              },
+                              st));
+    }
     HIP_TRY(launch_merit(b, prm, d_xu, h->dz, d_ee_ref, h->xs, nullptr, 0, 8, h->knot_merits, st));
     HIP_TRY(launch_merit_reduce(b, h->knot_merits, 8, h->merits, st));
     HIP_TRY(hipMemcpyAsync(h->h_merits, h->merits, sizeof(float) * 8, hipMemcpyDeviceToHost, st));
@@ -505,6 +569,12 @@ int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
     }
   }
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return horizons[a] > horizons[c]; });
+  {
+    std::vector<uint32_t> all(n_inst);
+    for (uint32_t i = 0; i < n_inst; ++i) all[i] = i;
+    std::stable_sort(all.begin(), all.end(), [&](uint32_t a, uint32_t c) { return horizons[a] > horizons[c]; });
+    HIP_TRY(hipMemcpy(h->dir_order, all.data(), sizeof(uint32_t) * n_inst, hipMemcpyHostToDevice));
+  }
   struct BinB { uint32_t used = 0; std::vector<uint32_t> inst; };
   std::vector<BinB> bins;
   for (uint32_t i : order) {
@@ -554,26 +624,6 @@ int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
 }
 
 // ---- differentiable QP layer --------------------------------------------------
-
-// refine_steps steps of iterative refinement of K [dz; lam] = [g; c]
-// (constraint row 0: dx_0 = 0; c == nullptr: c = 0) on the workspace's G, C,
-// G~^-1, S and Pinv; dz in the xu layout, lam 14 per knot, both in place.
-static hipError_t refine_kkt(mpc_handle *h, const float *d_rho, const float *g, const float *c, float *dz,
-                             float *lam, uint32_t steps, float rel_tol, hipStream_t st) {
-  const Batch b = batch_of(h);
-  const float tol = -(rel_tol > 0.f ? rel_tol : 1e-6f);  // negative: relative PCG exit
-  for (uint32_t s = 0; s < steps; ++s) {
-    hipError_t e = launch_kkt_residual(b, h->G, h->C, g, c, d_rho, dz, lam, h->ref_g, h->ref_c, st);
-    if (e == hipSuccess) e = launch_rhs_normalise(b, h->ref_g, h->ref_c, h->ref_scale, st);
-    if (e == hipSuccess) e = launch_form_gamma(b, h->Ginv, h->C, h->ref_g, h->ref_c, h->gamma, st);
-    if (e == hipSuccess) e = launch_fill(NX * h->total, 0.f, h->ref_lam, st);
-    if (e == hipSuccess) e = pcg_batch(h, h->ref_lam, nullptr, h->cfg.pcg_max_iter, tol, nullptr, st);
-    if (e == hipSuccess) e = launch_compute_dz(b, h->Ginv, h->C, h->ref_g, h->ref_lam, nullptr, h->ref_dz, st);
-    if (e == hipSuccess) e = launch_refine_update(b, h->ref_scale, h->ref_dz, h->ref_lam, dz, lam, st);
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
-}
 
 int mpc_batch_qp_layer(mpc_handle *h, const float *d_xu, const float *d_ee_ref, const float *d_rho,
                        float *d_lambda, float pcg_tol, uint32_t refine_steps, float refine_tol, float *d_dz,
@@ -664,6 +714,12 @@ static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_st
                             hipEvent_t mid, hipStream_t st) {
   const Batch b = batch_of(h);
   hipError_t e;
+  if (h->cfg.flags & FLAG_DIRECT_SOLVE) {
+    e = launch_direct_solve(b, h->dir_order, h->n_inst, h->S, h->gamma, d_status, h->P, d_lambda, kDirectRefine,
+                            h->pcg_iters, h->pcg_flag, st);
+    if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
+    return e;
+  }
   if (h->cfg.flags & FLAG_PCG_STREAM) {
     e = launch_pcg_stream(b, h->max_n, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, nullptr, 0,
                           h->pcg_iters, h->pcg_flag, st);
@@ -709,6 +765,8 @@ static hipError_t batch_sqp_run(mpc_handle *h, float *d_xu, float *d_lambda, con
     MPC_RUN(pcg_batch(h, d_lambda, d_status, h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, tm ? h->ev[3] : nullptr, st));
     if (tm) MPC_RUN(hipEventRecord(h->ev[4], st));
     MPC_RUN(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
+    if (h->cfg.flags & FLAG_DIRECT_SOLVE)
+      MPC_RUN(refine_kkt(h, d_rho, h->g, h->c, h->dz, d_lambda, kDirectKktRefine, 0.f, st));
     if (tm) MPC_RUN(hipEventRecord(h->ev[5], st));
     MPC_RUN(launch_merit(b, prm, d_xu, h->dz, d_ee_ref, h->xs, d_status, 0, 8, h->knot_merits, st));
     if (tm) MPC_RUN(hipEventRecord(h->ev[6], st));
@@ -764,7 +822,8 @@ int mpc_closed_loop(mpc_handle *h, const mpc_closed_loop_config *cl, uint32_t N,
   const uint32_t horizon[1] = {N};
   int rc = mpc_batch_setup(h, 1, horizon);
   if (rc != MPC_OK) return rc;
-  if (cl->use_graph && h->n_teams) return MPC_ERR_ARG;  // cooperative launches are not captured
+  // cooperative launches (PCG teams) are not captured
+  if (cl->use_graph && h->n_teams && !(h->cfg.flags & FLAG_DIRECT_SOLVE)) return MPC_ERR_ARG;
   hipStream_t user = (hipStream_t)stream;
   hipStream_t st = cl->use_graph ? h->cl_stream : user;
   const ClStep step{N, cl->substeps, cl->sqp_iters, cl->sim_dt, cl->noise_sigma, cl->noise_seed};

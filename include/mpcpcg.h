@@ -80,6 +80,11 @@ typedef struct mpc_config {
 /* Always use the streaming PCG kernel (S read from memory every iteration)
  * instead of the on-chip resident kernel; for kernel comparisons. */
 #define MPC_FLAG_PCG_STREAM 2u
+/* Solve the Schur system directly (block LDL^T of -S on the GPU, exact up
+ * to fp32 rounding) instead of by PCG, in the plain and batched SQP and the
+ * QP layer: the GPU counterpart of the reference's QDLDL path
+ * (PCG_SOLVE = 0, toplevel.cuh:160-215). */
+#define MPC_FLAG_DIRECT_SOLVE 4u
 
 typedef struct mpc_handle mpc_handle;
 
@@ -155,6 +160,15 @@ int mpc_pcg_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_Pi
                   const float *d_gamma, float *d_lambda, uint32_t max_iter, float exit_tol,
                   uint32_t *d_iters, uint32_t *d_max_iter_exit, mpc_stream_t stream);
 
+/* Direct solve of S lambda = gamma (one instance of N knots) by block LDL^T
+ * of -S on the GPU, then refine_steps (<= 16) steps of iterative refinement
+ * with an fp64 residual; d_work: 588 N floats of scratch (the layout of S:
+ * the elimination factors and a vector per block row).  GPU counterpart of
+ * the reference's QDLDL solve (qdldl_helper.cuh:62-180, toplevel.cuh:314-335);
+ * the SQP drivers use 2 refinement steps. */
+int mpc_direct_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_gamma, float *d_lambda,
+                     uint32_t refine_steps, float *d_work, mpc_stream_t stream);
+
 /* dz from G^-1, C, g, lambda.  compute_dz (schur.cuh:562-566). */
 int mpc_compute_dz(mpc_handle *h, uint32_t N, const float *d_Ginv, const float *d_C,
                    const float *d_g, const float *d_lambda, float *d_dz, mpc_stream_t stream);
@@ -223,8 +237,8 @@ int mpc_batch_phase_times(mpc_handle *h, double *us7);
  * out: last plan), d_lambda (14N, in/out), d_states (14 (steps + 1), out:
  * plant states), d_ee_err (2 steps, out: Euclidean and L1 norm of the xyz
  * error per step), d_pcg_iters (steps x sqp_iters, out, may be NULL).  use_graph captures
- * one control step in a HIP graph and replays it (N <= 128 only, where the
- * PCG has no cooperative launch); timing adds per-phase HIP event timing
+ * one control step in a HIP graph and replays it (N <= 128 with PCG, where
+ * the PCG has no cooperative launch; any N with MPC_FLAG_DIRECT_SOLVE); timing adds per-phase HIP event timing
  * (one synchronisation per step, no graph). */
 typedef struct mpc_closed_loop_config {
   uint32_t steps;
