@@ -326,6 +326,31 @@ def main():
                          f"one instance per horizon at its initial iterate, weighted by the workload's horizon mix; "
                          f"{desc}; ~{args.cpu_seconds:.0f} s of CPU work"}
 
+    # the same workload with the direct Schur solve (block LDL^T on the GPU,
+    # one fp64-residual refinement of each KKT step) instead of PCG
+    solver.set_config(flags=pkg.FLAG_DIRECT_SOLVE)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_direct = e0.elapsed_time(e1) / args.steps
+    solver.batch_set_timing(True)
+    step()
+    torch.cuda.synchronize()
+    ph_d = solver.batch_phase_times()
+    solver.batch_set_timing(False)
+    st_d = status.cpu().numpy()
+    kkt_d = int((pcg_log.cpu().numpy() >= 0).sum())
+    direct = {"kkt_solves_per_sec": round(kkt_d / (ms_direct * 1e-3), 1), "ms_per_step": round(ms_direct, 4),
+              "phase_us_per_sqp_iter": {name: round(v / K, 2) for name, v in zip(PHASES, ph_d)},
+              "status": {"aborted": int((st_d & 4 != 0).sum()), "nonfinite": int((st_d & 8 != 0).sum())},
+              "note": "phase 'pcg_team' holds the direct solve, 'dz' the dz recovery and its KKT refinement"}
+    solver.set_config(flags=0)
+
     diff = None if args.no_diff else differentiation(pkg, torch, args.seed)
     closed = None if args.no_closed_loop else closed_loop(pkg, torch, args.seed)
 
@@ -368,6 +393,7 @@ def main():
                      "fp32": {"achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": round(tflops / FP32_PEAK_TFLOPS, 4)}},
         "cpu_baseline": cpu,
+        "direct_solver": direct,
         "differentiation": diff,
         "closed_loop": closed,
         "wall_s_timed": round(t_wall, 3),

@@ -64,7 +64,8 @@ class Config(C.Structure):
         ("dt", C.c_float), ("q_ee", C.c_float), ("qd_cost", C.c_float), ("r_cost", C.c_float),
         ("mu", C.c_float), ("rho_factor", C.c_float), ("rho_max", C.c_float), ("rho_min", C.c_float),
         ("rho_reset", C.c_float), ("pcg_exit_tol", C.c_float), ("pcg_max_iter", C.c_uint32),
-        ("sqp_max_iter", C.c_uint32), ("flags", C.c_uint32),
+        ("sqp_max_iter", C.c_uint32), ("flags", C.c_uint32), ("direct_refine", C.c_uint32),
+        ("direct_kkt_refine", C.c_uint32),
     ]
 
 

@@ -156,10 +156,11 @@ hipError_t launch_fill(uint32_t n, float v, float *o, hipStream_t st);
 // direct block-tridiagonal solve of the Schur system (k_direct.hip); order:
 // instance indices longest first (NULL: instance = list index); work: the
 // factors E_b, D_b^-1 and a vector per knot in each knot's S_STRIDE slot;
-// refine: iterative-refinement steps with an fp64 residual
+// refine: iterative-refinement steps with an fp64 residual; do_factor =
+// false reuses the factors a previous call left in work for the same S
 hipError_t launch_direct_solve(const Batch &b, const uint32_t *order, uint32_t n_list, const float *S,
                                const float *gamma, const uint32_t *status, float *work, float *lambda,
-                               uint32_t refine, uint32_t *iters, uint32_t *flag, hipStream_t st);
+                               uint32_t refine, bool do_factor, uint32_t *iters, uint32_t *flag, hipStream_t st);
 // closed-loop harness (k_closed_loop.hip)
 hipError_t launch_cl_init(uint32_t N, const float *x0, const float *path, float *win, float *xu, float *xp,
                           uint32_t *ctr, float *states, float *rho, float rho0, hipStream_t st);

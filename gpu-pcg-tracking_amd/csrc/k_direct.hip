@@ -2,25 +2,32 @@
 // instance by block LDL^T (block Thomas) of the symmetric positive definite
 // A = -S (gfx950).
 //
-// One 16-lane group per instance, 4 groups per 64-lane workgroup; the
-// instances are taken longest first, so the groups of a wave run similar
-// trip counts.  Lane j holds column j of the current 14x14 blocks.  Forward
-// sweep over the knots (A_b = -S(b,b), B_b = -S(b,b-1), r = -gamma):
-//   E_b = B_b D_{b-1}^-1,   D_b = A_b - E_b B_b^T,
-//   y_b = r_b - E_b y_{b-1},   z_b = D_b^-1 y_b        (D_0 = A_0, y_0 = r_0)
+// Factorisation (A_b = -S(b,b), B_b = -S(b,b-1)):
+//   D_0 = A_0,   E_b = B_b D_{b-1}^-1,   D_b = A_b - E_b B_b^T,
 // with D_b^-1 by Gauss-Jordan in registers (D_b is SPD: no pivoting); E_b
-// and D_b^-1 go to the workspace for the backward sweep
-//   x_{N-1} = z_{N-1},   x_b = z_b - E_{b+1}^T x_{b+1},   lambda = x
-// and for `refine` steps of iterative refinement: the residual
-// r = -gamma + S lambda of A lambda = -gamma in fp64 from the fp32 blocks,
-// the same two sweeps on it with the stored factors, lambda += x.  The fp32
-// elimination alone is off by about cond(S) x eps (1e-4..1e-3 relative on
-// these systems); each step multiplies that error by about the same
-// factor, down to the fp32 storage of lambda.
-// The next knot's blocks are loaded while the current one is processed.
+// and D_b^-1 are stored in the workspace (the S layout: E_b in the left
+// block slot, D_b^-1 in the diagonal slot, a vector in the right slot).
+// Solve of A x = r:
+//   forward   y_b = r_b - E_b y_{b-1},   z_b = D_b^-1 y_b
+//   backward  x_{N-1} = z_{N-1},   x_b = z_b - E_{b+1}^T x_{b+1}
+// with r = -gamma, then `refine` steps of iterative refinement: the
+// residual -gamma + S lambda in fp64 from the fp32 blocks, the same solve on
+// it, lambda += x.  The fp32 elimination alone is off by about cond(S) x eps
+// (1e-4..1e-2 relative on these systems); each step multiplies the error by
+// about that factor, down to the fp32 storage of lambda.
+//
+// Mapping: one 16-lane group per instance, 4 groups per 64-lane workgroup
+// (one wave), instances longest first.  The factorisation stages B_b and E_b
+// in LDS (every group runs the wave's longest trip count; wave-level ordering
+// suffices); the solve sweeps keep one vector element per lane and broadcast the
+// vector inside the group by ds_swizzle, so they need no LDS and no barrier,
+// and each loads the next knot's factor rows ahead.  The residual is formed
+// for all knots of all instances at once, one group per knot.
 //
 // Reference: the QDLDL path of sqpSolve (PCG_SOLVE = 0, toplevel.cuh:160-215,
 // :314-335) solves the same system by a sparse LDL^T on the host.
+#include <utility>
+
 #include "common.h"
 #include "gj16.h"
 
@@ -33,105 +40,84 @@ constexpr int GROUPS_PER_WG = 4;
 constexpr int WG = GROUP * GROUPS_PER_WG;
 constexpr int BPL = (NXX + GROUP - 1) / GROUP;  // B entries per lane (13)
 
-struct DirLds {
-  float B[NXX];  // B_b (column-major); refinement: D_b^-1
+struct FacLds {
+  float B[NXX];  // B_b (column-major)
   float E[NXX];  // E_b (column-major)
-  float y[16];   // y_{b-1}, then y_b
-  float x[16];   // backward sweep: x_{b+1}
 };
 
-// workspace slots of knot row (S layout, 588 floats per knot)
 __device__ __forceinline__ float *slot_E(float *work, size_t row) { return work + (size_t)S_STRIDE * row; }
 __device__ __forceinline__ float *slot_Dinv(float *work, size_t row) { return work + (size_t)S_STRIDE * row + NXX; }
 __device__ __forceinline__ float *slot_z(float *work, size_t row) { return work + (size_t)S_STRIDE * row + 2 * NXX; }
 
-struct KnotLoad {
-  float a[NX];   // column j of A_b
-  float b[BPL];  // lane's share of B_b
-  float r;       // r_b, element j
+// LDS ordering among the lanes of the (single-wave) workgroup: the wave's
+// LDS accesses execute in order, so only compiler reordering must be
+// prevented; unlike __syncthreads() this does not wait for outstanding
+// global loads (the next knot's prefetch)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// sum_m w[m] v_m, v_m the value v of lane m of the group (m < 14), in order
+template <int... M>
+__device__ __forceinline__ float bdot_seq(const float (&w)[NX], float v, std::integer_sequence<int, M...>) {
+  float s = 0.f;
+  ((s += w[M] * bcast16<M>(v)), ...);
+  return s;
+}
+__device__ __forceinline__ float bdot(const float (&w)[NX], float v) {
+  return bdot_seq(w, v, std::make_integer_sequence<int, NX>{});
+}
+
+// instance of list slot `slot` and the workgroup's longest horizon
+struct Pick {
+  uint32_t inst, K0, N, Nmax;
+  bool live;
 };
 
-__device__ __forceinline__ void load_knot(KnotLoad &o, const float *__restrict__ S, const float *__restrict__ gamma,
-                                          size_t row, bool first, int lane, int j) {
-  const float *Sr = S + (size_t)S_STRIDE * row;
-#pragma unroll
-  for (int r = 0; r < NX; ++r) o.a[r] = -Sr[NXX + r + NX * j];
-#pragma unroll
-  for (int t = 0; t < BPL; ++t) {
-    const int i = lane + GROUP * t;
-    o.b[t] = (!first && i < NXX) ? -Sr[i] : 0.f;
-  }
-  o.r = -gamma[(size_t)NX * row + j];
-}
-
-// x_{N-1} = z_{N-1}, x_b = z_b - E_{b+1}^T x_{b+1} (every group runs Nmax
-// steps so the barriers line up).  zsrc == nullptr: z_b is in lambda and
-// lambda <- x; else z_b is in the workspace's z slot and lambda += x.
-__device__ __forceinline__ void backward(DirLds &L, float *__restrict__ work, float *__restrict__ lambda,
-                                         const float *zsrc, uint32_t K0, uint32_t N, uint32_t Nmax, bool live,
-                                         int lane) {
-  if (lane < 16) L.x[lane] = 0.f;
-  __syncthreads();
-  for (int k = (int)Nmax - 1; k >= 0; --k) {
-    const bool on = live && (uint32_t)k < N;
-    float xj = 0.f;
-    if (on && lane < NX) {
-      const size_t row = (size_t)K0 + (uint32_t)k;
-      xj = zsrc ? slot_z(work, row)[lane] : lambda[NX * row + lane];
-      if ((uint32_t)k + 1 < N) {
-        const float *Ec = slot_E(work, row + 1) + NX * lane;  // column j of E_{k+1}
-#pragma unroll
-        for (int r = 0; r < NX; ++r) xj -= Ec[r] * L.x[r];
-      }
-    }
-    __syncthreads();
-    if (on && lane < NX) {
-      L.x[lane] = xj;
-      float *l = lambda + NX * ((size_t)K0 + (uint32_t)k) + lane;
-      *l = zsrc ? *l + xj : xj;
-    }
-    __syncthreads();
-  }
-}
-
-__global__ void __launch_bounds__(WG) k_direct_solve(Batch b, const uint32_t *__restrict__ order, uint32_t n_list,
-                                                     const float *__restrict__ S, const float *__restrict__ gamma,
-                                                     const uint32_t *__restrict__ status, float *__restrict__ work,
-                                                     float *__restrict__ lambda, uint32_t refine,
-                                                     uint32_t *__restrict__ iters, uint32_t *__restrict__ flag) {
-  __shared__ DirLds lds[GROUPS_PER_WG];
-  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
-  const uint32_t slot = blockIdx.x * GROUPS_PER_WG + grp;
+__device__ __forceinline__ Pick pick_instance(const Batch &b, const uint32_t *order, uint32_t n_list,
+                                              const uint32_t *status, uint32_t slot) {
+  Pick p;
   const bool in_list = slot < n_list;
-  const uint32_t pick = in_list ? slot : n_list - 1;
-  const uint32_t inst = order ? order[pick] : pick;
-  const uint32_t K0 = inst_first(b, inst), N = inst_len(b, inst);
-  const bool live = in_list && !(status && !(status[inst] & ST_ACTIVE));
-  // the workgroup's first group holds its longest instance
-  const uint32_t first = blockIdx.x * GROUPS_PER_WG;
-  const uint32_t Nmax = inst_len(b, order ? order[first] : first);
+  const uint32_t s = in_list ? slot : n_list - 1;
+  p.inst = order ? order[s] : s;
+  p.K0 = inst_first(b, p.inst);
+  p.N = inst_len(b, p.inst);
+  p.live = in_list && !(status && !(status[p.inst] & ST_ACTIVE));
+  const uint32_t first = (slot / GROUPS_PER_WG) * GROUPS_PER_WG;
+  p.Nmax = inst_len(b, order ? order[first] : first);
+  return p;
+}
+
+// factorisation sweep of one instance: E_b, D_b^-1 into the workspace
+__device__ __forceinline__ void factor(FacLds &L, const float *__restrict__ S, float *__restrict__ work, const Pick &p,
+                                      int lane) {
   const int j = lane < NX ? lane : NX - 1;
-  DirLds &L = lds[grp];
-  if (lane < 16) {
-    L.y[lane] = 0.f;
-    L.x[lane] = 0.f;
-  }
   float dinv[NX];  // column j of D_{b-1}^-1
 #pragma unroll
   for (int r = 0; r < NX; ++r) dinv[r] = 0.f;
-  KnotLoad cur, nxt;
-  load_knot(cur, S, gamma, K0, true, lane, j);
-
-  for (uint32_t k = 0; k < Nmax; ++k) {
-    const bool on = live && k < N;
-    const uint32_t kn = k + 1 < N ? k + 1 : N - 1;  // next knot (clamped)
-    load_knot(nxt, S, gamma, (size_t)K0 + kn, kn == 0, lane, j);
+  float a_cur[NX], b_cur[BPL], a_nxt[NX], b_nxt[BPL];
+  auto load = [&](float (&a)[NX], float (&bb)[BPL], uint32_t k) {
+    const float *Sr = S + (size_t)S_STRIDE * (p.K0 + k);
+#pragma unroll
+    for (int r = 0; r < NX; ++r) a[r] = -Sr[NXX + r + NX * j];
 #pragma unroll
     for (int t = 0; t < BPL; ++t) {
       const int i = lane + GROUP * t;
-      if (i < NXX) L.B[i] = cur.b[t];
+      bb[t] = (k > 0 && i < NXX) ? -Sr[i] : 0.f;
     }
-    __syncthreads();
+  };
+  load(a_cur, b_cur, 0);
+  for (uint32_t k = 0; k < p.Nmax; ++k) {
+    const bool on = p.live && k < p.N;
+    load(a_nxt, b_nxt, k + 1 < p.N ? k + 1 : p.N - 1);
+#pragma unroll
+    for (int t = 0; t < BPL; ++t) {
+      const int i = lane + GROUP * t;
+      if (i < NXX) L.B[i] = b_cur[t];
+    }
+    wave_sync();
     // E_b (column j) = sum_m B(:, m) D_{b-1}^-1(m, j)
     float e[NX];
 #pragma unroll
@@ -145,101 +131,185 @@ __global__ void __launch_bounds__(WG) k_direct_solve(Batch b, const uint32_t *__
     if (lane < NX) {
 #pragma unroll
       for (int r = 0; r < NX; ++r) L.E[r + NX * lane] = e[r];
-      if (on && k > 0) {
-        float *w = slot_E(work, K0 + k) + NX * lane;
+      if (on) {
+        float *w = slot_E(work, p.K0 + k) + NX * lane;
 #pragma unroll
         for (int r = 0; r < NX; ++r) w[r] = e[r];
       }
     }
-    __syncthreads();
-    // D_b (column j) = A_b(:, j) - sum_m E(:, m) B(j, m);  y_b(j) = r_j - sum_m E(j, m) y_{b-1}(m)
+    wave_sync();
+    // D_b (column j) = A_b(:, j) - sum_m E(:, m) B(j, m)
     float a[NX];
 #pragma unroll
-    for (int r = 0; r < NX; ++r) a[r] = cur.a[r];
-    float yj = cur.r;
+    for (int r = 0; r < NX; ++r) a[r] = a_cur[r];
 #pragma unroll
     for (int m = 0; m < NX; ++m) {
       const float bjm = L.B[j + NX * m];
 #pragma unroll
       for (int r = 0; r < NX; ++r) a[r] -= L.E[r + NX * m] * bjm;
-      yj -= L.E[j + NX * m] * L.y[m];
     }
     pin(a);
     gj_inverse<NX>(lane, a, dinv);
     if (on && lane < NX) {
-      float *w = slot_Dinv(work, K0 + k) + NX * lane;
+      float *w = slot_Dinv(work, p.K0 + k) + NX * lane;
 #pragma unroll
       for (int r = 0; r < NX; ++r) w[r] = dinv[r];
     }
-    __syncthreads();  // every read of y_{b-1}, B and E is done
-    if (lane < NX) L.y[lane] = yj;
-    __syncthreads();
-    // z_b(j) = sum_m D_b^-1(j, m) y_b(m); D_b^-1 is symmetric, so row j is the lane's column
-    float z = 0.f;
+    wave_sync();  // every read of B and E is done before the next knot overwrites them
 #pragma unroll
-    for (int m = 0; m < NX; ++m) z += dinv[m] * L.y[m];
-    if (on && lane < NX) lambda[(size_t)NX * (K0 + k) + lane] = z;
-    cur = nxt;
+    for (int r = 0; r < NX; ++r) a_cur[r] = a_nxt[r];
+#pragma unroll
+    for (int t = 0; t < BPL; ++t) b_cur[t] = b_nxt[t];
   }
+}
 
-  // backward sweep: x_b = z_b - E_{b+1}^T x_{b+1}, z_b in lambda
-  backward(L, work, lambda, nullptr, K0, N, Nmax, live, lane);
-
-  for (uint32_t pass = 0; pass < refine; ++pass) {
-    // forward: r_b = -gamma_b + (S lambda)_b (fp64), y_b = r_b - E_b y_{b-1}, z_b = D_b^-1 y_b
-    if (lane < 16) L.y[lane] = 0.f;
-    for (uint32_t k = 0; k < Nmax; ++k) {
-      const bool on = live && k < N;
-      const uint32_t kk = k < N ? k : N - 1;
-      const size_t row = (size_t)K0 + kk;
-      const float *Sr = S + (size_t)S_STRIDE * row;
-      double res = -(double)gamma[(size_t)NX * row + j];
+// forward and backward sweeps of one instance on the right-hand side
+// r = -gamma (gamma != nullptr) or the workspace's z slot; lambda = x, or
+// lambda += x with accumulate.  No barriers: the group's lanes share its
+// trip count.
+__device__ __forceinline__ void substitute(const float *__restrict__ gamma, float *__restrict__ work,
+                                          float *__restrict__ lambda, const Pick &p, bool accumulate, int lane) {
+  const int j = lane < NX ? lane : NX - 1;
+  const bool wr = p.live && lane < NX;
+  auto rhs = [&](uint32_t k) {
+    return gamma ? -gamma[(size_t)NX * (p.K0 + k) + j] : slot_z(work, p.K0 + k)[j];
+  };
+  // row j of E_k (strided) and of D_k^-1 (= column j, D^-1 symmetric)
+  auto load_rows = [&](float (&er)[NX], float (&dr)[NX], uint32_t k) {
+    const float *E = slot_E(work, p.K0 + k), *D = slot_Dinv(work, p.K0 + k) + NX * j;
 #pragma unroll
-      for (int m = 0; m < NX; ++m) res += (double)Sr[NXX + j + NX * m] * (double)lambda[(size_t)NX * row + m];
-      if (kk > 0) {
-#pragma unroll
-        for (int m = 0; m < NX; ++m) res += (double)Sr[j + NX * m] * (double)lambda[(size_t)NX * (row - 1) + m];
-      }
-      if (kk + 1 < N) {
-#pragma unroll
-        for (int m = 0; m < NX; ++m)
-          res += (double)Sr[2 * NXX + j + NX * m] * (double)lambda[(size_t)NX * (row + 1) + m];
-      }
-      const float *Eg = slot_E(work, row), *Dg = slot_Dinv(work, row);
-      for (int i = lane; i < NXX; i += GROUP) {
-        L.E[i] = kk > 0 ? Eg[i] : 0.f;
-        L.B[i] = Dg[i];
-      }
-      __syncthreads();
-      float yj = (float)res;
-#pragma unroll
-      for (int m = 0; m < NX; ++m) yj -= L.E[j + NX * m] * L.y[m];
-      __syncthreads();
-      if (lane < NX) L.y[lane] = yj;
-      __syncthreads();
-      float z = 0.f;
-#pragma unroll
-      for (int m = 0; m < NX; ++m) z += L.B[m + NX * j] * L.y[m];
-      if (on && lane < NX) slot_z(work, row)[lane] = z;
-      __syncthreads();
+    for (int m = 0; m < NX; ++m) {
+      er[m] = k > 0 ? E[j + NX * m] : 0.f;
+      dr[m] = D[m];
     }
-    backward(L, work, lambda, work, K0, N, Nmax, live, lane);
+  };
+  // two buffers, knot k in A when k is even: each knot's rows are loaded two
+  // knots ahead and only waited for where they are used
+  float erA[NX], drA[NX], erB[NX], drB[NX];
+  float rA = rhs(0), rB = 0.f, y = 0.f;
+  load_rows(erA, drA, 0);
+  if (p.N > 1) {
+    load_rows(erB, drB, 1);
+    rB = rhs(1);
   }
-  if (live && lane == 0) {
-    if (iters) iters[inst] = 0;
-    if (flag) flag[inst] = 0;
+  auto step = [&](const float (&er)[NX], const float (&dr)[NX], float r, uint32_t k) {
+    const float yk = r - bdot(er, y);
+    const float zk = bdot(dr, yk);
+    if (wr) slot_z(work, p.K0 + k)[lane] = zk;
+    y = yk;
+  };
+  for (uint32_t k = 0; k < p.N; k += 2) {
+    step(erA, drA, rA, k);
+    if (k + 2 < p.N) {
+      load_rows(erA, drA, k + 2);
+      rA = rhs(k + 2);
+    }
+    if (k + 1 >= p.N) break;
+    step(erB, drB, rB, k + 1);
+    if (k + 3 < p.N) {
+      load_rows(erB, drB, k + 3);
+      rB = rhs(k + 3);
+    }
   }
+  // backward: column j of E_{k+1} (contiguous), double-buffered the same way
+  const float *zs = slot_z(work, p.K0);
+  float ecA[NX], ecB[NX];
+  float zA = zs[(size_t)S_STRIDE * (p.N - 1) + j], zB = 0.f, x = 0.f;
+  auto load_col = [&](float (&ec)[NX], float &z, int k) {  // E_{k+1} column, z_k
+    const float *E = slot_E(work, p.K0 + (uint32_t)k + 1) + NX * j;
+#pragma unroll
+    for (int m = 0; m < NX; ++m) ec[m] = (uint32_t)k + 1 < p.N ? E[m] : 0.f;
+    z = zs[(size_t)S_STRIDE * (uint32_t)k + j];
+  };
+#pragma unroll
+  for (int m = 0; m < NX; ++m) ecA[m] = 0.f;
+  if (p.N > 1) load_col(ecB, zB, (int)p.N - 2);
+  auto bstep = [&](const float (&ec)[NX], float z, int k) {
+    const float xk = z - bdot(ec, x);
+    if (wr) {
+      float *l = lambda + (size_t)NX * (p.K0 + (uint32_t)k) + lane;
+      *l = accumulate ? *l + xk : xk;
+    }
+    x = xk;
+  };
+  for (int k = (int)p.N - 1; k >= 0; k -= 2) {
+    bstep(ecA, zA, k);
+    if (k - 2 >= 0) load_col(ecA, zA, k - 2);
+    if (k - 1 < 0) break;
+    bstep(ecB, zB, k - 1);
+    if (k - 3 >= 0) load_col(ecB, zB, k - 3);
+  }
+}
+
+__global__ void __launch_bounds__(WG) k_direct_factor_solve(Batch b, const uint32_t *__restrict__ order,
+                                                            uint32_t n_list, const float *__restrict__ S,
+                                                            const float *__restrict__ gamma,
+                                                            const uint32_t *__restrict__ status,
+                                                            float *__restrict__ work, float *__restrict__ lambda,
+                                                            int do_factor, uint32_t *__restrict__ iters,
+                                                            uint32_t *__restrict__ flag) {
+  __shared__ FacLds lds[GROUPS_PER_WG];
+  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
+  const Pick p = pick_instance(b, order, n_list, status, blockIdx.x * GROUPS_PER_WG + grp);
+  if (do_factor) factor(lds[grp], S, work, p, lane);
+  substitute(gamma, work, lambda, p, false, lane);
+  if (p.live && lane == 0) {
+    if (iters) iters[p.inst] = 0;
+    if (flag) flag[p.inst] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(WG) k_direct_refine(Batch b, const uint32_t *__restrict__ order, uint32_t n_list,
+                                                      const uint32_t *__restrict__ status, float *__restrict__ work,
+                                                      float *__restrict__ lambda) {
+  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
+  const Pick p = pick_instance(b, order, n_list, status, blockIdx.x * GROUPS_PER_WG + grp);
+  substitute(nullptr, work, lambda, p, true, lane);
+}
+
+// z slot of every knot <- -gamma + S lambda (fp64 accumulation), one group per knot
+__global__ void __launch_bounds__(WG) k_schur_residual(Batch b, const float *__restrict__ S,
+                                                       const float *__restrict__ gamma,
+                                                       const float *__restrict__ lambda, float *__restrict__ work) {
+  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
+  const uint32_t gk = blockIdx.x * GROUPS_PER_WG + grp;
+  if (gk >= b.total_knots || lane >= NX) return;
+  const KnotRef kr = locate(b, gk);
+  const float *Sr = S + (size_t)S_STRIDE * gk;
+  const float *l = lambda + (size_t)NX * gk;
+  double res = -(double)gamma[(size_t)NX * gk + lane];
+#pragma unroll
+  for (int m = 0; m < NX; ++m) res += (double)Sr[NXX + lane + NX * m] * (double)l[m];
+  if (kr.k > 0) {
+#pragma unroll
+    for (int m = 0; m < NX; ++m) res += (double)Sr[lane + NX * m] * (double)l[m - NX];
+  }
+  if (kr.k + 1 < kr.N) {
+#pragma unroll
+    for (int m = 0; m < NX; ++m) res += (double)Sr[2 * NXX + lane + NX * m] * (double)l[m + NX];
+  }
+  slot_z(work, gk)[lane] = (float)res;
 }
 
 }  // namespace
 
 hipError_t launch_direct_solve(const Batch &b, const uint32_t *order, uint32_t n_list, const float *S,
                                const float *gamma, const uint32_t *status, float *work, float *lambda,
-                               uint32_t refine, uint32_t *iters, uint32_t *flag, hipStream_t st) {
+                               uint32_t refine, bool do_factor, uint32_t *iters, uint32_t *flag, hipStream_t st) {
   if (n_list == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_direct_solve, dim3((n_list + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG), 0, st, b, order,
-                     n_list, S, gamma, status, work, lambda, refine, iters, flag);
-  return hipGetLastError();
+  const dim3 grid((n_list + GROUPS_PER_WG - 1) / GROUPS_PER_WG);
+  hipLaunchKernelGGL(k_direct_factor_solve, grid, dim3(WG), 0, st, b, order, n_list, S, gamma, status, work, lambda,
+                     do_factor ? 1 : 0, iters, flag);
+  hipError_t e = hipGetLastError();
+  for (uint32_t s = 0; s < refine && e == hipSuccess; ++s) {
+    hipLaunchKernelGGL(k_schur_residual, dim3((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG), 0, st,
+                       b, S, gamma, (const float *)lambda, work);
+    e = hipGetLastError();
+    if (e != hipSuccess) break;
+    hipLaunchKernelGGL(k_direct_refine, grid, dim3(WG), 0, st, b, order, n_list, status, work, lambda);
+    e = hipGetLastError();
+  }
+  return e;
 }
 
 }  // namespace mpc

@@ -100,11 +100,7 @@ static Params make_params(const mpc_handle *h) {
 
 static Batch single(uint32_t N) { return Batch{nullptr, nullptr, 1u, N}; }
 
-// iterative-refinement steps of the direct Schur solve, and of the KKT
-// solve (fp64 residual of the whole step) after it, in the SQP drivers: the
-// dz recovery G~^-1 (g - C^T lambda) cancels, so even a lambda at fp32
-// storage precision leaves dz off by up to 1/rho times that
-constexpr uint32_t kDirectRefine = 2, kDirectKktRefine = 2;
+
 
 // PCG of one instance: resident kernel when its Schur system fits one
 // workgroup, a two-workgroup team up to twice that, streaming otherwise.
@@ -114,7 +110,7 @@ static hipError_t pcg_single(mpc_handle *h, uint32_t N, const float *S, const fl
   const uint32_t flags = h->cfg.flags;
   if (flags & FLAG_DIRECT_SOLVE)
     return launch_direct_solve(single(N), nullptr, 1, S, gamma, nullptr, const_cast<float *>(P), lambda,
-                               kDirectRefine, iters, flag, st);
+                               h->cfg.direct_refine, true, iters, flag, st);
   if (!(flags & FLAG_PCG_STREAM)) {
     if (N <= pcg_res_max_knots())
       return launch_pcg_res(single(N), nullptr, nullptr, 1, S, P, gamma, lambda, max_iter, tol, nullptr, iters,
@@ -160,8 +156,14 @@ static hipError_t refine_kkt_with(mpc_handle *h, const Batch &b, uint32_t T, con
 // the same for the batch layout
 static hipError_t refine_kkt(mpc_handle *h, const float *d_rho, const float *g, const float *c, float *dz,
                              float *lam, uint32_t steps, float rel_tol, hipStream_t st) {
+  // direct mode: the corrections reuse the factors of the solve just made on S
   return refine_kkt_with(h, batch_of(h), h->total, d_rho, g, c, dz, lam, steps, rel_tol,
-                         [&](float *l, float tol) { return pcg_batch(h, l, nullptr, h->cfg.pcg_max_iter, tol, nullptr, st); },
+                         [&](float *l, float tol) {
+                           if (h->cfg.flags & FLAG_DIRECT_SOLVE)
+                             return launch_direct_solve(batch_of(h), h->dir_order, h->n_inst, h->S, h->gamma, nullptr,
+                                                        h->P, l, h->cfg.direct_refine, false, nullptr, nullptr, st);
+                           return pcg_batch(h, l, nullptr, h->cfg.pcg_max_iter, tol, nullptr, st);
+                         },
                          st);
 }
 
@@ -169,7 +171,7 @@ static hipError_t refine_kkt(mpc_handle *h, const float *d_rho, const float *g, 
 static bool valid_numbers(const mpc_config &c) {
   return c.dt > 0 && std::isfinite(c.dt) && c.q_ee >= 0 && c.qd_cost >= 0 && c.r_cost >= 0 &&
          c.mu >= 0 && c.rho_factor > 1 && c.rho_max > 0 && c.rho_min > 0 && c.pcg_exit_tol >= 0 &&
-         c.sqp_max_iter <= MPC_MAX_SQP_ITERS;
+         c.sqp_max_iter <= MPC_MAX_SQP_ITERS && c.direct_refine <= 16 && c.direct_kkt_refine <= 16;
 }
 
 extern "C" {
@@ -193,6 +195,8 @@ void mpc_default_config(mpc_config *cfg) {
   cfg->pcg_max_iter = 173;
   cfg->sqp_max_iter = 3;
   cfg->flags = 0;
+  cfg->direct_refine = 0;
+  cfg->direct_kkt_refine = 1;
 }
 
 const char *mpc_status_string(int s) {
@@ -411,8 +415,8 @@ int mpc_pcg_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_Pi
 int mpc_direct_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_gamma, float *d_lambda,
                      uint32_t refine_steps, float *d_work, mpc_stream_t stream) {
   if (!h || !n_ok(h, N) || !d_S || !d_gamma || !d_lambda || !d_work || refine_steps > 16) return MPC_ERR_ARG;
-  HIP_TRY(launch_direct_solve(single(N), nullptr, 1, d_S, d_gamma, nullptr, d_work, d_lambda, refine_steps, nullptr,
-                              nullptr, (hipStream_t)stream));
+  HIP_TRY(launch_direct_solve(single(N), nullptr, 1, d_S, d_gamma, nullptr, d_work, d_lambda, refine_steps, true,
+                              nullptr, nullptr, (hipStream_t)stream));
   return MPC_OK;
 }
 
@@ -485,10 +489,10 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
     if (cfg.flags & FLAG_DIRECT_SOLVE) {
       h->h_rho[3] = *rho;
       HIP_TRY(hipMemcpyAsync(h->rho_one, h->h_rho + 3, sizeof(float), hipMemcpyHostToDevice, st));
-      HIP_TRY(refine_kkt_with(h, b, N, h->rho_one, h->g, h->c, h->dz, d_lambda, kDirectKktRefine, 0.f,
-                              [&](float *l, float tol) {
-                                return pcg_single(h, N, h->S, h->P, h->gamma, l, cfg.pcg_max_iter, tol, nullptr,
-                                                  nullptr, st);
+      HIP_TRY(refine_kkt_with(h, b, N, h->rho_one, h->g, h->c, h->dz, d_lambda, h->cfg.direct_kkt_refine, 0.f,
+                              [&](float *l, float) {
+                                return launch_direct_solve(b, nullptr, 1, h->S, h->gamma, nullptr, h->P, l,
+                                                           h->cfg.direct_refine, false, nullptr, nullptr, st);
                               },
                               st));
     }
@@ -715,8 +719,8 @@ static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_st
   const Batch b = batch_of(h);
   hipError_t e;
   if (h->cfg.flags & FLAG_DIRECT_SOLVE) {
-    e = launch_direct_solve(b, h->dir_order, h->n_inst, h->S, h->gamma, d_status, h->P, d_lambda, kDirectRefine,
-                            h->pcg_iters, h->pcg_flag, st);
+    e = launch_direct_solve(b, h->dir_order, h->n_inst, h->S, h->gamma, d_status, h->P, d_lambda, h->cfg.direct_refine,
+                            true, h->pcg_iters, h->pcg_flag, st);
     if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
     return e;
   }
@@ -766,7 +770,7 @@ static hipError_t batch_sqp_run(mpc_handle *h, float *d_xu, float *d_lambda, con
     if (tm) MPC_RUN(hipEventRecord(h->ev[4], st));
     MPC_RUN(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
     if (h->cfg.flags & FLAG_DIRECT_SOLVE)
-      MPC_RUN(refine_kkt(h, d_rho, h->g, h->c, h->dz, d_lambda, kDirectKktRefine, 0.f, st));
+      MPC_RUN(refine_kkt(h, d_rho, h->g, h->c, h->dz, d_lambda, h->cfg.direct_kkt_refine, 0.f, st));
     if (tm) MPC_RUN(hipEventRecord(h->ev[5], st));
     MPC_RUN(launch_merit(b, prm, d_xu, h->dz, d_ee_ref, h->xs, d_status, 0, 8, h->knot_merits, st));
     if (tm) MPC_RUN(hipEventRecord(h->ev[6], st));

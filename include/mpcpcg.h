@@ -72,6 +72,13 @@ typedef struct mpc_config {
   uint32_t pcg_max_iter;     /* pcg_config.pcg_max_iter */
   uint32_t sqp_max_iter;     /* SQP_MAX_ITER */
   uint32_t flags;            /* MPC_FLAG_* */
+  /* MPC_FLAG_DIRECT_SOLVE: iterative-refinement steps (fp64 residuals) of the
+   * Schur solve and of the whole KKT step (dz, lambda) after it (<= 16 each;
+   * defaults 0 and 1).  The dz recovery cancels, so the step needs the
+   * latter: with one KKT step the SQP follows the fp64 direct SQP to ~3e-5,
+   * with two to ~1e-6. */
+  uint32_t direct_refine;
+  uint32_t direct_kkt_refine;
 } mpc_config;
 
 /* The terminal knot's cost uses x_{N-1} instead of the reference's x_{N-2}
