@@ -76,6 +76,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-closed-loop", action="store_true")
+    ap.add_argument("--stress", action="store_true",
+                    help="also run the stress configuration (4096 instances, N in {128, 256, 512}, Q/R swept)")
     return ap.parse_args()
 
 
@@ -180,6 +182,7 @@ def closed_loop(pkg, torch, seed, N=128, steps=2000, timed_steps=200, K=3):
     _, pcg_t, st_t = run(timed_steps, 0, 1)
     solver.close()
     n_it = timed_steps * K
+    ref_traj = reference_trajectory(pkg, torch, N, K)
     return {"workload": f"1 instance, N={N}, {steps} control steps, {K} SQP iterations per step, PCG tol 5e-6, "
                         f"state noise 1e-3, plant 50 x 2e-4 s Euler substeps, precomputed-trajectory warm start",
             "us_per_control_step": round(st["total_us"] / steps, 1), "graph": bool(st["graph"]),
@@ -189,7 +192,119 @@ def closed_loop(pkg, torch, seed, N=128, steps=2000, timed_steps=200, K=3):
             "timed_run": {"steps": timed_steps, "linsys_us_per_sqp_iter": round(st_t["linsys_us"] / n_it, 1),
                           "sqp_us_per_sqp_iter": round(st_t["solve_us"] / n_it, 1),
                           "plant_us_per_step": round(st_t["plant_us"] / timed_steps, 1)},
-            "reference_pcg_linsys_us": {"N128_tol1e-5_rtx4090": 448.79}}
+            "reference_pcg_linsys_us": {"N128_tol1e-5_rtx4090": 448.79},
+            "reference_trajectory_0_0": ref_traj}
+
+
+def reference_trajectory(pkg, torch, N, K):
+    """The reference's own closed-loop input (testfiles/0_0_traj.csv and
+    0_0_eepos.traj, kept in tests/golden/fk_0_0.npz) with its settings:
+    dt = 1/64 s, QD = R = 1e-4, PCG tol 1e-5 with its cap 167 at N = 128, no
+    noise; one control step per knot (78 plant substeps) over the whole
+    trajectory; PCG and direct solver.  The reference reports a mean L1
+    end-effector error of 0.0489 m (PCG, tol 1e-5) and 0.0318 m (QDLDL)."""
+    import importlib
+    fx = importlib.import_module(pkg.__name__ + ".fixtures")
+    g = np.load(os.path.join(ROOT, "tests", "golden", "fk_0_0.npz"))
+    xu_rows, ee_rows = g["xu"].astype(np.float32), g["ee"].astype(np.float32)
+    dt = 1.0 / 64
+    out = {"settings": f"N={N}, {K} SQP iterations per step, dt=1/64, QD=R=1e-4, PCG tol 1e-5 cap 167, no noise",
+           "reference_mean_l1_m": {"pcg_tol1e-5": 0.0489, "qdldl": 0.0318}}
+    for mode, flags in (("pcg", 0), ("direct", pkg.FLAG_DIRECT_SOLVE)):
+        s = pkg.Solver(pkg.default_config(max_knots=N, max_instances=1, max_total_knots=N, dt=dt, qd_cost=1e-4,
+                                          r_cost=1e-4, pcg_exit_tol=1e-5, pcg_max_iter=167, flags=flags))
+        x0, path, traj, steps = fx.closed_loop_inputs(xu_rows, ee_rows, N)
+        xu = traj[:21 * N - 7].clone()
+        lam = torch.zeros(14 * N, device="cuda")
+        cl = pkg.default_closed_loop(steps=steps, sqp_iters=K, substeps=78, sim_dt=dt / 78, noise_sigma=0.0)
+        _, err, _, st = s.closed_loop(N, x0, path, xu, lam, cl, traj)
+        e = err.cpu().numpy()
+        s.close()
+        out[mode] = {"steps": steps, "mean_l1_ee_error_m": round(float(e[:, 1].mean()), 4),
+                     "rms_ee_error_m": round(float(np.sqrt(np.mean(e[:, 0] ** 2))), 4),
+                     "us_per_control_step": round(st["total_us"] / steps, 1)}
+    return out
+
+
+def stress(pkg, torch, seed, n=4096, ratios=(1e2, 1e4, 1e6), K=3):
+    """Stress configuration: n instances with horizons drawn from {128, 256,
+    512}; per Q/R ratio (R = Q_ee / ratio, Q_ee = 1) one batched solve of K
+    SQP iterations (PCG, then the direct solver), then the QP layer and its
+    adjoint at the final iterate."""
+    horizons = pkg.synth_horizons(seed + 5, n, (128, 256, 512)).astype(np.int64)
+    T = int(horizons.sum())
+    cfg = pkg.default_config(max_knots=512, max_instances=n, max_total_knots=T, sqp_max_iter=K)
+    solver = pkg.Solver(cfg)
+    xus, qrefs = [], []
+    for i, N in enumerate(horizons):
+        x0, qref = pkg.synth_instance(seed * 7919 + 200000 + i, int(N), 0.01)
+        xu = np.zeros(21 * N - 7, np.float32)  # x_k = x0, u_k = 0
+        for k in range(N):
+            xu[21 * k:21 * k + 14] = x0
+        xus.append(xu)
+        qrefs.append(qref)
+    ee = solver.fk(torch.from_numpy(np.concatenate(qrefs)).cuda()).reshape(-1).contiguous()
+    xu0 = torch.from_numpy(np.concatenate(xus)).cuda()
+    del xus, qrefs
+    solver.batch_setup(horizons.tolist())
+    xu, lam = torch.empty_like(xu0), torch.zeros(14 * T, device="cuda")
+    rho, status = torch.empty(n, device="cuda"), torch.zeros(n, dtype=torch.int32, device="cuda")
+    log = torch.zeros(n * K, dtype=torch.int32, device="cuda")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    out = {"workload": f"{n} instances, N ~ U{{128, 256, 512}} (total {T} knots), {K} SQP iterations, PCG tol 1e-5, "
+                       f"max_iter 173; Q_ee = 1, QD = 1e-2, R = 1 / ratio",
+           "pcg_layout": solver.batch_pcg_layout(), "ratios": {}}
+    for ratio in ratios:
+        r_cost = 1.0 / ratio
+        solver.set_config(r_cost=r_cost, flags=0)  # also resets the weights to (0, QD) / R
+        row = {}
+        for mode, flags in (("pcg", 0), ("direct", pkg.FLAG_DIRECT_SOLVE)):
+            solver.set_config(r_cost=r_cost, flags=flags)
+            xu.copy_(xu0)
+            lam.zero_()
+            rho.fill_(1e-3)
+            log.fill_(-1)
+            torch.cuda.synchronize()
+            ev[0].record()
+            solver.batch_sqp(xu, lam, ee, rho, K, status, log)
+            ev[1].record()
+            torch.cuda.synchronize()
+            ms = ev[0].elapsed_time(ev[1])
+            lg, st = log.cpu().numpy().reshape(n, K), status.cpu().numpy()
+            ex = lg >= 0
+            row[mode] = {"ms": round(ms, 2), "kkt_solves_per_sec": round(int(ex.sum()) / (ms * 1e-3), 1),
+                         "aborted": int((st & 4 != 0).sum()), "nonfinite": int((st & 8 != 0).sum())}
+            if mode == "pcg":
+                its = lg[ex]
+                row[mode]["pcg_iters"] = {"mean": round(float(its.mean()), 1), "max": int(its.max()),
+                                          "hit_cap_frac": round(float((its >= 173).mean()), 3)}
+                by_n = {}
+                for N in (128, 256, 512):
+                    m = (horizons == N)[:, None] & ex
+                    by_n[str(N)] = round(float(lg[m].mean()), 1)
+                row[mode]["pcg_iters_mean_by_N"] = by_n
+        # forward (QP layer + loss) and adjoint at the final (direct) iterate, PCG mode
+        solver.set_config(r_cost=r_cost, flags=0)
+        dz, xs = torch.empty_like(xu), torch.empty_like(xu)
+        loss = torch.empty(n, device="cuda")
+        dLdx, dLdref = torch.empty_like(xu), torch.empty_like(ee)
+        gw, gr = torch.empty(21 * n, device="cuda"), torch.empty_like(ee)
+        lam.zero_()
+        ev[0].record()
+        solver.qp_layer(xu, ee, rho, lam, dz, xs)
+        solver.qp_loss(xs, ee, loss, dLdx, dLdref)
+        ev[1].record()
+        gr.copy_(dLdref)
+        solver.qp_adjoint(xu, ee, rho, dz, dLdx, gw, gr)
+        ev[2].record()
+        torch.cuda.synchronize()
+        fwd, adj = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+        row["qp_layer"] = {"forward_ms": round(fwd, 2), "adjoint_ms": round(adj, 2),
+                           "adjoint_over_forward": round(adj / fwd, 3),
+                           "gradients_finite": bool(torch.isfinite(gw).all().item() and torch.isfinite(gr).all().item())}
+        out["ratios"][f"{ratio:.0e}"] = row
+    solver.close()
+    return out
 
 
 def cpu_baseline(orc, helpers, horizons, seconds, seed):
@@ -353,6 +468,7 @@ def main():
 
     diff = None if args.no_diff else differentiation(pkg, torch, args.seed)
     closed = None if args.no_closed_loop else closed_loop(pkg, torch, args.seed)
+    stress_out = stress(pkg, torch, args.seed) if args.stress else None
 
     its = log[executed]
     out = {
@@ -396,6 +512,7 @@ def main():
         "direct_solver": direct,
         "differentiation": diff,
         "closed_loop": closed,
+        "stress": stress_out,
         "wall_s_timed": round(t_wall, 3),
     }
     print(json.dumps(out))

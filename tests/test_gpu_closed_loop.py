@@ -106,3 +106,30 @@ def test_closed_loop_n128_2000_steps(pkg):
     assert np.all(np.isfinite(r["states"])) and np.all(np.isfinite(e))
     assert np.array_equal(r["states"][:51], r2["states"])
     assert rms < 0.5
+
+
+@pytest.mark.parametrize("direct", [False, True])
+def test_closed_loop_on_reference_trajectory(pkg, direct):
+    """The reference's own closed-loop input (testfiles/0_0, dt = 1/64,
+    QD = R = 1e-4, no noise) at N = 32, 3 SQP iterations per step: the arm
+    follows the recorded trajectory (mean L1 end-effector error measured
+    0.050 m with PCG and 0.050 m direct; the reference reports 0.049 m with
+    PCG at N = 128)."""
+    import importlib
+    fx = importlib.import_module(pkg.__name__ + ".fixtures")
+    g = np.load(f"{helpers.GOLDEN}/fk_0_0.npz")
+    N, dt = 32, 1.0 / 64
+    s = pkg.Solver(pkg.default_config(max_knots=N, max_instances=1, max_total_knots=N, dt=dt, qd_cost=1e-4,
+                                      r_cost=1e-4, pcg_exit_tol=1e-5, flags=pkg.FLAG_DIRECT_SOLVE if direct else 0))
+    try:
+        x0, path, traj, steps = fx.closed_loop_inputs(g["xu"].astype(np.float32), g["ee"].astype(np.float32), N)
+        xu, lam = traj[:21 * N - 7].clone(), torch.zeros(14 * N, device="cuda")
+        cl = pkg.default_closed_loop(steps=steps, sqp_iters=3, substeps=78, sim_dt=dt / 78, noise_sigma=0.0)
+        states, err, _, _ = s.closed_loop(N, x0, path, xu, lam, cl, traj)
+        e, st = host(err), host(states)
+    finally:
+        s.close()
+    l1 = float(e[:, 1].mean())
+    print(f"0_0 trajectory, N=32, {'direct' if direct else 'PCG'}: mean L1 {l1:.4f} m over {steps} steps")
+    assert steps == 665 and np.all(np.isfinite(st))
+    assert l1 < 0.08
