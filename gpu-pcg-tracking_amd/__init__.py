@@ -34,6 +34,7 @@ ST_ACTIVE, ST_PCG_MAXITER, ST_RHO_ABORT, ST_NONFINITE = 1, 2, 4, 8
 FLAG_LAST_KNOT_OWN_STATE = 1
 FLAG_PCG_STREAM = 2
 FLAG_DIRECT_SOLVE = 4
+FLAG_PCG_STAIR = 8
 
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = (

@@ -92,7 +92,8 @@ struct ClStep {
 };
 
 constexpr uint32_t ST_ACTIVE = 1u, ST_PCG_MAXITER = 2u, ST_RHO_ABORT = 4u, ST_NONFINITE = 8u;
-constexpr uint32_t FLAG_LAST_KNOT_OWN_STATE = 1u, FLAG_PCG_STREAM = 2u, FLAG_DIRECT_SOLVE = 4u;
+constexpr uint32_t FLAG_LAST_KNOT_OWN_STATE = 1u, FLAG_PCG_STREAM = 2u, FLAG_DIRECT_SOLVE = 4u,
+                   FLAG_PCG_STAIR = 8u;
 
 // host-side launchers (implemented in the .hip files)
 hipError_t launch_fk(uint32_t n, const float *q, uint32_t q_stride, float *ee, hipStream_t st);
@@ -117,14 +118,14 @@ hipError_t launch_compute_dz(const Batch &b, const float *Ginv, const float *C, 
 hipError_t launch_pcg_stream(const Batch &b, uint32_t max_n, const float *S, const float *Pinv,
                              const float *gamma, float *lambda, uint32_t max_iter, float tol,
                              const uint32_t *status, const uint32_t *inst_list, uint32_t n_list,
-                             uint32_t *iters, uint32_t *flag, hipStream_t st);
+                             uint32_t *iters, uint32_t *flag, bool stair, hipStream_t st);
 // PCG with S resident on chip (N <= pcg_res_max_knots()); one workgroup per
 // bin of instances.  items[bin_off[w] .. bin_off[w+1]) = {inst, first group,
 // groups, 0}; items == nullptr: a single instance (0) in one bin.
 hipError_t launch_pcg_res(const Batch &b, const uint4 *items, const uint32_t *bin_off, uint32_t n_bins,
                           const float *S, const float *Pinv, const float *gamma, float *lambda,
                           uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters,
-                          uint32_t *flag, hipStream_t st);
+                          uint32_t *flag, bool stair, hipStream_t st);
 // PCG of 128 < N <= pcg_team_max_knots() on a team of two workgroups
 // (cooperative launch); err receives 1 if a team wait timed out.
 hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n_teams, TeamMailbox *mb,

@@ -10,7 +10,8 @@
 // the per-wave partials, so results are bitwise reproducible.
 //
 // Algorithm (GPU-PCG pcg<T,14,N>, call site toplevel.cuh:143-157; the
-// submodule source is absent from the reference snapshot):
+// submodule source is absent from the reference snapshot), with z = M r for
+// the block-Jacobi M = D^-1 or (stair) M r = D^-1 (2 r - S D^-1 r):
 //   r = gamma - S lambda, z = Pinv r, p = z, eta = r.z
 //   exit if |eta| < tol (checked once before the loop as well)
 //   loop: y = S p; alpha = eta / p.y; lambda += alpha p; r -= alpha y;
@@ -51,7 +52,7 @@ __device__ __forceinline__ float blk_row_dot(const float *__restrict__ blk, int 
   return acc;
 }
 
-template <int NG, int R>
+template <int NG, int R, bool STAIR>
 __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restrict__ S,
                                                  const float *__restrict__ Pinv,
                                                  const float *__restrict__ gamma,
@@ -91,8 +92,8 @@ __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restric
       out[t] = acc;
     }
   };
-  // z = Pinv r for my rows (r staged in rv), returns r.z partial
-  auto precond = [&](float *z) {
+  // z = D^-1 v for my rows (v staged in rv), returns the r.z partial
+  auto jacobi = [&](float *z) {
     float part = 0.f;
 #pragma unroll
     for (int t = 0; t < R; ++t) {
@@ -105,6 +106,33 @@ __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restric
       z[t] = acc;
     }
     return part;
+  };
+  // z = M r (r staged in rv), returns the r.z partial.  Stair: M r =
+  // D^-1 (2 r - S D^-1 r); w = D^-1 r is staged in pv (p lives in
+  // registers), 2 r - S w in rv.  Ends with rv, pv free to overwrite after
+  // the caller's next barrier.
+  auto precond = [&](float *z) {
+    if constexpr (!STAIR) {
+      return jacobi(z);
+    } else {
+      float wv[R], y2[R];
+      jacobi(wv);
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        const uint32_t bk = grp + NG * t;
+        if (lane_ok && bk < N) pv[NX * bk + i] = wv[t];
+      }
+      __syncthreads();
+      smv(y2);
+      __syncthreads();  // every read of rv and pv done
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        const uint32_t bk = grp + NG * t;
+        if (lane_ok && bk < N) rv[NX * bk + i] = 2.f * r[t] - y2[t];
+      }
+      __syncthreads();
+      return jacobi(z);
+    }
   };
 
 #pragma unroll
@@ -184,12 +212,16 @@ template <int NG, int R>
 hipError_t launch_pcg_t(const Batch &b, uint32_t max_n, const float *S, const float *Pinv,
                         const float *gamma, float *lambda, uint32_t max_iter, float tol,
                         const uint32_t *status, const uint32_t *inst_list, uint32_t n_list,
-                        uint32_t *iters, uint32_t *flag, hipStream_t st) {
+                        uint32_t *iters, uint32_t *flag, bool stair, hipStream_t st) {
   const size_t smem = sizeof(float) * (2 * NX * (size_t)max_n + 2 * (NG * 16 / 64));
   const uint32_t grid = inst_list ? n_list : b.n_inst;
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_pcg<NG, R>), dim3(grid), dim3(NG * 16), smem, st, b, S, Pinv, gamma, lambda,
-                     max_iter, tol, status, inst_list, iters, flag);
+  if (stair)
+    hipLaunchKernelGGL((k_pcg<NG, R, true>), dim3(grid), dim3(NG * 16), smem, st, b, S, Pinv, gamma, lambda,
+                       max_iter, tol, status, inst_list, iters, flag);
+  else
+    hipLaunchKernelGGL((k_pcg<NG, R, false>), dim3(grid), dim3(NG * 16), smem, st, b, S, Pinv, gamma, lambda,
+                       max_iter, tol, status, inst_list, iters, flag);
   return hipGetLastError();
 }
 
@@ -198,9 +230,9 @@ hipError_t launch_pcg_t(const Batch &b, uint32_t max_n, const float *S, const fl
 hipError_t launch_pcg_stream(const Batch &b, uint32_t max_n, const float *S, const float *Pinv,
                              const float *gamma, float *lambda, uint32_t max_iter, float tol,
                              const uint32_t *status, const uint32_t *inst_list, uint32_t n_list,
-                             uint32_t *iters, uint32_t *flag, hipStream_t st) {
+                             uint32_t *iters, uint32_t *flag, bool stair, hipStream_t st) {
   if (b.n_inst == 0) return hipSuccess;
-#define MPC_PCG_ARGS b, max_n, S, Pinv, gamma, lambda, max_iter, tol, status, inst_list, n_list, iters, flag, st
+#define MPC_PCG_ARGS b, max_n, S, Pinv, gamma, lambda, max_iter, tol, status, inst_list, n_list, iters, flag, stair, st
   // 16 groups (256 lanes) up to N = 256, then 32 / 64 groups; R = rows per group.
   if (max_n <= 16) return launch_pcg_t<16, 1>(MPC_PCG_ARGS);
   if (max_n <= 32) return launch_pcg_t<16, 2>(MPC_PCG_ARGS);

@@ -13,7 +13,10 @@
 // into.  Instances of a bin iterate in lockstep; a converged instance
 // skips its arithmetic until the bin is done.
 //
-// Per iteration, two workgroup barriers:
+// With the stair preconditioner (STAIR), z = D^-1 (2 r - S D^-1 r): one more
+// block product, one more staged vector and one more barrier per iteration.
+//
+// Per iteration, two workgroup barriers (block Jacobi):
 //   y = S p (registers + L from LDS); p.y -> B1 -> alpha; lambda, r;
 //   z = Pinv r; stage z of the group's first and last knot; r.z -> B2 ->
 //   beta; p = z + beta p for the own knots and, from the staged z, for the
@@ -39,6 +42,7 @@ constexpr int MAXI = 8;        // instances per bin
 struct ResShared {
   float L[(KNOTS + 1) * L_SLOT + L_GUARD];  // S(b,b-1) per knot slot (+ one zero slot), pre-rotated rows
   float zb[(KNOTS + 1) * 16];         // staged vector element per knot slot and lane
+  float wb[(KNOTS + 1) * 16];         // stair preconditioner: staged D^-1 r
   float red[2][NWAV];
   uint32_t act[NWAV];
 };
@@ -50,6 +54,7 @@ __device__ __forceinline__ float seg_sum(const float *red, uint32_t w0, uint32_t
   return s;
 }
 
+template <bool STAIR>
 __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict__ items,
                                                 const uint32_t *__restrict__ bin_off,
                                                 const float *__restrict__ S,
@@ -118,6 +123,16 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
     vm = in_m ? zb0[-16] : 0.f;
     vp = in_p ? zb0[RK * 16] : 0.f;
   };
+  // stair preconditioner: the first / last element of w = D^-1 r, the same way
+  float *wb0 = sh.wb + s0 * 16 + i;
+  auto stage_w = [&](const float (&v)[RK]) {
+    wb0[0] = v[0];
+    wb0[(RK - 1) * 16] = v[RK - 1];
+  };
+  auto neighbours_w = [&](float &vm, float &vp) {
+    vm = in_m ? wb0[-16] : 0.f;
+    vp = in_p ? wb0[RK * 16] : 0.f;
+  };
   // y = S x for the group's knots; xm / xp: elements of knots 4lg-1, 4lg+4
   auto smv = [&](const float (&x)[RK], float xm, float xp) { group_smv<RK>(R, sh.L + s0 * L_SLOT, i, x, xm, xp, y); };
   stage(lam);
@@ -133,7 +148,20 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
     const uint32_t bk = b0 + (uint32_t)t;
     r[t] = live && bk < N && lane_ok ? gamma[(size_t)NX * (K0 + bk) + i] - y[t] : 0.f;
   }
-  group_precond<RK>(R, r, z);
+  if constexpr (STAIR) {
+    // z = D^-1 (2 r - S w), w = D^-1 r
+    float wv[RK], v[RK], wm, wp;
+    group_precond<RK>(R, r, wv);
+    stage_w(wv);
+    __syncthreads();
+    neighbours_w(wm, wp);
+    smv(wv, wm, wp);
+#pragma unroll
+    for (int t = 0; t < RK; ++t) v[t] = lane_ok ? 2.f * r[t] - y[t] : 0.f;
+    group_precond<RK>(R, v, z);
+  } else {
+    group_precond<RK>(R, r, z);
+  }
 #pragma unroll
   for (int t = 0; t < RK; ++t) part += r[t] * z[t];
   part = wave_sum(part);
@@ -169,6 +197,7 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
     for (int v = 0; v < NWAV; ++v) any |= sh.act[v];
     if (!any) break;
     float rz = 0.f;
+    float wv[RK];
     if (run) {
       const float alpha = eta / seg_sum(sh.red[0], w0, nw);
 #pragma unroll
@@ -176,11 +205,31 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
         lam[t] += alpha * p[t];
         r[t] = lane_ok ? r[t] - alpha * y[t] : 0.f;
       }
-      group_precond<RK>(R, r, z);
+      if constexpr (STAIR) {
+        group_precond<RK>(R, r, wv);
+        stage_w(wv);
+      } else {
+        group_precond<RK>(R, r, z);
 #pragma unroll
-      for (int t = 0; t < RK; ++t) rz += r[t] * z[t];
-      stage(z);
-      rz = wave_sum(rz);
+        for (int t = 0; t < RK; ++t) rz += r[t] * z[t];
+        stage(z);
+        rz = wave_sum(rz);
+      }
+    }
+    if constexpr (STAIR) {
+      __syncthreads();  // staged w visible
+      if (run) {
+        float v[RK], wm, wp;
+        neighbours_w(wm, wp);
+        smv(wv, wm, wp);
+#pragma unroll
+        for (int t = 0; t < RK; ++t) v[t] = lane_ok ? 2.f * r[t] - y[t] : 0.f;
+        group_precond<RK>(R, v, z);
+#pragma unroll
+        for (int t = 0; t < RK; ++t) rz += r[t] * z[t];
+        stage(z);
+        rz = wave_sum(rz);
+      }
     }
     if ((tid & 63u) == 0) sh.red[1][w] = rz;
     __syncthreads();
@@ -225,18 +274,23 @@ uint32_t pcg_res_max_items() { return MAXI; }
 hipError_t launch_pcg_res(const Batch &b, const uint4 *items, const uint32_t *bin_off, uint32_t n_bins,
                           const float *S, const float *Pinv, const float *gamma, float *lambda,
                           uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters,
-                          uint32_t *flag, hipStream_t st) {
+                          uint32_t *flag, bool stair, hipStream_t st) {
   if (n_bins == 0) return hipSuccess;
   static bool attr_set = false;
   const size_t smem = sizeof(ResShared);
   if (!attr_set) {
     // allow more than the default dynamic LDS; the launch reports any error
-    (void)hipFuncSetAttribute((const void *)k_pcg_res, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    (void)hipFuncSetAttribute((const void *)k_pcg_res<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    (void)hipFuncSetAttribute((const void *)k_pcg_res<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     (void)hipGetLastError();
     attr_set = true;
   }
-  hipLaunchKernelGGL(k_pcg_res, dim3(n_bins), dim3(NT), smem, st, b, items, bin_off, S, Pinv, gamma, lambda,
-                     max_iter, tol, status, iters, flag);
+  if (stair)
+    hipLaunchKernelGGL(k_pcg_res<true>, dim3(n_bins), dim3(NT), smem, st, b, items, bin_off, S, Pinv, gamma, lambda,
+                       max_iter, tol, status, iters, flag);
+  else
+    hipLaunchKernelGGL(k_pcg_res<false>, dim3(n_bins), dim3(NT), smem, st, b, items, bin_off, S, Pinv, gamma,
+                       lambda, max_iter, tol, status, iters, flag);
   return hipGetLastError();
 }
 

@@ -92,6 +92,14 @@ typedef struct mpc_config {
  * QP layer: the GPU counterpart of the reference's QDLDL path
  * (PCG_SOLVE = 0, toplevel.cuh:160-215). */
 #define MPC_FLAG_DIRECT_SOLVE 4u
+/* PCG with the stair preconditioner M = D^-1 - D^-1 (L + L^T) D^-1 (D, L: the
+ * block diagonal and the lower block part of S) instead of block Jacobi
+ * D^-1: the reference's gato_form_ss_inner (oldschur.cuh:334-494), present
+ * but unused there.  Applied as M r = D^-1 (2 r - S D^-1 r), it halves the
+ * iteration count on the block-tridiagonal Schur system for one more block
+ * product per iteration.  Instances of 128 < N <= 256 run on the streaming
+ * kernel in this mode. */
+#define MPC_FLAG_PCG_STAIR 8u
 
 typedef struct mpc_handle mpc_handle;
 

@@ -437,6 +437,7 @@ typedef struct {
    * the reference's cost is wx = (0 x 7, QD x 7), wu = (R x 7) */
   double wx[14];
   double wu[7];
+  int pcg_stair;    /* 1: stair preconditioner instead of block Jacobi */
 } orc_cfg;
 
 /* Cost gradient/Hessian at one knot (rbdfiles/iiwa_eepos_plant.cuh:293-378),
@@ -615,14 +616,29 @@ void F(orc_bd_mv)(int N, const real *S, const real *x, real *y) {
  * before the first iteration (a warm start already inside the tolerance
  * returns 0 iterations).  *iters = number of S products taken in the loop;
  * *max_iter_exit = 1 if max_iter was reached without meeting the tolerance. */
-void F(orc_pcg)(int N, const real *S, const real *Pinv, const real *gamma, real *lambda,
-                int max_iter, real tol, int *iters, int *max_iter_exit) {
+/* Preconditioner application z = M r.  Block Jacobi (stair == 0): M = Pinv
+ * (its block diagonal D^-1).  Stair (stair != 0; gato_form_ss_inner,
+ * oldschur.cuh:334-494, where the reference stores the off-diagonal blocks
+ * -D_b^-1 S(b,b-1) D_{b-1}^-1): M = D^-1 - D^-1 (L + L^T) D^-1 with L the
+ * strictly lower block part of S, applied as z = D^-1 (2 r - S D^-1 r)
+ * (since (L + L^T) D^-1 r = S D^-1 r - r).  t is n scratch values. */
+static void F(orc_precond)(int N, const real *S, const real *Pinv, const real *r, real *z, real *t, int stair) {
+  F(orc_bd_mv)(N, Pinv, r, z);
+  if (!stair) return;
+  F(orc_bd_mv)(N, S, z, t);
+  for (int i = 0; i < NX * N; ++i) t[i] = 2 * r[i] - t[i];
+  F(orc_bd_mv)(N, Pinv, t, z);
+}
+
+void F(orc_pcg_ex)(int N, const real *S, const real *Pinv, const real *gamma, real *lambda,
+                   int max_iter, real tol, int stair, int *iters, int *max_iter_exit) {
   int n = NX * N;
   real *r = (real *)malloc(sizeof(real) * n), *z = (real *)malloc(sizeof(real) * n);
   real *p = (real *)malloc(sizeof(real) * n), *y = (real *)malloc(sizeof(real) * n);
+  real *t = (real *)malloc(sizeof(real) * n);
   F(orc_bd_mv)(N, S, lambda, y);
   for (int i = 0; i < n; ++i) r[i] = gamma[i] - y[i];
-  F(orc_bd_mv)(N, Pinv, r, z);
+  F(orc_precond)(N, S, Pinv, r, z, t, stair);
   real eta = 0;
   for (int i = 0; i < n; ++i) { p[i] = z[i]; eta += r[i] * z[i]; }
   int it = 0, conv = 0;
@@ -633,7 +649,7 @@ void F(orc_pcg)(int N, const real *S, const real *Pinv, const real *gamma, real 
     for (int i = 0; i < n; ++i) py += p[i] * y[i];
     real alpha = eta / py;
     for (int i = 0; i < n; ++i) { lambda[i] += alpha * p[i]; r[i] -= alpha * y[i]; }
-    F(orc_bd_mv)(N, Pinv, r, z);
+    F(orc_precond)(N, S, Pinv, r, z, t, stair);
     real eta_new = 0;
     for (int i = 0; i < n; ++i) eta_new += r[i] * z[i];
     ++it;
@@ -644,7 +660,12 @@ void F(orc_pcg)(int N, const real *S, const real *Pinv, const real *gamma, real 
   }
   *iters = it;
   *max_iter_exit = !conv;
-  free(r); free(z); free(p); free(y);
+  free(r); free(z); free(p); free(y); free(t);
+}
+
+void F(orc_pcg)(int N, const real *S, const real *Pinv, const real *gamma, real *lambda,
+                int max_iter, real tol, int *iters, int *max_iter_exit) {
+  F(orc_pcg_ex)(N, S, Pinv, gamma, lambda, max_iter, tol, 0, iters, max_iter_exit);
 }
 
 /* dz (oldschur.cuh:966-1105):
@@ -917,7 +938,7 @@ int F(orc_sqp_solve)(const orc_cfg *cfg, int N, const real *ee_ref, real *lambda
       if (pcg_maxiter_out) pcg_maxiter_out[it] = 0;
     } else {
       int pit, pmx;
-      F(orc_pcg)(N, S, P, gam, lambda, cfg->pcg_max_iter, (real)cfg->pcg_tol, &pit, &pmx);
+      F(orc_pcg_ex)(N, S, P, gam, lambda, cfg->pcg_max_iter, (real)cfg->pcg_tol, cfg->pcg_stair, &pit, &pmx);
       if (pcg_iters_out) pcg_iters_out[it] = pit;
       if (pcg_maxiter_out) pcg_maxiter_out[it] = pmx;
     }

@@ -56,6 +56,7 @@ class Cfg(C.Structure):
         ("last_knot_own_state", C.c_int),
         ("wx", C.c_double * 14),
         ("wu", C.c_double * 7),
+        ("pcg_stair", C.c_int),
     ]
 
 
@@ -66,12 +67,12 @@ def default_weights(qd_cost=1e-2, r_cost=1e-4):
 
 def make_cfg(dt=0.01, q_ee=1.0, qd_cost=1e-2, r_cost=1e-4, mu=10.0, rho_factor=1.2, rho_max=10.0,
              rho_min=1e-3, pcg_tol=1e-5, pcg_max_iter=173, sqp_max_iter=3, last_knot_own_state=0,
-             wx=None, wu=None):
+             wx=None, wu=None, pcg_stair=0):
     dwx, dwu = default_weights(qd_cost, r_cost)
     wx = dwx if wx is None else np.asarray(wx, np.float64)
     wu = dwu if wu is None else np.asarray(wu, np.float64)
     return Cfg(dt, q_ee, qd_cost, r_cost, mu, rho_factor, rho_max, rho_min, pcg_tol, pcg_max_iter,
-               sqp_max_iter, last_knot_own_state, (C.c_double * 14)(*wx), (C.c_double * 7)(*wu))
+               sqp_max_iter, last_knot_own_state, (C.c_double * 14)(*wx), (C.c_double * 7)(*wu), int(pcg_stair))
 
 
 def _sfx(dtype):
@@ -190,13 +191,13 @@ def bd_mv(N, S, x, dtype=np.float64):
     return y
 
 
-def pcg(N, S, P, gamma, lam0, max_iter, tol, dtype=np.float64):
+def pcg(N, S, P, gamma, lam0, max_iter, tol, dtype=np.float64, stair=False):
     S, P, gamma = (_arr(a, dtype) for a in (S, P, gamma))
     lam = _arr(lam0, dtype).copy()
     it, mx = C.c_int(), C.c_int()
     tolc = (C.c_double if dtype == np.float64 else C.c_float)(tol)
-    _fn("orc_pcg", dtype)(N, _ptr(S), _ptr(P), _ptr(gamma), _ptr(lam), max_iter, tolc, C.byref(it),
-                          C.byref(mx))
+    _fn("orc_pcg_ex", dtype)(N, _ptr(S), _ptr(P), _ptr(gamma), _ptr(lam), max_iter, tolc, int(stair),
+                             C.byref(it), C.byref(mx))
     return lam, it.value, mx.value
 
 

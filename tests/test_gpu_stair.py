@@ -1,0 +1,101 @@
+"""GPU PCG with the stair preconditioner (MPC_FLAG_PCG_STAIR) against the
+oracle's stair PCG (orc_pcg_ex, pinned in test_oracle_stair.py): the
+resident kernel (N <= 128) and the streaming kernel (the team range and
+beyond), the plain SQP in stair mode, and the batched SQP in stair mode
+against the plain one.
+
+Tolerances as for block Jacobi (test_gpu_stages.py): the same exit rule on
+both sides, iteration counts within 2 %, solutions within
+max(2e-4, 4 x the fp32 twin's distance to the fp64 oracle)."""
+import numpy as np
+import pytest
+
+import helpers
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.float32))).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def _schur(orc, seed, N, perturbed=False):
+    x0, ee, xu = (helpers.perturbed_instance if perturbed else helpers.make_instance)(orc, seed, N)
+    cfg = orc.make_cfg()
+    G, Cm, g, c = orc.form_kkt(cfg, N, xu.astype(np.float64), ee.astype(np.float64), xu[:14].astype(np.float64))
+    S, P, gam, _ = orc.form_schur(N, G, Cm, g, c, 1e-3)
+    return (np.asarray(a, np.float32) for a in (S, P, gam))
+
+
+@pytest.fixture
+def stair(solver, pkg):
+    saved = solver.cfg.flags, solver.cfg.sqp_max_iter
+    solver.set_config(flags=pkg.FLAG_PCG_STAIR)
+    yield
+    solver.set_config(flags=saved[0], sqp_max_iter=saved[1])
+
+
+@pytest.mark.parametrize("N,perturbed", [(2, False), (16, False), (64, True), (128, False), (128, True),
+                                         (200, False), (256, True), (300, False)])
+def test_stair_pcg_matches_oracle(solver, orc, stair, N, perturbed):
+    S32, P32, gam32 = _schur(orc, 110 + N, N, perturbed)
+    lam = torch.zeros(14 * N, device="cuda")
+    it = torch.zeros(1, dtype=torch.int32, device="cuda")
+    fl = torch.zeros(1, dtype=torch.int32, device="cuda")
+    solver.pcg(N, dev(S32), dev(P32), dev(gam32), lam, 5000, 1e-5, it, fl)
+    lam_g, it_g, fl_g = host(lam), int(it.item()), int(fl.item())
+    lam_o, it_o, fl_o = orc.pcg(N, S32, P32, gam32, np.zeros(14 * N), 5000, 1e-5, stair=True)
+    lam_t, _, _ = orc.pcg(N, S32, P32, gam32, np.zeros(14 * N), 5000, 1e-5, dtype=np.float32, stair=True)
+    _, it_j, _ = orc.pcg(N, S32, P32, gam32, np.zeros(14 * N), 5000, 1e-5)
+    floor = helpers.rel_err(lam_t, lam_o)
+    err = helpers.rel_err(lam_g, lam_o)
+    print(f"N={N} perturbed={perturbed}: stair iters gpu={it_g} oracle={it_o} (block Jacobi {it_j}); "
+          f"rel(gpu, oracle)={err:.2e} fp32 floor={floor:.2e}")
+    assert fl_g == 0 and fl_o == 0
+    assert abs(it_g - it_o) <= max(1, it_o // 50)
+    assert err < max(2e-4, 4 * floor)
+
+
+@pytest.mark.parametrize("N", [8, 32, 128])
+def test_plain_stair_sqp_vs_oracle(solver, orc, stair, N):
+    x0, ee, xu = helpers.make_instance(orc, 150 + N, N)
+    cfg = orc.make_cfg(sqp_max_iter=3, pcg_stair=1)
+    r = orc.sqp_solve(cfg, N, ee.astype(np.float64), np.zeros(14 * N), xu.astype(np.float64), 1e-3)
+    solver.set_config(sqp_max_iter=3)
+    dxu, dlam = dev(xu), torch.zeros(14 * N, device="cuda")
+    pcg_iters, _, _, sqp_iters, _, _, rho, steps = solver.sqp_solve(N, dev(ee), dlam, dxu, 1e-3)
+    print(f"N={N} gpu steps={steps} pcg={pcg_iters} oracle steps={r['ls_steps']} pcg={r['pcg_iters']}")
+    assert steps == r["ls_steps"] and sqp_iters == r["sqp_iters"]
+    assert all(abs(a - b) <= max(1, b // 50) for a, b in zip(pcg_iters, r["pcg_iters"]))
+    assert helpers.rel_err(host(dxu), r["xu"]) < 1e-3
+
+
+@pytest.mark.parametrize("horizons", [[32, 64, 17, 128, 2, 45], [256, 129, 200, 64, 300, 3]])
+def test_batched_stair_matches_plain_stair(solver, orc, stair, horizons):
+    xus, ees = [], []
+    for i, N in enumerate(horizons):
+        _, ee, xu = helpers.make_instance(orc, 3000 + i, N)
+        xus.append(xu)
+        ees.append(ee)
+    n, T = len(horizons), sum(horizons)
+    xu, lam = dev(np.concatenate(xus)), torch.zeros(14 * T, device="cuda")
+    rho, st = torch.full((n,), 1e-3, device="cuda"), torch.zeros(n, dtype=torch.int32, device="cuda")
+    solver.batch_setup(horizons)
+    solver.batch_sqp(xu, lam, dev(np.concatenate(ees)), rho, 3, st)
+    xb, lb = host(xu), host(lam)
+    off = loff = 0
+    solver.set_config(sqp_max_iter=3)
+    for i, N in enumerate(horizons):
+        dxu, dlam = dev(xus[i]), torch.zeros(14 * N, device="cuda")
+        solver.sqp_solve(N, dev(ees[i]), dlam, dxu, 1e-3)
+        L = 21 * N - 7
+        assert helpers.rel_err(xb[off:off + L], host(dxu)) < 1e-5, i
+        assert helpers.rel_err(lb[loff:loff + 14 * N], host(dlam)) < 1e-5, i
+        off += L
+        loff += 14 * N
