@@ -229,8 +229,9 @@ def reference_trajectory(pkg, torch, N, K):
 def stress(pkg, torch, seed, n=4096, ratios=(1e2, 1e4, 1e6), K=3):
     """Stress configuration: n instances with horizons drawn from {128, 256,
     512}; per Q/R ratio (R = Q_ee / ratio, Q_ee = 1) one batched solve of K
-    SQP iterations (PCG, then the direct solver), then the QP layer and its
-    adjoint at the final iterate."""
+    SQP iterations each with block-Jacobi PCG, stair-preconditioned PCG and
+    the direct solver, then the QP layer and its adjoint at the final
+    iterate."""
     horizons = pkg.synth_horizons(seed + 5, n, (128, 256, 512)).astype(np.int64)
     T = int(horizons.sum())
     cfg = pkg.default_config(max_knots=512, max_instances=n, max_total_knots=T, sqp_max_iter=K)
@@ -258,7 +259,7 @@ def stress(pkg, torch, seed, n=4096, ratios=(1e2, 1e4, 1e6), K=3):
         r_cost = 1.0 / ratio
         solver.set_config(r_cost=r_cost, flags=0)  # also resets the weights to (0, QD) / R
         row = {}
-        for mode, flags in (("pcg", 0), ("direct", pkg.FLAG_DIRECT_SOLVE)):
+        for mode, flags in (("pcg", 0), ("pcg_stair", pkg.FLAG_PCG_STAIR), ("direct", pkg.FLAG_DIRECT_SOLVE)):
             solver.set_config(r_cost=r_cost, flags=flags)
             xu.copy_(xu0)
             lam.zero_()
@@ -274,7 +275,7 @@ def stress(pkg, torch, seed, n=4096, ratios=(1e2, 1e4, 1e6), K=3):
             ex = lg >= 0
             row[mode] = {"ms": round(ms, 2), "kkt_solves_per_sec": round(int(ex.sum()) / (ms * 1e-3), 1),
                          "aborted": int((st & 4 != 0).sum()), "nonfinite": int((st & 8 != 0).sum())}
-            if mode == "pcg":
+            if mode != "direct":
                 its = lg[ex]
                 row[mode]["pcg_iters"] = {"mean": round(float(its.mean()), 1), "max": int(its.max()),
                                           "hit_cap_frac": round(float((its >= 173).mean()), 3)}
@@ -441,6 +442,26 @@ def main():
                          f"one instance per horizon at its initial iterate, weighted by the workload's horizon mix; "
                          f"{desc}; ~{args.cpu_seconds:.0f} s of CPU work"}
 
+    # the same workload with the stair preconditioner (half the PCG iterations
+    # for one more block product each; the 128 < N <= 256 instances on the
+    # streaming kernel)
+    solver.set_config(flags=pkg.FLAG_PCG_STAIR)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_stair = e0.elapsed_time(e1) / args.steps
+    lg_s, st_s = pcg_log.cpu().numpy(), status.cpu().numpy()
+    its_s = lg_s[lg_s >= 0]
+    stair = {"kkt_solves_per_sec": round(int((lg_s >= 0).sum()) / (ms_stair * 1e-3), 1),
+             "ms_per_step": round(ms_stair, 4),
+             "pcg_iters": {"mean": round(float(its_s.mean()), 1), "max": int(its_s.max()),
+                           "hit_max_iter": int((st_s & 2 != 0).sum())}}
+
     # the same workload with the direct Schur solve (block LDL^T on the GPU,
     # one fp64-residual refinement of each KKT step) instead of PCG
     solver.set_config(flags=pkg.FLAG_DIRECT_SOLVE)
@@ -509,6 +530,7 @@ def main():
                      "fp32": {"achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": round(tflops / FP32_PEAK_TFLOPS, 4)}},
         "cpu_baseline": cpu,
+        "pcg_stair": stair,
         "direct_solver": direct,
         "differentiation": diff,
         "closed_loop": closed,

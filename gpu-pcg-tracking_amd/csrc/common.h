@@ -131,7 +131,7 @@ hipError_t launch_pcg_res(const Batch &b, const uint4 *items, const uint32_t *bi
 hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n_teams, TeamMailbox *mb,
                            uint32_t n_mb, const float *S, const float *Pinv, const float *gamma,
                            float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
-                           uint32_t *iters, uint32_t *flag, uint32_t *err, hipStream_t st);
+                           uint32_t *iters, uint32_t *flag, uint32_t *err, bool stair, hipStream_t st);
 uint32_t pcg_team_max_knots();
 uint32_t pcg_team_max_grid();
 uint32_t pcg_res_groups(uint32_t N);

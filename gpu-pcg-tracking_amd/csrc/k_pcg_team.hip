@@ -11,6 +11,9 @@
 // 0's group N0/4 - 1, the first of half 1's group 0) take their outer
 // neighbour's vector element from the partner.
 //
+// With the stair preconditioner (STAIR) a third exchange per iteration
+// carries the row of w = D^-1 r at the split for the extra block product.
+//
 // Exchange through an L2 mailbox, twice per iteration (p.Sp, then r.z with
 // z of the row next to the split): before the workgroup barrier that
 // publishes the wave partials in LDS, every wave also posts its partial to
@@ -50,6 +53,7 @@ static_assert(MB_ROW == NWAV && MB_ROW + NX <= 32, "mailbox layout");
 struct TeamShared {
   float L[(KNOTS + 1) * L_SLOT + L_GUARD];
   float zb[(KNOTS + 1) * 16];
+  float wb[(KNOTS + 1) * 16];  // stair preconditioner: staged D^-1 r
   float red[2][NWAV];
   float recv[16];
   uint32_t fail;
@@ -105,6 +109,7 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+template <bool STAIR>
 __global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__restrict__ team_inst,
                                                  uint32_t n_teams, uint32_t team0, TeamMailbox *mboxes,
                                                  const float *__restrict__ S,
@@ -196,6 +201,18 @@ __global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__rest
     vm = own_m ? zb0[-16] : (far_m && lane_ok ? sh.recv[i] : 0.f);
     vp = own_p ? zb0[RK * 16] : (far_p && lane_ok ? sh.recv[i] : 0.f);
   };
+  // stair preconditioner: w = D^-1 r staged like z, in its own buffer, and
+  // its row at the split exchanged on an epoch of its own (row only)
+  float *wb0 = sh.wb + b0 * 16 + i;
+  auto stage_w = [&](const float (&v)[RK]) {
+    wb0[0] = v[0];
+    wb0[(RK - 1) * 16] = v[RK - 1];
+    if (row_lane) mb_post(mb, half, e, MB_ROW + i, half ? v[0] : v[RK - 1]);
+  };
+  auto neighbours_w = [&](float &vm, float &vp) {
+    vm = own_m ? wb0[-16] : (far_m && lane_ok ? sh.recv[i] : 0.f);
+    vp = own_p ? wb0[RK * 16] : (far_p && lane_ok ? sh.recv[i] : 0.f);
+  };
   auto smv = [&](const float (&x)[RK], float xm, float xp) { group_smv<RK>(R, sh.L + b0 * L_SLOT, i, x, xm, xp, y); };
   auto total = [&](float mine, float theirs) { return half ? theirs + mine : mine + theirs; };
 
@@ -215,7 +232,23 @@ __global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__rest
     const uint32_t lk = b0 + (uint32_t)t;
     r[t] = lk < n_own && lane_ok ? gamma[(size_t)NX * (K0 + kb + lk) + i] - y[t] : 0.f;
   }
-  group_precond<RK>(R, r, z);
+  if constexpr (STAIR) {
+    float wv[RK], v[RK], wm, wp;
+    group_precond<RK>(R, r, wv);
+    __syncthreads();  // every read of the lambda row in recv is done
+    ++e;
+    stage_w(wv);
+    __syncthreads();
+    if (sh.fail) return;
+    collect(false, true);
+    neighbours_w(wm, wp);
+    smv(wv, wm, wp);
+#pragma unroll
+    for (int t = 0; t < RK; ++t) v[t] = lane_ok ? 2.f * r[t] - y[t] : 0.f;
+    group_precond<RK>(R, v, z);
+  } else {
+    group_precond<RK>(R, r, z);
+  }
 #pragma unroll
   for (int t = 0; t < RK; ++t) part += r[t] * z[t];
   part = wave_sum(part);
@@ -257,7 +290,24 @@ __global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__rest
       lam[t] += alpha * p[t];
       r[t] = lane_ok ? r[t] - alpha * y[t] : 0.f;
     }
-    group_precond<RK>(R, r, z);
+    if constexpr (STAIR) {
+      // z = D^-1 (2 r - S w), w = D^-1 r; the previous z row in recv was
+      // consumed before this iteration's first barrier
+      float wv[RK], wm, wp;
+      group_precond<RK>(R, r, wv);
+      ++e;
+      stage_w(wv);
+      __syncthreads();
+      if (sh.fail) return;
+      collect(false, true);
+      neighbours_w(wm, wp);
+      smv(wv, wm, wp);
+#pragma unroll
+      for (int t = 0; t < RK; ++t) wv[t] = lane_ok ? 2.f * r[t] - y[t] : 0.f;  // w is no longer needed
+      group_precond<RK>(R, wv, z);
+    } else {
+      group_precond<RK>(R, r, z);
+    }
 #pragma unroll
     for (int t = 0; t < RK; ++t) rz += r[t] * z[t];
     rz = wave_sum(rz);
@@ -320,10 +370,15 @@ uint32_t pcg_team_max_grid() {
     cached = 0;
     return 0;
   }
-  (void)hipFuncSetAttribute((const void *)k_pcg_team, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  // both variants (block Jacobi, stair) must fit; the launch uses the smaller
+  int per_cu_s = 0;
+  (void)hipFuncSetAttribute((const void *)k_pcg_team<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  (void)hipFuncSetAttribute((const void *)k_pcg_team<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   (void)hipGetLastError();
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_team, NT, smem) != hipSuccess) per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_team<false>, NT, smem) != hipSuccess) per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, k_pcg_team<true>, NT, smem) != hipSuccess) per_cu_s = 0;
   (void)hipGetLastError();
+  per_cu = per_cu < per_cu_s ? per_cu : per_cu_s;
   cached = (per_cu * cus) & ~15;  // whole groups of 16 (8 pairs, one per XCD slot)
   return (uint32_t)cached;
 }
@@ -331,7 +386,7 @@ uint32_t pcg_team_max_grid() {
 hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n_teams, TeamMailbox *mb,
                            uint32_t n_mb, const float *S, const float *Pinv, const float *gamma,
                            float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
-                           uint32_t *iters, uint32_t *flag, uint32_t *err, hipStream_t st) {
+                           uint32_t *iters, uint32_t *flag, uint32_t *err, bool stair, hipStream_t st) {
   if (n_teams == 0) return hipSuccess;
   const uint32_t cap = pcg_team_max_grid();
   if (cap < 16) return hipErrorNotSupported;
@@ -348,8 +403,8 @@ hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n
     uint32_t t0 = team0;
     void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, (void *)&S, (void *)&Pinv, (void *)&gamma,
                     &lambda, &max_iter, &tol, (void *)&status, &iters, &flag, &err};
-    e = hipLaunchCooperativeKernel((const void *)k_pcg_team, dim3(grid), dim3(NT), args,
-                                   (unsigned)sizeof(TeamShared), st);
+    const void *fn = stair ? (const void *)k_pcg_team<true> : (const void *)k_pcg_team<false>;
+    e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(NT), args, (unsigned)sizeof(TeamShared), st);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

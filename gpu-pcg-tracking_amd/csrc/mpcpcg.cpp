@@ -116,9 +116,9 @@ static hipError_t pcg_single(mpc_handle *h, uint32_t N, const float *S, const fl
     if (N <= pcg_res_max_knots())
       return launch_pcg_res(single(N), nullptr, nullptr, 1, S, P, gamma, lambda, max_iter, tol, nullptr, iters,
                             flag, stair, st);
-    if (!stair && N <= pcg_team_max_knots() && pcg_team_max_grid() >= 16)
+    if (N <= pcg_team_max_knots() && pcg_team_max_grid() >= 16)
       return launch_pcg_team(single(N), h->team_one, 1, h->mbox, h->n_mbox, S, P, gamma, lambda, max_iter, tol,
-                             nullptr, iters, flag, h->team_err, st);
+                             nullptr, iters, flag, h->team_err, stair, st);
   }
   return launch_pcg_stream(single(N), N, S, P, gamma, lambda, max_iter, tol, nullptr, nullptr, 0, iters, flag,
                            stair, st);
@@ -694,7 +694,7 @@ int mpc_batch_pcg_layout(const mpc_handle *h, uint32_t *out4) {
   out4[0] = h->n_bins;
   out4[1] = h->n_teams;
   out4[2] = h->n_big;
-  out4[3] = (h->cfg.flags & (FLAG_PCG_STREAM | FLAG_PCG_STAIR)) || per_launch == 0
+  out4[3] = (h->cfg.flags & FLAG_PCG_STREAM) || per_launch == 0
                 ? 0u
                 : (h->n_teams + per_launch - 1) / per_launch;
   return MPC_OK;
@@ -734,13 +734,8 @@ static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_st
     if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
     return e;
   }
-  // the team kernel has no stair variant: those instances go to the streaming kernel
-  if (stair)
-    e = launch_pcg_stream(b, pcg_team_max_knots(), h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status,
-                          h->team_list, h->n_teams, h->pcg_iters, h->pcg_flag, true, st);
-  else
-    e = launch_pcg_team(b, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->S, h->P, h->gamma, d_lambda, max_iter,
-                        tol, d_status, h->pcg_iters, h->pcg_flag, h->team_err, st);
+  e = launch_pcg_team(b, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->S, h->P, h->gamma, d_lambda, max_iter,
+                      tol, d_status, h->pcg_iters, h->pcg_flag, h->team_err, stair, st);
   if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
   if (e == hipSuccess)
     e = launch_pcg_stream(b, h->max_big_n, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, h->big_list,

@@ -97,8 +97,7 @@ typedef struct mpc_config {
  * D^-1: the reference's gato_form_ss_inner (oldschur.cuh:334-494), present
  * but unused there.  Applied as M r = D^-1 (2 r - S D^-1 r), it halves the
  * iteration count on the block-tridiagonal Schur system for one more block
- * product per iteration.  Instances of 128 < N <= 256 run on the streaming
- * kernel in this mode. */
+ * product (and one more neighbour exchange) per iteration. */
 #define MPC_FLAG_PCG_STAIR 8u
 
 typedef struct mpc_handle mpc_handle;
