@@ -120,16 +120,21 @@ hipError_t launch_pcg_stream(const Batch &b, uint32_t max_n, const float *S, con
                              const uint32_t *status, const uint32_t *inst_list, uint32_t n_list,
                              uint32_t *iters, uint32_t *flag, bool stair, hipStream_t st);
 // PCG with S resident on chip (N <= pcg_res_max_knots()); one workgroup per
-// bin of instances.  items[bin_off[w] .. bin_off[w+1]) = {inst, first group,
-// groups, 0}; items == nullptr: a single instance (0) in one bin.
-hipError_t launch_pcg_res(const Batch &b, const uint4 *items, const uint32_t *bin_off, uint32_t n_bins,
+// bin of instances, bins bin0 .. bin0 + n_bins - 1.  items[bin_off[w] ..
+// bin_off[w+1]) = {inst, first group, groups, 0}; items == nullptr: a single
+// instance (0) in one bin.
+hipError_t launch_pcg_res(const Batch &b, const uint4 *items, const uint32_t *bin_off, uint32_t bin0, uint32_t n_bins,
                           const float *S, const float *Pinv, const float *gamma, float *lambda,
                           uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters,
                           uint32_t *flag, bool stair, hipStream_t st);
 // PCG of 128 < N <= pcg_team_max_knots() on a team of two workgroups
-// (cooperative launch); err receives 1 if a team wait timed out.
+// (cooperative launch); err receives 1 if a team wait timed out.  With a bin
+// table (items, bin_off, n_bins) and bins_taken, the spare workgroups of the
+// last launch solve the first *bins_taken bins (the caller launches the
+// rest with launch_pcg_res from bin *bins_taken).
 hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n_teams, TeamMailbox *mb,
-                           uint32_t n_mb, const float *S, const float *Pinv, const float *gamma,
+                           uint32_t n_mb, const uint4 *items, const uint32_t *bin_off, uint32_t n_bins,
+                           uint32_t *bins_taken, const float *S, const float *Pinv, const float *gamma,
                            float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
                            uint32_t *iters, uint32_t *flag, uint32_t *err, bool stair, hipStream_t st);
 uint32_t pcg_team_max_knots();

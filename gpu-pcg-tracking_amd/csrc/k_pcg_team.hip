@@ -34,6 +34,7 @@
 // as well.
 #include "common.h"
 #include "pcg_onchip.h"
+#include "pcg_res_body.h"
 
 namespace mpc {
 
@@ -109,23 +110,22 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Half `(blk >> 3) & 1` of team team0 + pair(blk) on a 512-lane workgroup
+// with the TeamShared block at smem; blk is the workgroup's id among the
+// team workgroups of the launch.
 template <bool STAIR>
-__global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__restrict__ team_inst,
-                                                 uint32_t n_teams, uint32_t team0, TeamMailbox *mboxes,
-                                                 const float *__restrict__ S,
-                                                 const float *__restrict__ Pinv,
-                                                 const float *__restrict__ gamma,
-                                                 float *__restrict__ lambda, uint32_t max_iter,
-                                                 float tol, const uint32_t *__restrict__ status,
-                                                 uint32_t *__restrict__ iters_out,
-                                                 uint32_t *__restrict__ flag_out,
-                                                 uint32_t *__restrict__ err) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  TeamShared &sh = *reinterpret_cast<TeamShared *>(smem_raw);
+__device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Batch b,
+                                          const uint32_t *__restrict__ team_inst, uint32_t n_teams, uint32_t team0,
+                                          TeamMailbox *mboxes, const float *__restrict__ S,
+                                          const float *__restrict__ Pinv, const float *__restrict__ gamma,
+                                          float *__restrict__ lambda, uint32_t max_iter, float tol,
+                                          const uint32_t *__restrict__ status, uint32_t *__restrict__ iters_out,
+                                          uint32_t *__restrict__ flag_out, uint32_t *__restrict__ err) {
+  TeamShared &sh = *reinterpret_cast<TeamShared *>(smem);
   const uint32_t tid = threadIdx.x, g = tid >> 4, i = tid & 15u, w = tid >> 6, lane64 = tid & 63u;
   const uint32_t d = ror_step();
-  const int half = (int)((blockIdx.x >> 3) & 1u);
-  const uint32_t pair = (blockIdx.x >> 4) * 8u + (blockIdx.x & 7u);
+  const int half = (int)((blk >> 3) & 1u);
+  const uint32_t pair = (blk >> 4) * 8u + (blk & 7u);
   const uint32_t team = team0 + pair;
   if (team >= n_teams) return;
   const uint32_t inst = team_inst[team];
@@ -348,6 +348,42 @@ __global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__rest
   }
 }
 
+template <bool STAIR>
+__global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__restrict__ team_inst,
+                                                 uint32_t n_teams, uint32_t team0, TeamMailbox *mboxes,
+                                                 const float *__restrict__ S, const float *__restrict__ Pinv,
+                                                 const float *__restrict__ gamma, float *__restrict__ lambda,
+                                                 uint32_t max_iter, float tol, const uint32_t *__restrict__ status,
+                                                 uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out,
+                                                 uint32_t *__restrict__ err) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  team_body<STAIR>(blockIdx.x, smem_raw, b, team_inst, n_teams, team0, mboxes, S, Pinv, gamma, lambda, max_iter, tol,
+                   status, iters_out, flag_out, err);
+}
+
+// The last, partly filled team launch with resident bins in its spare
+// workgroups: ids below team_grid run the team body, the others bins
+// bin0 + (id - team_grid).  Bins need no co-residency; they only fill
+// workgroup slots that would idle until the launch ends.
+template <bool STAIR>
+__global__ void __launch_bounds__(NT) k_pcg_mixed(Batch b, const uint32_t *__restrict__ team_inst,
+                                                  uint32_t n_teams, uint32_t team0, TeamMailbox *mboxes,
+                                                  uint32_t team_grid, const uint4 *__restrict__ items,
+                                                  const uint32_t *__restrict__ bin_off, uint32_t bin0,
+                                                  const float *__restrict__ S, const float *__restrict__ Pinv,
+                                                  const float *__restrict__ gamma, float *__restrict__ lambda,
+                                                  uint32_t max_iter, float tol, const uint32_t *__restrict__ status,
+                                                  uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out,
+                                                  uint32_t *__restrict__ err) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  if (blockIdx.x < team_grid)
+    team_body<STAIR>(blockIdx.x, smem_raw, b, team_inst, n_teams, team0, mboxes, S, Pinv, gamma, lambda, max_iter,
+                     tol, status, iters_out, flag_out, err);
+  else
+    resident::body<STAIR>(bin0 + (blockIdx.x - team_grid), smem_raw, b, items, bin_off, S, Pinv, gamma, lambda,
+                          max_iter, tol, status, iters_out, flag_out);
+}
+
 __global__ void k_reset_mailboxes(TeamMailbox *mb, uint32_t n) {
   constexpr uint32_t W = sizeof(TeamMailbox) / sizeof(unsigned long long);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -358,35 +394,44 @@ __global__ void k_reset_mailboxes(TeamMailbox *mb, uint32_t n) {
 
 uint32_t pcg_team_max_knots() { return 2u * KNOTS; }
 
-// Largest cooperative grid of k_pcg_team on this device (0: not supported).
+constexpr size_t MIXED_SMEM =
+    sizeof(TeamShared) > sizeof(resident::ResShared) ? sizeof(TeamShared) : sizeof(resident::ResShared);
+
+// Largest cooperative grid of k_pcg_team (and k_pcg_mixed) on this device
+// (0: not supported).
 uint32_t pcg_team_max_grid() {
   static int cached = -1;
   if (cached >= 0) return (uint32_t)cached;
-  int dev = 0, coop = 0, cus = 0, per_cu = 0;
-  const size_t smem = sizeof(TeamShared);
+  int dev = 0, coop = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
     cached = 0;
     return 0;
   }
-  // both variants (block Jacobi, stair) must fit; the launch uses the smaller
-  int per_cu_s = 0;
-  (void)hipFuncSetAttribute((const void *)k_pcg_team<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-  (void)hipFuncSetAttribute((const void *)k_pcg_team<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  // every variant (block Jacobi, stair; teams only, mixed) must fit; the
+  // launches use the smallest
+  const void *fns[4] = {(const void *)k_pcg_team<false>, (const void *)k_pcg_team<true>,
+                        (const void *)k_pcg_mixed<false>, (const void *)k_pcg_mixed<true>};
+  const size_t smem[4] = {sizeof(TeamShared), sizeof(TeamShared), MIXED_SMEM, MIXED_SMEM};
+  int per_cu = 1 << 30;
+  for (int k = 0; k < 4; ++k) {
+    int n = 0;
+    (void)hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem[k]);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fns[k], NT, smem[k]) != hipSuccess) n = 0;
+    per_cu = n < per_cu ? n : per_cu;
+  }
   (void)hipGetLastError();
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_team<false>, NT, smem) != hipSuccess) per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, k_pcg_team<true>, NT, smem) != hipSuccess) per_cu_s = 0;
-  (void)hipGetLastError();
-  per_cu = per_cu < per_cu_s ? per_cu : per_cu_s;
   cached = (per_cu * cus) & ~15;  // whole groups of 16 (8 pairs, one per XCD slot)
   return (uint32_t)cached;
 }
 
 hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n_teams, TeamMailbox *mb,
-                           uint32_t n_mb, const float *S, const float *Pinv, const float *gamma,
+                           uint32_t n_mb, const uint4 *items, const uint32_t *bin_off, uint32_t n_bins,
+                           uint32_t *bins_taken, const float *S, const float *Pinv, const float *gamma,
                            float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
                            uint32_t *iters, uint32_t *flag, uint32_t *err, bool stair, hipStream_t st) {
+  if (bins_taken) *bins_taken = 0;
   if (n_teams == 0) return hipSuccess;
   const uint32_t cap = pcg_team_max_grid();
   if (cap < 16) return hipErrorNotSupported;
@@ -401,10 +446,24 @@ hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n
     if (e != hipSuccess) return e;
     Batch bb = b;
     uint32_t t0 = team0;
-    void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, (void *)&S, (void *)&Pinv, (void *)&gamma,
-                    &lambda, &max_iter, &tol, (void *)&status, &iters, &flag, &err};
-    const void *fn = stair ? (const void *)k_pcg_team<true> : (const void *)k_pcg_team<false>;
-    e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(NT), args, (unsigned)sizeof(TeamShared), st);
+    // the last launch takes the first bins into its spare workgroups
+    const bool last = team0 + n == n_teams;
+    uint32_t nb = last && items && bins_taken ? cap - grid : 0u;
+    nb = nb < n_bins ? nb : n_bins;
+    if (nb == 0) {
+      void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, (void *)&S, (void *)&Pinv, (void *)&gamma,
+                      &lambda, &max_iter, &tol, (void *)&status, &iters, &flag, &err};
+      const void *fn = stair ? (const void *)k_pcg_team<true> : (const void *)k_pcg_team<false>;
+      e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(NT), args, (unsigned)sizeof(TeamShared), st);
+    } else {
+      uint32_t tg = grid, bin0 = 0;
+      void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, &tg, (void *)&items, (void *)&bin_off, &bin0,
+                      (void *)&S, (void *)&Pinv, (void *)&gamma, &lambda, &max_iter, &tol, (void *)&status, &iters,
+                      &flag, &err};
+      const void *fn = stair ? (const void *)k_pcg_mixed<true> : (const void *)k_pcg_mixed<false>;
+      e = hipLaunchCooperativeKernel(fn, dim3(grid + nb), dim3(NT), args, (unsigned)MIXED_SMEM, st);
+      if (e == hipSuccess) *bins_taken = nb;
+    }
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

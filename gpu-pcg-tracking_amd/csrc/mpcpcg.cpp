@@ -114,11 +114,11 @@ static hipError_t pcg_single(mpc_handle *h, uint32_t N, const float *S, const fl
   const bool stair = flags & FLAG_PCG_STAIR;
   if (!(flags & FLAG_PCG_STREAM)) {
     if (N <= pcg_res_max_knots())
-      return launch_pcg_res(single(N), nullptr, nullptr, 1, S, P, gamma, lambda, max_iter, tol, nullptr, iters,
+      return launch_pcg_res(single(N), nullptr, nullptr, 0, 1, S, P, gamma, lambda, max_iter, tol, nullptr, iters,
                             flag, stair, st);
     if (N <= pcg_team_max_knots() && pcg_team_max_grid() >= 16)
-      return launch_pcg_team(single(N), h->team_one, 1, h->mbox, h->n_mbox, S, P, gamma, lambda, max_iter, tol,
-                             nullptr, iters, flag, h->team_err, stair, st);
+      return launch_pcg_team(single(N), h->team_one, 1, h->mbox, h->n_mbox, nullptr, nullptr, 0, nullptr, S, P,
+                             gamma, lambda, max_iter, tol, nullptr, iters, flag, h->team_err, stair, st);
   }
   return launch_pcg_stream(single(N), N, S, P, gamma, lambda, max_iter, tol, nullptr, nullptr, 0, iters, flag,
                            stair, st);
@@ -713,10 +713,11 @@ int mpc_batch_phase_times(mpc_handle *h, double *us7) {
 }
 
 // PCG of every instance of the batch on S, P, gamma of the workspace: long
-// horizons first on two-workgroup teams, then any beyond their reach on the
-// streaming kernel, then the bins of the resident kernel (or everything
-// streaming with FLAG_PCG_STREAM).  mid (optional) is recorded after the
-// team launches.
+// horizons first on two-workgroup teams (the last team launch also solves
+// the first bins of the resident kernel in its spare workgroups), then any
+// beyond their reach on the streaming kernel, then the remaining bins (or
+// everything streaming with FLAG_PCG_STREAM).  mid (optional) is recorded
+// after the team launches.
 static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_status, uint32_t max_iter, float tol,
                             hipEvent_t mid, hipStream_t st) {
   const Batch b = batch_of(h);
@@ -734,15 +735,17 @@ static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_st
     if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
     return e;
   }
-  e = launch_pcg_team(b, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->S, h->P, h->gamma, d_lambda, max_iter,
-                      tol, d_status, h->pcg_iters, h->pcg_flag, h->team_err, stair, st);
+  uint32_t taken = 0;
+  e = launch_pcg_team(b, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->bin_items, h->bin_off, h->n_bins, &taken,
+                      h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, h->pcg_iters, h->pcg_flag,
+                      h->team_err, stair, st);
   if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
   if (e == hipSuccess)
     e = launch_pcg_stream(b, h->max_big_n, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, h->big_list,
                           h->n_big, h->pcg_iters, h->pcg_flag, stair, st);
   if (e == hipSuccess)
-    e = launch_pcg_res(b, h->bin_items, h->bin_off, h->n_bins, h->S, h->P, h->gamma, d_lambda, max_iter, tol,
-                       d_status, h->pcg_iters, h->pcg_flag, stair, st);
+    e = launch_pcg_res(b, h->bin_items, h->bin_off, taken, h->n_bins - taken, h->S, h->P, h->gamma, d_lambda,
+                       max_iter, tol, d_status, h->pcg_iters, h->pcg_flag, stair, st);
   return e;
 }
 
