@@ -302,108 +302,93 @@ __global__ void __launch_bounds__(WG) k_form_kkt(Batch b, Params p, const float 
 }
 
 // --------------------------------------------------------------------------
-// merit terms, one group per (alpha, global knot)
+// merit terms, one lane per (alpha, global knot)
 // --------------------------------------------------------------------------
-struct MeritLds {
-  float z[40];  // trial x_k, u_k, x_{k+1}
-  float z0[16]; // trial x_0 (terminal slot)
-  float ref[4];
-  float sn[8], cs[8];
-  float M[49];
-  float bias[8];
-  float ee[4];
-};
+// A wave serves 64 consecutive global knots of one step length.  Their
+// trial points x_k, u_k, x_{k+1} form one contiguous range of the batch's xu
+// (instances lie back to back), staged once through LDS with coalesced
+// loads as xu + alpha dz; each lane then evaluates its knot alone: the lower
+// triangle of M, the bias forces, the end-effector position, the Cholesky
+// solve for qdd and the cost, with no divergence between lanes.
+constexpr int MERIT_NT = 64;
+constexpr int MERIT_SPAN = NXU * (MERIT_NT - 1) + NXU + NX;  // floats staged per wave
 
-__global__ void __launch_bounds__(WG) k_merit(Batch b, Params p, const float *__restrict__ xu,
-                                              const float *__restrict__ dz,
-                                              const float *__restrict__ ee_ref,
-                                              const float *__restrict__ xs,
-                                              const uint32_t *__restrict__ status, int p0, int np,
-                                              float *__restrict__ out) {
-  __shared__ MeritLds lds[GROUPS_PER_WG];
-  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
+__global__ void __launch_bounds__(MERIT_NT) k_merit(Batch b, Params p, const float *__restrict__ xu,
+                                                    const float *__restrict__ dz,
+                                                    const float *__restrict__ ee_ref,
+                                                    const float *__restrict__ xs,
+                                                    const uint32_t *__restrict__ status, int p0,
+                                                    float *__restrict__ out) {
+  __shared__ float Z[MERIT_SPAN];
+  const uint32_t lane = threadIdx.x;
   const uint32_t T = b.total_knots;
-  const uint32_t items = T * (uint32_t)np;
-  const uint32_t item_raw = blockIdx.x * GROUPS_PER_WG + grp;
-  const bool valid = item_raw < items;
-  const uint32_t item = valid ? item_raw : items - 1;
-  const uint32_t slot = item / T;  // 0 .. np-1
-  const uint32_t gk = item % T;
+  const uint32_t gk_raw = blockIdx.x * MERIT_NT + lane;
+  const bool valid = gk_raw < T;
+  const uint32_t gk = valid ? gk_raw : T - 1;
+  const uint32_t slot = blockIdx.y;
   const int pidx = p0 + (int)slot;
-  const KnotRef kr = locate(b, gk);
-  bool active = !(status && !(status[kr.inst] & ST_ACTIVE));
   const float alpha = (pidx >= 8) ? 0.f : -1.f / (float)(1u << pidx);
   const bool with_x0 = pidx < 8;
-  MeritLds &L = lds[grp];
-  const float *xi = xu + (size_t)NXU * kr.K0 - (size_t)NU * kr.inst;
-  const float *di = dz ? dz + (size_t)NXU * kr.K0 - (size_t)NU * kr.inst : nullptr;
-  const bool dyn = kr.k + 1 < kr.N;
-  const int len = dyn ? NXU + NX : NX;
-  for (int i = lane; i < len; i += GROUP) {
-    const size_t o = (size_t)NXU * kr.k + i;
-    L.z[i] = di ? xi[o] + alpha * di[o] : xi[o];
-  }
-  if (!dyn && with_x0)
-    for (int i = lane; i < NX; i += GROUP) L.z0[i] = di ? xi[i] + alpha * di[i] : xi[i];
-  if (lane < 3) L.ref[lane] = ee_ref[(size_t)6 * (kr.K0 + kr.k) + lane];
-  if (lane < NQ) {
-    float s, c;
-    sincosf(L.z[lane], &s, &c);  // z written by this lane above (lane < 16 covers i = lane)
-    L.sn[lane] = s;
-    L.cs[lane] = c;
-  }
+  const KnotRef kr = locate(b, gk);
+  const uint32_t o = NXU * (kr.K0 + kr.k) - NU * kr.inst;  // x_k in the batch xu
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)o);
+  const uint32_t n_xu = NXU * T - NU * b.n_inst;
+  const uint32_t n_stage = n_xu - lo < (uint32_t)MERIT_SPAN ? n_xu - lo : (uint32_t)MERIT_SPAN;
+  for (uint32_t j = lane; j < n_stage; j += MERIT_NT) Z[j] = dz ? xu[lo + j] + alpha * dz[lo + j] : xu[lo + j];
   __syncthreads();
-  float s[NQ], c[NQ];
+  if (!valid) return;
+  const bool active = !(status && !(status[kr.inst] & ST_ACTIVE));
+  if (!active) return;
+  const float *z = Z + (o - lo);
+  const bool dyn = kr.k + 1 < kr.N;
+  float s[NQ], c[NQ], ee[3];
 #pragma unroll
-  for (int i = 0; i < NQ; ++i) {
-    s[i] = L.sn[i];
-    c[i] = L.cs[i];
+  for (int i = 0; i < NQ; ++i) sincosf(z[i], &s[i], &c[i]);
+  fk_pos(s, c, ee);
+  // cost: 0.5 sum wx x^2 + 0.5 sum wu u^2 + 0.5 w |ee - ref|^2 (reference: wx = (0, QD), wu = R)
+  float J = 0.f;
+#pragma unroll
+  for (int i = 0; i < NX; ++i) J += 0.5f * (p.wx[i] * z[i] * z[i]);
+  if (dyn) {
+#pragma unroll
+    for (int i = 0; i < NU; ++i) J += 0.5f * (p.wu[i] * z[NX + i] * z[NX + i]);
   }
-  if (dyn && lane < NQ) {
-    float col[NQ];
-    mass_column(lane, s, c, col);
+  const float *r = ee_ref + (size_t)6 * gk;
 #pragma unroll
-    for (int r = 0; r < NQ; ++r) L.M[r + 7 * lane] = col[r];
-  } else if (dyn && lane == NQ) {
-    float qdv[NQ], zz[NQ], tau[NQ];
+  for (int i = 0; i < 3; ++i) {
+    const float e = ee[i] - r[i];
+    J += 0.5f * p.q_ee * e * e;
+  }
+  float ck = 0.f;
+  if (dyn) {
+    float M[49], Lc[49], bias[NQ], zero[NQ], qd[NQ], rhs[NQ], qdd[NQ];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) mass_column_lower(j, s, c, M + 7 * j);
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
-      qdv[i] = L.z[NQ + i];
-      zz[i] = 0.f;
+      qd[i] = z[NQ + i];
+      zero[i] = 0.f;
     }
-    rnea<false>(s, c, qdv, zz, tau, nullptr, nullptr, nullptr, nullptr, nullptr);
+    rnea<false>(s, c, qd, zero, bias, nullptr, nullptr, nullptr, nullptr, nullptr);
+    cholesky7(M, Lc);
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) L.bias[i] = tau[i];
-  } else if (lane == NQ + 1) {
-    fk_pos(s, c, L.ee);
-  }
-  __syncthreads();
-  if (lane == 0 && active && valid) {
-    // cost: 0.5 sum wx x^2 + 0.5 sum wu u^2 + 0.5 w |ee - ref|^2 (reference: wx = (0, QD), wu = R)
-    float J = 0.f;
-    for (int i = 0; i < NX; ++i) J += 0.5f * (p.wx[i] * L.z[i] * L.z[i]);
-    if (dyn)
-      for (int i = 0; i < NU; ++i) J += 0.5f * (p.wu[i] * L.z[NX + i] * L.z[NX + i]);
-    for (int i = 0; i < 3; ++i) {
-      const float e = L.ee[i] - L.ref[i];
-      J += 0.5f * p.q_ee * e * e;
+    for (int i = 0; i < NQ; ++i) rhs[i] = z[NX + i] - bias[i];
+    chol_solve7(Lc, rhs, qdd);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      ck += fabsf(z[NXU + i] - (z[i] + p.dt * z[NQ + i]));
+      ck += fabsf(z[NXU + NQ + i] - (z[NQ + i] + p.dt * qdd[i]));
     }
-    float ck = 0.f;
-    if (dyn) {
-      float Lc[49], rhs[NQ], qdd[NQ];
-      cholesky7(L.M, Lc);
-      for (int i = 0; i < NQ; ++i) rhs[i] = L.z[NX + i] - L.bias[i];
-      chol_solve7(Lc, rhs, qdd);
-      for (int i = 0; i < NQ; ++i) {
-        ck += fabsf(L.z[NXU + i] - (L.z[i] + p.dt * L.z[NQ + i]));
-        ck += fabsf(L.z[NXU + NQ + i] - (L.z[NQ + i] + p.dt * qdd[i]));
-      }
-    } else if (with_x0) {
-      const float *x0s = xs + (size_t)NX * kr.inst;
-      for (int i = 0; i < NX; ++i) ck += fabsf(L.z0[i] - x0s[i]);
+  } else if (with_x0) {
+    // the terminal slot also carries the initial-state constraint of the instance
+    const size_t o0 = (size_t)NXU * kr.K0 - (size_t)NU * kr.inst;
+    const float *x0s = xs + (size_t)NX * kr.inst;
+    for (int i = 0; i < NX; ++i) {
+      const float z0 = dz ? xu[o0 + i] + alpha * dz[o0 + i] : xu[o0 + i];
+      ck += fabsf(z0 - x0s[i]);
     }
-    out[(size_t)slot * T + gk] = J + p.mu * ck;
   }
+  out[(size_t)slot * T + gk] = J + p.mu * ck;
 }
 
 }  // namespace
@@ -440,9 +425,8 @@ hipError_t launch_merit(const Batch &b, const Params &p, const float *xu, const 
                         const float *ee_ref, const float *xs, const uint32_t *status, int p0, int np,
                         float *out, hipStream_t st) {
   if (b.total_knots == 0 || np <= 0) return hipSuccess;
-  const uint32_t items = b.total_knots * (uint32_t)np;
-  hipLaunchKernelGGL(k_merit, dim3((items + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG), 0, st, b, p,
-                     xu, dz, ee_ref, xs, status, p0, np, out);
+  hipLaunchKernelGGL(k_merit, dim3((b.total_knots + MERIT_NT - 1) / MERIT_NT, (uint32_t)np), dim3(MERIT_NT), 0, st,
+                     b, p, xu, dz, ee_ref, xs, status, p0, out);
   return hipGetLastError();
 }
 

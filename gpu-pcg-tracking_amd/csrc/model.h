@@ -186,6 +186,38 @@ __device__ __forceinline__ void mass_column(int j, const float *s, const float *
   }
 }
 
+// Rows j..6 of column j of M(q), the lower triangle that a Cholesky factor
+// reads: the recursion of mass_column started at joint j (the accelerations
+// of the joints before j are exactly zero there, and the backward pass below
+// joint j only carries rows above the diagonal).
+__device__ __forceinline__ void mass_column_lower(int j, const float *s, const float *c, float *col) {
+  float a[NQ][6];
+#pragma unroll
+  for (int i = j; i < NQ; ++i) {
+    if (i == j) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) a[i][k] = k == 2 ? 1.f : 0.f;
+    } else {
+      X_mv(i, s[i], c[i], a[i - 1], a[i]);
+    }
+  }
+  float f[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int i = NQ - 1; i >= j; --i) {
+    float fi[6];
+    inertia_mv(i, a[i], fi);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) f[k] += fi[k];
+    col[i] = f[2];
+    if (i > j) {
+      float t[6];
+      XT_fv(i, s[i], c[i], f, t);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) f[k] = t[k];
+    }
+  }
+}
+
 // RNEA with gravity 0.  Returns tau; optionally stores per-link v, a, the
 // total (backward-accumulated) f, X_i v_{i-1} and X_i a_{i-1} (6 each,
 // link-major, stride 6) for the derivative passes.
@@ -381,19 +413,24 @@ __device__ __forceinline__ void fk_pos(const float *s, const float *c, float *p)
 }
 
 // Lower Cholesky factor L of a 7x7 symmetric positive definite M (both
-// column-major), one lane.
+// column-major; only the lower triangle of M is read), one lane.
 __device__ __forceinline__ void cholesky7(const float *M, float *L) {
+#pragma unroll
   for (int j = 0; j < NQ; ++j) {
     float d = M[j + 7 * j];
+#pragma unroll
     for (int k = 0; k < j; ++k) d -= L[j + 7 * k] * L[j + 7 * k];
     d = sqrtf(d);
     L[j + 7 * j] = d;
     const float inv = 1.f / d;
+#pragma unroll
     for (int i = j + 1; i < NQ; ++i) {
       float s = M[i + 7 * j];
+#pragma unroll
       for (int k = 0; k < j; ++k) s -= L[i + 7 * k] * L[j + 7 * k];
       L[i + 7 * j] = s * inv;
     }
+#pragma unroll
     for (int i = 0; i < j; ++i) L[i + 7 * j] = 0.f;
   }
 }
