@@ -59,7 +59,6 @@ PCG_FLOPS_PER_KNOT_ITER = 1708
 PCG_LDS_BYTES_PER_KNOT_ITER = 4 * 84 * 14
 FP32_PEAK_TFLOPS = 157.3
 LDS_PEAK_GBS = 128 * 256 * 2.4  # 128 B/clk/CU x 256 CUs x 2.4 GHz
-TEAM_MIN_N = 129
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batched_1024.json")
 
 
@@ -393,14 +392,16 @@ def main():
     ph = solver.batch_phase_times()
     solver.batch_set_timing(False)
     log_t = pcg_log.cpu().numpy().reshape(n, K)
-    team = horizons >= TEAM_MIN_N
-    units = float(sum(int(horizons[i]) * int(log_t[i, j]) for i in range(n) for j in range(K)
-                      if team[i] and log_t[i, j] >= 0)) / K
     # the team PCG of one SQP iteration is `launches` cooperative launches
-    # (at most one workgroup per CU, two per team); per launch: the phase's
-    # knot-iterations and time divided by that count
+    # (at most one workgroup per CU, two per team; the last launch also
+    # solves the first resident bins in its spare workgroups); per launch:
+    # the knot-iterations of every instance those launches solve and the
+    # phase time, divided by that count
     layout = solver.batch_pcg_layout()
     launches = max(layout["team_launches"], 1)
+    in_team_phase = solver.batch_pcg_launch_map() < layout["team_launches"]
+    units = float(sum(int(horizons[i]) * int(log_t[i, j]) for i in range(n) for j in range(K)
+                      if in_team_phase[i] and log_t[i, j] >= 0)) / K
     t_phase = ph[2] / K
     t_launch = t_phase / launches
     units_launch = units / launches
@@ -410,7 +411,7 @@ def main():
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f)["kernels"].get("k_pcg_team", {}).get("hbm_bytes_per_launch")
+            traffic = json.load(f).get("team_phase", {}).get("hbm_bytes_per_launch")
     phases = {name: round(v / K, 2) for name, v in zip(PHASES, ph)}
     kkt_phase_us = (ph[1] + ph[2] + ph[3] + ph[4]) / K
 
@@ -524,7 +525,8 @@ def main():
                      "launches_per_sqp_iter": launches, "pcg_layout": layout,
                      "note": "algorithmic bytes = 2296 B per knot and PCG iteration (S, Pinv and vectors); "
                              "the kernel keeps S and Pinv on chip, so they come from LDS and registers and "
-                             "the HBM traffic (PMC, per launch) is the one-time load",
+                             "the HBM traffic (PMC, per launch) is the one-time load; the last team launch "
+                             "(k_pcg_mixed) also solves the first resident bins, counted with it",
                      "lds": {"achieved": round(lds_gbs, 1), "peak": round(LDS_PEAK_GBS, 1), "unit": "GB/s",
                              "frac": round(lds_gbs / LDS_PEAK_GBS, 4)},
                      "fp32": {"achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -42,7 +42,7 @@ EXPORTS = (
     "mpc_status_string", "mpc_fk", "mpc_dynamics_gradient", "mpc_form_kkt", "mpc_form_schur",
     "mpc_pcg_solve", "mpc_compute_dz", "mpc_compute_merit", "mpc_line_search_merits",
     "mpc_sqp_solve", "mpc_batch_setup", "mpc_batch_sqp", "mpc_batch_set_timing",
-    "mpc_batch_phase_times", "mpc_batch_pcg_layout", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
+    "mpc_batch_phase_times", "mpc_batch_pcg_layout", "mpc_batch_pcg_launch_map", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
     "mpc_batch_qp_loss", "mpc_batch_qp_adjoint", "mpc_synth_instance", "mpc_synth_horizons",
     "mpc_inverse_dynamics", "mpc_default_closed_loop", "mpc_closed_loop", "mpc_synth_track", "mpc_direct_solve",
 )
@@ -123,6 +123,7 @@ def lib():
             "mpc_batch_set_timing": [vp, C.c_int],
             "mpc_batch_phase_times": [vp, C.POINTER(C.c_double)],
             "mpc_batch_pcg_layout": [vp, C.POINTER(C.c_uint32)],
+            "mpc_batch_pcg_launch_map": [vp, vp],
             "mpc_set_cost_weights": [vp, vp, vp],
             "mpc_get_cost_weights": [vp, vp, vp],
             "mpc_batch_qp_layer": [vp, vp, vp, vp, vp, f32, u32, f32, vp, vp, vp],
@@ -316,6 +317,7 @@ class Solver:
     def batch_setup(self, horizons):
         arr = (C.c_uint32 * len(horizons))(*[int(x) for x in horizons])
         _check(lib().mpc_batch_setup(self.h, len(horizons), arr), "mpc_batch_setup")
+        self._n_inst = len(horizons)
 
     def batch_sqp(self, xu, lam, ee_ref, rho, sqp_iters, status, pcg_iters=None):
         _check(lib().mpc_batch_sqp(self.h, _ptr(xu), _ptr(lam), _ptr(ee_ref), _ptr(rho), sqp_iters,
@@ -399,6 +401,18 @@ class Solver:
         out = (C.c_uint32 * 4)()
         _check(lib().mpc_batch_pcg_layout(self.h, out), "mpc_batch_pcg_layout")
         return dict(zip(("bins", "teams", "streamed", "team_launches"), list(out)))
+
+    def batch_pcg_launch_map(self):
+        """Per instance, the PCG launch of a phase that solves it: 0 .. L-1
+        the team launches (the last also holds the first resident bins), then
+        the streaming launch (if any), then the resident launch."""
+        import numpy as np
+        n = getattr(self, "_n_inst", None)
+        if n is None:
+            raise MpcError("batch_pcg_launch_map: no batch layout (call batch_setup first)")
+        out = np.zeros(n, np.uint32)
+        _check(lib().mpc_batch_pcg_launch_map(self.h, out.ctypes.data), "mpc_batch_pcg_launch_map")
+        return out
 
     def batch_phase_times(self):
         out = (C.c_double * 7)()

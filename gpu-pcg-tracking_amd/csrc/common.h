@@ -139,6 +139,7 @@ hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n
                            uint32_t *iters, uint32_t *flag, uint32_t *err, bool stair, hipStream_t st);
 uint32_t pcg_team_max_knots();
 uint32_t pcg_team_max_grid();
+void pcg_team_plan(uint32_t n_teams, uint32_t n_bins, uint32_t *launches, uint32_t *bins_taken);
 uint32_t pcg_res_groups(uint32_t N);
 uint32_t pcg_res_bin_groups();  // 16-lane groups per workgroup
 uint32_t pcg_res_max_knots();

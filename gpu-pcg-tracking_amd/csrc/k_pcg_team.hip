@@ -426,6 +426,20 @@ uint32_t pcg_team_max_grid() {
   return (uint32_t)cached;
 }
 
+// Launch plan of n_teams teams: the number of cooperative launches and how
+// many of n_bins resident bins the last one solves in its spare workgroups.
+void pcg_team_plan(uint32_t n_teams, uint32_t n_bins, uint32_t *launches, uint32_t *bins_taken) {
+  *launches = 0;
+  *bins_taken = 0;
+  const uint32_t cap = pcg_team_max_grid();
+  if (n_teams == 0 || cap < 16) return;
+  const uint32_t per_launch = cap / 2;
+  *launches = (n_teams + per_launch - 1) / per_launch;
+  const uint32_t n_last = n_teams - (*launches - 1) * per_launch;
+  const uint32_t spare = cap - 16u * ((n_last + 7u) / 8u);
+  *bins_taken = spare < n_bins ? spare : n_bins;
+}
+
 hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n_teams, TeamMailbox *mb,
                            uint32_t n_mb, const uint4 *items, const uint32_t *bin_off, uint32_t n_bins,
                            uint32_t *bins_taken, const float *S, const float *Pinv, const float *gamma,
@@ -447,9 +461,11 @@ hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n
     Batch bb = b;
     uint32_t t0 = team0;
     // the last launch takes the first bins into its spare workgroups
-    const bool last = team0 + n == n_teams;
-    uint32_t nb = last && items && bins_taken ? cap - grid : 0u;
-    nb = nb < n_bins ? nb : n_bins;
+    uint32_t nb = 0;
+    if (team0 + n == n_teams && items && bins_taken) {
+      uint32_t nl = 0;
+      pcg_team_plan(n_teams, n_bins, &nl, &nb);
+    }
     if (nb == 0) {
       void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, (void *)&S, (void *)&Pinv, (void *)&gamma,
                       &lambda, &max_iter, &tol, (void *)&status, &iters, &flag, &err};

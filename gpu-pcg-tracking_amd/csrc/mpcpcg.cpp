@@ -64,6 +64,7 @@ struct mpc_handle {
   float *h_rho = nullptr;
   // batch layout
   std::vector<uint32_t> horizons, k0;
+  std::vector<uint32_t> pcg_launch;  // PCG launch of each instance within a phase (mpc_batch_pcg_launch_map)
   uint32_t n_inst = 0, total = 0, max_n = 0;
   bool batch_ready = false;
   // per-dimension cost weights (mpc_set_cost_weights; derived from cfg by default)
@@ -615,6 +616,19 @@ int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
     HIP_TRY(hipMemcpy(h->big_list, big.data(), sizeof(uint32_t) * big.size(), hipMemcpyHostToDevice));
   if (!teams.empty())
     HIP_TRY(hipMemcpy(h->team_list, teams.data(), sizeof(uint32_t) * teams.size(), hipMemcpyHostToDevice));
+  // launch of each instance within a PCG phase: team launches 0 .. nl-1
+  // (the last also takes the first bins), then the streaming launch, then
+  // the resident launch
+  {
+    uint32_t nl = 0, taken = 0;
+    pcg_team_plan((uint32_t)teams.size(), (uint32_t)bins.size(), &nl, &taken);
+    const uint32_t per_launch = pcg_team_max_grid() / 2;
+    h->pcg_launch.assign(n_inst, 0u);
+    for (size_t t = 0; t < teams.size(); ++t) h->pcg_launch[teams[t]] = (uint32_t)(t / per_launch);
+    for (uint32_t i : big) h->pcg_launch[i] = nl;
+    for (size_t w = 0; w < bins.size(); ++w)
+      for (uint32_t i : bins[w].inst) h->pcg_launch[i] = w < taken ? nl - 1 : nl + (big.empty() ? 0u : 1u);
+  }
   h->n_teams = (uint32_t)teams.size();
   h->n_bins = (uint32_t)bins.size();
   h->n_big = (uint32_t)big.size();
@@ -697,6 +711,14 @@ int mpc_batch_pcg_layout(const mpc_handle *h, uint32_t *out4) {
   out4[3] = (h->cfg.flags & FLAG_PCG_STREAM) || per_launch == 0
                 ? 0u
                 : (h->n_teams + per_launch - 1) / per_launch;
+  return MPC_OK;
+}
+
+int mpc_batch_pcg_launch_map(const mpc_handle *h, uint32_t *launch_of_inst) {
+  if (!h || !launch_of_inst) return MPC_ERR_ARG;
+  if (!h->batch_ready) return MPC_ERR_NOT_READY;
+  const bool one = h->cfg.flags & (FLAG_PCG_STREAM | FLAG_DIRECT_SOLVE);
+  for (uint32_t i = 0; i < h->n_inst; ++i) launch_of_inst[i] = one ? 0u : h->pcg_launch[i];
   return MPC_OK;
 }
 

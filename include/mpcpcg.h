@@ -225,13 +225,23 @@ int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee
 
 /* Device-event timing of the phases of the last mpc_batch_sqp call made
  * with timing enabled (mpc_batch_set_timing(h, 1)), summed over its SQP
- * iterations, in microseconds: linearisation, Schur formation, PCG on
- * two-workgroup teams, the rest of PCG, dz, line-search merits, update. */
+ * iterations, in microseconds: linearisation, Schur formation, PCG team
+ * launches (with the bins that share the last one), the rest of PCG, dz,
+ * line-search merits, update. */
 int mpc_batch_set_timing(mpc_handle *h, int enable);
 /* PCG packing of the batch layout: out4 = {resident-kernel bins, two-
  * workgroup teams, instances on the streaming kernel, cooperative team
  * launches per PCG phase}. */
 int mpc_batch_pcg_layout(const mpc_handle *h, uint32_t *out4);
+/* The PCG launch that solves each instance (n_inst values) within one PCG
+ * phase: 0 .. L-1 the cooperative team launches (the last one also solves
+ * the first resident bins in its spare workgroups), L the streaming launch
+ * when there are instances beyond the teams' reach, then the launch of the
+ * remaining resident bins; L = out4[3] of mpc_batch_pcg_layout.  All 0 with
+ * MPC_FLAG_PCG_STREAM or MPC_FLAG_DIRECT_SOLVE (one launch).  No
+ * counterpart in the reference (its batch is one instance); for timing
+ * analyses such as the per-launch roofline. */
+int mpc_batch_pcg_launch_map(const mpc_handle *h, uint32_t *launch_of_inst);
 int mpc_batch_phase_times(mpc_handle *h, double *us7);
 
 /* ---- closed-loop tracking ----------------------------------------------------

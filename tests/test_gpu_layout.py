@@ -1,0 +1,71 @@
+"""GPU: the PCG launch plan of a batch.  Long horizons run on cooperative
+two-workgroup team launches; the last team launch also solves the first
+(longest) resident bins in its spare workgroups; horizons beyond a team go
+to the streaming kernel; the remaining bins to the resident kernel.  Every
+instance must be solved exactly once, whichever launch holds it."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def test_launch_map_and_every_instance_solved(pkg, orc):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    # 140 teams (two launches at 128 teams per launch), bins of every
+    # packing (one 128, two 64s, several short instances), two streamed
+    horizons = [200] * 140 + [64] * 20 + [128] * 10 + [300] * 2 + [3] * 5
+    rng = np.random.default_rng(7)
+    horizons = [int(h) for h in rng.permutation(horizons)]
+    n, T = len(horizons), sum(horizons)
+    cfg = pkg.default_config(max_knots=512, max_instances=n, max_total_knots=T)
+    s = pkg.Solver(cfg)
+    try:
+        s.batch_setup(horizons)
+        lay = s.batch_pcg_layout()
+        L = lay["team_launches"]
+        m = s.batch_pcg_launch_map()
+        h = np.asarray(horizons)
+        print("layout", lay, "launch counts", np.bincount(m))
+        team, big, res = (h > 128) & (h <= 256), h > 256, h <= 128
+        assert lay["teams"] == team.sum() and lay["streamed"] == big.sum() and L >= 1
+        assert (m[team] < L).all()
+        # teams fill the launches in instance order, at most one launch's share each
+        per = np.bincount(m[team], minlength=L)
+        assert per[:-1].min() == per.max() and per[-1] <= per[0]
+        assert np.all(np.diff(m[team].astype(np.int64)) >= 0)
+        assert (m[big] == L).all()
+        rest = L + 1 if big.any() else L
+        assert np.isin(m[res], [L - 1, rest]).all()
+        taken, left = h[res & (m == L - 1)], h[res & (m == rest)]
+        assert taken.size > 0
+        if left.size:
+            assert taken.min() >= left.max()  # the longest bins go into the team launch
+        # a batched solve touches every instance: each one's PCG ran (or it
+        # started converged) and its multipliers were written
+        xu = torch.zeros(21 * T - 7 * n, device="cuda")
+        ee = torch.zeros(6 * T, device="cuda")
+        k0 = np.concatenate([[0], np.cumsum(h)])
+        xo = np.concatenate([[0], np.cumsum(21 * h - 7)])
+        for i, N in enumerate(horizons):
+            x0, qref = pkg.synth_instance(500 + i, N)
+            e = np.concatenate([orc.fk(q.astype(np.float64)) for q in qref]).astype(np.float32)
+            ee[6 * k0[i]:6 * k0[i + 1]] = torch.from_numpy(e).cuda()
+            xi = np.zeros(21 * N - 7, np.float32)
+            for k in range(N):
+                xi[21 * k:21 * k + 14] = x0
+            xu[xo[i]:xo[i + 1]] = torch.from_numpy(xi).cuda()
+        lam = torch.zeros(14 * T, device="cuda")
+        rho = torch.full((n,), 1e-3, device="cuda")
+        st = torch.zeros(n, dtype=torch.int32, device="cuda")
+        pit = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+        s.batch_sqp(xu, lam, ee, rho, 1, st, pit)
+        torch.cuda.synchronize()
+        pit, lam_h = pit.cpu().numpy(), lam.cpu().numpy()
+        assert (pit > 0).all(), np.nonzero(pit <= 0)
+        for i in range(n):
+            assert np.abs(lam_h[14 * k0[i]:14 * k0[i + 1]]).max() > 0, (i, horizons[i], m[i])
+        assert (st.cpu().numpy() & ~3).max() == 0
+    finally:
+        s.close()
