@@ -11,8 +11,10 @@ random phase); start trajectory x_k = x0, u_k = 0, lambda = 0, rho = 1e-3.
 One step = one batched SQP solve (3 SQP iterations, every phase on the
 device) over all instances, starting from the same initial iterate; each
 SQP iteration of an instance is one KKT solve (linearisation, Schur +
-block-Jacobi, PCG, dz, 8-alpha line search, update).  value = KKT
-solves / s over the whole step.  Also reported: the plain
+block-Jacobi, PCG, dz, 8-alpha line search, update).  The solve is
+captured once as a HIP graph and replayed; value = KKT solves / s over the
+whole step.  The same launch sequence issued without capture is timed
+beside it ("graph"), with a bitwise comparison of the two results.  Also reported: the plain
 one-instance-per-launch solve (reference call pattern) on a sample, the
 per-phase split, the roofline of the dominant kernel and a CPU baseline
 (the sparse LDL^T solve of the Schur system, fp32, one core).
@@ -364,21 +366,32 @@ def main():
         pcg_log.fill_(-1)
         solver.batch_sqp(xu, lam, ee, rho, K, status, pcg_log)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t_wall = time.perf_counter()
-    e0.record()
-    for _ in range(args.steps):
-        step()
-    e1.record()
-    torch.cuda.synchronize()
-    t_wall = time.perf_counter() - t_wall
-    ms_total = e0.elapsed_time(e1)
-    ms_step = ms_total / args.steps
-    log = pcg_log.cpu().numpy().reshape(n, K)
-    st = status.cpu().numpy()
+    def timed(graph):
+        # graph: the batched solve is captured once (in the first warm-up
+        # step) and replayed as a HIP graph
+        solver.batch_set_graph(graph)
+        for _ in range(max(args.warmup, 1 if graph else 0)):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t = time.perf_counter()
+        e0.record()
+        for _ in range(args.steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t
+        solver.batch_set_graph(False)
+        res = [a.cpu().numpy().copy() for a in (xu, lam, rho, status, pcg_log)]
+        return e0.elapsed_time(e1) / args.steps, t, res
+
+    # headline: the solve as a captured HIP graph; beside it the same launch
+    # sequence issued eagerly (cooperative team launches)
+    ms_step, t_wall, res_graph = timed(True)
+    ms_eager, _, res_eager = timed(False)
+    graph_equal = all(np.array_equal(a, b) for a, b in zip(res_graph, res_eager))
+    log = res_graph[4].reshape(n, K)
+    st = res_graph[3]
     executed = log >= 0
     kkt_per_step = int(executed.sum())
     value = kkt_per_step / (ms_step * 1e-3)
@@ -447,16 +460,8 @@ def main():
     # for one more block product each; the 128 < N <= 256 instances on the
     # streaming kernel)
     solver.set_config(flags=pkg.FLAG_PCG_STAIR)
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(args.steps):
-        step()
-    e1.record()
-    torch.cuda.synchronize()
-    ms_stair = e0.elapsed_time(e1) / args.steps
-    lg_s, st_s = pcg_log.cpu().numpy(), status.cpu().numpy()
+    ms_stair, _, r = timed(True)
+    lg_s, st_s = r[4], r[3]
     its_s = lg_s[lg_s >= 0]
     stair = {"kkt_solves_per_sec": round(int((lg_s >= 0).sum()) / (ms_stair * 1e-3), 1),
              "ms_per_step": round(ms_stair, 4),
@@ -466,22 +471,14 @@ def main():
     # the same workload with the direct Schur solve (block LDL^T on the GPU,
     # one fp64-residual refinement of each KKT step) instead of PCG
     solver.set_config(flags=pkg.FLAG_DIRECT_SOLVE)
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(args.steps):
-        step()
-    e1.record()
-    torch.cuda.synchronize()
-    ms_direct = e0.elapsed_time(e1) / args.steps
+    ms_direct, _, r = timed(True)
+    st_d = r[3]
+    kkt_d = int((r[4] >= 0).sum())
     solver.batch_set_timing(True)
     step()
     torch.cuda.synchronize()
     ph_d = solver.batch_phase_times()
     solver.batch_set_timing(False)
-    st_d = status.cpu().numpy()
-    kkt_d = int((pcg_log.cpu().numpy() >= 0).sum())
     direct = {"kkt_solves_per_sec": round(kkt_d / (ms_direct * 1e-3), 1), "ms_per_step": round(ms_direct, 4),
               "phase_us_per_sqp_iter": {name: round(v / K, 2) for name, v in zip(PHASES, ph_d)},
               "status": {"aborted": int((st_d & 4 != 0).sum()), "nonfinite": int((st_d & 8 != 0).sum())},
@@ -507,7 +504,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded splitmix64 start states and sinusoid EE references)",
         "config": {"workload": "batched SQP: 1024 KUKA iiwa EE-tracking instances, N ~ U{32,64,128,256}, "
-                               "3 SQP iterations, block-Jacobi PCG tol 1e-5",
+                               "3 SQP iterations, block-Jacobi PCG tol 1e-5, replayed as a captured HIP graph",
                    "instances": n, "total_knots": T, "sqp_iters": K, "pcg_exit_tol": cfg.pcg_exit_tol,
                    "pcg_max_iter": cfg.pcg_max_iter},
         "sqp_iters_per_sec": round(value, 1),
@@ -517,7 +514,13 @@ def main():
         "phase_us_per_sqp_iter": phases,
         "pcg_iters": {"mean": round(float(its.mean()), 1), "max": int(its.max()), "min": int(its.min()),
                       "hit_max_iter": int((st & 2 != 0).sum())},
-        "status": {"aborted": int((st & 4 != 0).sum()), "nonfinite": int((st & 8 != 0).sum())},
+        "status": {"aborted": int((st & 4 != 0).sum()), "nonfinite": int((st & 8 != 0).sum()),
+                   "pcg_fault": int((st & 16 != 0).sum())},
+        "graph": {"headline_is_graph_replay": True, "bitwise_equal_to_eager": bool(graph_equal),
+                  "eager_ms_per_step": round(ms_eager, 4),
+                  "eager_kkt_solves_per_sec": round(kkt_per_step / (ms_eager * 1e-3), 1),
+                  "note": "eager = the same launch sequence without capture, PCG teams as cooperative launches; "
+                          "in the graph they are ordinary kernel nodes (bounded waits, fault status)"},
         "roofline": {"kernel": "k_pcg_team", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": int(PCG_BYTES_PER_KNOT_ITER * units_launch),

@@ -29,8 +29,8 @@ NQ, NX, NU, NXU = 7, 14, 7, 21
 MAX_SQP_ITERS = 64
 
 # status codes / bits (include/mpcpcg.h)
-OK, ERR_ARG, ERR_HIP, ERR_NOT_READY, ERR_NONFINITE, ERR_NO_DEVICE = range(6)
-ST_ACTIVE, ST_PCG_MAXITER, ST_RHO_ABORT, ST_NONFINITE = 1, 2, 4, 8
+OK, ERR_ARG, ERR_HIP, ERR_NOT_READY, ERR_NONFINITE, ERR_NO_DEVICE, ERR_PCG_FAULT = range(7)
+ST_ACTIVE, ST_PCG_MAXITER, ST_RHO_ABORT, ST_NONFINITE, ST_PCG_FAULT = 1, 2, 4, 8, 16
 FLAG_LAST_KNOT_OWN_STATE = 1
 FLAG_PCG_STREAM = 2
 FLAG_DIRECT_SOLVE = 4
@@ -42,7 +42,7 @@ EXPORTS = (
     "mpc_status_string", "mpc_fk", "mpc_dynamics_gradient", "mpc_form_kkt", "mpc_form_schur",
     "mpc_pcg_solve", "mpc_compute_dz", "mpc_compute_merit", "mpc_line_search_merits",
     "mpc_sqp_solve", "mpc_batch_setup", "mpc_batch_sqp", "mpc_batch_set_timing",
-    "mpc_batch_phase_times", "mpc_batch_pcg_layout", "mpc_batch_pcg_launch_map", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
+    "mpc_batch_phase_times", "mpc_batch_pcg_layout", "mpc_batch_pcg_launch_map", "mpc_batch_set_graph", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
     "mpc_batch_qp_loss", "mpc_batch_qp_adjoint", "mpc_synth_instance", "mpc_synth_horizons",
     "mpc_inverse_dynamics", "mpc_default_closed_loop", "mpc_closed_loop", "mpc_synth_track", "mpc_direct_solve",
 )
@@ -121,6 +121,7 @@ def lib():
             "mpc_batch_setup": [vp, u32, C.POINTER(u32)],
             "mpc_batch_sqp": [vp, vp, vp, vp, vp, u32, vp, vp, vp],
             "mpc_batch_set_timing": [vp, C.c_int],
+            "mpc_batch_set_graph": [vp, C.c_int],
             "mpc_batch_phase_times": [vp, C.POINTER(C.c_double)],
             "mpc_batch_pcg_layout": [vp, C.POINTER(C.c_uint32)],
             "mpc_batch_pcg_launch_map": [vp, vp],
@@ -326,6 +327,11 @@ class Solver:
     def batch_set_timing(self, on=True):
         _check(lib().mpc_batch_set_timing(self.h, int(on)), "mpc_batch_set_timing")
 
+    def batch_set_graph(self, on=True):
+        """Run batch_sqp as a captured HIP graph, replayed while its buffers,
+        iteration count, configuration and layout stay the same."""
+        _check(lib().mpc_batch_set_graph(self.h, int(on)), "mpc_batch_set_graph")
+
     def set_cost_weights(self, wx, wu):
         import numpy as np
         wx = np.ascontiguousarray(np.asarray(wx, np.float32))
@@ -393,6 +399,7 @@ class Solver:
         _check(lib().mpc_closed_loop(self.h, C.byref(cl), N, _ptr(x0), _ptr(ee_path), _ptr(traj), _ptr(xu), _ptr(lam),
                                      _ptr(states), _ptr(ee_err), _ptr(pcg_iters), C.byref(st), _stream()),
                "mpc_closed_loop")
+        self._n_inst = 1  # the run sets the batch layout to its one instance
         stats = {k: getattr(st, k) for k, _ in ClosedLoopStats._fields_ if k != "reserved"}
         return states, ee_err, pcg_iters, stats
 

@@ -91,7 +91,10 @@ struct ClStep {
   uint64_t seed;
 };
 
-constexpr uint32_t ST_ACTIVE = 1u, ST_PCG_MAXITER = 2u, ST_RHO_ABORT = 4u, ST_NONFINITE = 8u;
+constexpr uint32_t ST_ACTIVE = 1u, ST_PCG_MAXITER = 2u, ST_RHO_ABORT = 4u, ST_NONFINITE = 8u, ST_PCG_FAULT = 16u;
+// per-instance PCG flag: 0 converged, 1 max_iter reached, PCG_FLAG_FAULT the
+// solve was abandoned (a team exchange timed out)
+constexpr uint32_t PCG_FLAG_FAULT = 2u;
 constexpr uint32_t FLAG_LAST_KNOT_OWN_STATE = 1u, FLAG_PCG_STREAM = 2u, FLAG_DIRECT_SOLVE = 4u,
                    FLAG_PCG_STAIR = 8u;
 

@@ -30,8 +30,12 @@
 // so they land on the same XCD and share its L2.  Mailbox words carry their
 // epoch, so relaxed (fence-free) atomics suffice.  Every wait is bounded:
 // a wave that gives up sets the error word and the workgroup's fail flag,
-// and the workgroup leaves at its next barrier; its partner then times out
-// as well.
+// and the workgroup leaves at its next barrier with the instance's PCG flag
+// set to PCG_FLAG_FAULT; its partner then times out as well.  Under stream
+// capture (HIP graphs) the launches are ordinary kernel launches of the same
+// grid, which fits the device at once (at most one workgroup per CU); the
+// bounded waits turn a partner that is not co-resident into that flag
+// instead of a hang.
 #include "common.h"
 #include "pcg_onchip.h"
 #include "pcg_res_body.h"
@@ -173,6 +177,11 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
   float *zb0 = sh.zb + b0 * 16 + i;
   uint32_t e = 0;  // mailbox epoch
   bool ok = true;
+  // a wait timed out somewhere in the workgroup: the solve is abandoned and
+  // the instance flagged (both halves write the same value)
+  auto fault = [&]() {
+    if (tid == 0 && flag_out) flag_out[inst] = PCG_FLAG_FAULT;
+  };
   // own first / last element to LDS, the row at the split to the partner
   auto stage = [&](const float (&v)[RK]) {
     zb0[0] = v[0];
@@ -239,7 +248,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     ++e;
     stage_w(wv);
     __syncthreads();
-    if (sh.fail) return;
+    if (sh.fail) return fault();
     collect(false, true);
     neighbours_w(wm, wp);
     smv(wv, wm, wp);
@@ -260,7 +269,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
   }
   stage(z);
   __syncthreads();
-  if (sh.fail) return;
+  if (sh.fail) return fault();
   float eta = total(sum8(sh.red[1]), collect(true, true));
   neighbours(pm, pp);
 #pragma unroll
@@ -282,7 +291,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
       mb_post(mb, half, e, w, pyp);
     }
     __syncthreads();
-    if (sh.fail) return;
+    if (sh.fail) return fault();
     const float alpha = eta / total(sum8(sh.red[0]), collect(true, false));
     float rz = 0.f;
 #pragma unroll
@@ -298,7 +307,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
       ++e;
       stage_w(wv);
       __syncthreads();
-      if (sh.fail) return;
+      if (sh.fail) return fault();
       collect(false, true);
       neighbours_w(wm, wp);
       smv(wv, wm, wp);
@@ -318,7 +327,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     }
     stage(z);
     __syncthreads();
-    if (sh.fail) return;
+    if (sh.fail) return fault();
     float zm = own_m ? zb0[-16] : 0.f, zp = own_p ? zb0[RK * 16] : 0.f;  // issued before the wait
     const float eta_new = total(sum8(sh.red[1]), collect(true, true));
     if (far_m && lane_ok) zm = sh.recv[i];
@@ -336,7 +345,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     pp = fmaf(beta, pp, zp);
   }
   __syncthreads();
-  if (sh.fail) return;
+  if (sh.fail) return fault();
 #pragma unroll
   for (int t = 0; t < RK; ++t) {
     const uint32_t lk = b0 + (uint32_t)t;
@@ -451,6 +460,18 @@ hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n
   if (cap < 16) return hipErrorNotSupported;
   const uint32_t per_launch = cap / 2;  // teams per launch
   constexpr uint32_t W = sizeof(TeamMailbox) / sizeof(unsigned long long);
+  // under stream capture: ordinary launches of the same grids (see the
+  // header comment), since a graph node carries no co-residency guarantee
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  {
+    const hipError_t ec = hipStreamIsCapturing(st, &cs);
+    if (ec != hipSuccess) return ec;
+  }
+  const bool capturing = cs == hipStreamCaptureStatusActive;
+  auto launch = [&](const void *fn, uint32_t grid, void **args, size_t smem) {
+    return capturing ? hipLaunchKernel(fn, dim3(grid), dim3(NT), args, smem, st)
+                     : hipLaunchCooperativeKernel(fn, dim3(grid), dim3(NT), args, (unsigned)smem, st);
+  };
   for (uint32_t team0 = 0; team0 < n_teams; team0 += per_launch) {
     const uint32_t n = n_teams - team0 < per_launch ? n_teams - team0 : per_launch;
     const uint32_t grid = 16u * ((n + 7u) / 8u);  // pairs in whole groups of 8
@@ -470,14 +491,14 @@ hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n
       void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, (void *)&S, (void *)&Pinv, (void *)&gamma,
                       &lambda, &max_iter, &tol, (void *)&status, &iters, &flag, &err};
       const void *fn = stair ? (const void *)k_pcg_team<true> : (const void *)k_pcg_team<false>;
-      e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(NT), args, (unsigned)sizeof(TeamShared), st);
+      e = launch(fn, grid, args, sizeof(TeamShared));
     } else {
       uint32_t tg = grid, bin0 = 0;
       void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, &tg, (void *)&items, (void *)&bin_off, &bin0,
                       (void *)&S, (void *)&Pinv, (void *)&gamma, &lambda, &max_iter, &tol, (void *)&status, &iters,
                       &flag, &err};
       const void *fn = stair ? (const void *)k_pcg_mixed<true> : (const void *)k_pcg_mixed<false>;
-      e = hipLaunchCooperativeKernel(fn, dim3(grid + nb), dim3(NT), args, (unsigned)MIXED_SMEM, st);
+      e = launch(fn, grid + nb, args, MIXED_SMEM);
       if (e == hipSuccess) *bins_taken = nb;
     }
     if (e != hipSuccess) return e;

@@ -103,7 +103,9 @@ __global__ void __launch_bounds__(NT) k_ls_update(Batch b, Params p, const float
   if (threadIdx.x == 0) {
     uint32_t st = s.status[inst];
     if (pcg_log) pcg_log[(size_t)inst * log_stride + it] = pcg_iters_now ? pcg_iters_now[inst] : 0u;
-    if (pcg_flag_now && pcg_flag_now[inst]) st |= ST_PCG_MAXITER;
+    const uint32_t pf = pcg_flag_now ? pcg_flag_now[inst] : 0u;
+    if (pf == PCG_FLAG_FAULT) st |= ST_PCG_FAULT;
+    else if (pf) st |= ST_PCG_MAXITER;
     const float m0 = s.merit0[inst];
     float minm = m0;
     int step = -1;
@@ -128,6 +130,10 @@ __global__ void __launch_bounds__(NT) k_ls_update(Batch b, Params p, const float
       drho = fminf(drho / p.rho_factor, 1.f / p.rho_factor);
       rho = fmaxf(rho * drho, p.rho_min);
       s.merit0[inst] = minm;
+    }
+    if (st & ST_PCG_FAULT) {  // the step came from an abandoned solve: not taken
+      st &= ~ST_ACTIVE;
+      alpha = 0.f;
     }
     s.rho[inst] = rho;
     s.drho[inst] = drho;

@@ -58,6 +58,22 @@ struct mpc_handle {
   float *rho_one = nullptr;  // the plain driver's rho on the device (direct mode)
   uint32_t *cl_ctr = nullptr, *cl_status = nullptr, *cl_pcg = nullptr;
   hipStream_t cl_stream = nullptr;
+  // the batched solve as a HIP graph (mpc_batch_set_graph): captured once per
+  // key and replayed; graph_epoch changes with everything a captured launch
+  // bakes in (configuration, weights, batch layout)
+  struct GraphKey {
+    uint64_t epoch = 0;
+    const void *ptr[6]{};
+    uint32_t sqp_iters = 0;
+    bool operator==(const GraphKey &o) const {
+      return epoch == o.epoch && sqp_iters == o.sqp_iters && std::memcmp(ptr, o.ptr, sizeof(ptr)) == 0;
+    }
+  };
+  bool graph_on = false;
+  uint64_t graph_epoch = 1;
+  GraphKey graph_key;
+  hipGraph_t bgraph = nullptr;
+  hipGraphExec_t bexec = nullptr;
   // pinned staging for the host round trips of the plain driver
   float *h_merits = nullptr;
   uint32_t *h_pcg = nullptr;
@@ -228,6 +244,8 @@ static void free_all(mpc_handle *h) {
   if (h->h_rho) (void)hipHostFree(h->h_rho);
   for (auto &e : h->ev)
     if (e) (void)hipEventDestroy(e);
+  if (h->bexec) (void)hipGraphExecDestroy(h->bexec);
+  if (h->bgraph) (void)hipGraphDestroy(h->bgraph);
   if (h->cl_stream) (void)hipStreamDestroy(h->cl_stream);
 }
 
@@ -331,6 +349,7 @@ int mpc_set_config(mpc_handle *h, const mpc_config *cfg) {
   c.max_total_knots = h->cfg.max_total_knots;
   h->cfg = c;
   weights_from_config(h);
+  ++h->graph_epoch;
   return MPC_OK;
 }
 
@@ -342,6 +361,7 @@ int mpc_set_cost_weights(mpc_handle *h, const float *wx14, const float *wu7) {
     if (!(wu7[j] >= 0.f) || !std::isfinite(wu7[j])) return MPC_ERR_ARG;
   std::memcpy(h->wx, wx14, sizeof(h->wx));
   std::memcpy(h->wu, wu7, sizeof(h->wu));
+  ++h->graph_epoch;
   return MPC_OK;
 }
 
@@ -504,6 +524,7 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
     HIP_TRY(hipStreamSynchronize(st));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+    if (h->h_pcg[1] == PCG_FLAG_FAULT) return MPC_ERR_PCG_FAULT;
     s.pcg_iters[it] = h->h_pcg[0];
     s.pcg_maxiter[it] = h->h_pcg[1];
     s.linsys_us[it] = 1e3 * ms;
@@ -544,6 +565,7 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
 
 int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
   if (!h || !horizons || n_inst == 0 || n_inst > h->cfg.max_instances) return MPC_ERR_ARG;
+  ++h->graph_epoch;
   std::vector<uint32_t> k0(n_inst + 1, 0);
   uint32_t max_n = 0;
   for (uint32_t i = 0; i < n_inst; ++i) {
@@ -819,12 +841,46 @@ static hipError_t batch_sqp_run(mpc_handle *h, float *d_xu, float *d_lambda, con
   return hipSuccess;
 }
 
+static void drop_batch_graph(mpc_handle *h) {
+  if (h->bexec) (void)hipGraphExecDestroy(h->bexec);
+  if (h->bgraph) (void)hipGraphDestroy(h->bgraph);
+  h->bexec = nullptr;
+  h->bgraph = nullptr;
+}
+
+int mpc_batch_set_graph(mpc_handle *h, int enable) {
+  if (!h) return MPC_ERR_ARG;
+  h->graph_on = enable != 0;
+  if (!h->graph_on) drop_batch_graph(h);
+  return MPC_OK;
+}
+
 int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee_ref, float *d_rho,
                   uint32_t sqp_iters, uint32_t *d_status, uint32_t *d_pcg_iters, mpc_stream_t stream) {
   if (!h || !d_xu || !d_lambda || !d_ee_ref || !d_rho || !d_status) return MPC_ERR_ARG;
   if (!h->batch_ready) return MPC_ERR_NOT_READY;
   if (sqp_iters > MPC_MAX_SQP_ITERS) return MPC_ERR_ARG;
   for (double &v : h->phase_us) v = 0;
+  if (h->graph_on && !h->timing) {
+    mpc_handle::GraphKey k;
+    k.epoch = h->graph_epoch;
+    k.sqp_iters = sqp_iters;
+    const void *p[6] = {d_xu, d_lambda, d_ee_ref, d_rho, d_status, d_pcg_iters};
+    std::memcpy(k.ptr, p, sizeof(p));
+    if (!h->bexec || !(k == h->graph_key)) {
+      drop_batch_graph(h);
+      hipStream_t cs = h->cl_stream;
+      HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      const hipError_t er = batch_sqp_run(h, d_xu, d_lambda, d_ee_ref, d_rho, sqp_iters, d_status, d_pcg_iters, cs);
+      const hipError_t ee = hipStreamEndCapture(cs, &h->bgraph);
+      HIP_TRY(er);
+      HIP_TRY(ee);
+      HIP_TRY(hipGraphInstantiate(&h->bexec, h->bgraph, nullptr, nullptr, 0));
+      h->graph_key = k;
+    }
+    HIP_TRY(hipGraphLaunch(h->bexec, (hipStream_t)stream));
+    return MPC_OK;
+  }
   HIP_TRY(batch_sqp_run(h, d_xu, d_lambda, d_ee_ref, d_rho, sqp_iters, d_status, d_pcg_iters,
                         (hipStream_t)stream));
   return MPC_OK;
@@ -855,8 +911,6 @@ int mpc_closed_loop(mpc_handle *h, const mpc_closed_loop_config *cl, uint32_t N,
   const uint32_t horizon[1] = {N};
   int rc = mpc_batch_setup(h, 1, horizon);
   if (rc != MPC_OK) return rc;
-  // cooperative launches (PCG teams) are not captured
-  if (cl->use_graph && h->n_teams && !(h->cfg.flags & FLAG_DIRECT_SOLVE)) return MPC_ERR_ARG;
   hipStream_t user = (hipStream_t)stream;
   hipStream_t st = cl->use_graph ? h->cl_stream : user;
   const ClStep step{N, cl->substeps, cl->sqp_iters, cl->sim_dt, cl->noise_sigma, cl->noise_seed};

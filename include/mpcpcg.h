@@ -42,7 +42,8 @@ enum {
   MPC_ERR_HIP = 2,         /* a HIP runtime call failed */
   MPC_ERR_NOT_READY = 3,   /* batch layout not set up */
   MPC_ERR_NONFINITE = 4,   /* non-finite value detected */
-  MPC_ERR_NO_DEVICE = 5    /* no usable GPU */
+  MPC_ERR_NO_DEVICE = 5,   /* no usable GPU */
+  MPC_ERR_PCG_FAULT = 6    /* a PCG team exchange timed out (the solve was abandoned) */
 };
 
 /* per-instance status bits (batched solve) */
@@ -50,7 +51,8 @@ enum {
   MPC_ST_ACTIVE = 1,       /* still iterating */
   MPC_ST_PCG_MAXITER = 2,  /* some PCG solve hit max_iter */
   MPC_ST_RHO_ABORT = 4,    /* line search failed with rho > rho_max (toplevel.cuh:396-407) */
-  MPC_ST_NONFINITE = 8     /* a merit or step was not finite */
+  MPC_ST_NONFINITE = 8,    /* a merit or step was not finite */
+  MPC_ST_PCG_FAULT = 16    /* a PCG team exchange timed out: the instance was stopped, its step not taken */
 };
 
 /* Runtime configuration.  Reference equivalents: settings.cuh and the -D
@@ -168,7 +170,8 @@ int mpc_form_schur(mpc_handle *h, uint32_t N, float *d_G, const float *d_C, cons
 
 /* PCG on S lambda = gamma, warm-started from lambda.  Exit when
  * |r^T Pinv r| < exit_tol (also tested before the first iteration); writes
- * the iteration count and a max-iter flag (1 = max_iter reached).
+ * the iteration count and a max-iter flag (1 = max_iter reached, 2 = a
+ * two-workgroup team's exchange timed out and the solve was abandoned).
  * GPU-PCG pcg<T,14,N> (call site toplevel.cuh:143-157, :292-294). */
 int mpc_pcg_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_Pinv,
                   const float *d_gamma, float *d_lambda, uint32_t max_iter, float exit_tol,
@@ -229,6 +232,16 @@ int mpc_batch_sqp(mpc_handle *h, float *d_xu, float *d_lambda, const float *d_ee
  * launches (with the bins that share the last one), the rest of PCG, dz,
  * line-search merits, update. */
 int mpc_batch_set_timing(mpc_handle *h, int enable);
+/* With enable != 0, mpc_batch_sqp runs its whole launch sequence (initial
+ * merit, then sqp_iters x {linearise, Schur, PCG, dz, merits, line search})
+ * as a HIP graph: captured on the first call and replayed on the caller's
+ * stream while the buffers, sqp_iters, configuration, cost weights and batch
+ * layout stay the same (any change re-captures).  In the graph the PCG team
+ * launches are ordinary kernel nodes of grids that fit the device at once;
+ * an instance whose team exchange timed out ends with MPC_ST_PCG_FAULT.
+ * Not used while timing is enabled.  No counterpart in the reference (its
+ * SQP loop decides on the host between launches, toplevel.cuh:254-446). */
+int mpc_batch_set_graph(mpc_handle *h, int enable);
 /* PCG packing of the batch layout: out4 = {resident-kernel bins, two-
  * workgroup teams, instances on the streaming kernel, cooperative team
  * launches per PCG phase}. */
