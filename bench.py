@@ -352,6 +352,11 @@ def main():
     horizons, xu0, ee = build_workload(pkg, solver, torch, n, args.seed, dt)
     T = int(horizons.sum())
     solver.batch_setup(horizons.tolist())
+    # launch plan of the batched PCG: every packing that differs for this
+    # layout timed on the device, the fastest kept (mpc_batch_autotune)
+    t_tune = time.perf_counter()
+    tune = solver.batch_autotune(iters=16, reps=3)
+    t_tune = time.perf_counter() - t_tune
     xu = torch.empty_like(xu0)
     lam = torch.zeros(14 * T, device="cuda")
     rho = torch.empty(n, device="cuda")
@@ -516,6 +521,8 @@ def main():
                       "hit_max_iter": int((st & 2 != 0).sum())},
         "status": {"aborted": int((st & 4 != 0).sum()), "nonfinite": int((st & 8 != 0).sum()),
                    "pcg_fault": int((st & 16 != 0).sum())},
+        "autotune": {"chosen": solver.batch_get_plan(), "iters": 16, "reps": 3, "seconds": round(t_tune, 3),
+                     "candidates_us": sorted(tune, key=lambda r: r["us"])},
         "graph": {"headline_is_graph_replay": True, "bitwise_equal_to_eager": bool(graph_equal),
                   "eager_ms_per_step": round(ms_eager, 4),
                   "eager_kkt_solves_per_sec": round(kkt_per_step / (ms_eager * 1e-3), 1),

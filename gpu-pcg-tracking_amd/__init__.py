@@ -42,7 +42,8 @@ EXPORTS = (
     "mpc_status_string", "mpc_fk", "mpc_dynamics_gradient", "mpc_form_kkt", "mpc_form_schur",
     "mpc_pcg_solve", "mpc_compute_dz", "mpc_compute_merit", "mpc_line_search_merits",
     "mpc_sqp_solve", "mpc_batch_setup", "mpc_batch_sqp", "mpc_batch_set_timing",
-    "mpc_batch_phase_times", "mpc_batch_pcg_layout", "mpc_batch_pcg_launch_map", "mpc_batch_set_graph", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
+    "mpc_batch_phase_times", "mpc_batch_pcg_layout", "mpc_batch_pcg_launch_map", "mpc_batch_set_graph", "mpc_batch_get_plan",
+    "mpc_batch_set_plan", "mpc_batch_autotune", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
     "mpc_batch_qp_loss", "mpc_batch_qp_adjoint", "mpc_synth_instance", "mpc_synth_horizons",
     "mpc_inverse_dynamics", "mpc_default_closed_loop", "mpc_closed_loop", "mpc_synth_track", "mpc_direct_solve",
 )
@@ -93,6 +94,17 @@ class ClosedLoopStats(C.Structure):
     ]
 
 
+class PcgPlan(C.Structure):
+    """mpc_pcg_plan: launch plan of the batched PCG."""
+    _fields_ = [
+        ("small_nt", C.c_uint32), ("max_items", C.c_uint32), ("mixed", C.c_uint32), ("teams", C.c_uint32),
+        ("us", C.c_float), ("reserved", C.c_uint32),
+    ]
+
+
+_PLAN_KEYS = ("small_nt", "max_items", "mixed", "teams")
+
+
 _lib = None
 
 
@@ -122,6 +134,9 @@ def lib():
             "mpc_batch_sqp": [vp, vp, vp, vp, vp, u32, vp, vp, vp],
             "mpc_batch_set_timing": [vp, C.c_int],
             "mpc_batch_set_graph": [vp, C.c_int],
+            "mpc_batch_get_plan": [vp, C.POINTER(PcgPlan)],
+            "mpc_batch_set_plan": [vp, C.POINTER(PcgPlan)],
+            "mpc_batch_autotune": [vp, u32, u32, C.POINTER(PcgPlan), u32, C.POINTER(u32), vp],
             "mpc_batch_phase_times": [vp, C.POINTER(C.c_double)],
             "mpc_batch_pcg_layout": [vp, C.POINTER(C.c_uint32)],
             "mpc_batch_pcg_launch_map": [vp, vp],
@@ -420,6 +435,31 @@ class Solver:
         out = np.zeros(n, np.uint32)
         _check(lib().mpc_batch_pcg_launch_map(self.h, out.ctypes.data), "mpc_batch_pcg_launch_map")
         return out
+
+    def batch_get_plan(self):
+        """The current PCG launch plan {small_nt, max_items, mixed, teams}."""
+        p = PcgPlan()
+        _check(lib().mpc_batch_get_plan(self.h, C.byref(p)), "mpc_batch_get_plan")
+        return {k: int(getattr(p, k)) for k in _PLAN_KEYS}
+
+    def batch_set_plan(self, **kw):
+        """Set fields of the PCG launch plan (the others kept) and re-pack."""
+        cur = self.batch_get_plan()
+        unknown = set(kw) - set(_PLAN_KEYS)
+        if unknown:
+            raise MpcError(f"batch_set_plan: unknown fields {sorted(unknown)}")
+        cur.update({k: int(v) for k, v in kw.items()})
+        p = PcgPlan(**cur)
+        _check(lib().mpc_batch_set_plan(self.h, C.byref(p)), "mpc_batch_set_plan")
+
+    def batch_autotune(self, iters=16, reps=3):
+        """Time every launch plan that differs for the current layout and keep
+        the fastest; returns the timed plans (dicts with "us") in order."""
+        res = (PcgPlan * 64)()
+        n = C.c_uint32(0)
+        _check(lib().mpc_batch_autotune(self.h, int(iters), int(reps), res, 64, C.byref(n), _stream()),
+               "mpc_batch_autotune")
+        return [dict({k: int(getattr(r, k)) for k in _PLAN_KEYS}, us=float(r.us)) for r in res[:n.value]]
 
     def batch_phase_times(self):
         out = (C.c_double * 7)()

@@ -122,12 +122,13 @@ hipError_t launch_pcg_stream(const Batch &b, uint32_t max_n, const float *S, con
                              const float *gamma, float *lambda, uint32_t max_iter, float tol,
                              const uint32_t *status, const uint32_t *inst_list, uint32_t n_list,
                              uint32_t *iters, uint32_t *flag, bool stair, hipStream_t st);
-// PCG with S resident on chip (N <= pcg_res_max_knots()); one workgroup per
-// bin of instances, bins bin0 .. bin0 + n_bins - 1.  items[bin_off[w] ..
+// PCG with S resident on chip (N <= pcg_res_max_knots(nt)); one workgroup of
+// nt lanes (512: 128 knots, one per CU; 256: 64 knots, two per CU) per bin
+// of instances, bins bin0 .. bin0 + n_bins - 1.  items[bin_off[w] ..
 // bin_off[w+1]) = {inst, first group, groups, 0}; items == nullptr: a single
 // instance (0) in one bin.
 hipError_t launch_pcg_res(const Batch &b, const uint4 *items, const uint32_t *bin_off, uint32_t bin0, uint32_t n_bins,
-                          const float *S, const float *Pinv, const float *gamma, float *lambda,
+                          uint32_t nt, const float *S, const float *Pinv, const float *gamma, float *lambda,
                           uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters,
                           uint32_t *flag, bool stair, hipStream_t st);
 // PCG of 128 < N <= pcg_team_max_knots() on a team of two workgroups
@@ -144,9 +145,12 @@ uint32_t pcg_team_max_knots();
 uint32_t pcg_team_max_grid();
 void pcg_team_plan(uint32_t n_teams, uint32_t n_bins, uint32_t *launches, uint32_t *bins_taken);
 uint32_t pcg_res_groups(uint32_t N);
-uint32_t pcg_res_bin_groups();  // 16-lane groups per workgroup
-uint32_t pcg_res_max_knots();
+uint32_t pcg_res_bin_groups(uint32_t nt);  // 16-lane groups per workgroup of nt lanes
+uint32_t pcg_res_max_knots(uint32_t nt);
 uint32_t pcg_res_max_items();
+// a synthetic block-tridiagonal Schur system (diag -2.05 I, off-diagonal I,
+// block-Jacobi Pinv, gamma = 1, lambda = 0) for timing launch plans
+hipError_t launch_tune_system(const Batch &b, float *S, float *Pinv, float *gamma, float *lambda, hipStream_t st);
 hipError_t launch_merit_reduce(const Batch &b, const float *knot_merits, int np, float *out,
                                hipStream_t st);
 hipError_t launch_ls_update(const Batch &b, const Params &p, const float *knot_merits,

@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(NT) k_pcg_mixed(Batch b, const uint32_t *__res
     team_body<STAIR>(blockIdx.x, smem_raw, b, team_inst, n_teams, team0, mboxes, S, Pinv, gamma, lambda, max_iter,
                      tol, status, iters_out, flag_out, err);
   else
-    resident::body<STAIR>(bin0 + (blockIdx.x - team_grid), smem_raw, b, items, bin_off, S, Pinv, gamma, lambda,
+    resident::body<STAIR, 512>(bin0 + (blockIdx.x - team_grid), smem_raw, b, items, bin_off, S, Pinv, gamma, lambda,
                           max_iter, tol, status, iters_out, flag_out);
 }
 
@@ -404,7 +404,7 @@ __global__ void k_reset_mailboxes(TeamMailbox *mb, uint32_t n) {
 uint32_t pcg_team_max_knots() { return 2u * KNOTS; }
 
 constexpr size_t MIXED_SMEM =
-    sizeof(TeamShared) > sizeof(resident::ResShared) ? sizeof(TeamShared) : sizeof(resident::ResShared);
+    sizeof(TeamShared) > sizeof(resident::ResShared<512>) ? sizeof(TeamShared) : sizeof(resident::ResShared<512>);
 
 // Largest cooperative grid of k_pcg_team (and k_pcg_mixed) on this device
 // (0: not supported).

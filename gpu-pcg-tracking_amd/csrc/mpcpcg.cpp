@@ -24,6 +24,14 @@ using namespace mpc;
     if (e_ != hipSuccess) return MPC_ERR_HIP;  \
   } while (0)
 
+// Launch plan of the batched PCG (mpc_batch_autotune / mpc_batch_set_plan)
+struct PcgPlan {
+  uint32_t small_nt = 512;  // workgroup of the bins of N <= 64: 512 or 256 lanes
+  uint32_t max_items = 8;   // instances per bin
+  bool mixed = true;        // the last team launch also solves the first bins
+  bool teams = true;        // 128 < N <= 256 on two-workgroup teams (else streamed)
+};
+
 struct mpc_handle {
   mpc_config cfg{};
   int device = 0;
@@ -40,7 +48,7 @@ struct mpc_handle {
   // the list of long instances (streaming kernel)
   uint4 *bin_items = nullptr;
   uint32_t *bin_off = nullptr, *big_list = nullptr;
-  uint32_t n_bins = 0, n_big = 0, max_big_n = 0;
+  uint32_t n_bins = 0, n_bins_small = 0, n_big = 0, max_big_n = 0;
   // two-workgroup PCG teams (128 < N <= 256)
   uint32_t *team_list = nullptr, *team_one = nullptr, *team_err = nullptr;
   uint32_t *dir_order = nullptr;  // direct solve: instances longest first
@@ -81,6 +89,7 @@ struct mpc_handle {
   // batch layout
   std::vector<uint32_t> horizons, k0;
   std::vector<uint32_t> pcg_launch;  // PCG launch of each instance within a phase (mpc_batch_pcg_launch_map)
+  PcgPlan plan;
   uint32_t n_inst = 0, total = 0, max_n = 0;
   bool batch_ready = false;
   // per-dimension cost weights (mpc_set_cost_weights; derived from cfg by default)
@@ -130,9 +139,9 @@ static hipError_t pcg_single(mpc_handle *h, uint32_t N, const float *S, const fl
                                h->cfg.direct_refine, true, iters, flag, st);
   const bool stair = flags & FLAG_PCG_STAIR;
   if (!(flags & FLAG_PCG_STREAM)) {
-    if (N <= pcg_res_max_knots())
-      return launch_pcg_res(single(N), nullptr, nullptr, 0, 1, S, P, gamma, lambda, max_iter, tol, nullptr, iters,
-                            flag, stair, st);
+    if (N <= pcg_res_max_knots(512))
+      return launch_pcg_res(single(N), nullptr, nullptr, 0, 1, 512, S, P, gamma, lambda, max_iter, tol, nullptr,
+                            iters, flag, stair, st);
     if (N <= pcg_team_max_knots() && pcg_team_max_grid() >= 16)
       return launch_pcg_team(single(N), h->team_one, 1, h->mbox, h->n_mbox, nullptr, nullptr, 0, nullptr, S, P,
                              gamma, lambda, max_iter, tol, nullptr, iters, flag, h->team_err, stair, st);
@@ -561,65 +570,60 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
   return MPC_OK;
 }
 
-// ---- batched SQP ------------------------------------------------------------
-
-int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
-  if (!h || !horizons || n_inst == 0 || n_inst > h->cfg.max_instances) return MPC_ERR_ARG;
-  ++h->graph_epoch;
-  std::vector<uint32_t> k0(n_inst + 1, 0);
-  uint32_t max_n = 0;
-  for (uint32_t i = 0; i < n_inst; ++i) {
-    if (horizons[i] < 2 || horizons[i] > h->cfg.max_knots) return MPC_ERR_ARG;
-    k0[i + 1] = k0[i] + horizons[i];
-    if (k0[i + 1] > h->cfg.max_total_knots) return MPC_ERR_ARG;
-    max_n = horizons[i] > max_n ? horizons[i] : max_n;
-  }
-  const uint32_t T = k0[n_inst];
-  std::vector<uint32_t> ki(T);
-  for (uint32_t i = 0; i < n_inst; ++i)
-    for (uint32_t k = k0[i]; k < k0[i + 1]; ++k) ki[k] = i;
-  HIP_TRY(hipMemcpy(h->knot_inst, ki.data(), sizeof(uint32_t) * T, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(h->inst_k0, k0.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice));
-  // Bins for the resident PCG: instances with N <= 128 packed first-fit
-  // decreasing into workgroups of pcg_res_bin_groups() 16-lane groups (whole waves per instance, at
-  // most pcg_res_max_items() instances per bin); longer instances go to the
-  // streaming kernel.
-  const uint32_t cap_groups = pcg_res_bin_groups(), max_items = pcg_res_max_items();
-  std::vector<uint32_t> order, big, teams;
+// PCG launch plan of the batch layout under h->plan.  Instances with
+// N <= 128 are packed first-fit decreasing into bins of the resident kernel
+// (whole waves per instance, at most plan.max_items per bin): workgroups of
+// 512 lanes, or for N <= 64 of 256 lanes when plan.small_nt is 256 (listed
+// after the 512-lane bins in the same tables).  128 < N <= 256 run on
+// two-workgroup teams (plan.teams) and longer ones, or all of them without
+// teams, on the streaming kernel.  Also records the launch of each instance
+// within a PCG phase (mpc_batch_pcg_launch_map).
+static int build_pcg_plan(mpc_handle *h) {
+  const PcgPlan &pl = h->plan;
+  const uint32_t n_inst = (uint32_t)h->horizons.size();
+  const uint32_t *horizons = h->horizons.data();
+  const uint32_t max_items = pl.max_items < pcg_res_max_items() ? pl.max_items : pcg_res_max_items();
+  const uint32_t small_max = pl.small_nt == 256 ? pcg_res_max_knots(256) : 0u;
+  std::vector<uint32_t> order, order_s, big, teams;
   uint32_t max_big = 0;
-  const bool team_ok = pcg_team_max_grid() >= 16;
+  const bool team_ok = pl.teams && pcg_team_max_grid() >= 16;
   for (uint32_t i = 0; i < n_inst; ++i) {
-    if (horizons[i] <= pcg_res_max_knots()) order.push_back(i);
+    if (horizons[i] <= small_max) order_s.push_back(i);
+    else if (horizons[i] <= pcg_res_max_knots(512)) order.push_back(i);
     else if (team_ok && horizons[i] <= pcg_team_max_knots()) teams.push_back(i);
     else {
       big.push_back(i);
       max_big = horizons[i] > max_big ? horizons[i] : max_big;
     }
   }
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return horizons[a] > horizons[c]; });
-  {
-    std::vector<uint32_t> all(n_inst);
-    for (uint32_t i = 0; i < n_inst; ++i) all[i] = i;
-    std::stable_sort(all.begin(), all.end(), [&](uint32_t a, uint32_t c) { return horizons[a] > horizons[c]; });
-    HIP_TRY(hipMemcpy(h->dir_order, all.data(), sizeof(uint32_t) * n_inst, hipMemcpyHostToDevice));
-  }
-  struct BinB { uint32_t used = 0; std::vector<uint32_t> inst; };
-  std::vector<BinB> bins;
-  for (uint32_t i : order) {
-    const uint32_t need = pcg_res_groups(horizons[i]);
-    BinB *dst = nullptr;
-    for (auto &bb : bins)
-      if (bb.used + need <= cap_groups && bb.inst.size() < max_items) {
-        dst = &bb;
-        break;
+  struct BinB {
+    uint32_t used = 0;
+    std::vector<uint32_t> inst;
+  };
+  auto pack = [&](std::vector<uint32_t> &ord, uint32_t cap_groups) {
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t c) { return horizons[a] > horizons[c]; });
+    std::vector<BinB> bins;
+    for (uint32_t i : ord) {
+      const uint32_t need = pcg_res_groups(horizons[i]);
+      BinB *dst = nullptr;
+      for (auto &bb : bins)
+        if (bb.used + need <= cap_groups && bb.inst.size() < max_items) {
+          dst = &bb;
+          break;
+        }
+      if (!dst) {
+        bins.emplace_back();
+        dst = &bins.back();
       }
-    if (!dst) {
-      bins.emplace_back();
-      dst = &bins.back();
+      dst->inst.push_back(i);
+      dst->used += need;
     }
-    dst->inst.push_back(i);
-    dst->used += need;
-  }
+    return bins;
+  };
+  std::vector<BinB> bins = pack(order, pcg_res_bin_groups(512));
+  const std::vector<BinB> bins_s = pack(order_s, pcg_res_bin_groups(256));
+  const uint32_t n512 = (uint32_t)bins.size();
+  bins.insert(bins.end(), bins_s.begin(), bins_s.end());
   std::vector<uint4> items;
   std::vector<uint32_t> boff(1, 0);
   for (auto &bb : bins) {
@@ -640,26 +644,62 @@ int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
     HIP_TRY(hipMemcpy(h->team_list, teams.data(), sizeof(uint32_t) * teams.size(), hipMemcpyHostToDevice));
   // launch of each instance within a PCG phase: team launches 0 .. nl-1
   // (the last also takes the first bins), then the streaming launch, then
-  // the resident launch
-  {
-    uint32_t nl = 0, taken = 0;
-    pcg_team_plan((uint32_t)teams.size(), (uint32_t)bins.size(), &nl, &taken);
-    const uint32_t per_launch = pcg_team_max_grid() / 2;
-    h->pcg_launch.assign(n_inst, 0u);
-    for (size_t t = 0; t < teams.size(); ++t) h->pcg_launch[teams[t]] = (uint32_t)(t / per_launch);
-    for (uint32_t i : big) h->pcg_launch[i] = nl;
-    for (size_t w = 0; w < bins.size(); ++w)
-      for (uint32_t i : bins[w].inst) h->pcg_launch[i] = w < taken ? nl - 1 : nl + (big.empty() ? 0u : 1u);
-  }
+  // the resident launches (512-lane bins, then 256-lane bins)
+  uint32_t nl = 0, taken = 0;
+  pcg_team_plan((uint32_t)teams.size(), pl.mixed ? n512 : 0u, &nl, &taken);
+  const uint32_t per_launch = pcg_team_max_grid() / 2;
+  h->pcg_launch.assign(n_inst, 0u);
+  for (size_t t = 0; t < teams.size(); ++t) h->pcg_launch[teams[t]] = (uint32_t)(t / per_launch);
+  uint32_t next = nl;
+  if (!big.empty())
+    for (uint32_t i : big) h->pcg_launch[i] = next;
+  next += big.empty() ? 0u : 1u;
+  const uint32_t res512 = next;
+  next += n512 > taken ? 1u : 0u;
+  for (size_t w = 0; w < bins.size(); ++w)
+    for (uint32_t i : bins[w].inst) h->pcg_launch[i] = w < taken ? nl - 1 : (w < n512 ? res512 : next);
   h->n_teams = (uint32_t)teams.size();
-  h->n_bins = (uint32_t)bins.size();
+  h->n_bins = n512;
+  h->n_bins_small = (uint32_t)bins_s.size();
   h->n_big = (uint32_t)big.size();
   h->max_big_n = max_big;
+  ++h->graph_epoch;
+  return MPC_OK;
+}
+
+// ---- batched SQP ------------------------------------------------------------
+
+int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons) {
+  if (!h || !horizons || n_inst == 0 || n_inst > h->cfg.max_instances) return MPC_ERR_ARG;
+  ++h->graph_epoch;
+  std::vector<uint32_t> k0(n_inst + 1, 0);
+  uint32_t max_n = 0;
+  for (uint32_t i = 0; i < n_inst; ++i) {
+    if (horizons[i] < 2 || horizons[i] > h->cfg.max_knots) return MPC_ERR_ARG;
+    k0[i + 1] = k0[i] + horizons[i];
+    if (k0[i + 1] > h->cfg.max_total_knots) return MPC_ERR_ARG;
+    max_n = horizons[i] > max_n ? horizons[i] : max_n;
+  }
+  const uint32_t T = k0[n_inst];
+  h->batch_ready = false;
+  std::vector<uint32_t> ki(T);
+  for (uint32_t i = 0; i < n_inst; ++i)
+    for (uint32_t k = k0[i]; k < k0[i + 1]; ++k) ki[k] = i;
+  HIP_TRY(hipMemcpy(h->knot_inst, ki.data(), sizeof(uint32_t) * T, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->inst_k0, k0.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice));
+  {
+    std::vector<uint32_t> all(n_inst);
+    for (uint32_t i = 0; i < n_inst; ++i) all[i] = i;
+    std::stable_sort(all.begin(), all.end(), [&](uint32_t a, uint32_t c) { return horizons[a] > horizons[c]; });
+    HIP_TRY(hipMemcpy(h->dir_order, all.data(), sizeof(uint32_t) * n_inst, hipMemcpyHostToDevice));
+  }
   h->horizons.assign(horizons, horizons + n_inst);
   h->k0 = k0;
   h->n_inst = n_inst;
   h->total = T;
   h->max_n = max_n;
+  const int rc = build_pcg_plan(h);
+  if (rc != MPC_OK) return rc;
   h->batch_ready = true;
   return MPC_OK;
 }
@@ -727,13 +767,111 @@ int mpc_batch_pcg_layout(const mpc_handle *h, uint32_t *out4) {
   if (!h || !out4) return MPC_ERR_ARG;
   if (!h->batch_ready) return MPC_ERR_NOT_READY;
   const uint32_t per_launch = pcg_team_max_grid() / 2;
-  out4[0] = h->n_bins;
+  out4[0] = h->n_bins + h->n_bins_small;
   out4[1] = h->n_teams;
   out4[2] = h->n_big;
   out4[3] = (h->cfg.flags & FLAG_PCG_STREAM) || per_launch == 0
                 ? 0u
                 : (h->n_teams + per_launch - 1) / per_launch;
   return MPC_OK;
+}
+
+static void plan_to_c(const PcgPlan &p, float us, mpc_pcg_plan *o) {
+  o->small_nt = p.small_nt;
+  o->max_items = p.max_items;
+  o->mixed = p.mixed ? 1u : 0u;
+  o->teams = p.teams ? 1u : 0u;
+  o->us = us;
+  o->reserved = 0;
+}
+
+int mpc_batch_get_plan(const mpc_handle *h, mpc_pcg_plan *plan) {
+  if (!h || !plan) return MPC_ERR_ARG;
+  plan_to_c(h->plan, 0.f, plan);
+  return MPC_OK;
+}
+
+int mpc_batch_set_plan(mpc_handle *h, const mpc_pcg_plan *plan) {
+  if (!h || !plan || (plan->small_nt != 256 && plan->small_nt != 512) || plan->max_items < 1 ||
+      plan->max_items > pcg_res_max_items() || plan->mixed > 1 || plan->teams > 1)
+    return MPC_ERR_ARG;
+  h->plan.small_nt = plan->small_nt;
+  h->plan.max_items = plan->max_items;
+  h->plan.mixed = plan->mixed != 0;
+  h->plan.teams = plan->teams != 0;
+  return h->batch_ready ? build_pcg_plan(h) : MPC_OK;
+}
+
+// Times every launch plan that differs for this layout on the synthetic
+// system of launch_tune_system (`iters` PCG iterations everywhere, tol 0,
+// the best of `reps` passes after one untimed pass) and keeps the fastest.
+int mpc_batch_autotune(mpc_handle *h, uint32_t iters, uint32_t reps, mpc_pcg_plan *results, uint32_t max_results,
+                       uint32_t *n_results, mpc_stream_t stream) {
+  if (!h || iters == 0 || iters > 100000 || reps == 0 || reps > 100 || (max_results && !results)) return MPC_ERR_ARG;
+  if (!h->batch_ready) return MPC_ERR_NOT_READY;
+  if (h->cfg.flags & (FLAG_PCG_STREAM | FLAG_DIRECT_SOLVE)) return MPC_ERR_ARG;  // no launch plan in these modes
+  hipStream_t st = (hipStream_t)stream;
+  const Batch b = batch_of(h);
+  bool has_small = false, has_bins = false, has_team = false;
+  for (uint32_t n : h->horizons) {
+    has_small |= n <= pcg_res_max_knots(256);
+    has_bins |= n <= pcg_res_max_knots(512);
+    has_team |= n > pcg_res_max_knots(512) && n <= pcg_team_max_knots();
+  }
+  has_team = has_team && pcg_team_max_grid() >= 16;
+  std::vector<PcgPlan> cands;
+  for (uint32_t nt : {512u, 256u}) {
+    if (nt == 256 && !has_small) continue;
+    for (uint32_t mi : {8u, 4u, 2u}) {
+      if (mi != 8 && !has_bins) continue;
+      for (int teams = 1; teams >= 0; --teams) {
+        if (!teams && !has_team) continue;
+        for (int mixed = 1; mixed >= 0; --mixed) {
+          if (!mixed && !(teams && has_team && has_bins)) continue;
+          PcgPlan c;
+          c.small_nt = nt;
+          c.max_items = mi;
+          c.teams = teams != 0;
+          c.mixed = mixed != 0;
+          cands.push_back(c);
+        }
+      }
+    }
+  }
+  const PcgPlan saved = h->plan;
+  PcgPlan best = saved;
+  float best_ms = INFINITY;
+  uint32_t nres = 0;
+  hipError_t e = hipSuccess;
+  for (const PcgPlan &c : cands) {
+    h->plan = c;
+    if (build_pcg_plan(h) != MPC_OK) {
+      e = hipErrorUnknown;
+      break;
+    }
+    float tmin = INFINITY;
+    for (uint32_t r = 0; e == hipSuccess && r <= reps; ++r) {
+      e = launch_tune_system(b, h->S, h->P, h->gamma, h->adj_mu, st);
+      if (e == hipSuccess) e = hipEventRecord(h->ev[0], st);
+      if (e == hipSuccess) e = pcg_batch(h, h->adj_mu, nullptr, iters, 0.f, nullptr, st);
+      if (e == hipSuccess) e = hipEventRecord(h->ev[1], st);
+      if (e == hipSuccess) e = hipEventSynchronize(h->ev[1]);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+      if (e == hipSuccess && r > 0) tmin = ms < tmin ? ms : tmin;  // pass 0 is the untimed warm-up
+    }
+    if (e != hipSuccess) break;
+    if (nres < max_results) plan_to_c(c, 1e3f * tmin, &results[nres++]);
+    if (tmin < best_ms) {
+      best_ms = tmin;
+      best = c;
+    }
+  }
+  h->plan = e == hipSuccess ? best : saved;
+  const int rc = build_pcg_plan(h);
+  if (n_results) *n_results = nres;
+  if (e != hipSuccess) return MPC_ERR_HIP;
+  return rc;
 }
 
 int mpc_batch_pcg_launch_map(const mpc_handle *h, uint32_t *launch_of_inst) {
@@ -780,16 +918,19 @@ static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_st
     return e;
   }
   uint32_t taken = 0;
-  e = launch_pcg_team(b, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->bin_items, h->bin_off, h->n_bins, &taken,
-                      h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, h->pcg_iters, h->pcg_flag,
-                      h->team_err, stair, st);
+  e = launch_pcg_team(b, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->plan.mixed ? h->bin_items : nullptr,
+                      h->bin_off, h->n_bins, &taken, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status,
+                      h->pcg_iters, h->pcg_flag, h->team_err, stair, st);
   if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
   if (e == hipSuccess)
     e = launch_pcg_stream(b, h->max_big_n, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, h->big_list,
                           h->n_big, h->pcg_iters, h->pcg_flag, stair, st);
   if (e == hipSuccess)
-    e = launch_pcg_res(b, h->bin_items, h->bin_off, taken, h->n_bins - taken, h->S, h->P, h->gamma, d_lambda,
+    e = launch_pcg_res(b, h->bin_items, h->bin_off, taken, h->n_bins - taken, 512, h->S, h->P, h->gamma, d_lambda,
                        max_iter, tol, d_status, h->pcg_iters, h->pcg_flag, stair, st);
+  if (e == hipSuccess)
+    e = launch_pcg_res(b, h->bin_items, h->bin_off, h->n_bins, h->n_bins_small, 256, h->S, h->P, h->gamma,
+                       d_lambda, max_iter, tol, d_status, h->pcg_iters, h->pcg_flag, stair, st);
   return e;
 }
 

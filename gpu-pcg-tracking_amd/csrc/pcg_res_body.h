@@ -8,7 +8,8 @@
 // neighbour exchange per iteration (block-row arithmetic: pcg_onchip.h).
 //
 // A 512-lane workgroup holds up to 128 knots: 32 groups of 16 lanes, each
-// serving RK = 4 consecutive knots (4 lg + t of its instance, t = 0..3).
+// serving RK = 4 consecutive knots (4 lg + t of its instance, t = 0..3); a
+// 256-lane one holds 64 knots, and two of them share a CU.
 // Several instances share a workgroup ("bin"): each owns whole waves, so
 // its dot products are a wave64 reduction plus a fixed-order sum over its
 // own waves, and its arithmetic does not depend on which bin it is packed
@@ -33,21 +34,27 @@ namespace resident {
 
 using namespace onchip;
 
-constexpr int NT = 512;
-constexpr int NGRP = NT / 16;  // 32 groups
-constexpr int NWAV = NT / 64;  // 8 waves
-constexpr int RK = 4;          // knots per group
-constexpr int KNOTS = NGRP * RK;
-constexpr int MAXI = 8;        // instances per bin
+constexpr int RK = 4;    // knots per group
+constexpr int MAXI = 8;  // instances per bin
 
+// Workgroup shapes: NT = 512 lanes (32 groups, 8 waves, 128 knots; one per
+// CU) or 256 (16 groups, 4 waves, 64 knots; two per CU share its LDS).
+template <int NT>
+struct ResShape {
+  static_assert(NT == 256 || NT == 512, "workgroup of 4 or 8 waves");
+  static constexpr int NGRP = NT / 16, NWAV = NT / 64, KNOTS = NGRP * RK;
+};
+
+template <int NT>
 struct ResShared {
+  static constexpr int KNOTS = ResShape<NT>::KNOTS, NWAV = ResShape<NT>::NWAV;
   float L[(KNOTS + 1) * L_SLOT + L_GUARD];  // S(b,b-1) per knot slot (+ one zero slot), pre-rotated rows
   float zb[(KNOTS + 1) * 16];         // staged vector element per knot slot and lane
   float wb[(KNOTS + 1) * 16];         // stair preconditioner: staged D^-1 r
   float red[2][NWAV];
   uint32_t act[NWAV];
+  static_assert(((KNOTS + 1) * L_SLOT + L_GUARD) % 4 == 0, "L and zb are cleared as one float4 range");
 };
-static_assert(((KNOTS + 1) * L_SLOT + L_GUARD) % 4 == 0, "L and zb are cleared as one float4 range");
 
 __device__ __forceinline__ float seg_sum(const float *red, uint32_t w0, uint32_t nw) {
   float s = 0.f;
@@ -56,15 +63,16 @@ __device__ __forceinline__ float seg_sum(const float *red, uint32_t w0, uint32_t
 }
 
 // The PCG of bin `bin` (its instances: items[bin_off[bin] .. bin_off[bin+1]))
-// on a 512-lane workgroup with the ResShared block at smem.
-template <bool STAIR>
+// on an NT-lane workgroup with the ResShared<NT> block at smem.
+template <bool STAIR, int NT>
 __device__ __forceinline__ void body(uint32_t bin, unsigned char *smem, Batch b, const uint4 *__restrict__ items,
                                      const uint32_t *__restrict__ bin_off, const float *__restrict__ S,
                                      const float *__restrict__ Pinv, const float *__restrict__ gamma,
                                      float *__restrict__ lambda, uint32_t max_iter, float tol,
                                      const uint32_t *__restrict__ status, uint32_t *__restrict__ iters_out,
                                      uint32_t *__restrict__ flag_out) {
-  ResShared &sh = *reinterpret_cast<ResShared *>(smem);
+  constexpr int NGRP = ResShape<NT>::NGRP, NWAV = ResShape<NT>::NWAV, KNOTS = ResShape<NT>::KNOTS;
+  ResShared<NT> &sh = *reinterpret_cast<ResShared<NT> *>(smem);
   const uint32_t tid = threadIdx.x, g = tid >> 4, i = tid & 15u, w = tid >> 6;
   const uint32_t d = ror_step();
   {

@@ -249,12 +249,43 @@ int mpc_batch_pcg_layout(const mpc_handle *h, uint32_t *out4);
 /* The PCG launch that solves each instance (n_inst values) within one PCG
  * phase: 0 .. L-1 the cooperative team launches (the last one also solves
  * the first resident bins in its spare workgroups), L the streaming launch
- * when there are instances beyond the teams' reach, then the launch of the
- * remaining resident bins; L = out4[3] of mpc_batch_pcg_layout.  All 0 with
+ * when there are instances beyond the teams' reach, then the launches of the
+ * remaining resident bins (512-lane bins, then 256-lane bins); L = out4[3]
+ * of mpc_batch_pcg_layout.  All 0 with
  * MPC_FLAG_PCG_STREAM or MPC_FLAG_DIRECT_SOLVE (one launch).  No
  * counterpart in the reference (its batch is one instance); for timing
  * analyses such as the per-launch roofline. */
 int mpc_batch_pcg_launch_map(const mpc_handle *h, uint32_t *launch_of_inst);
+
+/* Launch plan of the batched PCG.  Instances with N <= 128 are packed into
+ * bins of the resident kernel (workgroups of 512 lanes = 128 knots, one per
+ * CU; with small_nt = 256, instances with N <= 64 go to bins of 256 lanes =
+ * 64 knots, two per CU), at most max_items instances per bin; 128 < N <= 256
+ * run on two-workgroup teams (teams = 1) or on the streaming kernel; with
+ * mixed = 1 the last team launch also solves the first bins in its spare
+ * workgroups.  Every plan gives the same arithmetic per instance (bins and
+ * teams: bitwise), only the packing changes.  Default {512, 8, 1, 1}.  No
+ * counterpart in the reference (one instance per solve). */
+typedef struct mpc_pcg_plan {
+  uint32_t small_nt;
+  uint32_t max_items;
+  uint32_t mixed;
+  uint32_t teams;
+  float us;          /* mpc_batch_autotune: PCG phase time of this plan, microseconds */
+  uint32_t reserved;
+} mpc_pcg_plan;
+int mpc_batch_get_plan(const mpc_handle *h, mpc_pcg_plan *plan);
+/* Sets the plan (us ignored) and re-packs the current layout. */
+int mpc_batch_set_plan(mpc_handle *h, const mpc_pcg_plan *plan);
+/* Autotuner: times every plan that differs for the current layout on a
+ * synthetic block-tridiagonal system (diag -2.05 I, off-diagonal I), iters
+ * PCG iterations in every instance (tol 0), best of reps passes after one
+ * untimed pass, and keeps the fastest for later solves.  Writes up to
+ * max_results timed plans to results and their count to *n_results.
+ * Overwrites the Schur workspace (call it between solves); not in
+ * MPC_FLAG_PCG_STREAM or MPC_FLAG_DIRECT_SOLVE mode. */
+int mpc_batch_autotune(mpc_handle *h, uint32_t iters, uint32_t reps, mpc_pcg_plan *results, uint32_t max_results,
+                       uint32_t *n_results, mpc_stream_t stream);
 int mpc_batch_phase_times(mpc_handle *h, double *us7);
 
 /* ---- closed-loop tracking ----------------------------------------------------
