@@ -77,6 +77,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-closed-loop", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="issue every solve eagerly (no HIP graph replay), e.g. under counter profiling")
     ap.add_argument("--stress", action="store_true",
                     help="also run the stress configuration (4096 instances, N in {128, 256, 512}, Q/R swept)")
     return ap.parse_args()
@@ -374,6 +376,7 @@ def main():
     def timed(graph):
         # graph: the batched solve is captured once (in the first warm-up
         # step) and replayed as a HIP graph
+        graph = graph and not args.no_graph
         solver.batch_set_graph(graph)
         for _ in range(max(args.warmup, 1 if graph else 0)):
             step()
@@ -509,7 +512,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded splitmix64 start states and sinusoid EE references)",
         "config": {"workload": "batched SQP: 1024 KUKA iiwa EE-tracking instances, N ~ U{32,64,128,256}, "
-                               "3 SQP iterations, block-Jacobi PCG tol 1e-5, replayed as a captured HIP graph",
+                               "3 SQP iterations, block-Jacobi PCG tol 1e-5"
+                               + ("" if args.no_graph else ", replayed as a captured HIP graph"),
                    "instances": n, "total_knots": T, "sqp_iters": K, "pcg_exit_tol": cfg.pcg_exit_tol,
                    "pcg_max_iter": cfg.pcg_max_iter},
         "sqp_iters_per_sec": round(value, 1),
@@ -523,7 +527,7 @@ def main():
                    "pcg_fault": int((st & 16 != 0).sum())},
         "autotune": {"chosen": solver.batch_get_plan(), "iters": 16, "reps": 3, "seconds": round(t_tune, 3),
                      "candidates_us": sorted(tune, key=lambda r: r["us"])},
-        "graph": {"headline_is_graph_replay": True, "bitwise_equal_to_eager": bool(graph_equal),
+        "graph": {"headline_is_graph_replay": not args.no_graph, "bitwise_equal_to_eager": bool(graph_equal),
                   "eager_ms_per_step": round(ms_eager, 4),
                   "eager_kkt_solves_per_sec": round(kkt_per_step / (ms_eager * 1e-3), 1),
                   "note": "eager = the same launch sequence without capture, PCG teams as cooperative launches; "
