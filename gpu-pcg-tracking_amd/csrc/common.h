@@ -66,12 +66,13 @@ struct Params {
   uint32_t flags;
 };
 
-// mailbox of a two-workgroup PCG team: per epoch parity and half, 32
-// self-validating words {epoch (high 32 bits), float bits (low 32 bits)}:
-// words 0..7 the partial sums of the half's 8 waves, 8..21 the vector
-// element of the half's row next to the split
+// mailbox of a PCG team of up to 4 workgroups: per epoch parity and part,
+// 40 self-validating words {epoch (high 32 bits), float bits (low 32 bits)}:
+// words 0..7 the partial sums of the part's 8 waves, 8..21 the vector
+// elements of its first row (for the part before it), 24..37 of its last
+// row (for the part after it)
 struct TeamMailbox {
-  unsigned long long w[2][2][32];
+  unsigned long long w[2][4][40];
 };
 
 // per-instance device state of the batched SQP (struct of arrays)
@@ -131,19 +132,21 @@ hipError_t launch_pcg_res(const Batch &b, const uint4 *items, const uint32_t *bi
                           uint32_t nt, const float *S, const float *Pinv, const float *gamma, float *lambda,
                           uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters,
                           uint32_t *flag, bool stair, hipStream_t st);
-// PCG of 128 < N <= pcg_team_max_knots() on a team of two workgroups
-// (cooperative launch); err receives 1 if a team wait timed out.  With a bin
-// table (items, bin_off, n_bins) and bins_taken, the spare workgroups of the
-// last launch solve the first *bins_taken bins (the caller launches the
-// rest with launch_pcg_res from bin *bins_taken).
-hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n_teams, TeamMailbox *mb,
-                           uint32_t n_mb, const uint4 *items, const uint32_t *bin_off, uint32_t n_bins,
-                           uint32_t *bins_taken, const float *S, const float *Pinv, const float *gamma,
-                           float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
+// PCG of 128 < N <= pcg_team_max_knots(q) on a team of q = 2 or 4
+// workgroups (cooperative launches); err receives 1 if a team wait timed
+// out.  With a bin table (items, bin_off, n_bins) and bins_taken, the spare
+// workgroups of the last two-workgroup team launch solve the first
+// *bins_taken bins (the caller launches the rest with launch_pcg_res from
+// bin *bins_taken).
+hipError_t launch_pcg_team(const Batch &b, uint32_t q, const uint32_t *team_inst, uint32_t n_teams,
+                           TeamMailbox *mb, uint32_t n_mb, const uint4 *items, const uint32_t *bin_off,
+                           uint32_t n_bins, uint32_t *bins_taken, const float *S, const float *Pinv,
+                           const float *gamma, float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
                            uint32_t *iters, uint32_t *flag, uint32_t *err, bool stair, hipStream_t st);
-uint32_t pcg_team_max_knots();
+uint32_t pcg_team_max_knots(uint32_t q);
 uint32_t pcg_team_max_grid();
-void pcg_team_plan(uint32_t n_teams, uint32_t n_bins, uint32_t *launches, uint32_t *bins_taken);
+uint32_t pcg_team_per_launch(uint32_t q);
+void pcg_team_plan(uint32_t n_teams, uint32_t q, uint32_t n_bins, uint32_t *launches, uint32_t *bins_taken);
 uint32_t pcg_res_groups(uint32_t N);
 uint32_t pcg_res_bin_groups(uint32_t nt);  // 16-lane groups per workgroup of nt lanes
 uint32_t pcg_res_max_knots(uint32_t nt);

@@ -1,33 +1,34 @@
-// k_pcg_team.hip -- PCG for horizons 128 < N <= 256: one instance split
-// over a team of two workgroups (two CUs), each holding its half of the
-// Schur system on chip with the block-row arithmetic of pcg_onchip.h
-// (gfx950).
+// k_pcg_team.hip -- PCG for horizons 128 < N <= 512: one instance split
+// over a team of Q = 2 (N <= 256) or Q = 4 (N <= 512) workgroups (Q CUs),
+// each holding its part of the Schur system on chip with the block-row
+// arithmetic of pcg_onchip.h (gfx950).
 //
-// Half 0 owns knots [0, N0), half 1 owns [N0, N), N0 = ceil(N/2) rounded up
-// to a multiple of 4; each workgroup's 32 groups of 16 lanes serve 4
-// consecutive knots each (local knot 4 g + t).  Half 0 also keeps
-// L_{N0} = S(N0, N0-1) in the LDS slot after its own, for its last row's
-// S(N0-1, N0) p_{N0} term.  The knots next to the split (the last of half
-// 0's group N0/4 - 1, the first of half 1's group 0) take their outer
-// neighbour's vector element from the partner.
+// Part p owns knots [p C, min(N, (p+1) C)), C = ceil(N/Q) rounded up to a
+// multiple of 4 (at most 128); each workgroup's 32 groups of 16 lanes serve 4
+// consecutive knots each (local knot 4 g + t).  Every part but the last also
+// keeps L_ke = S(ke, ke-1) of the next part's first knot in the LDS slot
+// after its own, for its last row's S(ke-1, ke) p_ke term.  The knots next
+// to a split (the last of a part's last group, the first of the next part's
+// group 0) take their outer neighbour's vector element from that part.
 //
 // With the stair preconditioner (STAIR) a third exchange per iteration
-// carries the row of w = D^-1 r at the split for the extra block product.
+// carries the rows of w = D^-1 r at the splits for the extra block product.
 //
 // Exchange through an L2 mailbox, twice per iteration (p.Sp, then r.z with
-// z of the row next to the split): before the workgroup barrier that
+// z of the rows next to the splits): before the workgroup barrier that
 // publishes the wave partials in LDS, every wave also posts its partial to
-// the partner, and the boundary group posts its row; after the barrier
-// every wave polls the partner's 8 partials itself (the boundary wave also
-// its row), so no second barrier is needed.  Totals are
-// (sum of half 0's waves) + (sum of half 1's waves), each in wave order, on
-// both sides, so both halves take identical decisions.  Same algorithm and
-// exit rule as k_pcg.hip (GPU-PCG pcg<T,14,N>, toplevel.cuh:143-157).
+// the mailbox, and the boundary groups post their rows; after the barrier
+// every wave polls the other parts' partials itself (the boundary waves also
+// the rows they need), so no second barrier is needed.  Totals are
+// ((part 0 + part 1) + part 2) + part 3, each part's the sum of its waves in
+// wave order, on every part, so all parts take identical decisions.  Same
+// algorithm and exit rule as k_pcg.hip (GPU-PCG pcg<T,14,N>,
+// toplevel.cuh:143-157).
 //
 // Each launch is cooperative (all workgroups co-resident) and solves one
-// team per workgroup pair; more teams than fit at once are solved by
-// consecutive launches.  The two halves of a team sit 8 workgroup ids apart
-// so they land on the same XCD and share its L2.  Mailbox words carry their
+// team per Q workgroups; more teams than fit at once are solved by
+// consecutive launches.  The parts of a team sit 8 workgroup ids apart so
+// they land on the same XCD and share its L2.  Mailbox words carry their
 // epoch, so relaxed (fence-free) atomics suffice.  Every wait is bounded:
 // a wave that gives up sets the error word and the workgroup's fail flag,
 // and the workgroup leaves at its next barrier with the instance's PCG flag
@@ -50,17 +51,21 @@ constexpr int NT = 512;
 constexpr int NGRP = NT / 16;  // 32 groups
 constexpr int NWAV = NT / 64;  // 8 waves
 constexpr int RK = 4;
-constexpr int KNOTS = NGRP * RK;  // knots per half
+constexpr int KNOTS = NGRP * RK;  // knots per part
 constexpr uint32_t SPIN_LIMIT = 1u << 16;
-constexpr uint32_t MB_ROW = 8;  // first mailbox word of the boundary row
-static_assert(MB_ROW == NWAV && MB_ROW + NX <= 32, "mailbox layout");
+// mailbox words of a part: 0..7 wave partials, MB_FIRST.. its first row,
+// MB_LAST.. its last row; lanes of a polling wave: 0 .. 8(Q-1)-1 the other
+// parts' partials, LANE_M.. the row before the part, LANE_P.. the row after
+constexpr uint32_t MB_FIRST = 8, MB_LAST = 24, LANE_M = 32, LANE_P = 48;
+static_assert(MB_FIRST == NWAV && MB_LAST >= MB_FIRST + NX && MB_LAST + NX <= 40, "mailbox layout");
+static_assert(8 * 3 <= LANE_M && LANE_M + NX <= LANE_P && LANE_P + NX <= 64, "polling lanes");
 
 struct TeamShared {
   float L[(KNOTS + 1) * L_SLOT + L_GUARD];
   float zb[(KNOTS + 1) * 16];
   float wb[(KNOTS + 1) * 16];  // stair preconditioner: staged D^-1 r
   float red[2][NWAV];
-  float recv[16];
+  float recv[2][16];  // rows from the part before / after
   uint32_t fail;
 };
 static_assert(((KNOTS + 1) * L_SLOT + L_GUARD) % 4 == 0, "L and zb are cleared as one float4 range");
@@ -72,33 +77,31 @@ __device__ __forceinline__ float sum8(const float *red) {
   return s;
 }
 
-// x of lanes 0..7, summed in lane order (the order of sum8)
-__device__ __forceinline__ float sum_lanes8(float x) {
+// x of lanes l0 .. l0+7 (l0 uniform), summed in lane order (the order of sum8)
+__device__ __forceinline__ float sum_lanes8(float x, uint32_t l0) {
   float s = 0.f;
 #pragma unroll
-  for (int v = 0; v < NWAV; ++v) s += __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), v));
+  for (uint32_t v = 0; v < (uint32_t)NWAV; ++v)
+    s += __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), (int)(l0 + v)));
   return s;
 }
 
-__device__ __forceinline__ void mb_post(TeamMailbox *mb, int half, uint32_t e, uint32_t word, float v) {
-  __hip_atomic_store(&mb->w[e & 1u][half][word], ((unsigned long long)e << 32) | __float_as_uint(v),
+__device__ __forceinline__ void mb_post(TeamMailbox *mb, uint32_t part, uint32_t e, uint32_t word, float v) {
+  __hip_atomic_store(&mb->w[e & 1u][part][word], ((unsigned long long)e << 32) | __float_as_uint(v),
                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Lanes lo <= lane < hi of the calling wave wait for the partner's mailbox
-// word (lane) of epoch e and return its value (0 in other lanes); ok is
-// cleared when a word did not arrive in time.  Every mailbox word carries
-// its epoch, so a word is taken only once its tag shows the current epoch.
-__device__ __forceinline__ float mb_wait(const TeamMailbox *mb, int half, uint32_t e, uint32_t lane, uint32_t lo,
-                                         uint32_t hi, bool &ok) {
-  const unsigned long long *theirs = mb->w[e & 1u][1 - half];
-  const bool want = lane >= lo && lane < hi;
+// Lanes with `want` wait for the mailbox word at src to carry epoch e and
+// return its value (0 in other lanes, whose src is not read); ok is cleared
+// when a word did not arrive in time.  Every mailbox word carries its epoch,
+// so a word is taken only once its tag shows the current epoch.
+__device__ __forceinline__ float mb_wait(const unsigned long long *src, bool want, uint32_t e, bool &ok) {
   bool got = !want;
   unsigned long long v = 0;
   uint32_t spins = 0;
   while (!__all(got)) {
     if (!got) {
-      v = __hip_atomic_load(theirs + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      v = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       got = (uint32_t)(v >> 32) == e;
     }
     if (++spins > SPIN_LIMIT) break;
@@ -114,10 +117,10 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Half `(blk >> 3) & 1` of team team0 + pair(blk) on a 512-lane workgroup
-// with the TeamShared block at smem; blk is the workgroup's id among the
-// team workgroups of the launch.
-template <bool STAIR>
+// Part `(blk >> 3) & (Q-1)` of team team0 + slot(blk) on a 512-lane
+// workgroup with the TeamShared block at smem; blk is the workgroup's id
+// among the team workgroups of the launch.
+template <bool STAIR, int Q>
 __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Batch b,
                                           const uint32_t *__restrict__ team_inst, uint32_t n_teams, uint32_t team0,
                                           TeamMailbox *mboxes, const float *__restrict__ S,
@@ -128,18 +131,21 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
   TeamShared &sh = *reinterpret_cast<TeamShared *>(smem);
   const uint32_t tid = threadIdx.x, g = tid >> 4, i = tid & 15u, w = tid >> 6, lane64 = tid & 63u;
   const uint32_t d = ror_step();
-  const int half = (int)((blk >> 3) & 1u);
-  const uint32_t pair = (blk >> 4) * 8u + (blk & 7u);
-  const uint32_t team = team0 + pair;
+  static_assert(Q == 2 || Q == 4, "teams of 2 or 4 workgroups");
+  constexpr uint32_t QS = Q == 4 ? 2u : 1u;  // log2 Q
+  const uint32_t part = (blk >> 3) & (uint32_t)(Q - 1);
+  const uint32_t slot = (blk >> (3u + QS)) * 8u + (blk & 7u);
+  const uint32_t team = team0 + slot;
   if (team >= n_teams) return;
   const uint32_t inst = team_inst[team];
-  if (status && !(status[inst] & ST_ACTIVE)) return;  // same decision on both halves
-  TeamMailbox *mb = mboxes + pair;
+  if (status && !(status[inst] & ST_ACTIVE)) return;  // same decision on every part
+  TeamMailbox *mb = mboxes + slot;
   const bool lane_ok = i < (uint32_t)NX;
   const uint32_t N = inst_len(b, inst), K0 = inst_first(b, inst);
-  const uint32_t N0 = (((N + 1) / 2) + 3u) & ~3u;  // <= 128 for N <= 256
-  const uint32_t kb = half ? N0 : 0u, ke = half ? N : N0;  // own knots [kb, ke)
-  const uint32_t n_own = ke - kb;
+  const uint32_t chunk = (((N + Q - 1) / Q) + 3u) & ~3u;  // <= 128 for N <= 128 Q
+  const uint32_t kb = part * chunk, ke = kb + chunk < N ? kb + chunk : N;  // own knots [kb, ke)
+  const uint32_t n_own = ke - kb;  // > 0: the host gives Q = 4 teams N > 256
+  const bool has_prev = part > 0, has_next = ke < N;
 
   {
     float4 *z4 = reinterpret_cast<float4 *>(sh.L);
@@ -148,9 +154,9 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     if (tid == 0) sh.fail = 0u;
   }
   __syncthreads();
-  // half 0: L_{N0} into the slot after its own knots
-  if (half == 0 && N0 < N && g == 0 && lane_ok) {
-    const float *Sn = S + (size_t)S_STRIDE * (K0 + N0);
+  // every part but the last: L_ke into the slot after its own knots
+  if (has_next && g == 0 && lane_ok) {
+    const float *Sn = S + (size_t)S_STRIDE * (K0 + ke);
     float *Ls = sh.L + n_own * L_SLOT;
 #pragma unroll
     for (int n = 0; n < 16; ++n) {
@@ -169,46 +175,68 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     load_row(R[t], sh.L + lk * L_SLOT, has, i, d, S, Pinv, K0, kb + lk);
     lam[t] = has && lane_ok ? lambda[(size_t)NX * (K0 + kb + lk) + i] : 0.f;
   }
-  // the group's outer neighbours: in the own zb, from the partner, or none
+  // the group's outer neighbours: in the own zb, from the next / previous
+  // part, or none
   const bool own_m = g > 0 && b0 < n_own, own_p = b0 + RK < n_own;
-  const bool far_m = half == 1 && g == 0, far_p = half == 0 && b0 + RK == n_own && ke < N;
-  const uint32_t gb = half ? 0u : n_own / RK - 1u, wb = gb >> 2;  // group / wave at the split
-  const bool row_lane = g == gb && lane_ok;
+  const bool far_m = has_prev && g == 0, far_p = has_next && b0 + RK == n_own;
+  const uint32_t gl = n_own / RK - 1u, wl = gl >> 2;  // last group / its wave (n_own % RK == 0 with has_next)
+  const bool first_lane = has_prev && g == 0 && lane_ok, last_lane = has_next && g == gl && lane_ok;
   float *zb0 = sh.zb + b0 * 16 + i;
   uint32_t e = 0;  // mailbox epoch
   bool ok = true;
   // a wait timed out somewhere in the workgroup: the solve is abandoned and
-  // the instance flagged (both halves write the same value)
+  // the instance flagged (every part writes the same value)
   auto fault = [&]() {
     if (tid == 0 && flag_out) flag_out[inst] = PCG_FLAG_FAULT;
   };
-  // own first / last element to LDS, the row at the split to the partner
+  // own first / last element to LDS, the rows at the splits to the mailbox
+  auto post_rows = [&](const float (&v)[RK]) {
+    if (first_lane) mb_post(mb, part, e, MB_FIRST + i, v[0]);
+    if (last_lane) mb_post(mb, part, e, MB_LAST + i, v[RK - 1]);
+  };
   auto stage = [&](const float (&v)[RK]) {
     zb0[0] = v[0];
     zb0[(RK - 1) * 16] = v[RK - 1];
-    if (row_lane) mb_post(mb, half, e, MB_ROW + i, half ? v[0] : v[RK - 1]);
+    post_rows(v);
   };
-  // after the barrier: the partner's partials (every wave) and, when the
-  // exchange carries it, its row (wave wb, into sh.recv)
-  auto collect = [&](bool partials, bool row) {
-    const bool rw = row && w == wb;
+  // after the barrier: the other parts' partials (every wave) and, when the
+  // exchange carries them, the rows next to the part (the waves of group 0
+  // and of the last group, into sh.recv); returns the partial lanes
+  // each polling lane's role and mailbox word (within one epoch's parity):
+  // 1 = a partial of another part, 2 = the row before the part (wave 0),
+  // 3 = the row after it (wave wl)
+  constexpr uint32_t PART_WORDS = sizeof(TeamMailbox) / sizeof(unsigned long long) / 8u;
+  uint32_t role = 0, word = 0;
+  if (lane64 < 8u * (Q - 1)) {
+    const uint32_t q = lane64 >> 3;
+    role = 1u;
+    word = (q < part ? q : q + 1u) * PART_WORDS + (lane64 & 7u);
+  } else if (has_prev && w == 0 && lane64 >= LANE_M && lane64 < LANE_M + NX) {
+    role = 2u;
+    word = (part - 1u) * PART_WORDS + MB_LAST + lane64 - LANE_M;
+  } else if (has_next && w == wl && lane64 >= LANE_P && lane64 < LANE_P + NX) {
+    role = 3u;
+    word = (part + 1u) * PART_WORDS + MB_FIRST + lane64 - LANE_P;
+  }
+  auto collect = [&](bool partials, bool rows) {
+    const bool rm = rows && has_prev && w == 0, rp = rows && has_next && w == wl;
+    const bool want = (role == 1u && partials) || (role == 2u && rows) || (role == 3u && rows);
     float x = 0.f;
-    if (partials || rw) {
-      x = mb_wait(mb, half, e, lane64, partials ? 0u : MB_ROW, rw ? MB_ROW + NX : MB_ROW, ok);
-      if (rw) {
-        if (lane64 >= MB_ROW && lane64 < MB_ROW + NX) sh.recv[lane64 - MB_ROW] = x;
-        wave_sync();
-      }
+    if (partials || rm || rp) {
+      x = mb_wait(&mb->w[e & 1u][0][0] + word, want, e, ok);
+      if (want && role == 2u) sh.recv[0][lane64 - LANE_M] = x;
+      if (want && role == 3u) sh.recv[1][lane64 - LANE_P] = x;
+      if (rm || rp) wave_sync();
     }
     if (!ok) {
       sh.fail = 1u;
       if (lane64 == 0) atomicOr(err, 1u);
     }
-    return partials ? sum_lanes8(x) : 0.f;
+    return x;
   };
   auto neighbours = [&](float &vm, float &vp) {
-    vm = own_m ? zb0[-16] : (far_m && lane_ok ? sh.recv[i] : 0.f);
-    vp = own_p ? zb0[RK * 16] : (far_p && lane_ok ? sh.recv[i] : 0.f);
+    vm = own_m ? zb0[-16] : (far_m && lane_ok ? sh.recv[0][i] : 0.f);
+    vp = own_p ? zb0[RK * 16] : (far_p && lane_ok ? sh.recv[1][i] : 0.f);
   };
   // stair preconditioner: w = D^-1 r staged like z, in its own buffer, and
   // its row at the split exchanged on an epoch of its own (row only)
@@ -216,14 +244,21 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
   auto stage_w = [&](const float (&v)[RK]) {
     wb0[0] = v[0];
     wb0[(RK - 1) * 16] = v[RK - 1];
-    if (row_lane) mb_post(mb, half, e, MB_ROW + i, half ? v[0] : v[RK - 1]);
+    post_rows(v);
   };
   auto neighbours_w = [&](float &vm, float &vp) {
-    vm = own_m ? wb0[-16] : (far_m && lane_ok ? sh.recv[i] : 0.f);
-    vp = own_p ? wb0[RK * 16] : (far_p && lane_ok ? sh.recv[i] : 0.f);
+    vm = own_m ? wb0[-16] : (far_m && lane_ok ? sh.recv[0][i] : 0.f);
+    vp = own_p ? wb0[RK * 16] : (far_p && lane_ok ? sh.recv[1][i] : 0.f);
   };
   auto smv = [&](const float (&x)[RK], float xm, float xp) { group_smv<RK>(R, sh.L + b0 * L_SLOT, i, x, xm, xp, y); };
-  auto total = [&](float mine, float theirs) { return half ? theirs + mine : mine + theirs; };
+  // ((part 0 + part 1) + part 2) + part 3: the own part from LDS, the others
+  // from the partial lanes x of collect
+  auto total = [&](float mine, float x) {
+    float t = 0.f;
+#pragma unroll
+    for (uint32_t q = 0; q < (uint32_t)Q; ++q) t += q == part ? mine : sum_lanes8(x, 8u * (q < part ? q : q - 1u));
+    return t;
+  };
 
   // boundary lambda to the partner
   ++e;
@@ -235,7 +270,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
 
   // prologue: r = gamma - S lambda, z = Pinv r, p = z, eta = r.z
   smv(lam, pm, pp);
-  float part = 0.f;
+  float part_sum = 0.f;
 #pragma unroll
   for (int t = 0; t < RK; ++t) {
     const uint32_t lk = b0 + (uint32_t)t;
@@ -259,13 +294,13 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     group_precond<RK>(R, r, z);
   }
 #pragma unroll
-  for (int t = 0; t < RK; ++t) part += r[t] * z[t];
-  part = wave_sum(part);
+  for (int t = 0; t < RK; ++t) part_sum += r[t] * z[t];
+  part_sum = wave_sum(part_sum);
   __syncthreads();  // neighbour and row reads done before zb / recv are reused
   ++e;
   if (lane64 == 0) {
-    sh.red[1][w] = part;
-    mb_post(mb, half, e, w, part);
+    sh.red[1][w] = part_sum;
+    mb_post(mb, part, e, w, part_sum);
   }
   stage(z);
   __syncthreads();
@@ -288,7 +323,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     ++e;
     if (lane64 == 0) {
       sh.red[0][w] = pyp;
-      mb_post(mb, half, e, w, pyp);
+      mb_post(mb, part, e, w, pyp);
     }
     __syncthreads();
     if (sh.fail) return fault();
@@ -323,15 +358,15 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     ++e;
     if (lane64 == 0) {
       sh.red[1][w] = rz;
-      mb_post(mb, half, e, w, rz);
+      mb_post(mb, part, e, w, rz);
     }
     stage(z);
     __syncthreads();
     if (sh.fail) return fault();
     float zm = own_m ? zb0[-16] : 0.f, zp = own_p ? zb0[RK * 16] : 0.f;  // issued before the wait
     const float eta_new = total(sum8(sh.red[1]), collect(true, true));
-    if (far_m && lane_ok) zm = sh.recv[i];
-    if (far_p && lane_ok) zp = sh.recv[i];
+    if (far_m && lane_ok) zm = sh.recv[0][i];
+    if (far_p && lane_ok) zp = sh.recv[1][i];
     ++it;
     if (fabsf(eta_new) < tol_i) {
       conv = true;
@@ -351,13 +386,13 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     const uint32_t lk = b0 + (uint32_t)t;
     if (lk < n_own && lane_ok) lambda[(size_t)NX * (K0 + kb + lk) + i] = lam[t];
   }
-  if (half == 0 && tid == 0) {
+  if (part == 0 && tid == 0) {
     if (iters_out) iters_out[inst] = it;
     if (flag_out) flag_out[inst] = conv ? 0u : 1u;
   }
 }
 
-template <bool STAIR>
+template <bool STAIR, int Q>
 __global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__restrict__ team_inst,
                                                  uint32_t n_teams, uint32_t team0, TeamMailbox *mboxes,
                                                  const float *__restrict__ S, const float *__restrict__ Pinv,
@@ -366,12 +401,12 @@ __global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__rest
                                                  uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out,
                                                  uint32_t *__restrict__ err) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  team_body<STAIR>(blockIdx.x, smem_raw, b, team_inst, n_teams, team0, mboxes, S, Pinv, gamma, lambda, max_iter, tol,
-                   status, iters_out, flag_out, err);
+  team_body<STAIR, Q>(blockIdx.x, smem_raw, b, team_inst, n_teams, team0, mboxes, S, Pinv, gamma, lambda, max_iter,
+                      tol, status, iters_out, flag_out, err);
 }
 
-// The last, partly filled team launch with resident bins in its spare
-// workgroups: ids below team_grid run the team body, the others bins
+// The last, partly filled launch of two-workgroup teams with resident bins
+// in its spare workgroups: ids below team_grid run the team body, the others bins
 // bin0 + (id - team_grid).  Bins need no co-residency; they only fill
 // workgroup slots that would idle until the launch ends.
 template <bool STAIR>
@@ -386,8 +421,8 @@ __global__ void __launch_bounds__(NT) k_pcg_mixed(Batch b, const uint32_t *__res
                                                   uint32_t *__restrict__ err) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   if (blockIdx.x < team_grid)
-    team_body<STAIR>(blockIdx.x, smem_raw, b, team_inst, n_teams, team0, mboxes, S, Pinv, gamma, lambda, max_iter,
-                     tol, status, iters_out, flag_out, err);
+    team_body<STAIR, 2>(blockIdx.x, smem_raw, b, team_inst, n_teams, team0, mboxes, S, Pinv, gamma, lambda,
+                        max_iter, tol, status, iters_out, flag_out, err);
   else
     resident::body<STAIR, 512>(bin0 + (blockIdx.x - team_grid), smem_raw, b, items, bin_off, S, Pinv, gamma, lambda,
                           max_iter, tol, status, iters_out, flag_out);
@@ -401,7 +436,7 @@ __global__ void k_reset_mailboxes(TeamMailbox *mb, uint32_t n) {
 
 }  // namespace
 
-uint32_t pcg_team_max_knots() { return 2u * KNOTS; }
+uint32_t pcg_team_max_knots(uint32_t q) { return q * (uint32_t)KNOTS; }
 
 constexpr size_t MIXED_SMEM =
     sizeof(TeamShared) > sizeof(resident::ResShared<512>) ? sizeof(TeamShared) : sizeof(resident::ResShared<512>);
@@ -418,47 +453,54 @@ uint32_t pcg_team_max_grid() {
     cached = 0;
     return 0;
   }
-  // every variant (block Jacobi, stair; teams only, mixed) must fit; the
-  // launches use the smallest
-  const void *fns[4] = {(const void *)k_pcg_team<false>, (const void *)k_pcg_team<true>,
-                        (const void *)k_pcg_mixed<false>, (const void *)k_pcg_mixed<true>};
-  const size_t smem[4] = {sizeof(TeamShared), sizeof(TeamShared), MIXED_SMEM, MIXED_SMEM};
+  // every variant (block Jacobi, stair; two or four parts; mixed) must fit;
+  // the launches use the smallest
+  const void *fns[6] = {(const void *)k_pcg_team<false, 2>, (const void *)k_pcg_team<true, 2>,
+                        (const void *)k_pcg_team<false, 4>, (const void *)k_pcg_team<true, 4>,
+                        (const void *)k_pcg_mixed<false>,   (const void *)k_pcg_mixed<true>};
+  const size_t smem[6] = {sizeof(TeamShared), sizeof(TeamShared), sizeof(TeamShared),
+                          sizeof(TeamShared), MIXED_SMEM,         MIXED_SMEM};
   int per_cu = 1 << 30;
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 6; ++k) {
     int n = 0;
     (void)hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem[k]);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fns[k], NT, smem[k]) != hipSuccess) n = 0;
     per_cu = n < per_cu ? n : per_cu;
   }
   (void)hipGetLastError();
-  cached = (per_cu * cus) & ~15;  // whole groups of 16 (8 pairs, one per XCD slot)
+  cached = (per_cu * cus) & ~15;  // whole groups of 16 (8 two-part teams, one per XCD slot)
   return (uint32_t)cached;
 }
 
-// Launch plan of n_teams teams: the number of cooperative launches and how
-// many of n_bins resident bins the last one solves in its spare workgroups.
-void pcg_team_plan(uint32_t n_teams, uint32_t n_bins, uint32_t *launches, uint32_t *bins_taken) {
+// Teams of q workgroups per cooperative launch: whole groups of 8 teams (one
+// per XCD slot, each team's parts on one XCD) within the grid limit.
+uint32_t pcg_team_per_launch(uint32_t q) { return pcg_team_max_grid() / (8u * q) * 8u; }
+
+// Launch plan of n_teams teams of q workgroups: the number of cooperative
+// launches and how many of n_bins resident bins the last one solves in its
+// spare workgroups (two-workgroup teams only).
+void pcg_team_plan(uint32_t n_teams, uint32_t q, uint32_t n_bins, uint32_t *launches, uint32_t *bins_taken) {
   *launches = 0;
   *bins_taken = 0;
-  const uint32_t cap = pcg_team_max_grid();
-  if (n_teams == 0 || cap < 16) return;
-  const uint32_t per_launch = cap / 2;
+  const uint32_t per_launch = pcg_team_per_launch(q);
+  if (n_teams == 0 || per_launch == 0) return;
   *launches = (n_teams + per_launch - 1) / per_launch;
+  if (q != 2) return;
   const uint32_t n_last = n_teams - (*launches - 1) * per_launch;
-  const uint32_t spare = cap - 16u * ((n_last + 7u) / 8u);
+  const uint32_t spare = pcg_team_max_grid() - 16u * ((n_last + 7u) / 8u);
   *bins_taken = spare < n_bins ? spare : n_bins;
 }
 
-hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n_teams, TeamMailbox *mb,
-                           uint32_t n_mb, const uint4 *items, const uint32_t *bin_off, uint32_t n_bins,
-                           uint32_t *bins_taken, const float *S, const float *Pinv, const float *gamma,
-                           float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
+hipError_t launch_pcg_team(const Batch &b, uint32_t q, const uint32_t *team_inst, uint32_t n_teams,
+                           TeamMailbox *mb, uint32_t n_mb, const uint4 *items, const uint32_t *bin_off,
+                           uint32_t n_bins, uint32_t *bins_taken, const float *S, const float *Pinv,
+                           const float *gamma, float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
                            uint32_t *iters, uint32_t *flag, uint32_t *err, bool stair, hipStream_t st) {
   if (bins_taken) *bins_taken = 0;
   if (n_teams == 0) return hipSuccess;
-  const uint32_t cap = pcg_team_max_grid();
-  if (cap < 16) return hipErrorNotSupported;
-  const uint32_t per_launch = cap / 2;  // teams per launch
+  if (q != 2 && q != 4) return hipErrorInvalidValue;
+  const uint32_t per_launch = pcg_team_per_launch(q);  // teams per launch
+  if (per_launch == 0) return hipErrorNotSupported;
   constexpr uint32_t W = sizeof(TeamMailbox) / sizeof(unsigned long long);
   // under stream capture: ordinary launches of the same grids (see the
   // header comment), since a graph node carries no co-residency guarantee
@@ -474,9 +516,9 @@ hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n
   };
   for (uint32_t team0 = 0; team0 < n_teams; team0 += per_launch) {
     const uint32_t n = n_teams - team0 < per_launch ? n_teams - team0 : per_launch;
-    const uint32_t grid = 16u * ((n + 7u) / 8u);  // pairs in whole groups of 8
-    if (grid / 2 > n_mb) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_reset_mailboxes, dim3((grid / 2 * W + 255) / 256), dim3(256), 0, st, mb, grid / 2);
+    const uint32_t grid = 8u * q * ((n + 7u) / 8u);  // teams in whole groups of 8
+    if (grid / q > n_mb) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reset_mailboxes, dim3((grid / q * W + 255) / 256), dim3(256), 0, st, mb, grid / q);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     Batch bb = b;
@@ -485,12 +527,13 @@ hipError_t launch_pcg_team(const Batch &b, const uint32_t *team_inst, uint32_t n
     uint32_t nb = 0;
     if (team0 + n == n_teams && items && bins_taken) {
       uint32_t nl = 0;
-      pcg_team_plan(n_teams, n_bins, &nl, &nb);
+      pcg_team_plan(n_teams, q, n_bins, &nl, &nb);
     }
     if (nb == 0) {
       void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, (void *)&S, (void *)&Pinv, (void *)&gamma,
                       &lambda, &max_iter, &tol, (void *)&status, &iters, &flag, &err};
-      const void *fn = stair ? (const void *)k_pcg_team<true> : (const void *)k_pcg_team<false>;
+      const void *fn = q == 4 ? (stair ? (const void *)k_pcg_team<true, 4> : (const void *)k_pcg_team<false, 4>)
+                              : (stair ? (const void *)k_pcg_team<true, 2> : (const void *)k_pcg_team<false, 2>);
       e = launch(fn, grid, args, sizeof(TeamShared));
     } else {
       uint32_t tg = grid, bin0 = 0;

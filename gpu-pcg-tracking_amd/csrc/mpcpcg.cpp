@@ -53,7 +53,7 @@ struct mpc_handle {
   uint32_t *team_list = nullptr, *team_one = nullptr, *team_err = nullptr;
   uint32_t *dir_order = nullptr;  // direct solve: instances longest first
   TeamMailbox *mbox = nullptr;
-  uint32_t n_teams = 0, n_mbox = 256;
+  uint32_t n_teams = 0, n_quads = 0, n_mbox = 256;  // team_list: n_teams pairs, then n_quads quads
   // differentiable QP layer workspace: adjoint multipliers (14 T), per-knot
   // gradient terms (21 T) and per-knot loss terms (T)
   float *adj_mu = nullptr, *adj_knot = nullptr, *loss_knot = nullptr;
@@ -77,6 +77,7 @@ struct mpc_handle {
       return epoch == o.epoch && sqp_iters == o.sqp_iters && std::memcmp(ptr, o.ptr, sizeof(ptr)) == 0;
     }
   };
+  uint32_t team_launches = 0;  // cooperative team launches per PCG phase
   bool graph_on = false;
   uint64_t graph_epoch = 1;
   GraphKey graph_key;
@@ -142,9 +143,10 @@ static hipError_t pcg_single(mpc_handle *h, uint32_t N, const float *S, const fl
     if (N <= pcg_res_max_knots(512))
       return launch_pcg_res(single(N), nullptr, nullptr, 0, 1, 512, S, P, gamma, lambda, max_iter, tol, nullptr,
                             iters, flag, stair, st);
-    if (N <= pcg_team_max_knots() && pcg_team_max_grid() >= 16)
-      return launch_pcg_team(single(N), h->team_one, 1, h->mbox, h->n_mbox, nullptr, nullptr, 0, nullptr, S, P,
-                             gamma, lambda, max_iter, tol, nullptr, iters, flag, h->team_err, stair, st);
+    for (uint32_t q : {2u, 4u})
+      if (N <= pcg_team_max_knots(q) && pcg_team_per_launch(q) > 0)
+        return launch_pcg_team(single(N), q, h->team_one, 1, h->mbox, h->n_mbox, nullptr, nullptr, 0, nullptr, S, P,
+                               gamma, lambda, max_iter, tol, nullptr, iters, flag, h->team_err, stair, st);
   }
   return launch_pcg_stream(single(N), N, S, P, gamma, lambda, max_iter, tol, nullptr, nullptr, 0, iters, flag,
                            stair, st);
@@ -584,13 +586,14 @@ static int build_pcg_plan(mpc_handle *h) {
   const uint32_t *horizons = h->horizons.data();
   const uint32_t max_items = pl.max_items < pcg_res_max_items() ? pl.max_items : pcg_res_max_items();
   const uint32_t small_max = pl.small_nt == 256 ? pcg_res_max_knots(256) : 0u;
-  std::vector<uint32_t> order, order_s, big, teams;
+  std::vector<uint32_t> order, order_s, big, teams, quads;
   uint32_t max_big = 0;
-  const bool team_ok = pl.teams && pcg_team_max_grid() >= 16;
+  const bool pair_ok = pl.teams && pcg_team_per_launch(2) > 0, quad_ok = pl.teams && pcg_team_per_launch(4) > 0;
   for (uint32_t i = 0; i < n_inst; ++i) {
     if (horizons[i] <= small_max) order_s.push_back(i);
     else if (horizons[i] <= pcg_res_max_knots(512)) order.push_back(i);
-    else if (team_ok && horizons[i] <= pcg_team_max_knots()) teams.push_back(i);
+    else if (pair_ok && horizons[i] <= pcg_team_max_knots(2)) teams.push_back(i);
+    else if (quad_ok && horizons[i] <= pcg_team_max_knots(4)) quads.push_back(i);
     else {
       big.push_back(i);
       max_big = horizons[i] > max_big ? horizons[i] : max_big;
@@ -640,16 +643,20 @@ static int build_pcg_plan(mpc_handle *h) {
   HIP_TRY(hipMemcpy(h->bin_off, boff.data(), sizeof(uint32_t) * boff.size(), hipMemcpyHostToDevice));
   if (!big.empty())
     HIP_TRY(hipMemcpy(h->big_list, big.data(), sizeof(uint32_t) * big.size(), hipMemcpyHostToDevice));
-  if (!teams.empty())
-    HIP_TRY(hipMemcpy(h->team_list, teams.data(), sizeof(uint32_t) * teams.size(), hipMemcpyHostToDevice));
-  // launch of each instance within a PCG phase: team launches 0 .. nl-1
-  // (the last also takes the first bins), then the streaming launch, then
-  // the resident launches (512-lane bins, then 256-lane bins)
-  uint32_t nl = 0, taken = 0;
-  pcg_team_plan((uint32_t)teams.size(), pl.mixed ? n512 : 0u, &nl, &taken);
-  const uint32_t per_launch = pcg_team_max_grid() / 2;
+  std::vector<uint32_t> tl(teams);
+  tl.insert(tl.end(), quads.begin(), quads.end());
+  if (!tl.empty()) HIP_TRY(hipMemcpy(h->team_list, tl.data(), sizeof(uint32_t) * tl.size(), hipMemcpyHostToDevice));
+  // launch of each instance within a PCG phase: four-workgroup team
+  // launches 0 .. nq-1, two-workgroup ones nq .. nq+np-1 (the last also
+  // takes the first bins), then the streaming launch, then the resident
+  // launches (512-lane bins, then 256-lane bins)
+  uint32_t nq = 0, np = 0, taken = 0, unused = 0;
+  pcg_team_plan((uint32_t)quads.size(), 4, 0, &nq, &unused);
+  pcg_team_plan((uint32_t)teams.size(), 2, pl.mixed ? n512 : 0u, &np, &taken);
   h->pcg_launch.assign(n_inst, 0u);
-  for (size_t t = 0; t < teams.size(); ++t) h->pcg_launch[teams[t]] = (uint32_t)(t / per_launch);
+  for (size_t t = 0; t < quads.size(); ++t) h->pcg_launch[quads[t]] = (uint32_t)(t / pcg_team_per_launch(4));
+  for (size_t t = 0; t < teams.size(); ++t) h->pcg_launch[teams[t]] = nq + (uint32_t)(t / pcg_team_per_launch(2));
+  const uint32_t nl = nq + np;
   uint32_t next = nl;
   if (!big.empty())
     for (uint32_t i : big) h->pcg_launch[i] = next;
@@ -659,6 +666,8 @@ static int build_pcg_plan(mpc_handle *h) {
   for (size_t w = 0; w < bins.size(); ++w)
     for (uint32_t i : bins[w].inst) h->pcg_launch[i] = w < taken ? nl - 1 : (w < n512 ? res512 : next);
   h->n_teams = (uint32_t)teams.size();
+  h->n_quads = (uint32_t)quads.size();
+  h->team_launches = nl;
   h->n_bins = n512;
   h->n_bins_small = (uint32_t)bins_s.size();
   h->n_big = (uint32_t)big.size();
@@ -766,13 +775,10 @@ int mpc_batch_qp_adjoint(mpc_handle *h, const float *d_xu, const float *d_ee_ref
 int mpc_batch_pcg_layout(const mpc_handle *h, uint32_t *out4) {
   if (!h || !out4) return MPC_ERR_ARG;
   if (!h->batch_ready) return MPC_ERR_NOT_READY;
-  const uint32_t per_launch = pcg_team_max_grid() / 2;
   out4[0] = h->n_bins + h->n_bins_small;
-  out4[1] = h->n_teams;
+  out4[1] = h->n_teams + h->n_quads;
   out4[2] = h->n_big;
-  out4[3] = (h->cfg.flags & FLAG_PCG_STREAM) || per_launch == 0
-                ? 0u
-                : (h->n_teams + per_launch - 1) / per_launch;
+  out4[3] = h->cfg.flags & (FLAG_PCG_STREAM | FLAG_DIRECT_SOLVE) ? 0u : h->team_launches;
   return MPC_OK;
 }
 
@@ -816,9 +822,9 @@ int mpc_batch_autotune(mpc_handle *h, uint32_t iters, uint32_t reps, mpc_pcg_pla
   for (uint32_t n : h->horizons) {
     has_small |= n <= pcg_res_max_knots(256);
     has_bins |= n <= pcg_res_max_knots(512);
-    has_team |= n > pcg_res_max_knots(512) && n <= pcg_team_max_knots();
+    has_team |= n > pcg_res_max_knots(512) && n <= pcg_team_max_knots(4);
   }
-  has_team = has_team && pcg_team_max_grid() >= 16;
+  has_team = has_team && pcg_team_per_launch(4) > 0;
   std::vector<PcgPlan> cands;
   for (uint32_t nt : {512u, 256u}) {
     if (nt == 256 && !has_small) continue;
@@ -918,7 +924,11 @@ static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_st
     return e;
   }
   uint32_t taken = 0;
-  e = launch_pcg_team(b, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->plan.mixed ? h->bin_items : nullptr,
+  e = launch_pcg_team(b, 4, h->team_list + h->n_teams, h->n_quads, h->mbox, h->n_mbox, nullptr, nullptr, 0, nullptr,
+                      h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, h->pcg_iters, h->pcg_flag,
+                      h->team_err, stair, st);
+  if (e == hipSuccess)
+    e = launch_pcg_team(b, 2, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->plan.mixed ? h->bin_items : nullptr,
                       h->bin_off, h->n_bins, &taken, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status,
                       h->pcg_iters, h->pcg_flag, h->team_err, stair, st);
   if (e == hipSuccess && mid) e = hipEventRecord(mid, st);

@@ -242,13 +242,14 @@ int mpc_batch_set_timing(mpc_handle *h, int enable);
  * Not used while timing is enabled.  No counterpart in the reference (its
  * SQP loop decides on the host between launches, toplevel.cuh:254-446). */
 int mpc_batch_set_graph(mpc_handle *h, int enable);
-/* PCG packing of the batch layout: out4 = {resident-kernel bins, two-
- * workgroup teams, instances on the streaming kernel, cooperative team
- * launches per PCG phase}. */
+/* PCG packing of the batch layout: out4 = {resident-kernel bins, teams
+ * (two workgroups for N <= 256, four for N <= 512), instances on the
+ * streaming kernel, cooperative team launches per PCG phase}. */
 int mpc_batch_pcg_layout(const mpc_handle *h, uint32_t *out4);
 /* The PCG launch that solves each instance (n_inst values) within one PCG
- * phase: 0 .. L-1 the cooperative team launches (the last one also solves
- * the first resident bins in its spare workgroups), L the streaming launch
+ * phase: 0 .. L-1 the cooperative team launches (four-workgroup teams
+ * first; the last two-workgroup launch also solves the first resident bins
+ * in its spare workgroups), L the streaming launch
  * when there are instances beyond the teams' reach, then the launches of the
  * remaining resident bins (512-lane bins, then 256-lane bins); L = out4[3]
  * of mpc_batch_pcg_layout.  All 0 with
@@ -261,8 +262,9 @@ int mpc_batch_pcg_launch_map(const mpc_handle *h, uint32_t *launch_of_inst);
  * bins of the resident kernel (workgroups of 512 lanes = 128 knots, one per
  * CU; with small_nt = 256, instances with N <= 64 go to bins of 256 lanes =
  * 64 knots, two per CU), at most max_items instances per bin; 128 < N <= 256
- * run on two-workgroup teams (teams = 1) or on the streaming kernel; with
- * mixed = 1 the last team launch also solves the first bins in its spare
+ * run on two-workgroup and 256 < N <= 512 on four-workgroup teams
+ * (teams = 1) or on the streaming kernel; with mixed = 1 the last
+ * two-workgroup team launch also solves the first bins in its spare
  * workgroups.  Every plan gives the same arithmetic per instance (bins and
  * teams: bitwise), only the packing changes.  Default {512, 8, 1, 1}.  No
  * counterpart in the reference (one instance per solve). */

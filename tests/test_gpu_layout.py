@@ -1,8 +1,9 @@
 """GPU: the PCG launch plan of a batch.  Long horizons run on cooperative
-two-workgroup team launches; the last team launch also solves the first
-(longest) resident bins in its spare workgroups; horizons beyond a team go
-to the streaming kernel; the remaining bins to the resident kernel.  Every
-instance must be solved exactly once, whichever launch holds it."""
+team launches (four-workgroup teams for 256 < N <= 512 first, then
+two-workgroup teams for 128 < N <= 256); the last two-workgroup launch also
+solves the first (longest) resident bins in its spare workgroups; the
+remaining bins run on the resident kernel.  Every instance must be solved
+exactly once, whichever launch holds it."""
 import numpy as np
 import pytest
 
@@ -13,8 +14,9 @@ pytestmark = pytest.mark.gpu
 def test_launch_map_and_every_instance_solved(pkg, orc):
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    # 140 teams (two launches at 128 teams per launch), bins of every
-    # packing (one 128, two 64s, several short instances), two streamed
+    # 140 two-workgroup teams (two launches at 128 teams per launch), two
+    # four-workgroup teams, bins of every packing (one 128, two 64s,
+    # several short instances)
     horizons = [200] * 140 + [64] * 20 + [128] * 10 + [300] * 2 + [3] * 5
     rng = np.random.default_rng(7)
     horizons = [int(h) for h in rng.permutation(horizons)]
@@ -28,15 +30,15 @@ def test_launch_map_and_every_instance_solved(pkg, orc):
         m = s.batch_pcg_launch_map()
         h = np.asarray(horizons)
         print("layout", lay, "launch counts", np.bincount(m))
-        team, big, res = (h > 128) & (h <= 256), h > 256, h <= 128
-        assert lay["teams"] == team.sum() and lay["streamed"] == big.sum() and L >= 1
-        assert (m[team] < L).all()
+        team, quad, res = (h > 128) & (h <= 256), (h > 256) & (h <= 512), h <= 128
+        assert lay["teams"] == team.sum() + quad.sum() and lay["streamed"] == 0 and L >= 2
+        # four-workgroup teams first, then the two-workgroup ones
+        assert (m[quad] == 0).all() and (m[team] >= 1).all() and (m[team] < L).all()
         # teams fill the launches in instance order, at most one launch's share each
-        per = np.bincount(m[team], minlength=L)
+        per = np.bincount(m[team] - 1, minlength=L - 1)
         assert per[:-1].min() == per.max() and per[-1] <= per[0]
         assert np.all(np.diff(m[team].astype(np.int64)) >= 0)
-        assert (m[big] == L).all()
-        rest = L + 1 if big.any() else L
+        rest = L
         assert np.isin(m[res], [L - 1, rest]).all()
         taken, left = h[res & (m == L - 1)], h[res & (m == rest)]
         assert taken.size > 0

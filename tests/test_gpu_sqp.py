@@ -46,7 +46,8 @@ def _batch(orc, pkg, horizons, seed0=1000):
     return xus, ees
 
 
-@pytest.mark.parametrize("horizons", [[32, 64, 17, 128, 2, 45], [256, 129, 200, 64, 256, 3]])
+@pytest.mark.parametrize("horizons", [[32, 64, 17, 128, 2, 45], [256, 129, 200, 64, 256, 3],
+                                      [512, 300, 257, 64, 200, 400]])
 def test_batch_matches_plain_and_is_deterministic(solver, orc, pkg, horizons):
     xus, ees = _batch(orc, pkg, horizons)
     n = len(horizons)

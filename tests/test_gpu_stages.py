@@ -161,9 +161,12 @@ def test_pcg_converges_to_direct_solve(solver, orc, N):
     assert err < 1e-3
 
 
-# The resident (on-chip) and streaming PCG kernels take the same iterations
-# and agree to the fp32 floor on several seeds.
-@pytest.mark.parametrize("N", [16, 64, 128])
+# The on-chip PCG kernels (resident for N <= 128, teams of two workgroups
+# for N <= 256 and of four for N <= 512) and the streaming kernel follow the
+# oracle's iterations (exactly on one workgroup, within 2 % across the
+# teams' split reductions) and agree with it to the fp32 floor on several
+# seeds.
+@pytest.mark.parametrize("N", [16, 64, 128, 200, 400, 512])
 def test_pcg_kernels_agree(solver, orc, pkg, N):
     for seed in range(3):
         S32, P32, gam32 = _natural_schur(orc, 500 + 11 * seed + N, N)
@@ -177,7 +180,7 @@ def test_pcg_kernels_agree(solver, orc, pkg, N):
             finally:
                 solver.set_config(flags=0)
         for lam_g, it_g, fl_g in out:
-            assert fl_g == 0 and it_g == it_o
+            assert fl_g == 0 and (it_g == it_o if N <= 128 else abs(it_g - it_o) <= max(2, it_o // 50))
             assert helpers.rel_err(lam_g, lam_o) < max(2e-4, 4 * floor)
 
 

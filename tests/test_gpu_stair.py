@@ -1,8 +1,8 @@
 """GPU PCG with the stair preconditioner (MPC_FLAG_PCG_STAIR) against the
 oracle's stair PCG (orc_pcg_ex, pinned in test_oracle_stair.py): the
-resident kernel (N <= 128) and the streaming kernel (the team range and
-beyond), the plain SQP in stair mode, and the batched SQP in stair mode
-against the plain one.
+resident kernel (N <= 128) and the team kernels (two workgroups up to
+N = 256, four up to 512), the plain SQP in stair mode, and the batched SQP
+in stair mode against the plain one.
 
 Tolerances as for block Jacobi (test_gpu_stages.py): the same exit rule on
 both sides, iteration counts within 2 %, solutions within
