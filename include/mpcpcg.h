@@ -171,7 +171,7 @@ int mpc_form_schur(mpc_handle *h, uint32_t N, float *d_G, const float *d_C, cons
 /* PCG on S lambda = gamma, warm-started from lambda.  Exit when
  * |r^T Pinv r| < exit_tol (also tested before the first iteration); writes
  * the iteration count and a max-iter flag (1 = max_iter reached, 2 = a
- * two-workgroup team's exchange timed out and the solve was abandoned).
+ * PCG team's exchange timed out and the solve was abandoned).
  * GPU-PCG pcg<T,14,N> (call site toplevel.cuh:143-157, :292-294). */
 int mpc_pcg_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_Pinv,
                   const float *d_gamma, float *d_lambda, uint32_t max_iter, float exit_tol,
