@@ -18,7 +18,7 @@ def host(t):
     return t.detach().cpu().numpy().astype(np.float64)
 
 
-@pytest.mark.parametrize("N", [8, 32])
+@pytest.mark.parametrize("N", [8, 32, 200, 300, 512])
 def test_plain_sqp_vs_oracle(solver, orc, N):
     x0, ee, xu = helpers.make_instance(orc, 100 + N, N)
     cfg = orc.make_cfg(sqp_max_iter=3)
