@@ -110,21 +110,23 @@ __global__ void __launch_bounds__(NT) k_cl_step(ClStep p, const float *__restric
       s[i] = L.sn[i];
       c[i] = L.cs[i];
     }
-    if (t < NQ) {
-      float col[NQ];
-      mass_column((int)t, s, c, col);
-#pragma unroll
-      for (int r = 0; r < NQ; ++r) L.M[r + 7 * t] = col[r];
-    } else if (t == NQ) {
-      float qd[NQ], z[NQ], tau[NQ];
+    // lanes 0..6: column t of M = RNEA(q, 0, e_t); lane 7: bias RNEA(q, qd, 0),
+    // one code path for all eight lanes
+    if (t <= NQ) {
+      float qd[NQ], qdd[NQ], tau[NQ];
 #pragma unroll
       for (int i = 0; i < NQ; ++i) {
-        qd[i] = L.x[NQ + i];
-        z[i] = 0.f;
+        qd[i] = t == NQ ? L.x[NQ + i] : 0.f;
+        qdd[i] = i == (int)t ? 1.f : 0.f;
       }
-      rnea<false>(s, c, qd, z, tau, nullptr, nullptr, nullptr, nullptr, nullptr);
+      rnea<false>(s, c, qd, qdd, tau, nullptr, nullptr, nullptr, nullptr, nullptr);
+      if (t < NQ) {
 #pragma unroll
-      for (int i = 0; i < NQ; ++i) L.bias[i] = tau[i];
+        for (int r = 0; r < NQ; ++r) L.M[r + 7 * t] = tau[r];
+      } else {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) L.bias[i] = tau[i];
+      }
     }
     __syncthreads();
     if (t == 0) {

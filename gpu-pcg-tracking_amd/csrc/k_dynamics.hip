@@ -66,21 +66,24 @@ __device__ __forceinline__ void group_dyn_grad(int lane, const float *q, const f
     s[i] = D.sn[i];
     c[i] = D.cs[i];
   }
-  if (lane < NQ) {
-    float col[NQ];
-    mass_column(lane, s, c, col);
-#pragma unroll
-    for (int r = 0; r < NQ; ++r) D.M[r + 7 * lane] = col[r];
-  } else if (lane == NQ) {
-    float qdv[NQ], z[NQ], tau[NQ];
+  // one RNEA on lanes 0..7 with lane-dependent inputs (no divergent paths):
+  // lane j < 7: (qd, qdd) = (0, e_j) gives column j of M (the velocity terms
+  // are exact zeros); lane 7: (qd, 0) gives the bias forces c(q, qd)
+  if (lane <= NQ) {
+    float qdv[NQ], qddv[NQ], tau[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
-      qdv[i] = qd[i];
-      z[i] = 0.f;
+      qdv[i] = lane == NQ ? qd[i] : 0.f;
+      qddv[i] = i == lane ? 1.f : 0.f;
     }
-    rnea<false>(s, c, qdv, z, tau, nullptr, nullptr, nullptr, nullptr, nullptr);
+    rnea<false>(s, c, qdv, qddv, tau, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (lane < NQ) {
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) D.bias[i] = tau[i];
+      for (int r = 0; r < NQ; ++r) D.M[r + 7 * lane] = tau[r];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) D.bias[i] = tau[i];
+    }
   }
   __syncthreads();
   if (lane == 0) cholesky7(D.M, D.Lc);
