@@ -157,39 +157,12 @@ __device__ __forceinline__ void inertia_mv(int i, const float *m, float *o) {
   o[5] = mass * m[5] - c[2];
 }
 
-// Column j of the joint-space inertia matrix: tau of RNEA(q, 0, e_j) with
-// gravity 0 (v = 0 so only the unit-acceleration recursion remains).
-__device__ __forceinline__ void mass_column(int j, const float *s, const float *c, float *col) {
-  float a[NQ][6];
-  float ap[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-  for (int i = 0; i < NQ; ++i) {
-    X_mv(i, s[i], c[i], ap, a[i]);
-    if (i == j) a[i][2] += 1.f;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) ap[k] = a[i][k];
-  }
-  float f[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-  for (int i = NQ - 1; i >= 0; --i) {
-    float fi[6];
-    inertia_mv(i, a[i], fi);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) f[k] += fi[k];
-    col[i] = f[2];
-    if (i > 0) {
-      float t[6];
-      XT_fv(i, s[i], c[i], f, t);
-#pragma unroll
-      for (int k = 0; k < 6; ++k) f[k] = t[k];
-    }
-  }
-}
-
 // Rows j..6 of column j of M(q), the lower triangle that a Cholesky factor
-// reads: the recursion of mass_column started at joint j (the accelerations
-// of the joints before j are exactly zero there, and the backward pass below
-// joint j only carries rows above the diagonal).
+// reads, by the unit-acceleration recursion (RNEA at qd = 0, qdd = e_j,
+// gravity 0): link accelerations forward from joint j (those before j are
+// exactly zero), forces I a backward, row i = the joint-i component of the
+// accumulated force; the backward pass stops at joint j (rows above the
+// diagonal are not needed).
 __device__ __forceinline__ void mass_column_lower(int j, const float *s, const float *c, float *col) {
   float a[NQ][6];
 #pragma unroll
