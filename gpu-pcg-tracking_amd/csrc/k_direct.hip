@@ -163,6 +163,146 @@ __device__ __forceinline__ void factor(FacLds &L, const float *__restrict__ S, f
   }
 }
 
+// ---- factorisation with one wave per instance --------------------------------
+// Lane 16 q + j (q = 0..3) holds, for column j of a block, the rows
+// r = 4 t + q (t = 0..3, r < 14): the products and the Gauss-Jordan updates
+// of a column are split over four lanes, each element with the operation
+// sequence of factor() (so E_b and D_b^-1 are bitwise the same), and the
+// 1024-instance batch runs four waves per CU instead of one.  Blocks that
+// every lane reads in full (B_b, E_b, D_{b-1}^-1) go through LDS.
+struct FacWaveLds {
+  float B[NXX];     // B_b (column-major)
+  float E[NXX];     // E_b
+  float Dinv[NXX];  // D_{b-1}^-1
+};
+
+__device__ __forceinline__ float lane_value(float v, uint32_t src) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute((int)(src << 2), __builtin_bit_cast(int, v)));
+}
+
+template <int P>
+__device__ __forceinline__ void gj_wave_step(uint32_t q, uint32_t j, float (&a)[4], float (&x)[4]) {
+  constexpr uint32_t QP = P & 3, TP = P >> 2;  // owner quarter and slot of row P
+  float pc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) pc[t] = bcast16<P>(a[t]);  // column P, the lane's rows
+  const float piv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a[TP]), 16 * QP + P));
+  const float inv = 1.f / piv;
+  const float ap = lane_value(a[TP], 16u * QP + j) * inv, xp = lane_value(x[TP], 16u * QP + j) * inv;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const uint32_t r = 4u * t + q;
+    if (r == (uint32_t)P) {
+      a[t] = ap;
+      x[t] = xp;
+    } else {
+      a[t] -= pc[t] * ap;
+      x[t] -= pc[t] * xp;
+    }
+  }
+  pin(a);
+  pin(x);
+}
+
+template <int P = 0>
+__device__ __forceinline__ void gj_wave_steps(uint32_t q, uint32_t j, float (&a)[4], float (&x)[4]) {
+  if constexpr (P < NX) {
+    gj_wave_step<P>(q, j, a, x);
+    gj_wave_steps<P + 1>(q, j, a, x);
+  }
+}
+
+__device__ __forceinline__ void factor_wave(FacWaveLds &L, const float *__restrict__ S, float *__restrict__ work,
+                                           const Pick &p, uint32_t lane) {
+  const uint32_t q = lane >> 4, jl = lane & 15u, j = jl < (uint32_t)NX ? jl : NX - 1;
+  const bool col_ok = jl < (uint32_t)NX;
+  for (uint32_t i = lane; i < (uint32_t)NXX; i += 64) L.Dinv[i] = 0.f;
+  float a_cur[4], b_cur[4], a_nxt[4], b_nxt[4];
+  auto load = [&](float (&a)[4], float (&bb)[4], uint32_t k) {
+    const float *Sr = S + (size_t)S_STRIDE * (p.K0 + k);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const uint32_t r = 4u * t + q, i = lane + 64u * t;
+      a[t] = r < (uint32_t)NX ? -Sr[NXX + r + NX * j] : 0.f;
+      bb[t] = (k > 0 && i < (uint32_t)NXX) ? -Sr[i] : 0.f;
+    }
+  };
+  load(a_cur, b_cur, 0);
+  for (uint32_t k = 0; k < p.N; ++k) {
+    load(a_nxt, b_nxt, k + 1 < p.N ? k + 1 : p.N - 1);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const uint32_t i = lane + 64u * t;
+      if (i < (uint32_t)NXX) L.B[i] = b_cur[t];
+    }
+    wave_sync();
+    // E_b(r, j) = sum_m B(r, m) D_{b-1}^-1(m, j), m in order
+    float dv[NX];
+#pragma unroll
+    for (int m = 0; m < NX; ++m) dv[m] = L.Dinv[m + NX * j];
+    float e[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) e[t] = 0.f;
+#pragma unroll
+    for (int m = 0; m < NX; ++m) {
+      const float d = dv[m];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint32_t r = 4u * t + q;
+        e[t] += L.B[(r < (uint32_t)NX ? r : 0u) + NX * m] * d;
+      }
+    }
+    if (col_ok) {
+      float *w = slot_E(work, p.K0 + k) + NX * j;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint32_t r = 4u * t + q;
+        if (r < (uint32_t)NX) {
+          L.E[r + NX * j] = e[t];
+          if (p.live) w[r] = e[t];
+        }
+      }
+    }
+    wave_sync();
+    // D_b(r, j) = A_b(r, j) - sum_m E(r, m) B(j, m), m in order
+    float a[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) a[t] = a_cur[t];
+#pragma unroll
+    for (int m = 0; m < NX; ++m) {
+      const float bjm = L.B[j + NX * m];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint32_t r = 4u * t + q;
+        a[t] -= L.E[(r < (uint32_t)NX ? r : 0u) + NX * m] * bjm;
+      }
+    }
+    pin(a);
+    float x[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) x[t] = 4u * t + q == j ? 1.f : 0.f;
+    gj_wave_steps(q, jl, a, x);
+    wave_sync();  // every read of D_{b-1}^-1, B and E is done
+    if (col_ok) {
+      float *w = slot_Dinv(work, p.K0 + k) + NX * j;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint32_t r = 4u * t + q;
+        if (r < (uint32_t)NX) {
+          L.Dinv[r + NX * j] = x[t];
+          if (p.live) w[r] = x[t];
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      a_cur[t] = a_nxt[t];
+      b_cur[t] = b_nxt[t];
+    }
+  }
+  wave_sync();
+}
+
 // forward and backward sweeps of one instance on the right-hand side
 // r = -gamma (gamma != nullptr) or the workspace's z slot; lambda = x, or
 // lambda += x with accumulate.  No barriers: the group's lanes share its
@@ -259,6 +399,34 @@ __global__ void __launch_bounds__(WG) k_direct_factor_solve(Batch b, const uint3
   }
 }
 
+// one wave per instance: the factorisation on all 64 lanes, then the
+// substitution sweeps on the first 16-lane group (the others follow them
+// without writing)
+__global__ void __launch_bounds__(64) k_direct_factor_wave(Batch b, const uint32_t *__restrict__ order,
+                                                          uint32_t n_list, const float *__restrict__ S,
+                                                          const float *__restrict__ gamma,
+                                                          const uint32_t *__restrict__ status,
+                                                          float *__restrict__ work, float *__restrict__ lambda,
+                                                          uint32_t *__restrict__ iters, uint32_t *__restrict__ flag) {
+  __shared__ FacWaveLds lds;
+  const uint32_t lane = threadIdx.x;
+  Pick p;
+  p.inst = order ? order[blockIdx.x] : blockIdx.x;
+  p.K0 = inst_first(b, p.inst);
+  p.N = inst_len(b, p.inst);
+  p.Nmax = p.N;
+  p.live = !(status && !(status[p.inst] & ST_ACTIVE));
+  factor_wave(lds, S, work, p, lane);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // the factor rows written above are read below
+  Pick ps = p;
+  ps.live = p.live && lane < 16u;
+  substitute(gamma, work, lambda, ps, false, (int)(lane & 15u));
+  if (p.live && lane == 0) {
+    if (iters) iters[p.inst] = 0;
+    if (flag) flag[p.inst] = 0;
+  }
+}
+
 __global__ void __launch_bounds__(WG) k_direct_refine(Batch b, const uint32_t *__restrict__ order, uint32_t n_list,
                                                       const uint32_t *__restrict__ status, float *__restrict__ work,
                                                       float *__restrict__ lambda) {
@@ -298,8 +466,12 @@ hipError_t launch_direct_solve(const Batch &b, const uint32_t *order, uint32_t n
                                uint32_t refine, bool do_factor, uint32_t *iters, uint32_t *flag, hipStream_t st) {
   if (n_list == 0) return hipSuccess;
   const dim3 grid((n_list + GROUPS_PER_WG - 1) / GROUPS_PER_WG);
-  hipLaunchKernelGGL(k_direct_factor_solve, grid, dim3(WG), 0, st, b, order, n_list, S, gamma, status, work, lambda,
-                     do_factor ? 1 : 0, iters, flag);
+  if (do_factor)
+    hipLaunchKernelGGL(k_direct_factor_wave, dim3(n_list), dim3(64), 0, st, b, order, n_list, S, gamma, status, work,
+                       lambda, iters, flag);
+  else
+    hipLaunchKernelGGL(k_direct_factor_solve, grid, dim3(WG), 0, st, b, order, n_list, S, gamma, status, work, lambda,
+                       0, iters, flag);
   hipError_t e = hipGetLastError();
   for (uint32_t s = 0; s < refine && e == hipSuccess; ++s) {
     hipLaunchKernelGGL(k_schur_residual, dim3((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG), 0, st,
