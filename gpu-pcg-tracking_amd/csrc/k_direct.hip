@@ -170,10 +170,11 @@ __device__ __forceinline__ void factor(FacLds &L, const float *__restrict__ S, f
 // sequence of factor() (so E_b and D_b^-1 are bitwise the same), and the
 // 1024-instance batch runs four waves per CU instead of one.  Blocks that
 // every lane reads in full (B_b, E_b, D_{b-1}^-1) go through LDS.
+constexpr int RS = 16;  // LDS row stride of a block (16-byte aligned rows)
 struct FacWaveLds {
-  float B[NXX];     // B_b (column-major)
-  float E[NXX];     // E_b
-  float Dinv[NXX];  // D_{b-1}^-1
+  float B[NX * RS];     // B_b by rows: B(r, c) at c + RS r
+  float E[NX * RS];     // E_b by rows
+  float Dinv[NX * RS];  // D_{b-1}^-1 by columns: (m, j) at m + RS j
 };
 
 __device__ __forceinline__ float lane_value(float v, uint32_t src) {
@@ -216,7 +217,7 @@ __device__ __forceinline__ void factor_wave(FacWaveLds &L, const float *__restri
                                            const Pick &p, uint32_t lane) {
   const uint32_t q = lane >> 4, jl = lane & 15u, j = jl < (uint32_t)NX ? jl : NX - 1;
   const bool col_ok = jl < (uint32_t)NX;
-  for (uint32_t i = lane; i < (uint32_t)NXX; i += 64) L.Dinv[i] = 0.f;
+  for (uint32_t i = lane; i < (uint32_t)(NX * RS); i += 64) L.Dinv[i] = 0.f;
   float a_cur[4], b_cur[4], a_nxt[4], b_nxt[4];
   auto load = [&](float (&a)[4], float (&bb)[4], uint32_t k) {
     const float *Sr = S + (size_t)S_STRIDE * (p.K0 + k);
@@ -232,14 +233,14 @@ __device__ __forceinline__ void factor_wave(FacWaveLds &L, const float *__restri
     load(a_nxt, b_nxt, k + 1 < p.N ? k + 1 : p.N - 1);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const uint32_t i = lane + 64u * t;
-      if (i < (uint32_t)NXX) L.B[i] = b_cur[t];
+      const uint32_t i = lane + 64u * t;  // B(i % 14, i / 14)
+      if (i < (uint32_t)NXX) L.B[i / NX + RS * (i % NX)] = b_cur[t];
     }
     wave_sync();
     // E_b(r, j) = sum_m B(r, m) D_{b-1}^-1(m, j), m in order
     float dv[NX];
 #pragma unroll
-    for (int m = 0; m < NX; ++m) dv[m] = L.Dinv[m + NX * j];
+    for (int m = 0; m < NX; ++m) dv[m] = L.Dinv[m + RS * j];
     float e[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) e[t] = 0.f;
@@ -249,7 +250,7 @@ __device__ __forceinline__ void factor_wave(FacWaveLds &L, const float *__restri
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const uint32_t r = 4u * t + q;
-        e[t] += L.B[(r < (uint32_t)NX ? r : 0u) + NX * m] * d;
+        e[t] += L.B[m + RS * (r < (uint32_t)NX ? r : 0u)] * d;
       }
     }
     if (col_ok) {
@@ -258,7 +259,7 @@ __device__ __forceinline__ void factor_wave(FacWaveLds &L, const float *__restri
       for (int t = 0; t < 4; ++t) {
         const uint32_t r = 4u * t + q;
         if (r < (uint32_t)NX) {
-          L.E[r + NX * j] = e[t];
+          L.E[j + RS * r] = e[t];
           if (p.live) w[r] = e[t];
         }
       }
@@ -270,11 +271,11 @@ __device__ __forceinline__ void factor_wave(FacWaveLds &L, const float *__restri
     for (int t = 0; t < 4; ++t) a[t] = a_cur[t];
 #pragma unroll
     for (int m = 0; m < NX; ++m) {
-      const float bjm = L.B[j + NX * m];
+      const float bjm = L.B[m + RS * j];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const uint32_t r = 4u * t + q;
-        a[t] -= L.E[(r < (uint32_t)NX ? r : 0u) + NX * m] * bjm;
+        a[t] -= L.E[m + RS * (r < (uint32_t)NX ? r : 0u)] * bjm;
       }
     }
     pin(a);
@@ -289,7 +290,7 @@ __device__ __forceinline__ void factor_wave(FacWaveLds &L, const float *__restri
       for (int t = 0; t < 4; ++t) {
         const uint32_t r = 4u * t + q;
         if (r < (uint32_t)NX) {
-          L.Dinv[r + NX * j] = x[t];
+          L.Dinv[r + RS * j] = x[t];
           if (p.live) w[r] = x[t];
         }
       }
@@ -323,61 +324,62 @@ __device__ __forceinline__ void substitute(const float *__restrict__ gamma, floa
       dr[m] = D[m];
     }
   };
-  // two buffers, knot k in A when k is even: each knot's rows are loaded two
-  // knots ahead and only waited for where they are used
-  float erA[NX], drA[NX], erB[NX], drB[NX];
-  float rA = rhs(0), rB = 0.f, y = 0.f;
-  load_rows(erA, drA, 0);
-  if (p.N > 1) {
-    load_rows(erB, drB, 1);
-    rB = rhs(1);
-  }
-  auto step = [&](const float (&er)[NX], const float (&dr)[NX], float r, uint32_t k) {
-    const float yk = r - bdot(er, y);
-    const float zk = bdot(dr, yk);
+  // a ring of PF buffers, knot k in buffer k % PF: each knot's rows are
+  // loaded PF knots ahead and only waited for where they are used
+  constexpr int PF = 4;
+  float er[PF][NX], dr[PF][NX], rv[PF];
+  float y = 0.f;
+#pragma unroll
+  for (int d = 0; d < PF; ++d)
+    if ((uint32_t)d < p.N) {
+      load_rows(er[d], dr[d], (uint32_t)d);
+      rv[d] = rhs((uint32_t)d);
+    }
+  auto step = [&](const float (&e_)[NX], const float (&d_)[NX], float r, uint32_t k) {
+    const float yk = r - bdot(e_, y);
+    const float zk = bdot(d_, yk);
     if (wr) slot_z(work, p.K0 + k)[lane] = zk;
     y = yk;
   };
-  for (uint32_t k = 0; k < p.N; k += 2) {
-    step(erA, drA, rA, k);
-    if (k + 2 < p.N) {
-      load_rows(erA, drA, k + 2);
-      rA = rhs(k + 2);
-    }
-    if (k + 1 >= p.N) break;
-    step(erB, drB, rB, k + 1);
-    if (k + 3 < p.N) {
-      load_rows(erB, drB, k + 3);
-      rB = rhs(k + 3);
+  for (uint32_t k = 0; k < p.N; k += PF) {
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+      if (k + d >= p.N) break;
+      step(er[d], dr[d], rv[d], k + d);
+      if (k + d + PF < p.N) {
+        load_rows(er[d], dr[d], k + d + PF);
+        rv[d] = rhs(k + d + PF);
+      }
     }
   }
-  // backward: column j of E_{k+1} (contiguous), double-buffered the same way
+  // backward: column j of E_{k+1} (contiguous), through a ring the same way
   const float *zs = slot_z(work, p.K0);
-  float ecA[NX], ecB[NX];
-  float zA = zs[(size_t)S_STRIDE * (p.N - 1) + j], zB = 0.f, x = 0.f;
-  auto load_col = [&](float (&ec)[NX], float &z, int k) {  // E_{k+1} column, z_k
+  float ec[PF][NX], zv[PF];
+  float x = 0.f;
+  auto load_col = [&](float (&c_)[NX], float &z, int k) {  // E_{k+1} column, z_k
     const float *E = slot_E(work, p.K0 + (uint32_t)k + 1) + NX * j;
 #pragma unroll
-    for (int m = 0; m < NX; ++m) ec[m] = (uint32_t)k + 1 < p.N ? E[m] : 0.f;
+    for (int m = 0; m < NX; ++m) c_[m] = (uint32_t)k + 1 < p.N ? E[m] : 0.f;
     z = zs[(size_t)S_STRIDE * (uint32_t)k + j];
   };
 #pragma unroll
-  for (int m = 0; m < NX; ++m) ecA[m] = 0.f;
-  if (p.N > 1) load_col(ecB, zB, (int)p.N - 2);
-  auto bstep = [&](const float (&ec)[NX], float z, int k) {
-    const float xk = z - bdot(ec, x);
+  for (int d = 0; d < PF; ++d)
+    if ((int)p.N - 1 - d >= 0) load_col(ec[d], zv[d], (int)p.N - 1 - d);
+  auto bstep = [&](const float (&c_)[NX], float z, int k) {
+    const float xk = z - bdot(c_, x);
     if (wr) {
       float *l = lambda + (size_t)NX * (p.K0 + (uint32_t)k) + lane;
       *l = accumulate ? *l + xk : xk;
     }
     x = xk;
   };
-  for (int k = (int)p.N - 1; k >= 0; k -= 2) {
-    bstep(ecA, zA, k);
-    if (k - 2 >= 0) load_col(ecA, zA, k - 2);
-    if (k - 1 < 0) break;
-    bstep(ecB, zB, k - 1);
-    if (k - 3 >= 0) load_col(ecB, zB, k - 3);
+  for (int k = (int)p.N - 1; k >= 0; k -= PF) {
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+      if (k - d < 0) break;
+      bstep(ec[d], zv[d], k - d);
+      if (k - d - PF >= 0) load_col(ec[d], zv[d], k - d - PF);
+    }
   }
 }
 
