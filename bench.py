@@ -405,7 +405,7 @@ def main():
     value = kkt_per_step / (ms_step * 1e-3)
 
     # per-phase split and roofline of the dominant kernel, k_pcg_team (the
-    # PCG of the horizons above 128 on two-workgroup teams), timed with HIP
+    # PCG of the horizons above 128 on teams of workgroups), timed with HIP
     # events on the launch stream
     solver.batch_set_timing(True)
     step()
@@ -465,8 +465,7 @@ def main():
                          f"{desc}; ~{args.cpu_seconds:.0f} s of CPU work"}
 
     # the same workload with the stair preconditioner (half the PCG iterations
-    # for one more block product each; the 128 < N <= 256 instances on the
-    # streaming kernel)
+    # for one more block product each, on the same kernels)
     solver.set_config(flags=pkg.FLAG_PCG_STAIR)
     ms_stair, _, r = timed(True)
     lg_s, st_s = r[4], r[3]

@@ -29,7 +29,7 @@ struct PcgPlan {
   uint32_t small_nt = 512;  // workgroup of the bins of N <= 64: 512 or 256 lanes
   uint32_t max_items = 8;   // instances per bin
   bool mixed = true;        // the last team launch also solves the first bins
-  bool teams = true;        // 128 < N <= 256 on two-workgroup teams (else streamed)
+  bool teams = true;        // 128 < N <= 512 on teams of two or four workgroups (else streamed)
 };
 
 struct mpc_handle {
@@ -45,11 +45,11 @@ struct mpc_handle {
   uint32_t *status_ws = nullptr, *sqp_done = nullptr, *pcg_iters = nullptr, *pcg_flag = nullptr;
   uint32_t *knot_inst = nullptr, *inst_k0 = nullptr;
   // PCG dispatch of a batch: bins of small instances (resident kernel) and
-  // the list of long instances (streaming kernel)
+  // the list of instances beyond the teams (streaming kernel)
   uint4 *bin_items = nullptr;
   uint32_t *bin_off = nullptr, *big_list = nullptr;
   uint32_t n_bins = 0, n_bins_small = 0, n_big = 0, max_big_n = 0;
-  // two-workgroup PCG teams (128 < N <= 256)
+  // PCG teams: two workgroups for 128 < N <= 256, four for 256 < N <= 512
   uint32_t *team_list = nullptr, *team_one = nullptr, *team_err = nullptr;
   uint32_t *dir_order = nullptr;  // direct solve: instances longest first
   TeamMailbox *mbox = nullptr;
