@@ -71,13 +71,29 @@ __global__ void __launch_bounds__(WG) k_invert_g(Batch b, const float *__restric
   }
 }
 
-struct SchurLds {
+// 16-byte aligned blocks: AQ and A (reused for the S(b,b) and Pinv(b,b)
+// blocks on their way out) are read as float4 rows for coalesced stores
+struct alignas(16) SchurLds {
+  float AQ[NXX];  // A Q~_k^-1
   float A[NXX];   // A_k (column-major)
   float B[98];    // B_k
-  float AQ[NXX];  // A Q~_k^-1
   float BR[98];   // B R~_k^-1
   float qk[16], q1[16], rk[8], cb[16];
 };
+static_assert(NXX % 4 == 0 && sizeof(float) * NXX % 16 == 0, "float4 rows of a 14 x 14 block");
+
+// a 14 x 14 block (element i at src[f(i)]) from the group's LDS to global
+// memory as 49 float4 over the 16 lanes; dst is 16-byte aligned whenever
+// the S / Pinv base is (rows are 588 floats apart), else plain stores
+template <typename F>
+__device__ __forceinline__ void store_block(int lane, float *dst, F &&f) {
+  if ((reinterpret_cast<uintptr_t>(dst) & 15u) == 0) {
+    float4 *d = reinterpret_cast<float4 *>(dst);
+    for (int v = lane; v < NXX / 4; v += GROUP) d[v] = make_float4(f(4 * v), f(4 * v + 1), f(4 * v + 2), f(4 * v + 3));
+  } else {
+    for (int i = lane; i < NXX; i += GROUP) dst[i] = f(i);
+  }
+}
 
 __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restrict__ G,
                                                    const float *__restrict__ C,
@@ -174,6 +190,13 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
     for (int r = 0; r < NX; ++r) th[r] += L.BR[r + NX * l] * bjl;
   }
   pin(th);
+  __syncthreads();  // every lane is done reading A
+  // S(b,b) = -theta, transposed into rows through A for coalesced stores
+  if (lane < NX) {
+#pragma unroll
+    for (int r = 0; r < NX; ++r) L.A[r + NX * lane] = -th[r];
+  }
+  __syncthreads();
   if (valid && row0) {
     // S(0,0) = -Q~_0^-1, gamma_0 = -Q~_0^-1 q_0, Pinv(0,0) = -Q~_0
     const float *Q0 = G + (size_t)G_STRIDE * row;
@@ -195,17 +218,10 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
       }
   }
   if (valid && !row0) {
-    for (int i = lane; i < NXX; i += GROUP) Srow[i] = L.AQ[i];
-    if (lane < NX) {
-#pragma unroll
-      for (int r = 0; r < NX; ++r) Srow[NXX + r + NX * lane] = -th[r];
-    }
+    store_block(lane, Srow, [&](int i) { return L.AQ[i]; });
+    store_block(lane, Srow + NXX, [&](int i) { return L.A[i]; });
     // right block of the previous row: S(b-1, b) = AQ^T
-    float *Sprev = S + (size_t)S_STRIDE * kprev + 2 * NXX;
-    for (int i = lane; i < NXX; i += GROUP) {
-      const int r = i % NX, col = i / NX;
-      Sprev[i] = L.AQ[col + NX * r];
-    }
+    store_block(lane, S + (size_t)S_STRIDE * kprev + 2 * NXX, [&](int i) { return L.AQ[i / NX + NX * (i % NX)]; });
     if (lane < NX) {
       float a1 = 0.f, a2 = 0.f, a3 = 0.f;
 #pragma unroll
@@ -232,10 +248,13 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
   // Pinv(b,b) = -theta^-1
   float x[NX];
   gj_inverse<NX>(lane, th, x);
-  if (valid && !row0 && lane < NX) {
+  __syncthreads();  // every lane is done reading S(b,b) from A
+  if (lane < NX) {
 #pragma unroll
-    for (int r = 0; r < NX; ++r) Prow[NXX + r + NX * lane] = -x[r];
+    for (int r = 0; r < NX; ++r) L.A[r + NX * lane] = -x[r];
   }
+  __syncthreads();
+  if (valid && !row0) store_block(lane, Prow + NXX, [&](int i) { return L.A[i]; });
 }
 
 struct DzLds {
