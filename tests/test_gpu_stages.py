@@ -90,6 +90,26 @@ def test_form_schur(solver, orc, N, rho):
     assert helpers.rel_err(host(dG), rGi) < 5e-4
 
 
+def test_form_schur_unaligned_outputs(solver, orc):
+    """The Schur blocks leave the kernel as float4 rows when S and Pinv are
+    16-byte aligned and as plain stores otherwise; S and Pinv one float off
+    alignment must give the same bits."""
+    N = 32
+    G, Cm, g, c = _schur_inputs(orc, 52, N)
+    outs = []
+    for off in (0, 1):
+        dG = dev(G)
+        Sb = torch.full((588 * N + 4,), 7.0, device="cuda")
+        Pb = torch.full((588 * N + 4,), 7.0, device="cuda")
+        S, P = Sb[off:off + 588 * N], Pb[off:off + 588 * N]
+        assert (S.data_ptr() % 16 == 0) == (off == 0) and (P.data_ptr() % 16 == 0) == (off == 0)
+        gam = torch.zeros(14 * N, device="cuda")
+        solver.form_schur(N, dG, dev(Cm), dev(g), dev(c), S, P, gam, 1e-3)
+        outs.append([host(t) for t in (S, P, gam, dG)])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
 def _natural_schur(orc, seed, N):
     """Schur system of a synthetic instance at its initial iterate."""
     x0, ee, xu = helpers.make_instance(orc, seed, N)
