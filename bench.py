@@ -60,8 +60,26 @@ PCG_FLOPS_PER_KNOT_ITER = 1708
 # and r_b from LDS (84 floats)
 PCG_LDS_BYTES_PER_KNOT_ITER = 4 * 84 * 14
 FP32_PEAK_TFLOPS = 157.3
-LDS_PEAK_GBS = 128 * 256 * 2.4  # 128 B/clk/CU x 256 CUs x 2.4 GHz
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batched_1024.json")
+# algorithmic HBM bytes and flops per knot of the other phases of an SQP
+# iteration (fp32):
+# linearize: reads x_k, u_k, x_{k+1}, ref_k (21 + 14 + 6 floats), writes G
+#   (245), C (294), g (21), c (14); ~2e4 flops (dynamics, its gradient, FK)
+# schur: reads A, B, Q, R, q, r, c (574 floats), writes S (301 symmetric),
+#   Pinv (105), gamma (14); 3.24e4 flops (Gauss-Jordan inverses and GEMMs)
+# dz: reads G^-1, C, g, lambda (609 floats), writes dz (21); 1.1e3 flops
+# merit: reads xu, dz, ref (21 + 21 + 6 floats) per knot for all 8 step
+#   lengths; flops not counted (priced by bytes only)
+# pcg: per solve the S row, Jacobi block, gamma, lambda in and out
+#   (588 + 196 + 14 + 28 floats) from HBM; per iteration the
+#   PCG_BYTES_PER_KNOT_ITER / PCG_FLOPS_PER_KNOT_ITER above, on chip (LDS)
+PHASE_MODEL = {
+    "linearize": (4 * (21 + 14 + 6 + 245 + 294 + 21 + 14), 2.0e4),
+    "schur": (4 * (574 + 420), 3.24e4),
+    "dz": (4 * (609 + 21), 1.1e3),
+    "merit": (4 * (21 + 21 + 6), 0.0),
+}
+PCG_HBM_BYTES_PER_KNOT = 4 * (588 + 196 + 14 + 28)
 
 
 def parse():
@@ -337,6 +355,67 @@ def cpu_baseline(orc, helpers, horizons, seconds, seed):
     return 1.0 / mean_t, desc, t_per
 
 
+def measure_bandwidth(pkg, torch):
+    """HBM and LDS bandwidth of this device from the library's own probes
+    (mpc_bw_copy: a 1 GiB float4 copy, 2 GiB moved per launch; mpc_bw_lds:
+    4096 workgroups x 256 lanes x 2048 x 64 B of 128-bit LDS reads), timed
+    with events on the launch stream, best of 5 launches after 2 warm-ups."""
+    def best(fn):
+        for _ in range(2):
+            fn()
+        ts = []
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return min(ts)
+
+    n = 1 << 28
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    hbm = 8.0 * n / best(lambda: pkg.bw_copy(src, dst)) / 1e9
+    del src, dst
+    n_wg, reps = 4096, 2048
+    out = torch.empty(256 * n_wg, dtype=torch.int32, device="cuda")
+    lds = 64.0 * 256 * reps * n_wg / best(lambda: pkg.bw_lds(out, n_wg, reps)) / 1e9
+    return {"hbm_gbs": round(hbm, 1), "lds_gbs": round(lds, 1),
+            "method": "mpc_bw_copy (1 GiB float4 copy, read + write bytes) and mpc_bw_lds (128-bit LDS reads, "
+                      "4096 x 256 lanes x 2048 x 64 B), best of 5 event-timed launches"}
+
+
+def phase_roofline(phases, knots, pcg_units, bw):
+    """Per phase of one SQP iteration: roofline time = max(B_LDS / BW_LDS,
+    B_HBM / BW_HBM, F / fp32 peak) over the measured bandwidths, and the
+    fraction roofline time / measured time."""
+    rows = {}
+
+    def row(name, us, b_hbm, b_lds, flops):
+        t = {"hbm": b_hbm / (bw["hbm_gbs"] * 1e9), "lds": b_lds / (bw["lds_gbs"] * 1e9),
+             "fp32": flops / (FP32_PEAK_TFLOPS * 1e12)}
+        bound = max(t, key=t.get)
+        roof = t[bound] * 1e6
+        rows[name] = {"us": round(us, 2), "hbm_bytes": int(b_hbm), "lds_bytes": int(b_lds), "flops": int(flops),
+                      "roofline_us": round(roof, 2), "bound": bound, "frac": round(roof / us, 4) if us > 0 else None}
+
+    for name in ("linearize", "schur"):
+        b, f = PHASE_MODEL[name]
+        row(name, phases[name], b * knots, 0.0, f * knots)
+    row("pcg", phases["pcg_team"] + phases["pcg_rest"], PCG_HBM_BYTES_PER_KNOT * knots,
+        PCG_BYTES_PER_KNOT_ITER * pcg_units, PCG_FLOPS_PER_KNOT_ITER * pcg_units)
+    for name in ("dz", "merit"):
+        b, f = PHASE_MODEL[name]
+        row(name, phases[name], b * knots, 0.0, f * knots)
+    return {"bandwidth": bw, "fp32_peak_tflops": FP32_PEAK_TFLOPS, "knots_per_sqp_iter": int(knots),
+            "pcg_knot_iterations_per_sqp_iter": int(pcg_units), "phases": rows,
+            "note": "per SQP iteration of the headline workload; bytes and flops are the algorithmic ones per "
+                    "knot (PCG: per knot and iteration, priced at the LDS rate, its S kept on chip); fp32 peak "
+                    "is the published packed rate (this build issues unpacked fp32: half of it); merit is "
+                    "priced by bytes only"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -435,6 +514,11 @@ def main():
             traffic = json.load(f).get("team_phase", {}).get("hbm_bytes_per_launch")
     phases = {name: round(v / K, 2) for name, v in zip(PHASES, ph)}
     kkt_phase_us = (ph[1] + ph[2] + ph[3] + ph[4]) / K
+    # per-phase roofline against this device's measured HBM and LDS bandwidth
+    ran = log_t >= 0
+    knots_it = float((horizons[:, None] * ran).sum()) / K
+    pcg_it = float((horizons[:, None] * np.where(ran, log_t, 0)).sum()) / K
+    proof = phase_roofline(phases, knots_it, pcg_it, measure_bandwidth(pkg, torch))
 
     # plain: one instance per launch sequence, host decisions (sqpSolve pattern)
     rng = np.random.default_rng(args.seed)
@@ -540,10 +624,12 @@ def main():
                              "the kernel keeps S and Pinv on chip, so they come from LDS and registers and "
                              "the HBM traffic (PMC, per launch) is the one-time load; the last team launch "
                              "(k_pcg_mixed) also solves the first resident bins, counted with it",
-                     "lds": {"achieved": round(lds_gbs, 1), "peak": round(LDS_PEAK_GBS, 1), "unit": "GB/s",
-                             "frac": round(lds_gbs / LDS_PEAK_GBS, 4)},
+                     "lds": {"achieved": round(lds_gbs, 1), "peak": proof["bandwidth"]["lds_gbs"], "unit": "GB/s",
+                             "frac": round(lds_gbs / proof["bandwidth"]["lds_gbs"], 4),
+                             "peak_source": "measured on this device (mpc_bw_lds)"},
                      "fp32": {"achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": round(tflops / FP32_PEAK_TFLOPS, 4)}},
+        "phase_roofline": proof,
         "cpu_baseline": cpu,
         "pcg_stair": stair,
         "direct_solver": direct,

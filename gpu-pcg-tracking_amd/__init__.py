@@ -46,6 +46,7 @@ EXPORTS = (
     "mpc_batch_set_plan", "mpc_batch_autotune", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
     "mpc_batch_qp_loss", "mpc_batch_qp_adjoint", "mpc_synth_instance", "mpc_synth_horizons",
     "mpc_inverse_dynamics", "mpc_default_closed_loop", "mpc_closed_loop", "mpc_synth_track", "mpc_direct_solve",
+    "mpc_bw_copy", "mpc_bw_lds",
 )
 
 
@@ -146,6 +147,8 @@ def lib():
             "mpc_batch_qp_loss": [vp, vp, vp, vp, vp, vp, vp],
             "mpc_batch_qp_adjoint": [vp, vp, vp, vp, vp, vp, f32, u32, f32, vp, vp, vp],
             "mpc_inverse_dynamics": [vp, u32, vp, vp, vp],
+            "mpc_bw_copy": [vp, vp, C.c_size_t, vp],
+            "mpc_bw_lds": [vp, u32, u32, vp],
             "mpc_direct_solve": [vp, u32, vp, vp, vp, u32, vp, vp],
             "mpc_closed_loop": [vp, C.POINTER(ClosedLoopConfig), u32, vp, vp, vp, vp, vp, vp, vp, vp,
                                 C.POINTER(ClosedLoopStats), vp],
@@ -233,6 +236,23 @@ def _ptr(t):
     if not (t.is_cuda and t.is_contiguous()):
         raise MpcError("expected a contiguous CUDA tensor")
     return C.c_void_p(t.data_ptr())
+
+
+def bw_copy(src, dst):
+    """dst = src (float32 CUDA tensors, numel % 4 == 0, 16-byte aligned) by
+    the HBM bandwidth probe of the library; 8 * numel bytes moved."""
+    if src.numel() != dst.numel():
+        raise MpcError("bw_copy: sizes differ")
+    _check(lib().mpc_bw_copy(_ptr(src), _ptr(dst), src.numel(), _stream()), "mpc_bw_copy")
+
+
+def bw_lds(out, n_wg: int, reps: int):
+    """The LDS bandwidth probe: n_wg workgroups of 256 lanes, 64 * 256 * reps
+    bytes of LDS reads each; out (int32 CUDA tensor, n_wg * 256) gets each
+    lane's folded sum."""
+    if out.numel() < 256 * n_wg:
+        raise MpcError("bw_lds: out too small")
+    _check(lib().mpc_bw_lds(_ptr(out), n_wg, reps, _stream()), "mpc_bw_lds")
 
 
 def _stream():

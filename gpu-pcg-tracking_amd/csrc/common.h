@@ -154,6 +154,9 @@ uint32_t pcg_res_max_items();
 // a synthetic block-tridiagonal Schur system (diag -2.05 I, off-diagonal I,
 // block-Jacobi Pinv, gamma = 1, lambda = 0) for timing launch plans
 hipError_t launch_tune_system(const Batch &b, float *S, float *Pinv, float *gamma, float *lambda, hipStream_t st);
+// bandwidth probes (k_bandwidth.hip)
+hipError_t launch_bw_copy(const float *src, float *dst, size_t n, hipStream_t st);
+hipError_t launch_bw_lds(uint32_t *out, uint32_t n_wg, uint32_t reps, hipStream_t st);
 hipError_t launch_merit_reduce(const Batch &b, const float *knot_merits, int np, float *out,
                                hipStream_t st);
 hipError_t launch_ls_update(const Batch &b, const Params &p, const float *knot_merits,

@@ -398,6 +398,20 @@ int mpc_inverse_dynamics(mpc_handle *h, uint32_t n, const float *d_qqdqdd, float
   return MPC_OK;
 }
 
+int mpc_bw_copy(const float *d_src, float *d_dst, size_t n, mpc_stream_t stream) {
+  if (n && (!d_src || !d_dst || n % 4 != 0 || (reinterpret_cast<uintptr_t>(d_src) & 15u) ||
+            (reinterpret_cast<uintptr_t>(d_dst) & 15u)))
+    return MPC_ERR_ARG;
+  HIP_TRY(launch_bw_copy(d_src, d_dst, n, (hipStream_t)stream));
+  return MPC_OK;
+}
+
+int mpc_bw_lds(uint32_t *d_out, uint32_t n_wg, uint32_t reps, mpc_stream_t stream) {
+  if (n_wg && !d_out) return MPC_ERR_ARG;
+  HIP_TRY(launch_bw_lds(d_out, n_wg, reps, (hipStream_t)stream));
+  return MPC_OK;
+}
+
 int mpc_dynamics_gradient(mpc_handle *h, uint32_t n, const float *d_qqdu, float *d_qdd,
                           float *d_df_du, mpc_stream_t stream) {
   if (!h || (n && (!d_qqdu || !d_qdd || !d_df_du))) return MPC_ERR_ARG;

@@ -377,6 +377,22 @@ int mpc_batch_qp_adjoint(mpc_handle *h, const float *d_xu, const float *d_ee_ref
                          const float *d_dz, const float *d_a, float pcg_tol, uint32_t refine_steps,
                          float refine_tol, float *d_grad_w, float *d_grad_ref, mpc_stream_t stream);
 
+/* ---- bandwidth probes --------------------------------------------------------
+ * No counterpart in the reference: the device's own HBM and LDS bandwidth,
+ * against which bench.py prices the per-phase roofline of the SQP
+ * iteration.  Stream-ordered; time them with events on `stream`.
+ * mpc_bw_copy: d_dst = d_src, n floats with 128-bit loads and stores
+ *   (n % 4 == 0, both pointers 16-byte aligned; else MPC_ERR_ARG); moves
+ *   8 n bytes.
+ * mpc_bw_lds: n_wg workgroups of 256 lanes each fill a 16 KB LDS block with
+ *   the words 0, 1, 2, ... and every lane reads reps x 4 consecutive
+ *   128-bit words of it (word (lane + 256 k + r) mod 1024); reads
+ *   64 x 256 x reps bytes per workgroup.  d_out (n_wg x 256 uint32) gets,
+ *   per lane, (sum of the x and y parts) XOR (sum of the z and w parts) of
+ *   the words it read, mod 2^32. */
+int mpc_bw_copy(const float *d_src, float *d_dst, size_t n, mpc_stream_t stream);
+int mpc_bw_lds(uint32_t *d_out, uint32_t n_wg, uint32_t reps, mpc_stream_t stream);
+
 /* ---- synthetic workload (seeded, reproducible) ----------------------------- */
 
 /* Joint-space data of one synthetic instance: splitmix64 stream seeded with
