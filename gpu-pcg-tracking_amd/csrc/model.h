@@ -11,7 +11,9 @@
 // The spatial transform X_i (child <- parent) is E = (kRF[i] Rz(q_i))^T with
 // translation r = kPF[i]; motion vectors are [w; v], forces [n; f].  All
 // helpers are written for fully unrolled loops over joints so that the 0/+-1
-// rotation entries fold into the instruction stream.
+// rotation entries fold into the instruction stream: +-1 factors by the
+// compiler's exact rules, zero factors through pa / ps (below), since x * 0
+// is not foldable under IEEE rules (x may be infinite).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -41,10 +43,20 @@ constexpr float kLink[7][10] = {
     {1.2f, 0.0f, 0.0f, 0.024f, 0.00598f, 0.00598f, 0.005f, 0.0f, 0.0f, 0.0f},
 };
 
+// Products in which either factor may be a compile-time zero (a model
+// constant, or a literal zero of a sparse vector): such a product is an
+// exact zero for finite operands, so it is replaced by the zero that folds
+// out of the expression it sits in, -0 where it is added (-0 + y = y) and
+// +0 where it is subtracted (y - +0 = y); other products are plain.  Only
+// the sign of an exactly zero result can differ from the full product.
+__device__ __forceinline__ bool czero(float v) { return __builtin_constant_p(v) && v == 0.f; }
+__device__ __forceinline__ float pa(float x, float y) { return czero(x) || czero(y) ? -0.f : x * y; }
+__device__ __forceinline__ float ps(float x, float y) { return czero(x) || czero(y) ? 0.f : x * y; }
+
 __device__ __forceinline__ void cross3(const float *a, const float *b, float *o) {
-  float x = a[1] * b[2] - a[2] * b[1];
-  float y = a[2] * b[0] - a[0] * b[2];
-  float z = a[0] * b[1] - a[1] * b[0];
+  float x = pa(a[1], b[2]) - ps(a[2], b[1]);
+  float y = pa(a[2], b[0]) - ps(a[0], b[2]);
+  float z = pa(a[0], b[1]) - ps(a[1], b[0]);
   o[0] = x;
   o[1] = y;
   o[2] = z;
@@ -55,8 +67,8 @@ __device__ __forceinline__ void joint_rot(int i, float s, float c, float *R) {
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     const float a = kRF[i][3 * r + 0], b = kRF[i][3 * r + 1], d = kRF[i][3 * r + 2];
-    R[3 * r + 0] = a * c + b * s;
-    R[3 * r + 1] = -a * s + b * c;
+    R[3 * r + 0] = pa(a, c) + pa(b, s);
+    R[3 * r + 1] = pa(-a, s) + pa(b, c);
     R[3 * r + 2] = d;
   }
 }
@@ -69,12 +81,12 @@ __device__ __forceinline__ void X_mv(int i, float s, float c, const float *m, fl
   float t[3];
   cross3(r, m, t);
   const float u0 = m[3] - t[0], u1 = m[4] - t[1], u2 = m[5] - t[2];
-  const float w0 = R[0] * m[0] + R[3] * m[1] + R[6] * m[2];
-  const float w1 = R[1] * m[0] + R[4] * m[1] + R[7] * m[2];
-  const float w2 = R[2] * m[0] + R[5] * m[1] + R[8] * m[2];
-  o[3] = R[0] * u0 + R[3] * u1 + R[6] * u2;
-  o[4] = R[1] * u0 + R[4] * u1 + R[7] * u2;
-  o[5] = R[2] * u0 + R[5] * u1 + R[8] * u2;
+  const float w0 = pa(R[0], m[0]) + pa(R[3], m[1]) + pa(R[6], m[2]);
+  const float w1 = pa(R[1], m[0]) + pa(R[4], m[1]) + pa(R[7], m[2]);
+  const float w2 = pa(R[2], m[0]) + pa(R[5], m[1]) + pa(R[8], m[2]);
+  o[3] = pa(R[0], u0) + pa(R[3], u1) + pa(R[6], u2);
+  o[4] = pa(R[1], u0) + pa(R[4], u1) + pa(R[7], u2);
+  o[5] = pa(R[2], u0) + pa(R[5], u1) + pa(R[8], u2);
   o[0] = w0;
   o[1] = w1;
   o[2] = w2;
@@ -85,13 +97,13 @@ __device__ __forceinline__ void XT_fv(int i, float s, float c, const float *f, f
   float R[9];
   joint_rot(i, s, c, R);
   const float r[3] = {kPF[i][0], kPF[i][1], kPF[i][2]};
-  const float n0 = R[0] * f[0] + R[1] * f[1] + R[2] * f[2];
-  const float n1 = R[3] * f[0] + R[4] * f[1] + R[5] * f[2];
-  const float n2 = R[6] * f[0] + R[7] * f[1] + R[8] * f[2];
+  const float n0 = pa(R[0], f[0]) + pa(R[1], f[1]) + pa(R[2], f[2]);
+  const float n1 = pa(R[3], f[0]) + pa(R[4], f[1]) + pa(R[5], f[2]);
+  const float n2 = pa(R[6], f[0]) + pa(R[7], f[1]) + pa(R[8], f[2]);
   float ff[3];
-  ff[0] = R[0] * f[3] + R[1] * f[4] + R[2] * f[5];
-  ff[1] = R[3] * f[3] + R[4] * f[4] + R[5] * f[5];
-  ff[2] = R[6] * f[3] + R[7] * f[4] + R[8] * f[5];
+  ff[0] = pa(R[0], f[3]) + pa(R[1], f[4]) + pa(R[2], f[5]);
+  ff[1] = pa(R[3], f[3]) + pa(R[4], f[4]) + pa(R[5], f[5]);
+  ff[2] = pa(R[6], f[3]) + pa(R[7], f[4]) + pa(R[8], f[5]);
   float t[3];
   cross3(r, ff, t);
   o[0] = n0 + t[0];
@@ -149,12 +161,12 @@ __device__ __forceinline__ void inertia_mv(int i, const float *m, float *o) {
   float b[3], c[3];
   cross3(mc, m + 3, b);
   cross3(mc, m, c);
-  o[0] = Ixx * m[0] + Ixy * m[1] + Ixz * m[2] + b[0];
-  o[1] = Ixy * m[0] + Iyy * m[1] + Iyz * m[2] + b[1];
-  o[2] = Ixz * m[0] + Iyz * m[1] + Izz * m[2] + b[2];
-  o[3] = mass * m[3] - c[0];
-  o[4] = mass * m[4] - c[1];
-  o[5] = mass * m[5] - c[2];
+  o[0] = pa(Ixx, m[0]) + pa(Ixy, m[1]) + pa(Ixz, m[2]) + b[0];
+  o[1] = pa(Ixy, m[0]) + pa(Iyy, m[1]) + pa(Iyz, m[2]) + b[1];
+  o[2] = pa(Ixz, m[0]) + pa(Iyz, m[1]) + pa(Izz, m[2]) + b[2];
+  o[3] = pa(mass, m[3]) - c[0];
+  o[4] = pa(mass, m[4]) - c[1];
+  o[5] = pa(mass, m[5]) - c[2];
 }
 
 // Rows j..6 of column j of M(q), the lower triangle that a Cholesky factor
@@ -336,13 +348,13 @@ __device__ __forceinline__ void fk_frames(const float *s, const float *c, float 
     joint_rot(i, s[i], c[i], R);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-      pw[a] += Rw[3 * a + 0] * kPF[i][0] + Rw[3 * a + 1] * kPF[i][1] + Rw[3 * a + 2] * kPF[i][2];
+      pw[a] += pa(Rw[3 * a + 0], kPF[i][0]) + pa(Rw[3 * a + 1], kPF[i][1]) + pa(Rw[3 * a + 2], kPF[i][2]);
     float Rn[9];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc)
-        Rn[3 * r + cc] = Rw[3 * r + 0] * R[0 + cc] + Rw[3 * r + 1] * R[3 + cc] + Rw[3 * r + 2] * R[6 + cc];
+        Rn[3 * r + cc] = pa(Rw[3 * r + 0], R[0 + cc]) + pa(Rw[3 * r + 1], R[3 + cc]) + pa(Rw[3 * r + 2], R[6 + cc]);
 #pragma unroll
     for (int k = 0; k < 9; ++k) Rw[k] = Rn[k];
     zax[3 * i + 0] = Rw[2];
@@ -367,7 +379,7 @@ __device__ __forceinline__ void fk_pos(const float *s, const float *c, float *p)
     joint_rot(i, s[i], c[i], R);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-      pw[a] += Rw[3 * a + 0] * kPF[i][0] + Rw[3 * a + 1] * kPF[i][1] + Rw[3 * a + 2] * kPF[i][2];
+      pw[a] += pa(Rw[3 * a + 0], kPF[i][0]) + pa(Rw[3 * a + 1], kPF[i][1]) + pa(Rw[3 * a + 2], kPF[i][2]);
     if (i < NQ - 1) {
       float Rn[9];
 #pragma unroll
@@ -375,7 +387,7 @@ __device__ __forceinline__ void fk_pos(const float *s, const float *c, float *p)
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc)
           Rn[3 * r + cc] =
-              Rw[3 * r + 0] * R[0 + cc] + Rw[3 * r + 1] * R[3 + cc] + Rw[3 * r + 2] * R[6 + cc];
+              pa(Rw[3 * r + 0], R[0 + cc]) + pa(Rw[3 * r + 1], R[3 + cc]) + pa(Rw[3 * r + 2], R[6 + cc]);
 #pragma unroll
       for (int k = 0; k < 9; ++k) Rw[k] = Rn[k];
     }
