@@ -1,0 +1,64 @@
+"""CPU: the host-side arithmetic of bench.py that turns measurements into
+the reported figures: the per-phase roofline (time = max over the HBM,
+LDS and fp32 limits, fraction = that time over the measured one) and the
+threaded CPU baseline leg.  Checked against values worked out by hand."""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def bench():
+    spec = importlib.util.spec_from_file_location("bench_under_test", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_phase_roofline_bounds_and_fractions(bench):
+    bw = {"hbm_gbs": 5000.0, "lds_gbs": 100000.0, "method": "test"}
+    phases = {"linearize": 200.0, "schur": 400.0, "pcg_team": 1500.0, "pcg_rest": 500.0, "dz": 60.0,
+              "merit": 80.0, "update": 10.0}
+    knots, pcg_units = 1.0e5, 1.5e7
+    out = bench.phase_roofline(phases, knots, pcg_units, bw)
+    rows = out["phases"]
+    assert set(rows) == {"linearize", "schur", "pcg", "dz", "merit"}
+    # schur: 3976 B / knot over 5 TB/s against 3.24e4 flops / knot over 157.3 TFLOP/s
+    t_hbm = 3976 * knots / 5000e9
+    t_f = 3.24e4 * knots / 157.3e12
+    assert rows["schur"]["bound"] == ("hbm" if t_hbm > t_f else "fp32")
+    assert rows["schur"]["roofline_us"] == pytest.approx(max(t_hbm, t_f) * 1e6, abs=0.01)
+    assert rows["schur"]["frac"] == pytest.approx(max(t_hbm, t_f) * 1e6 / 400.0, abs=1e-4)  # 4 decimals
+    # pcg: both PCG phases together, 2296 B per knot-iteration at the LDS rate
+    assert rows["pcg"]["us"] == pytest.approx(2000.0)
+    t_lds = 2296 * pcg_units / 100000e9
+    t_pf = 1708 * pcg_units / 157.3e12
+    t_ph = 4 * (588 + 196 + 14 + 28) * knots / 5000e9
+    assert rows["pcg"]["roofline_us"] == pytest.approx(max(t_lds, t_pf, t_ph) * 1e6, abs=0.01)
+    assert rows["pcg"]["lds_bytes"] == int(2296 * pcg_units)
+    # merit has no flops: priced by bytes
+    assert rows["merit"]["flops"] == 0 and rows["merit"]["bound"] == "hbm"
+    for r in rows.values():
+        assert 0 < r["frac"] <= 1.5
+
+
+def test_phase_roofline_picks_the_largest_limit(bench):
+    # an LDS rate so low that every phase with LDS bytes is LDS-bound
+    bw = {"hbm_gbs": 1e9, "lds_gbs": 1.0, "method": "test"}
+    phases = dict.fromkeys(("linearize", "schur", "pcg_team", "pcg_rest", "dz", "merit", "update"), 100.0)
+    rows = bench.phase_roofline(phases, 10.0, 10.0, bw)["phases"]
+    assert rows["pcg"]["bound"] == "lds"
+    assert rows["dz"]["bound"] == "fp32"  # no LDS bytes, HBM nearly free
+
+
+def test_cpu_baseline_threads_counts_solves(bench, orc):
+    import helpers
+    horizons = np.array([8, 16] * 4)
+    one, _, t_per, per_n = bench.cpu_baseline(orc, helpers, horizons, 0.2, 5)
+    assert one > 0 and set(t_per) == {8, 16}
+    two = bench.cpu_baseline_threads(orc, per_n, t_per, horizons, 0.2, 2)
+    assert two > 0
