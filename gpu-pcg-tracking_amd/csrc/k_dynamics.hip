@@ -205,22 +205,30 @@ __global__ void __launch_bounds__(256) k_inverse_dynamics(uint32_t n, const floa
 }
 
 // --------------------------------------------------------------------------
-// KKT blocks, one group per global knot
+// KKT blocks, one group per global knot, 16 knots per 256-lane workgroup
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(WG) k_form_kkt(Batch b, Params p, const float *__restrict__ xu,
-                                                 const float *__restrict__ ee_ref,
-                                                 const float *__restrict__ xs,
-                                                 const uint32_t *__restrict__ status,
-                                                 float *__restrict__ G, float *__restrict__ C,
-                                                 float *__restrict__ g, float *__restrict__ cvec) {
-  __shared__ KktLds lds[GROUPS_PER_WG];
-  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
-  const uint32_t gk_raw = blockIdx.x * GROUPS_PER_WG + grp;
+// The steps of group_dyn_grad, with its single-lane steps (the Cholesky
+// factor of M, the RNEA base point) and the frames of the cost state run
+// one knot per lane for all 16 knots of the workgroup: one instruction
+// stream per 16 knots instead of one per 4.  Each knot's arithmetic is the
+// same function on the same inputs as in group_dyn_grad.
+constexpr int KKT_GROUPS = 16;
+constexpr int KKT_WG = GROUP * KKT_GROUPS;
+__global__ void __launch_bounds__(KKT_WG) k_form_kkt(Batch b, Params p, const float *__restrict__ xu,
+                                                     const float *__restrict__ ee_ref,
+                                                     const float *__restrict__ xs,
+                                                     const uint32_t *__restrict__ status,
+                                                     float *__restrict__ G, float *__restrict__ C,
+                                                     float *__restrict__ g, float *__restrict__ cvec) {
+  __shared__ KktLds lds[KKT_GROUPS];
+  const int tid = threadIdx.x, grp = tid / GROUP, lane = tid % GROUP, w = tid >> 6, l64 = tid & 63;
+  const uint32_t gk_raw = blockIdx.x * KKT_GROUPS + grp;
   bool valid = gk_raw < b.total_knots;
   const uint32_t gk = valid ? gk_raw : b.total_knots - 1;
   const KnotRef kr = locate(b, gk);
   if (status && !(status[kr.inst] & ST_ACTIVE)) valid = false;
   KktLds &L = lds[grp];
+  DynLds &D = L.dyn;
   const float *xi = xu + (size_t)NXU * kr.K0 - (size_t)NU * kr.inst;  // instance trajectory
   const bool dyn = kr.k + 1 < kr.N;
   const uint32_t kc = dyn ? kr.k : ((p.flags & FLAG_LAST_KNOT_OWN_STATE) ? kr.k : kr.k - 1);
@@ -234,30 +242,111 @@ __global__ void __launch_bounds__(WG) k_form_kkt(Batch b, Params p, const float 
   __syncthreads();
 
   // dynamics linearisation (every group runs it so barriers line up; the
-  // terminal knot's results are discarded)
-  group_dyn_grad(lane, L.x, L.x + NQ, L.u, L.dyn);
-
-  // tracking cost gradient / Gauss-Newton Hessian at the cost state
+  // terminal knot's results are discarded), and sin / cos of the cost state
+  const float *q = L.x, *qd = L.x + NQ;
   if (lane < NQ) {
-    float s, c;
-    sincosf(L.xc[lane], &s, &c);
-    L.csn[lane] = s;
-    L.ccs[lane] = c;
+    float sv, cv;
+    sincosf(q[lane], &sv, &cv);
+    D.sn[lane] = sv;
+    D.cs[lane] = cv;
+    sincosf(L.xc[lane], &sv, &cv);
+    L.csn[lane] = sv;
+    L.ccs[lane] = cv;
   }
   __syncthreads();
-  if (lane == 0) {
-    float s[NQ], c[NQ], R[9];
+  float s[NQ], c[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    s[i] = D.sn[i];
+    c[i] = D.cs[i];
+  }
+  // one RNEA on lanes 0..7 with lane-dependent inputs: lane j < 7: (qd, qdd)
+  // = (0, e_j) gives column j of M; lane 7: (qd, 0) the bias forces c(q, qd)
+  if (lane <= NQ) {
+    float qdv[NQ], qddv[NQ], tau[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
-      s[i] = L.csn[i];
-      c[i] = L.ccs[i];
+      qdv[i] = lane == NQ ? qd[i] : 0.f;
+      qddv[i] = i == lane ? 1.f : 0.f;
     }
-    fk_frames(s, c, L.zax, L.org, R);
-    L.ee[0] = L.org[18];
-    L.ee[1] = L.org[19];
-    L.ee[2] = L.org[20];
+    rnea<false>(s, c, qdv, qddv, tau, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (lane < NQ) {
+#pragma unroll
+      for (int r = 0; r < NQ; ++r) D.M[r + 7 * lane] = tau[r];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) D.bias[i] = tau[i];
+    }
   }
   __syncthreads();
+  // one knot per lane: wave 0 factors M, wave 1 forms the cost state's frames
+  if (w == 0 && l64 < KKT_GROUPS) cholesky7(lds[l64].dyn.M, lds[l64].dyn.Lc);
+  if (w == 1 && l64 < KKT_GROUPS) {
+    KktLds &K = lds[l64];
+    float sk[NQ], ck[NQ], R[9];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      sk[i] = K.csn[i];
+      ck[i] = K.ccs[i];
+    }
+    fk_frames(sk, ck, K.zax, K.org, R);
+    K.ee[0] = K.org[18];
+    K.ee[1] = K.org[19];
+    K.ee[2] = K.org[20];
+  }
+  __syncthreads();
+  if (lane < NQ) {
+    float Lf[49];
+#pragma unroll
+    for (int i = 0; i < 49; ++i) Lf[i] = D.Lc[i];
+    float e[NQ], x[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) e[i] = (i == lane) ? 1.f : 0.f;
+    chol_solve7(Lf, e, x);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) D.Minv[i + 7 * lane] = x[i];
+  }
+  __syncthreads();
+  if (lane < NQ) {
+    float acc = 0.f;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) acc += D.Minv[lane + 7 * m] * (L.u[m] - D.bias[m]);
+    D.qdd[lane] = acc;
+  }
+  __syncthreads();
+  // one knot per lane: the RNEA base point at (q, qd, qdd)
+  if (w == 0 && l64 < KKT_GROUPS) {
+    KktLds &K = lds[l64];
+    float sk[NQ], ck[NQ], qdv[NQ], qddv[NQ], tau[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      sk[i] = K.dyn.sn[i];
+      ck[i] = K.dyn.cs[i];
+      qdv[i] = K.x[NQ + i];
+      qddv[i] = K.dyn.qdd[i];
+    }
+    rnea<true>(sk, ck, qdv, qddv, tau, K.dyn.bv, K.dyn.ba, K.dyn.bf, K.dyn.bxv, K.dyn.bxa);
+  }
+  __syncthreads();
+  float dtau[NQ];
+  if (lane < NX) {
+    float qdv[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) qdv[i] = qd[i];
+    rnea_dir(lane, s, c, qdv, D.bv, D.bf, D.bxv, D.bxa, dtau);
+  }
+  __syncthreads();  // every lane is done reading the base point
+  if (lane < NX) {
+    // dqdd/d(dir) = -M^-1 dtau
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) {
+      float acc = 0.f;
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) acc += D.Minv[r + 7 * m] * dtau[m];
+      D.df[r + 7 * lane] = -acc;
+    }
+  }
+  // tracking cost gradient at the cost state: J^T Q_ee (ee - ref)
   if (lane < NQ) {
     const float d0 = L.ee[0] - L.org[3 * lane], d1 = L.ee[1] - L.org[3 * lane + 1],
                 d2 = L.ee[2] - L.org[3 * lane + 2];
@@ -419,8 +508,8 @@ hipError_t launch_form_kkt(const Batch &b, const Params &p, const float *xu, con
                            const float *xs, const uint32_t *status, float *G, float *C, float *g,
                            float *c, hipStream_t st) {
   if (b.total_knots == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_form_kkt, dim3((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG),
-                     0, st, b, p, xu, ee_ref, xs, status, G, C, g, c);
+  hipLaunchKernelGGL(k_form_kkt, dim3((b.total_knots + KKT_GROUPS - 1) / KKT_GROUPS), dim3(KKT_WG), 0, st, b, p,
+                     xu, ee_ref, xs, status, G, C, g, c);
   return hipGetLastError();
 }
 
