@@ -1,9 +1,11 @@
 // k_dynamics.hip -- linearisation (KKT blocks), forward kinematics,
 // dynamics gradient and merit kernels for gfx950.
 //
-// Mapping: one 16-lane group per knot, 4 groups per 64-lane workgroup (one
-// wave), so every __syncthreads() is a single-wave barrier.  Within a
-// group the lanes split the rigid-body work by direction:
+// Mapping: one 16-lane group per knot or point; k_dyn_grad runs 4 groups
+// per 64-lane workgroup (one wave, so its barriers are single-wave),
+// k_form_kkt 16 per 256-lane workgroup, with its single-lane steps one knot
+// per lane (below).  Within a group the lanes split the rigid-body work by
+// direction:
 //   lanes 0..6   column j of M(q) (unit-acceleration recursion)
 //   lane  7      bias forces c(q, qd) (RNEA at qdd = 0)
 //   lanes 0..13  directional derivatives d tau / d(q_j | qd_j)
