@@ -46,7 +46,7 @@ EXPORTS = (
     "mpc_batch_set_plan", "mpc_batch_autotune", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
     "mpc_batch_qp_loss", "mpc_batch_qp_adjoint", "mpc_synth_instance", "mpc_synth_horizons",
     "mpc_inverse_dynamics", "mpc_default_closed_loop", "mpc_closed_loop", "mpc_synth_track", "mpc_direct_solve",
-    "mpc_bw_copy", "mpc_bw_lds",
+    "mpc_bw_copy", "mpc_bw_lds", "mpc_direct_solve_static",
 )
 
 
@@ -150,6 +150,7 @@ def lib():
             "mpc_bw_copy": [vp, vp, C.c_size_t, vp],
             "mpc_bw_lds": [vp, u32, u32, vp],
             "mpc_direct_solve": [vp, u32, vp, vp, vp, u32, vp, vp],
+            "mpc_direct_solve_static": [u32, vp, vp, vp, u32, vp],
             "mpc_closed_loop": [vp, C.POINTER(ClosedLoopConfig), u32, vp, vp, vp, vp, vp, vp, vp, vp,
                                 C.POINTER(ClosedLoopStats), vp],
         }
@@ -253,6 +254,19 @@ def bw_lds(out, n_wg: int, reps: int):
     if out.numel() < 256 * n_wg:
         raise MpcError("bw_lds: out too small")
     _check(lib().mpc_bw_lds(_ptr(out), n_wg, reps, _stream()), "mpc_bw_lds")
+
+
+# largest horizon of direct_solve_static (MPC_DIRECT_STATIC_MAX_KNOTS)
+DIRECT_STATIC_MAX_KNOTS = 512
+
+
+def direct_solve_static(N, S, gamma, lam, refine=2):
+    """lambda = S^-1 gamma as Solver.direct_solve, through the entry point
+    that allocates nothing (no handle, factor workspace in a library-owned
+    device array for N <= DIRECT_STATIC_MAX_KNOTS; calls overlapping in
+    time on one GPU race on it)."""
+    _check(lib().mpc_direct_solve_static(N, _ptr(S), _ptr(gamma), _ptr(lam), int(refine), _stream()),
+           "mpc_direct_solve_static")
 
 
 def _stream():

@@ -186,6 +186,20 @@ int mpc_pcg_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_Pi
 int mpc_direct_solve(mpc_handle *h, uint32_t N, const float *d_S, const float *d_gamma, float *d_lambda,
                      uint32_t refine_steps, float *d_work, mpc_stream_t stream);
 
+/* mpc_direct_solve without a handle and without d_work: allocates nothing
+ * (no hipMalloc, no host allocation), for callers that cannot allocate
+ * device memory, such as code under HIP graph capture or tight serving
+ * loops.  No counterpart in the reference.  The factor workspace is a
+ * library-owned device array of 588 x MPC_DIRECT_STATIC_MAX_KNOTS floats
+ * (1.2 MB), one per GPU, held until the process ends; N beyond that limit
+ * (or N < 2, refine_steps > 16, a null pointer) returns MPC_ERR_ARG.  All
+ * calls on one GPU share that array, so calls that overlap in time on one
+ * GPU (other streams, concurrent graph replays) race; calls ordered on one
+ * stream do not.  Results are bitwise those of mpc_direct_solve. */
+#define MPC_DIRECT_STATIC_MAX_KNOTS 512u
+int mpc_direct_solve_static(uint32_t N, const float *d_S, const float *d_gamma, float *d_lambda,
+                            uint32_t refine_steps, mpc_stream_t stream);
+
 /* dz from G^-1, C, g, lambda.  compute_dz (schur.cuh:562-566). */
 int mpc_compute_dz(mpc_handle *h, uint32_t N, const float *d_Ginv, const float *d_C,
                    const float *d_g, const float *d_lambda, float *d_dz, mpc_stream_t stream);

@@ -64,3 +64,11 @@ def test_synth_distribution_moments(pkg):
     q0, qd0 = x0s[:, :7], x0s[:, 7:]
     assert np.all(np.abs(q0) <= 0.5) and abs(q0.mean()) < 0.02 and abs(q0.var() - 1 / 12) < 0.01
     assert abs(qd0.mean()) < 0.01 and abs(qd0.std() - 0.1) < 0.01
+
+
+def test_static_direct_solve_rejects_arguments_without_a_device(pkg):
+    lib = pkg.lib()
+    for N in (0, 1, pkg.DIRECT_STATIC_MAX_KNOTS + 1):
+        assert lib.mpc_direct_solve_static(N, 8, 8, 8, 2, None) == pkg.ERR_ARG
+    assert lib.mpc_direct_solve_static(8, None, 8, 8, 2, None) == pkg.ERR_ARG
+    assert lib.mpc_direct_solve_static(8, 8, 8, 8, 17, None) == pkg.ERR_ARG
