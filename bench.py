@@ -45,8 +45,8 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.join(ROOT, "tests"))
-sys.path.insert(0, os.path.join(ROOT, "oracle"))
+PKG_DIR = os.path.join(ROOT, "gpu-pcg-tracking_amd")
+sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the CPU baseline leg only
 
 HBM_PEAK_GBS = 8000.0
 PHASES = ("linearize", "schur", "pcg_team", "pcg_rest", "dz", "merit", "update")
@@ -56,12 +56,12 @@ PHASES = ("linearize", "schur", "pcg_team", "pcg_rest", "dz", "merit", "update")
 # dots 56, three axpys 84)
 PCG_BYTES_PER_KNOT_ITER = 4 * (105 + 196 + 105 + 12 * 14)
 PCG_FLOPS_PER_KNOT_ITER = 1708
-# on-chip bytes the resident / team kernels read per knot-iteration: each of
-# the 14 row lanes reads a row and a column of S(b,b-1) and p_{b-1,b,b+1}
-# and r_b from LDS (84 floats)
-PCG_LDS_BYTES_PER_KNOT_ITER = 4 * 84 * 14
 FP32_PEAK_TFLOPS = 157.3
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batched_1024.json")
+# per-kernel counters of the batched workload (rocprofv3 --pmc: LDS and VALU
+# instruction counts per launch, the LDS bytes per instruction of the PCG
+# loop body from the kernel's own ISA); see the .md beside it
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "counters_batched_1024.json")
 # algorithmic HBM bytes and flops per knot of the other phases of an SQP
 # iteration (fp32):
 # linearize: reads x_k, u_k, x_{k+1}, ref_k (21 + 14 + 6 floats), writes G
@@ -83,6 +83,34 @@ PHASE_MODEL = {
 PCG_HBM_BYTES_PER_KNOT = 4 * (588 + 196 + 14 + 28)
 
 
+def load_package():
+    """The package (ctypes front end of libmpcpcg.so) from its directory."""
+    import importlib.util
+    name = "gpu_pcg_tracking_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def host_info():
+    """Cores and CPU model of the host the CPU baseline runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0)), "model": model}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,14 +122,14 @@ def parse():
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16,
-                    help="host threads of the CPU baseline's parallel leg (capped by the visible cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline's parallel leg (0: every core this process may use)")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-closed-loop", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="issue every solve eagerly (no HIP graph replay), e.g. under counter profiling")
-    ap.add_argument("--stress", action="store_true",
-                    help="also run the stress configuration (4096 instances, N in {128, 256, 512}, Q/R swept)")
+    ap.add_argument("--no-stress", action="store_true",
+                    help="skip the stress configuration (4096 instances, N in {128, 256, 512}, Q/R swept)")
     return ap.parse_args()
 
 
@@ -119,6 +147,75 @@ def build_workload(pkg, solver, torch, n, seed, dt):
     ee = solver.fk(q_all).reshape(-1).contiguous()  # device FK of the reference path
     xu = torch.from_numpy(np.concatenate(xus)).cuda()
     return horizons, xu, ee
+
+
+def pcg_accuracy(solver, torch, horizons, lam, tol, cap):
+    """Accuracy of the last batched PCG solves of `solver` (their
+    multipliers in lam): the per-instance exit state (mpc_batch_pcg_exit)
+    and the relative error max|lam - lam_d| / max|lam_d| per instance
+    against the same Schur systems solved directly (block LDL^T with two
+    fp64-residual refinements, mpc_batch_direct_resolve; overwrites the
+    preconditioner workspace)."""
+    n = len(horizons)
+    it = torch.zeros(n, dtype=torch.int32, device="cuda")
+    fl = torch.zeros(n, dtype=torch.int32, device="cuda")
+    eta = torch.zeros(n, device="cuda")
+    solver.batch_pcg_exit(it, fl, eta)
+    lam_d = torch.empty_like(lam)
+    solver.batch_direct_resolve(lam_d, refine=2)
+    it, fl, eta = it.cpu().numpy(), fl.cpu().numpy(), eta.cpu().numpy().astype(np.float64)
+    a, b = lam.cpu().numpy().astype(np.float64), lam_d.cpu().numpy().astype(np.float64)
+    k0 = np.concatenate([[0], np.cumsum(horizons)]) * 14
+    err = np.array([np.abs(a[k0[i]:k0[i + 1]] - b[k0[i]:k0[i + 1]]).max() /
+                    max(np.abs(b[k0[i]:k0[i + 1]]).max(), 1e-30) for i in range(n)])
+    capped = fl == 1
+    q = lambda v, p: round(float(np.quantile(v, p)), 8) if len(v) else None
+    return {"pcg_exit_tol": tol, "pcg_max_iter": cap, "instances": n,
+            "capped_frac": round(float(capped.mean()), 4), "faulted": int((fl == 2).sum()),
+            "iters_mean": round(float(it.mean()), 1),
+            "eta_at_exit": {"median": q(eta, 0.5), "max": round(float(eta.max()), 8),
+                            "median_capped": q(eta[capped], 0.5), "median_converged": q(eta[~capped], 0.5)},
+            "rel_err_vs_direct": {"median": q(err, 0.5), "p90": q(err, 0.9), "max": round(float(err.max()), 8),
+                                  "median_capped": q(err[capped], 0.5), "median_converged": q(err[~capped], 0.5)},
+            "note": "eta = |r^T Pinv r| of each instance's last exit test (the exit rule's quantity); the error "
+                    "is of the PCG multipliers against a direct solve of the same Schur system"}
+
+
+def config1(pkg, torch, seed, n=64, N=32, reps=3):
+    """The plain single SQP step at N = 32 (one instance per launch
+    sequence, one SQP iteration, PCG tol 1e-5, cap 173), over n seeded
+    instances: whole-step wall time and the PCG kernel time (device events
+    around the PCG launch, the reference's 'linsys' without its two small
+    copies)."""
+    s = pkg.Solver(pkg.default_config(max_knots=N, max_instances=1, max_total_knots=N, sqp_max_iter=1))
+    step_us, lin_us, iters = [], [], []
+    for i in range(n):
+        x0, qref = pkg.synth_instance(seed * 7919 + 300000 + i, N)
+        ee = s.fk(torch.from_numpy(qref).cuda()).reshape(-1).contiguous()
+        xu0 = np.zeros(21 * N - 7, np.float32)
+        for k in range(N):
+            xu0[21 * k:21 * k + 14] = x0
+        d_xu0 = torch.from_numpy(xu0).cuda()
+        for r in range(reps + 1):
+            xu, lam = d_xu0.clone(), torch.zeros(14 * N, device="cuda")
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            out = s.sqp_solve(N, ee, lam, xu, 1e-3)
+            torch.cuda.synchronize()
+            if r > 0:  # the first solve of an instance is a warm-up
+                step_us.append((time.perf_counter() - t) * 1e6)
+                lin_us.append(out[1][0])
+                iters.append(out[0][0])
+    s.close()
+    it = np.asarray(iters)
+    return {"workload": f"plain single SQP step, N={N}, {n} seeded instances x {reps} solves, PCG tol 1e-5, cap 173",
+            "sqp_step_us": {"mean": round(float(np.mean(step_us)), 2), "median": round(float(np.median(step_us)), 2)},
+            "pcg_linsys_us": {"mean": round(float(np.mean(lin_us)), 2), "median": round(float(np.median(lin_us)), 2)},
+            "pcg_iters": {"mean": round(float(it.mean()), 1), "max": int(it.max()),
+                          "hit_cap_frac": round(float((it >= 173).mean()), 3)},
+            "pcg_linsys_us_per_iteration": round(float(np.sum(lin_us) / max(int(it.sum()), 1)), 3),
+            "reference_rtx4090": {"pcg_linsys_us_tol1e-5": 51.76, "pcg_linsys_us_173_iters": 742.23,
+                                  "source": "old-test-results/32-knot-point-100-iters-5-configs-test.txt:86, :91-96"}}
 
 
 def differentiation(pkg, torch, seed, reps=10, n=256, N=64, sqp_iters=3):
@@ -199,12 +296,40 @@ def closed_loop(pkg, torch, seed, N=128, steps=2000, timed_steps=200, K=3):
         pcg = torch.zeros(n * K, dtype=torch.int32, device="cuda")
         cl = pkg.default_closed_loop(steps=n, sqp_iters=K, use_graph=graph, timing=timing)
         _, err, _, st = solver.closed_loop(N, x0, ee, xu, lam, cl, traj, pcg_iters=pcg)
-        return err.cpu().numpy(), pcg.cpu().numpy(), st
+        return err.cpu().numpy(), pcg.cpu().numpy(), st, lam
+
+    def last_solve_accuracy(lam):
+        # the last control step's last PCG solve against a direct solve of
+        # its Schur system; the loop has since shifted lambda by one knot,
+        # so its knots 1 .. N-1 are lam[0 .. N-2]
+        it = torch.zeros(1, dtype=torch.int32, device="cuda")
+        fl = torch.zeros(1, dtype=torch.int32, device="cuda")
+        eta = torch.zeros(1, device="cuda")
+        solver.batch_pcg_exit(it, fl, eta)
+        lam_d = torch.empty_like(lam)
+        solver.batch_direct_resolve(lam_d, refine=2)
+        a = lam[:14 * (N - 1)].cpu().numpy().astype(np.float64)
+        b = lam_d[14:].cpu().numpy().astype(np.float64)
+        return (int(it.item()), int(fl.item()), float(eta.item()),
+                float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)))
 
     run(20, 1, 0)  # warm-up (first launches, graph instantiation paths)
-    err, pcg, st = run(steps, 1, 0)
-    _, pcg_t, st_t = run(timed_steps, 0, 1)
+    samples = []
+    for n_s in (50, 200, 800):  # accuracy of the last solve after n_s control steps
+        samples.append(last_solve_accuracy(run(n_s, 1, 0)[3]))
+    err, pcg, st, lam_end = run(steps, 1, 0)
+    samples.append(last_solve_accuracy(lam_end))
+    _, pcg_t, st_t, _ = run(timed_steps, 0, 1)
     solver.close()
+    s_eta = [s[2] for s in samples]
+    s_err = [s[3] for s in samples]
+    accuracy = {"pcg_exit_tol": 5e-6, "pcg_max_iter": 173, "samples": "the last PCG solve after 50, 200, 800 and 2000 "
+                                                                        "control steps",
+                "capped_frac": round(float(np.mean([s[1] == 1 for s in samples])), 3),
+                "eta_at_exit": {"median": round(float(np.median(s_eta)), 8), "max": round(float(max(s_eta)), 8)},
+                "rel_err_vs_direct": {"median": round(float(np.median(s_err)), 8),
+                                      "max": round(float(max(s_err)), 8)},
+                "note": "knots 1..N-1 of the multipliers (the loop shifts them by one knot after each solve)"}
     n_it = timed_steps * K
     ref_traj = reference_trajectory(pkg, torch, N, K)
     return {"workload": f"1 instance, N={N}, {steps} control steps, {K} SQP iterations per step, PCG tol 5e-6, "
@@ -213,6 +338,7 @@ def closed_loop(pkg, torch, seed, N=128, steps=2000, timed_steps=200, K=3):
             "rms_ee_error_m": round(float(np.sqrt(np.mean(err[:, 0] ** 2))), 4),
             "mean_l1_ee_error_m": round(float(err[:, 1].mean()), 4),
             "pcg_iters_mean": round(float(pcg.mean()), 1), "pcg_hit_cap_frac": round(float((pcg >= 173).mean()), 3),
+            "pcg_accuracy": accuracy,
             "timed_run": {"steps": timed_steps, "linsys_us_per_sqp_iter": round(st_t["linsys_us"] / n_it, 1),
                           "sqp_us_per_sqp_iter": round(st_t["solve_us"] / n_it, 1),
                           "plant_us_per_step": round(st_t["plant_us"] / timed_steps, 1)},
@@ -308,6 +434,7 @@ def stress(pkg, torch, seed, n=4096, ratios=(1e2, 1e4, 1e6), K=3):
                     m = (horizons == N)[:, None] & ex
                     by_n[str(N)] = round(float(lg[m].mean()), 1)
                 row[mode]["pcg_iters_mean_by_N"] = by_n
+                row[mode]["accuracy"] = pcg_accuracy(solver, torch, horizons, lam, 1e-5, 173)
         # forward (QP layer + loss) and adjoint at the final (direct) iterate, PCG mode
         solver.set_config(r_cost=r_cost, flags=0)
         dz, xs = torch.empty_like(xu), torch.empty_like(xu)
@@ -332,30 +459,80 @@ def stress(pkg, torch, seed, n=4096, ratios=(1e2, 1e4, 1e6), K=3):
     return out
 
 
-def cpu_baseline(orc, helpers, horizons, seconds, seed):
-    """fp32 sparse LDL^T factor + solve of the Schur system (oracle, one
-    core) on a sample of the workload's instances, weighted by the
-    horizon mix.  Returns KKT solves / s and the sample description."""
+def cpu_instance(orc, pkg, seed, N):
+    """A workload instance for the CPU legs: the library's seeded generator,
+    the oracle's FK for the reference path, x_k = x0 and u_k = 0."""
+    x0, qref = pkg.synth_instance(seed, N)
+    ee = np.concatenate([orc.fk(q.astype(np.float64)) for q in qref])
+    xu = np.zeros(21 * N - 7, np.float64)
+    for k in range(N):
+        xu[21 * k:21 * k + 14] = x0
+    return ee, xu
+
+
+def _rate(fn, budget):
+    """Seconds per call of fn(reps) -> elapsed, doubling reps up to ~budget s."""
+    reps, el = 1, 0.0
+    while el < budget:
+        el = fn(reps)
+        if el < budget:
+            reps = max(reps * 2, int(reps * budget / max(el, 1e-6)))
+    return el / reps
+
+
+def cpu_baseline(orc, pkg, torch, horizons, seconds, seed):
+    """The reference's QDLDL linsys restated on the host (the oracle's fp32
+    sparse LDL^T factor + solve of the Schur system, one core), one instance
+    per horizon of the workload at its initial iterate, weighted by the
+    horizon mix; also the host Schur formation (the oracle's fp32
+    form_schur) and the reference's copies around its solve
+    (toplevel.cuh:314-335: S values and gamma to the host, lambda back;
+    pageable host buffers as there), timed on this GPU.  Returns the rates,
+    their description and the per-horizon inputs."""
     cfg = orc.make_cfg()
-    per_n = {}
+    per_n, kkt = {}, {}
     for N in sorted(set(int(h) for h in horizons)):
-        x0, ee, xu = helpers.make_instance(orc, seed + N, N)
-        G, Cm, g, c = orc.form_kkt(cfg, N, xu.astype(np.float64), ee.astype(np.float64),
-                                   xu[:14].astype(np.float64), dtype=np.float32)
+        ee, xu = cpu_instance(orc, pkg, seed + N, N)
+        G, Cm, g, c = orc.form_kkt(cfg, N, xu, ee, xu[:14], dtype=np.float32)
         S, P, gam, _ = orc.form_schur(N, G, Cm, g, c, 1e-3, dtype=np.float32)
         per_n[N] = (S, gam)
+        kkt[N] = (G, Cm, g, c)
     budget = seconds / len(per_n)
-    t_per = {}
+    t_per, t_schur, t_copy = {}, {}, {}
     for N, (S, gam) in per_n.items():
-        reps, el = 1, 0.0
-        while el < budget:
-            el, _ = orc.ldl_bench(N, S, gam, reps)
-            if el < budget:
-                reps = max(reps * 2, int(reps * budget / max(el, 1e-6)))
-        t_per[N] = el / reps
-    mean_t = float(np.mean([t_per[int(h)] for h in horizons]))
-    desc = "; ".join(f"N={N}: {t_per[N] * 1e6:.0f} us" for N in sorted(t_per))
-    return 1.0 / mean_t, desc, t_per, per_n
+        t_per[N] = _rate(lambda reps: orc.ldl_bench(N, S, gam, reps)[0], budget)
+        G, Cm, g, c = kkt[N]
+
+        def schur(reps):
+            t = time.perf_counter()
+            for _ in range(reps):
+                orc.form_schur(N, G, Cm, g, c, 1e-3, dtype=np.float32)
+            return time.perf_counter() - t
+        t_schur[N] = _rate(schur, budget / 4)
+        if torch is None or not torch.cuda.is_available():
+            continue
+        nnz = len(orc.csc_values(N, S, dtype=np.float32))
+        d_val, d_gam, d_lam = (torch.zeros(m, device="cuda") for m in (nnz, 14 * N, 14 * N))
+        h_val, h_gam, h_lam = (torch.zeros(m) for m in (nnz, 14 * N, 14 * N))
+
+        def copies(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(reps):
+                h_val.copy_(d_val)
+                h_gam.copy_(d_gam)
+                d_lam.copy_(h_lam)
+                torch.cuda.synchronize()
+            return time.perf_counter() - t
+        t_copy[N] = _rate(copies, budget / 8)
+    mix = [int(h) for h in horizons]
+    mean = lambda t: float(np.mean([t[N] for N in mix]))
+    desc = "; ".join(f"N={N}: {t_per[N] * 1e6:.0f} us factor+solve, "
+                     + (f"{t_copy[N] * 1e6:.0f} us copies, " if t_copy else "")
+                     + f"{t_schur[N] * 1e6:.0f} us Schur formation" for N in sorted(t_per))
+    rates = {"ldl": 1.0 / mean(t_per), "ldl_with_copies": 1.0 / (mean(t_per) + mean(t_copy)) if t_copy else None,
+             "schur": 1.0 / mean(t_schur)}
+    return rates, desc, t_per, per_n
 
 
 def cpu_baseline_threads(orc, per_n, t_per, horizons, seconds, threads):
@@ -411,11 +588,22 @@ def measure_bandwidth(pkg, torch):
                       "4096 x 256 lanes x 2048 x 64 B), best of 5 event-timed launches"}
 
 
-def phase_roofline(phases, knots, pcg_units, bw):
+def phase_roofline(phases, knots, pcg_units, bw, counters=None):
     """Per phase of one SQP iteration: roofline time = max(B_LDS / BW_LDS,
     B_HBM / BW_HBM, F / fp32 peak) over the measured bandwidths, and the
-    fraction roofline time / measured time."""
+    fraction roofline time / measured time.  The linearisation and merit
+    flops per knot are counted from their kernels when the counter file
+    has them (fp32 VALU instructions x 64 lanes), else the estimate."""
     rows = {}
+    model = dict(PHASE_MODEL)
+    flop_source = {}
+    for name in ("linearize", "merit"):
+        per = (counters or {}).get("flops_per_knot", {}).get(name)
+        if per:
+            model[name] = (model[name][0], float(per))
+            flop_source[name] = "counted"
+        else:
+            flop_source[name] = "estimate" if name == "linearize" else "not counted"
 
     def row(name, us, b_hbm, b_lds, flops):
         t = {"hbm": b_hbm / (bw["hbm_gbs"] * 1e9), "lds": b_lds / (bw["lds_gbs"] * 1e9),
@@ -426,19 +614,20 @@ def phase_roofline(phases, knots, pcg_units, bw):
                       "roofline_us": round(roof, 2), "bound": bound, "frac": round(roof / us, 4) if us > 0 else None}
 
     for name in ("linearize", "schur"):
-        b, f = PHASE_MODEL[name]
+        b, f = model[name]
         row(name, phases[name], b * knots, 0.0, f * knots)
     row("pcg", phases["pcg_team"] + phases["pcg_rest"], PCG_HBM_BYTES_PER_KNOT * knots,
         PCG_BYTES_PER_KNOT_ITER * pcg_units, PCG_FLOPS_PER_KNOT_ITER * pcg_units)
     for name in ("dz", "merit"):
-        b, f = PHASE_MODEL[name]
+        b, f = model[name]
         row(name, phases[name], b * knots, 0.0, f * knots)
     return {"bandwidth": bw, "fp32_peak_tflops": FP32_PEAK_TFLOPS, "knots_per_sqp_iter": int(knots),
-            "pcg_knot_iterations_per_sqp_iter": int(pcg_units), "phases": rows,
-            "note": "per SQP iteration of the headline workload; bytes and flops are the algorithmic ones per "
-                    "knot (PCG: per knot and iteration, priced at the LDS rate, its S kept on chip); fp32 peak "
-                    "is the published packed rate (this build issues unpacked fp32: half of it); merit is "
-                    "priced by bytes only"}
+            "pcg_knot_iterations_per_sqp_iter": int(pcg_units), "phases": rows, "flops_source": flop_source,
+            "note": "per SQP iteration of the headline workload; bytes are the algorithmic ones per knot "
+                    "(PCG: per knot and iteration, priced at the LDS rate, its S kept on chip); flops: Schur, dz "
+                    "and PCG from the operation counts of their algorithms, linearisation and merit counted from "
+                    "their kernels' fp32 VALU instructions x 64 lanes (an upper bound: masked lanes count); fp32 "
+                    "peak is the published packed rate (this build issues unpacked fp32: half of it)"}
 
 
 def main():
@@ -448,8 +637,7 @@ def main():
                                    "split the instance list (see DESIGN.md); only --gpus 1 is measured"}))
         sys.exit(2)
     import torch
-    import helpers
-    pkg = helpers.load_pkg()
+    pkg = load_package()
 
     dt = 0.01
     n = args.instances
@@ -532,18 +720,32 @@ def main():
     units_launch = units / launches
     achieved = PCG_BYTES_PER_KNOT_ITER * units / (t_phase * 1e-6) / 1e9
     tflops = PCG_FLOPS_PER_KNOT_ITER * units / (t_phase * 1e-6) / 1e12
-    lds_gbs = PCG_LDS_BYTES_PER_KNOT_ITER * units / (t_phase * 1e-6) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("team_phase", {}).get("hbm_bytes_per_launch")
+    counters = None
+    if os.path.exists(COUNTERS_FILE):
+        with open(COUNTERS_FILE) as f:
+            counters = json.load(f)
+    # accuracy of the last SQP iteration's PCG solves (this timing step's)
+    accuracy = pcg_accuracy(solver, torch, horizons, lam, cfg.pcg_exit_tol, cfg.pcg_max_iter)
     phases = {name: round(v / K, 2) for name, v in zip(PHASES, ph)}
     kkt_phase_us = (ph[1] + ph[2] + ph[3] + ph[4]) / K
     # per-phase roofline against this device's measured HBM and LDS bandwidth
     ran = log_t >= 0
     knots_it = float((horizons[:, None] * ran).sum()) / K
     pcg_it = float((horizons[:, None] * np.where(ran, log_t, 0)).sum()) / K
-    proof = phase_roofline(phases, knots_it, pcg_it, measure_bandwidth(pkg, torch))
+    proof = phase_roofline(phases, knots_it, pcg_it, measure_bandwidth(pkg, torch), counters)
+    lds_peak = proof["bandwidth"]["lds_gbs"]
+    lds_counted = None
+    if counters and "team_phase" in counters:
+        tp = counters["team_phase"]
+        b_launch = tp["lds_wave_instructions_per_launch"] * tp["lds_bytes_per_wave_instruction"]
+        lds_counted = {"bytes_per_launch": int(b_launch), "achieved": round(b_launch / (t_launch * 1e-6) / 1e9, 1),
+                       "peak": lds_peak, "unit": "GB/s", "frac": round(b_launch / (t_launch * 1e-6) / 1e9 / lds_peak, 4),
+                       "source": "SQ_INSTS_LDS per launch (PMC) x LDS bytes per wave instruction of the loop body "
+                                 "(the kernel's ISA), from " + os.path.relpath(COUNTERS_FILE, ROOT)}
 
     # plain: one instance per launch sequence, host decisions (sqpSolve pattern)
     rng = np.random.default_rng(args.seed)
@@ -567,15 +769,24 @@ def main():
     cpu = None
     if not args.no_cpu:
         import oracle as orc
-        cval, desc, t_per, per_n = cpu_baseline(orc, helpers, horizons, args.cpu_seconds, args.seed)
-        threads = max(1, min(os.cpu_count() or 1, args.cpu_threads))
+        host = host_info()
+        rates, desc, t_per, per_n = cpu_baseline(orc, pkg, torch, horizons, args.cpu_seconds, args.seed)
+        threads = args.cpu_threads if args.cpu_threads > 0 else host["usable_cores"]
+        cval = rates["ldl"]
         tval = cpu_baseline_threads(orc, per_n, t_per, horizons, args.cpu_seconds, threads) if threads > 1 else cval
         cpu = {"value": round(tval, 1), "unit": "KKT solves/s", "cores": threads, "kind": "port",
+               "host": host,
                "single_core_value": round(cval, 1),
-               "sample": f"fp32 sparse LDL^T factor+solve of the Schur system (oracle, reference CSC pattern), "
-                         f"one instance per horizon at its initial iterate, weighted by the workload's horizon mix; "
-                         f"one core: {desc}, ~{args.cpu_seconds:.0f} s; then {threads} threads, one instance per "
-                         f"thread at a time, ~{args.cpu_seconds:.0f} s each"}
+               "single_core_with_copies_value": round(rates["ldl_with_copies"], 1)
+               if rates["ldl_with_copies"] else None,
+               "single_core_schur_formation_per_sec": round(rates["schur"], 1),
+               "sample": f"fp32 sparse LDL^T factor+solve of the Schur system (the oracle's restatement of the "
+                         f"reference's QDLDL path, its CSC pattern), one instance per horizon at its initial "
+                         f"iterate, weighted by the workload's horizon mix; one core: {desc}, ~{args.cpu_seconds:.0f} "
+                         f"s; then {threads} threads (every core this process may use), one instance per thread "
+                         f"at a time, ~{args.cpu_seconds:.0f} s each.  'with copies' adds the reference's "
+                         f"device-host copies around its solve (S values and gamma down, lambda up, pageable, "
+                         f"timed on this GPU); the Schur formation line is the oracle's fp32 form_schur"}
 
     # the same workload with the stair preconditioner (half the PCG iterations
     # for one more block product each, on the same kernels)
@@ -605,9 +816,10 @@ def main():
               "note": "phase 'pcg_team' holds the direct solve, 'dz' the dz recovery and its KKT refinement"}
     solver.set_config(flags=0)
 
+    c1 = config1(pkg, torch, args.seed)
     diff = None if args.no_diff else differentiation(pkg, torch, args.seed)
     closed = None if args.no_closed_loop else closed_loop(pkg, torch, args.seed)
-    stress_out = stress(pkg, torch, args.seed) if args.stress else None
+    stress_out = None if args.no_stress else stress(pkg, torch, args.seed)
 
     its = log[executed]
     out = {
@@ -633,6 +845,7 @@ def main():
         "plain_kkt_solves_per_sec": round(plain_value, 1),
         "speedup_vs_plain": round(value / plain_value, 2),
         "phase_us_per_sqp_iter": phases,
+        "pcg_accuracy": accuracy,
         "pcg_iters": {"mean": round(float(its.mean()), 1), "max": int(its.max()), "min": int(its.min()),
                       "hit_max_iter": int((st & 2 != 0).sum())},
         "status": {"aborted": int((st & 4 != 0).sum()), "nonfinite": int((st & 8 != 0).sum()),
@@ -644,22 +857,25 @@ def main():
                   "eager_kkt_solves_per_sec": round(kkt_per_step / (ms_eager * 1e-3), 1),
                   "note": "eager = the same launch sequence without capture, PCG teams as cooperative launches; "
                           "in the graph they are ordinary kernel nodes (bounded waits, fault status)"},
-        "roofline": {"kernel": "k_pcg_team", "bound": "hbm", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "roofline": {"kernel": "k_pcg_team", "bound": "lds", "achieved": round(achieved, 1),
+                     "peak": lds_peak, "unit": "GB/s", "frac": round(achieved / lds_peak, 4),
+                     "peak_source": "LDS bandwidth measured on this device in this run (mpc_bw_lds, 128-bit reads)",
                      "traffic": traffic, "algorithmic_bytes_per_launch": int(PCG_BYTES_PER_KNOT_ITER * units_launch),
                      "knot_iterations_per_launch": int(units_launch), "avg_launch_us": round(t_launch, 2),
                      "launches_per_sqp_iter": launches, "pcg_layout": layout,
-                     "note": "algorithmic bytes = 2296 B per knot and PCG iteration (S, Pinv and vectors); "
-                             "the kernel keeps S and Pinv on chip, so they come from LDS and registers and "
-                             "the HBM traffic (PMC, per launch) is the one-time load; the last team launch "
-                             "(k_pcg_mixed) also solves the first resident bins, counted with it",
-                     "lds": {"achieved": round(lds_gbs, 1), "peak": proof["bandwidth"]["lds_gbs"], "unit": "GB/s",
-                             "frac": round(lds_gbs / proof["bandwidth"]["lds_gbs"], 4),
-                             "peak_source": "measured on this device (mpc_bw_lds)"},
+                     "note": "algorithmic bytes = 2296 B per knot and PCG iteration (S, Pinv and vectors) x the "
+                             "knot-iterations of one launch; S and Pinv stay on chip (LDS and registers), so the "
+                             "roof is the LDS rate; traffic = HBM bytes per launch (PMC), the one-time load; the "
+                             "last team launch (k_pcg_mixed) also solves the first resident bins, counted with it",
+                     "hbm": None if traffic is None else
+                     {"achieved": round(traffic / (t_launch * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(traffic / (t_launch * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},
+                     "lds_counted": lds_counted,
                      "fp32": {"achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": round(tflops / FP32_PEAK_TFLOPS, 4)}},
         "phase_roofline": proof,
         "cpu_baseline": cpu,
+        "plain_single_step_n32": c1,
         "pcg_stair": stair,
         "direct_solver": direct,
         "differentiation": diff,

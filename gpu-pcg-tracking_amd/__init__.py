@@ -30,11 +30,12 @@ MAX_SQP_ITERS = 64
 
 # status codes / bits (include/mpcpcg.h)
 OK, ERR_ARG, ERR_HIP, ERR_NOT_READY, ERR_NONFINITE, ERR_NO_DEVICE, ERR_PCG_FAULT = range(7)
-ST_ACTIVE, ST_PCG_MAXITER, ST_RHO_ABORT, ST_NONFINITE, ST_PCG_FAULT = 1, 2, 4, 8, 16
+ST_ACTIVE, ST_PCG_MAXITER, ST_RHO_ABORT, ST_NONFINITE, ST_PCG_FAULT, ST_TIME_BUDGET = 1, 2, 4, 8, 16, 32
 FLAG_LAST_KNOT_OWN_STATE = 1
 FLAG_PCG_STREAM = 2
 FLAG_DIRECT_SOLVE = 4
 FLAG_PCG_STAIR = 8
+FLAG_PCG_CAP_BY_N = 16
 
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = (
@@ -46,7 +47,8 @@ EXPORTS = (
     "mpc_batch_set_plan", "mpc_batch_autotune", "mpc_set_cost_weights", "mpc_get_cost_weights", "mpc_batch_qp_layer",
     "mpc_batch_qp_loss", "mpc_batch_qp_adjoint", "mpc_synth_instance", "mpc_synth_horizons",
     "mpc_inverse_dynamics", "mpc_default_closed_loop", "mpc_closed_loop", "mpc_synth_track", "mpc_direct_solve",
-    "mpc_bw_copy", "mpc_bw_lds", "mpc_direct_solve_static",
+    "mpc_bw_copy", "mpc_bw_lds", "mpc_direct_solve_static", "mpc_set_fault_injection", "mpc_batch_pcg_exit",
+    "mpc_batch_direct_resolve",
 )
 
 
@@ -68,7 +70,7 @@ class Config(C.Structure):
         ("mu", C.c_float), ("rho_factor", C.c_float), ("rho_max", C.c_float), ("rho_min", C.c_float),
         ("rho_reset", C.c_float), ("pcg_exit_tol", C.c_float), ("pcg_max_iter", C.c_uint32),
         ("sqp_max_iter", C.c_uint32), ("flags", C.c_uint32), ("direct_refine", C.c_uint32),
-        ("direct_kkt_refine", C.c_uint32),
+        ("direct_kkt_refine", C.c_uint32), ("sqp_max_time_us", C.c_float),
     ]
 
 
@@ -77,6 +79,7 @@ class SqpStats(C.Structure):
         ("sqp_iters", C.c_uint32), ("not_aborted", C.c_uint32), ("sqp_us", C.c_double),
         ("pcg_iters", C.c_uint32 * MAX_SQP_ITERS), ("pcg_maxiter", C.c_uint32 * MAX_SQP_ITERS),
         ("linsys_us", C.c_double * MAX_SQP_ITERS), ("ls_step", C.c_int32 * MAX_SQP_ITERS),
+        ("time_exit", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
 
@@ -151,6 +154,9 @@ def lib():
             "mpc_bw_lds": [vp, u32, u32, vp],
             "mpc_direct_solve": [vp, u32, vp, vp, vp, u32, vp, vp],
             "mpc_direct_solve_static": [u32, vp, vp, vp, u32, vp],
+            "mpc_set_fault_injection": [vp, u32],
+            "mpc_batch_pcg_exit": [vp, vp, vp, vp, vp],
+            "mpc_batch_direct_resolve": [vp, vp, u32, vp],
             "mpc_closed_loop": [vp, C.POINTER(ClosedLoopConfig), u32, vp, vp, vp, vp, vp, vp, vp, vp,
                                 C.POINTER(ClosedLoopStats), vp],
         }
@@ -231,11 +237,23 @@ def synth_horizons(seed: int, n: int, choices=(32, 64, 128, 256)):
 
 # ---- solver -------------------------------------------------------------------------------------
 
-def _ptr(t):
+def _ptr(t, n=0, kind="f32", what="buffer"):
+    """Device address of a CUDA tensor after checking what the kernels
+    assume of it: contiguous, on the current device, float32 (kind "f32")
+    or 32-bit integer (kind "u32"), at least n elements.  None passes as a
+    null pointer (optional outputs)."""
     if t is None:
         return None
+    import torch
     if not (t.is_cuda and t.is_contiguous()):
-        raise MpcError("expected a contiguous CUDA tensor")
+        raise MpcError(f"{what}: expected a contiguous CUDA tensor")
+    if t.device.index != torch.cuda.current_device():
+        raise MpcError(f"{what}: tensor on cuda:{t.device.index}, current device is cuda:{torch.cuda.current_device()}")
+    ok = t.dtype == torch.float32 if kind == "f32" else t.dtype in (torch.int32, getattr(torch, "uint32", torch.int32))
+    if not ok:
+        raise MpcError(f"{what}: expected {'float32' if kind == 'f32' else 'int32 / uint32'}, got {t.dtype}")
+    if t.numel() < n:
+        raise MpcError(f"{what}: {t.numel()} elements, needs {n}")
     return C.c_void_p(t.data_ptr())
 
 
@@ -244,7 +262,7 @@ def bw_copy(src, dst):
     the HBM bandwidth probe of the library; 8 * numel bytes moved."""
     if src.numel() != dst.numel():
         raise MpcError("bw_copy: sizes differ")
-    _check(lib().mpc_bw_copy(_ptr(src), _ptr(dst), src.numel(), _stream()), "mpc_bw_copy")
+    _check(lib().mpc_bw_copy(_ptr(src, what="src"), _ptr(dst, what="dst"), src.numel(), _stream()), "mpc_bw_copy")
 
 
 def bw_lds(out, n_wg: int, reps: int):
@@ -253,7 +271,7 @@ def bw_lds(out, n_wg: int, reps: int):
     lane's folded sum."""
     if out.numel() < 256 * n_wg:
         raise MpcError("bw_lds: out too small")
-    _check(lib().mpc_bw_lds(_ptr(out), n_wg, reps, _stream()), "mpc_bw_lds")
+    _check(lib().mpc_bw_lds(_ptr(out, 256 * n_wg, "u32", "out"), n_wg, reps, _stream()), "mpc_bw_lds")
 
 
 # largest horizon of direct_solve_static (MPC_DIRECT_STATIC_MAX_KNOTS)
@@ -265,7 +283,8 @@ def direct_solve_static(N, S, gamma, lam, refine=2):
     that allocates nothing (no handle, factor workspace in a library-owned
     device array for N <= DIRECT_STATIC_MAX_KNOTS; calls overlapping in
     time on one GPU race on it)."""
-    _check(lib().mpc_direct_solve_static(N, _ptr(S), _ptr(gamma), _ptr(lam), int(refine), _stream()),
+    _check(lib().mpc_direct_solve_static(N, _ptr(S, 588 * N, what="S"), _ptr(gamma, 14 * N, what="gamma"),
+                                         _ptr(lam, 14 * N, what="lambda"), int(refine), _stream()),
            "mpc_direct_solve_static")
 
 
@@ -307,7 +326,8 @@ class Solver:
         import torch
         n = q.shape[0]
         out = out if out is not None else torch.empty((n, 6), device=q.device, dtype=torch.float32)
-        _check(lib().mpc_fk(self.h, n, _ptr(q), q.shape[1], _ptr(out), _stream()), "mpc_fk")
+        _check(lib().mpc_fk(self.h, n, _ptr(q, n * q.shape[1], what="q"), q.shape[1], _ptr(out, 6 * n, what="ee"),
+                            _stream()), "mpc_fk")
         return out
 
     def dynamics_gradient(self, qqdu):
@@ -315,50 +335,61 @@ class Solver:
         n = qqdu.shape[0]
         qdd = torch.empty((n, NQ), device=qqdu.device, dtype=torch.float32)
         df = torch.empty((n, 147), device=qqdu.device, dtype=torch.float32)
-        _check(lib().mpc_dynamics_gradient(self.h, n, _ptr(qqdu), _ptr(qdd), _ptr(df), _stream()),
-               "mpc_dynamics_gradient")
+        _check(lib().mpc_dynamics_gradient(self.h, n, _ptr(qqdu, 21 * n, what="qqdu"), _ptr(qdd), _ptr(df),
+                                           _stream()), "mpc_dynamics_gradient")
         return qdd, df
 
     # -- stages (one instance, reference layouts)
     def form_kkt(self, N, xu, ee_ref, xs, G, Cm, g, c):
-        _check(lib().mpc_form_kkt(self.h, N, _ptr(xu), _ptr(ee_ref), _ptr(xs), _ptr(G), _ptr(Cm), _ptr(g),
-                                  _ptr(c), _stream()), "mpc_form_kkt")
+        _check(lib().mpc_form_kkt(self.h, N, _ptr(xu, 21 * N - 7, what="xu"), _ptr(ee_ref, 6 * N, what="ee_ref"),
+                                  _ptr(xs, NX, what="xs"), _ptr(G, 245 * N - 49, what="G"),
+                                  _ptr(Cm, 294 * (N - 1), what="C"), _ptr(g, 21 * N - 7, what="g"),
+                                  _ptr(c, 14 * N, what="c"), _stream()), "mpc_form_kkt")
 
     def form_schur(self, N, G, Cm, g, c, S, Pinv, gamma, rho):
-        _check(lib().mpc_form_schur(self.h, N, _ptr(G), _ptr(Cm), _ptr(g), _ptr(c), _ptr(S), _ptr(Pinv),
-                                    _ptr(gamma), float(rho), _stream()), "mpc_form_schur")
+        _check(lib().mpc_form_schur(self.h, N, _ptr(G, 245 * N - 49, what="G"), _ptr(Cm, 294 * (N - 1), what="C"),
+                                    _ptr(g, 21 * N - 7, what="g"), _ptr(c, 14 * N, what="c"),
+                                    _ptr(S, 588 * N, what="S"), _ptr(Pinv, 588 * N, what="Pinv"),
+                                    _ptr(gamma, 14 * N, what="gamma"), float(rho), _stream()), "mpc_form_schur")
 
     def pcg(self, N, S, Pinv, gamma, lam, max_iter, exit_tol, iters=None, flag=None):
-        _check(lib().mpc_pcg_solve(self.h, N, _ptr(S), _ptr(Pinv), _ptr(gamma), _ptr(lam), max_iter,
-                                   float(exit_tol), _ptr(iters), _ptr(flag), _stream()), "mpc_pcg_solve")
+        _check(lib().mpc_pcg_solve(self.h, N, _ptr(S, 588 * N, what="S"), _ptr(Pinv, 588 * N, what="Pinv"),
+                                   _ptr(gamma, 14 * N, what="gamma"), _ptr(lam, 14 * N, what="lambda"), max_iter,
+                                   float(exit_tol), _ptr(iters, 1, "u32", "iters"), _ptr(flag, 1, "u32", "flag"),
+                                   _stream()), "mpc_pcg_solve")
 
     def direct_solve(self, N, S, gamma, lam, refine=2, work=None):
         """lambda = S^-1 gamma by block LDL^T on the GPU with `refine` steps of
         iterative refinement (work: 588 N floats)."""
         import torch
         work = torch.empty(588 * N, device="cuda") if work is None else work
-        _check(lib().mpc_direct_solve(self.h, N, _ptr(S), _ptr(gamma), _ptr(lam), int(refine), _ptr(work), _stream()),
-               "mpc_direct_solve")
+        _check(lib().mpc_direct_solve(self.h, N, _ptr(S, 588 * N, what="S"), _ptr(gamma, 14 * N, what="gamma"),
+                                      _ptr(lam, 14 * N, what="lambda"), int(refine), _ptr(work, 588 * N, what="work"),
+                                      _stream()), "mpc_direct_solve")
 
     def compute_dz(self, N, Ginv, Cm, g, lam, dz):
-        _check(lib().mpc_compute_dz(self.h, N, _ptr(Ginv), _ptr(Cm), _ptr(g), _ptr(lam), _ptr(dz), _stream()),
-               "mpc_compute_dz")
+        _check(lib().mpc_compute_dz(self.h, N, _ptr(Ginv, 245 * N - 49, what="Ginv"), _ptr(Cm, 294 * (N - 1), what="C"),
+                                    _ptr(g, 21 * N - 7, what="g"), _ptr(lam, 14 * N, what="lambda"),
+                                    _ptr(dz, 21 * N - 7, what="dz"), _stream()), "mpc_compute_dz")
 
     def compute_merit(self, N, xu, ee_ref, out):
-        _check(lib().mpc_compute_merit(self.h, N, _ptr(xu), _ptr(ee_ref), _ptr(out), _stream()),
-               "mpc_compute_merit")
+        _check(lib().mpc_compute_merit(self.h, N, _ptr(xu, 21 * N - 7, what="xu"), _ptr(ee_ref, 6 * N, what="ee_ref"),
+                                       _ptr(out, 1, what="merit"), _stream()), "mpc_compute_merit")
 
     def ls_compute_merit(self, N, xs, xu, ee_ref, dz, out):
-        _check(lib().mpc_line_search_merits(self.h, N, _ptr(xs), _ptr(xu), _ptr(ee_ref), _ptr(dz), _ptr(out),
-                                            _stream()), "mpc_line_search_merits")
+        _check(lib().mpc_line_search_merits(self.h, N, _ptr(xs, NX, what="xs"), _ptr(xu, 21 * N - 7, what="xu"),
+                                            _ptr(ee_ref, 6 * N, what="ee_ref"), _ptr(dz, 21 * N - 7, what="dz"),
+                                            _ptr(out, 8, what="merits"), _stream()), "mpc_line_search_merits")
 
-    def sqp_solve(self, N, ee_ref, lam, xu, rho, rho_reset=1e-3):
+    def sqp_solve(self, N, ee_ref, lam, xu, rho, rho_reset=1e-3, stats=None):
         """Drop-in for sqpSolve: updates xu and lam in place; returns
-        (pcg_iters, linsys_us, sqp_us, sqp_iters, not_aborted, pcg_maxiter, rho)."""
+        (pcg_iters, linsys_us, sqp_us, sqp_iters, not_aborted, pcg_maxiter, rho,
+        ls_steps); `stats` (an SqpStats) receives the raw statistics."""
         r = C.c_float(rho)
-        st = SqpStats()
-        _check(lib().mpc_sqp_solve(self.h, N, _ptr(ee_ref), _ptr(lam), _ptr(xu), C.byref(r), float(rho_reset),
-                                   C.byref(st), _stream()), "mpc_sqp_solve")
+        st = SqpStats() if stats is None else stats
+        _check(lib().mpc_sqp_solve(self.h, N, _ptr(ee_ref, 6 * N, what="ee_ref"), _ptr(lam, 14 * N, what="lambda"),
+                                   _ptr(xu, 21 * N - 7, what="xu"), C.byref(r), float(rho_reset), C.byref(st),
+                                   _stream()), "mpc_sqp_solve")
         k = st.sqp_iters
         return (list(st.pcg_iters[:k]), list(st.linsys_us[:k]), st.sqp_us, k, bool(st.not_aborted),
                 [bool(x) for x in st.pcg_maxiter[:k]], r.value, list(st.ls_step[:k]))
@@ -368,10 +399,19 @@ class Solver:
         arr = (C.c_uint32 * len(horizons))(*[int(x) for x in horizons])
         _check(lib().mpc_batch_setup(self.h, len(horizons), arr), "mpc_batch_setup")
         self._n_inst = len(horizons)
+        self._T = int(sum(int(x) for x in horizons))
+
+    def _layout(self):
+        if getattr(self, "_n_inst", None) is None:
+            raise MpcError("no batch layout (call batch_setup first)")
+        return self._n_inst, self._T
 
     def batch_sqp(self, xu, lam, ee_ref, rho, sqp_iters, status, pcg_iters=None):
-        _check(lib().mpc_batch_sqp(self.h, _ptr(xu), _ptr(lam), _ptr(ee_ref), _ptr(rho), sqp_iters,
-                                   _ptr(status), _ptr(pcg_iters), _stream()), "mpc_batch_sqp")
+        n, T = self._layout()
+        _check(lib().mpc_batch_sqp(self.h, _ptr(xu, 21 * T - 7 * n, what="xu"), _ptr(lam, 14 * T, what="lambda"),
+                                   _ptr(ee_ref, 6 * T, what="ee_ref"), _ptr(rho, n, what="rho"), sqp_iters,
+                                   _ptr(status, n, "u32", "status"), _ptr(pcg_iters, n * sqp_iters, "u32", "pcg_iters"),
+                                   _stream()), "mpc_batch_sqp")
 
     def batch_set_timing(self, on=True):
         _check(lib().mpc_batch_set_timing(self.h, int(on)), "mpc_batch_set_timing")
@@ -389,6 +429,11 @@ class Solver:
             raise MpcError("weights: wx has 14 entries, wu has 7")
         _check(lib().mpc_set_cost_weights(self.h, wx.ctypes.data, wu.ctypes.data), "mpc_set_cost_weights")
 
+    def set_fault_injection(self, part_plus_one):
+        """Test hook: part part_plus_one - 1 of every PCG team posts under a
+        wrong epoch tag, so team solves end with flag 2 (0 turns it off)."""
+        _check(lib().mpc_set_fault_injection(self.h, int(part_plus_one)), "mpc_set_fault_injection")
+
     def cost_weights(self):
         import numpy as np
         wx, wu = np.zeros(14, np.float32), np.zeros(7, np.float32)
@@ -396,27 +441,37 @@ class Solver:
         return wx, wu
 
     def qp_layer(self, xu, ee_ref, rho, lam, dz, xstar, pcg_tol=0.0, refine=REFINE_STEPS, refine_tol=0.0):
-        _check(lib().mpc_batch_qp_layer(self.h, _ptr(xu), _ptr(ee_ref), _ptr(rho), _ptr(lam), float(pcg_tol),
-                                        int(refine), float(refine_tol), _ptr(dz), _ptr(xstar), _stream()),
-               "mpc_batch_qp_layer")
+        n, T = self._layout()
+        L = 21 * T - 7 * n
+        _check(lib().mpc_batch_qp_layer(self.h, _ptr(xu, L, what="xu"), _ptr(ee_ref, 6 * T, what="ee_ref"),
+                                        _ptr(rho, n, what="rho"), _ptr(lam, 14 * T, what="lambda"), float(pcg_tol),
+                                        int(refine), float(refine_tol), _ptr(dz, L, what="dz"),
+                                        _ptr(xstar, L, what="xstar"), _stream()), "mpc_batch_qp_layer")
 
     def qp_loss(self, xu, ee_ref, loss, dLdx=None, dLdref=None):
-        _check(lib().mpc_batch_qp_loss(self.h, _ptr(xu), _ptr(ee_ref), _ptr(loss), _ptr(dLdx), _ptr(dLdref),
-                                       _stream()), "mpc_batch_qp_loss")
+        n, T = self._layout()
+        L = 21 * T - 7 * n
+        _check(lib().mpc_batch_qp_loss(self.h, _ptr(xu, L, what="xu"), _ptr(ee_ref, 6 * T, what="ee_ref"),
+                                       _ptr(loss, n, what="loss"), _ptr(dLdx, L, what="dLdx"),
+                                       _ptr(dLdref, 6 * T, what="dLdref"), _stream()), "mpc_batch_qp_loss")
 
     def qp_adjoint(self, xu, ee_ref, rho, dz, a, grad_w, grad_ref, pcg_tol=0.0, refine=REFINE_STEPS,
                    refine_tol=0.0):
-        _check(lib().mpc_batch_qp_adjoint(self.h, _ptr(xu), _ptr(ee_ref), _ptr(rho), _ptr(dz), _ptr(a),
-                                          float(pcg_tol), int(refine), float(refine_tol), _ptr(grad_w),
-                                          _ptr(grad_ref), _stream()),
-               "mpc_batch_qp_adjoint")
+        n, T = self._layout()
+        L = 21 * T - 7 * n
+        _check(lib().mpc_batch_qp_adjoint(self.h, _ptr(xu, L, what="xu"), _ptr(ee_ref, 6 * T, what="ee_ref"),
+                                          _ptr(rho, n, what="rho"), _ptr(dz, L, what="dz"), _ptr(a, L, what="a"),
+                                          float(pcg_tol), int(refine), float(refine_tol),
+                                          _ptr(grad_w, 21 * n, what="grad_w"), _ptr(grad_ref, 6 * T, what="grad_ref"),
+                                          _stream()), "mpc_batch_qp_adjoint")
 
     def inverse_dynamics(self, qqdqdd, out=None):
         """tau (n, 7) of n points [q qd qdd] (n, 21) on the device."""
         import torch
         n = qqdqdd.shape[0]
         out = torch.empty(n, NQ, device=qqdqdd.device) if out is None else out
-        _check(lib().mpc_inverse_dynamics(self.h, n, _ptr(qqdqdd), _ptr(out), _stream()), "mpc_inverse_dynamics")
+        _check(lib().mpc_inverse_dynamics(self.h, n, _ptr(qqdqdd, 21 * n, what="qqdqdd"), _ptr(out, NQ * n, what="tau"),
+                                          _stream()), "mpc_inverse_dynamics")
         return out
 
     def track_inputs(self, seed, steps, N, dt=0.01):
@@ -445,10 +500,15 @@ class Solver:
         states = torch.empty(steps + 1, NX, device="cuda") if states is None else states
         ee_err = torch.empty(steps, 2, device="cuda") if ee_err is None else ee_err
         st = ClosedLoopStats()
-        _check(lib().mpc_closed_loop(self.h, C.byref(cl), N, _ptr(x0), _ptr(ee_path), _ptr(traj), _ptr(xu), _ptr(lam),
-                                     _ptr(states), _ptr(ee_err), _ptr(pcg_iters), C.byref(st), _stream()),
+        K_all = steps + N
+        _check(lib().mpc_closed_loop(self.h, C.byref(cl), N, _ptr(x0, NX, what="x0"),
+                                     _ptr(ee_path, 6 * K_all, what="ee_path"),
+                                     _ptr(traj, 21 * K_all - 7, what="traj"), _ptr(xu, 21 * N - 7, what="xu"),
+                                     _ptr(lam, 14 * N, what="lambda"), _ptr(states, NX * (steps + 1), what="states"),
+                                     _ptr(ee_err, 2 * steps, what="ee_err"),
+                                     _ptr(pcg_iters, steps * K, "u32", "pcg_iters"), C.byref(st), _stream()),
                "mpc_closed_loop")
-        self._n_inst = 1  # the run sets the batch layout to its one instance
+        self._n_inst, self._T = 1, N  # the run sets the batch layout to its one instance
         stats = {k: getattr(st, k) for k, _ in ClosedLoopStats._fields_ if k != "reserved"}
         return states, ee_err, pcg_iters, stats
 
@@ -494,6 +554,20 @@ class Solver:
         _check(lib().mpc_batch_autotune(self.h, int(iters), int(reps), res, 64, C.byref(n), _stream()),
                "mpc_batch_autotune")
         return [dict({k: int(getattr(r, k)) for k in _PLAN_KEYS}, us=float(r.us)) for r in res[:n.value]]
+
+    def batch_pcg_exit(self, iters=None, flags=None, eta=None):
+        """Per instance, the last PCG solve's iterations, max-iter flag and
+        |r^T Pinv r| at its last exit test, into the given CUDA tensors."""
+        n, _ = self._layout()
+        _check(lib().mpc_batch_pcg_exit(self.h, _ptr(iters, n, "u32", "iters"), _ptr(flags, n, "u32", "flags"),
+                                        _ptr(eta, n, what="eta"), _stream()), "mpc_batch_pcg_exit")
+
+    def batch_direct_resolve(self, lam_out, refine=2):
+        """The last batched Schur systems solved directly into lam_out (the
+        batch's lambda layout); overwrites the preconditioner workspace."""
+        _, T = self._layout()
+        _check(lib().mpc_batch_direct_resolve(self.h, _ptr(lam_out, 14 * T, what="lambda"), int(refine), _stream()),
+               "mpc_batch_direct_resolve")
 
     def batch_phase_times(self):
         out = (C.c_double * 7)()

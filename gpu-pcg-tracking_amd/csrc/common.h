@@ -64,15 +64,21 @@ struct Params {
   float pcg_tol;
   uint32_t pcg_max_iter;
   uint32_t flags;
+  // SQP wall-clock budget in device clock ticks (wall_clock64), 0 = none
+  unsigned long long budget_ticks;
 };
 
-// mailbox of a PCG team of up to 4 workgroups: per epoch parity and part,
+// mailbox of a PCG team of up to 4 workgroups: per epoch (mod 4) and part,
 // 40 self-validating words {epoch (high 32 bits), float bits (low 32 bits)}:
 // words 0..7 the partial sums of the part's 8 waves, 8..21 the vector
 // elements of its first row (for the part before it), 24..37 of its last
-// row (for the part after it)
+// row (for the part after it).  Four epoch buffers: a part overwrites the
+// words of epoch e only when it posts epoch e + 4, and between the two it
+// has waited for the partials of every wave of every other part at an
+// all-to-all epoch, which each wave posts only after its reads of epoch e
+// (no two consecutive epochs are rows-only exchanges)
 struct TeamMailbox {
-  unsigned long long w[2][4][40];
+  unsigned long long w[4][4][40];
 };
 
 // per-instance device state of the batched SQP (struct of arrays)
@@ -83,6 +89,7 @@ struct InstState {
   uint32_t *status;  // [n] MPC_ST_* bits
   uint32_t *sqp_done;  // [n] SQP iterations taken
   float *xs;         // [14 n] initial state of each solve
+  unsigned long long *t0;  // [1] device clock at the start of the solve
 };
 
 // one control step of the closed-loop harness (k_closed_loop.hip)
@@ -92,12 +99,22 @@ struct ClStep {
   uint64_t seed;
 };
 
-constexpr uint32_t ST_ACTIVE = 1u, ST_PCG_MAXITER = 2u, ST_RHO_ABORT = 4u, ST_NONFINITE = 8u, ST_PCG_FAULT = 16u;
+constexpr uint32_t ST_ACTIVE = 1u, ST_PCG_MAXITER = 2u, ST_RHO_ABORT = 4u, ST_NONFINITE = 8u, ST_PCG_FAULT = 16u,
+                   ST_TIME_BUDGET = 32u;
 // per-instance PCG flag: 0 converged, 1 max_iter reached, PCG_FLAG_FAULT the
 // solve was abandoned (a team exchange timed out)
 constexpr uint32_t PCG_FLAG_FAULT = 2u;
 constexpr uint32_t FLAG_LAST_KNOT_OWN_STATE = 1u, FLAG_PCG_STREAM = 2u, FLAG_DIRECT_SOLVE = 4u,
-                   FLAG_PCG_STAIR = 8u;
+                   FLAG_PCG_STAIR = 8u, FLAG_PCG_CAP_BY_N = 16u;
+
+// PCG max_iter value that selects the reference experiments' cap per
+// horizon (perform_experiment.sh:14-19: 173 at N = 32, 167 at 64 and 128,
+// 118 at 256, 67 at 512), taken for the tabulated horizon at or above N
+constexpr uint32_t PCG_CAP_BY_N = 0xFFFFFFFFu;
+__host__ __device__ __forceinline__ uint32_t pcg_cap(uint32_t max_iter, uint32_t N) {
+  if (max_iter != PCG_CAP_BY_N) return max_iter;
+  return N <= 32u ? 173u : N <= 128u ? 167u : N <= 256u ? 118u : 67u;
+}
 
 // host-side launchers (implemented in the .hip files)
 hipError_t launch_fk(uint32_t n, const float *q, uint32_t q_stride, float *ee, hipStream_t st);
@@ -117,12 +134,16 @@ hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, con
                              hipStream_t st);
 hipError_t launch_compute_dz(const Batch &b, const float *Ginv, const float *C, const float *g,
                              const float *lambda, const uint32_t *status, float *dz, hipStream_t st);
+// PCG of every instance.  Outputs per instance: iters (S products), flag (0
+// converged, 1 max_iter reached, PCG_FLAG_FAULT abandoned) and eta (the
+// |r^T Pinv r| of the last exit test); each may be null.  max_iter ==
+// PCG_CAP_BY_N takes the cap of pcg_cap per instance.
 // PCG streaming S from memory every iteration (any N <= 512); inst_list
 // (n_list entries) restricts the launch to some instances of the batch.
 hipError_t launch_pcg_stream(const Batch &b, uint32_t max_n, const float *S, const float *Pinv,
                              const float *gamma, float *lambda, uint32_t max_iter, float tol,
                              const uint32_t *status, const uint32_t *inst_list, uint32_t n_list,
-                             uint32_t *iters, uint32_t *flag, bool stair, hipStream_t st);
+                             uint32_t *iters, uint32_t *flag, float *eta, bool stair, hipStream_t st);
 // PCG with S resident on chip (N <= pcg_res_max_knots(nt)); one workgroup of
 // nt lanes (512: 128 knots, one per CU; 256: 64 knots, two per CU) per bin
 // of instances, bins bin0 .. bin0 + n_bins - 1.  items[bin_off[w] ..
@@ -131,18 +152,21 @@ hipError_t launch_pcg_stream(const Batch &b, uint32_t max_n, const float *S, con
 hipError_t launch_pcg_res(const Batch &b, const uint4 *items, const uint32_t *bin_off, uint32_t bin0, uint32_t n_bins,
                           uint32_t nt, const float *S, const float *Pinv, const float *gamma, float *lambda,
                           uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters,
-                          uint32_t *flag, bool stair, hipStream_t st);
+                          uint32_t *flag, float *eta, bool stair, hipStream_t st);
 // PCG of 128 < N <= pcg_team_max_knots(q) on a team of q = 2 or 4
 // workgroups (cooperative launches); err receives 1 if a team wait timed
 // out.  With a bin table (items, bin_off, n_bins) and bins_taken, the spare
 // workgroups of the last two-workgroup team launch solve the first
 // *bins_taken bins (the caller launches the rest with launch_pcg_res from
-// bin *bins_taken).
+// bin *bins_taken).  poison (test hook, 0 = off): part poison - 1 of every
+// team posts its exchange words under a wrong epoch, so every team ends
+// with PCG_FLAG_FAULT.
 hipError_t launch_pcg_team(const Batch &b, uint32_t q, const uint32_t *team_inst, uint32_t n_teams,
                            TeamMailbox *mb, uint32_t n_mb, const uint4 *items, const uint32_t *bin_off,
                            uint32_t n_bins, uint32_t *bins_taken, const float *S, const float *Pinv,
                            const float *gamma, float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
-                           uint32_t *iters, uint32_t *flag, uint32_t *err, bool stair, hipStream_t st);
+                           uint32_t *iters, uint32_t *flag, float *eta, uint32_t *err, bool stair, uint32_t poison,
+                           hipStream_t st);
 uint32_t pcg_team_max_knots(uint32_t q);
 uint32_t pcg_team_max_grid();
 uint32_t pcg_team_per_launch(uint32_t q);

@@ -60,7 +60,8 @@ __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restric
                                                  float tol, const uint32_t *__restrict__ status,
                                                  const uint32_t *__restrict__ inst_list,
                                                  uint32_t *__restrict__ iters_out,
-                                                 uint32_t *__restrict__ flag_out) {
+                                                 uint32_t *__restrict__ flag_out,
+                                                 float *__restrict__ eta_out) {
   constexpr int NT = NG * 16;
   constexpr int NW = NT / 64;
   extern __shared__ float smem[];
@@ -163,8 +164,10 @@ __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restric
   // tol < 0: exit relative to the first residual, |eta| < -tol |eta_0|
   const float tol_i = tol < 0.f ? fmaxf(-tol * fabsf(eta), FLT_MIN) : tol;
   bool conv = fabsf(eta) < tol_i;
+  float eta_last = eta;  // of the last exit test
+  const uint32_t cap = pcg_cap(max_iter, N);
   int slot = 1;
-  while (!conv && it < max_iter) {
+  while (!conv && it < cap) {
     smv(y);
     float part = 0.f;
 #pragma unroll
@@ -181,6 +184,7 @@ __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restric
     }
     __syncthreads();
     const float eta_new = block_sum<NW>(precond(z), red + NW * slot);
+    eta_last = eta_new;
     slot ^= 1;
     ++it;
     if (fabsf(eta_new) < tol_i) {
@@ -205,6 +209,7 @@ __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restric
   if (threadIdx.x == 0) {
     if (iters_out) iters_out[inst] = it;
     if (flag_out) flag_out[inst] = conv ? 0u : 1u;
+    if (eta_out) eta_out[inst] = fabsf(eta_last);
   }
 }
 
@@ -212,16 +217,16 @@ template <int NG, int R>
 hipError_t launch_pcg_t(const Batch &b, uint32_t max_n, const float *S, const float *Pinv,
                         const float *gamma, float *lambda, uint32_t max_iter, float tol,
                         const uint32_t *status, const uint32_t *inst_list, uint32_t n_list,
-                        uint32_t *iters, uint32_t *flag, bool stair, hipStream_t st) {
+                        uint32_t *iters, uint32_t *flag, float *eta, bool stair, hipStream_t st) {
   const size_t smem = sizeof(float) * (2 * NX * (size_t)max_n + 2 * (NG * 16 / 64));
   const uint32_t grid = inst_list ? n_list : b.n_inst;
   if (grid == 0) return hipSuccess;
   if (stair)
     hipLaunchKernelGGL((k_pcg<NG, R, true>), dim3(grid), dim3(NG * 16), smem, st, b, S, Pinv, gamma, lambda,
-                       max_iter, tol, status, inst_list, iters, flag);
+                       max_iter, tol, status, inst_list, iters, flag, eta);
   else
     hipLaunchKernelGGL((k_pcg<NG, R, false>), dim3(grid), dim3(NG * 16), smem, st, b, S, Pinv, gamma, lambda,
-                       max_iter, tol, status, inst_list, iters, flag);
+                       max_iter, tol, status, inst_list, iters, flag, eta);
   return hipGetLastError();
 }
 
@@ -230,9 +235,10 @@ hipError_t launch_pcg_t(const Batch &b, uint32_t max_n, const float *S, const fl
 hipError_t launch_pcg_stream(const Batch &b, uint32_t max_n, const float *S, const float *Pinv,
                              const float *gamma, float *lambda, uint32_t max_iter, float tol,
                              const uint32_t *status, const uint32_t *inst_list, uint32_t n_list,
-                             uint32_t *iters, uint32_t *flag, bool stair, hipStream_t st) {
+                             uint32_t *iters, uint32_t *flag, float *eta, bool stair, hipStream_t st) {
   if (b.n_inst == 0) return hipSuccess;
-#define MPC_PCG_ARGS b, max_n, S, Pinv, gamma, lambda, max_iter, tol, status, inst_list, n_list, iters, flag, stair, st
+#define MPC_PCG_ARGS \
+  b, max_n, S, Pinv, gamma, lambda, max_iter, tol, status, inst_list, n_list, iters, flag, eta, stair, st
   // 16 groups (256 lanes) up to N = 256, then 32 / 64 groups; R = rows per group.
   if (max_n <= 16) return launch_pcg_t<16, 1>(MPC_PCG_ARGS);
   if (max_n <= 32) return launch_pcg_t<16, 2>(MPC_PCG_ARGS);

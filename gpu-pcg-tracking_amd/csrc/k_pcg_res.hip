@@ -16,16 +16,17 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
                                                 const float *__restrict__ S, const float *__restrict__ Pinv,
                                                 const float *__restrict__ gamma, float *__restrict__ lambda,
                                                 uint32_t max_iter, float tol, const uint32_t *__restrict__ status,
-                                                uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out) {
+                                                uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out,
+                                                float *__restrict__ eta_out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   body<STAIR, NT>(bin0 + blockIdx.x, smem_raw, b, items, bin_off, S, Pinv, gamma, lambda, max_iter, tol, status,
-                  iters_out, flag_out);
+                  iters_out, flag_out, eta_out);
 }
 
 template <int NT>
 hipError_t launch_shape(const Batch &b, const uint4 *items, const uint32_t *bin_off, uint32_t bin0, uint32_t n_bins,
                         const float *S, const float *Pinv, const float *gamma, float *lambda, uint32_t max_iter,
-                        float tol, const uint32_t *status, uint32_t *iters, uint32_t *flag, bool stair,
+                        float tol, const uint32_t *status, uint32_t *iters, uint32_t *flag, float *eta, bool stair,
                         hipStream_t st) {
   static bool attr_set = false;
   const size_t smem = sizeof(ResShared<NT>);
@@ -40,10 +41,10 @@ hipError_t launch_shape(const Batch &b, const uint4 *items, const uint32_t *bin_
   }
   if (stair)
     hipLaunchKernelGGL((k_pcg_res<true, NT>), dim3(n_bins), dim3(NT), smem, st, b, items, bin_off, bin0, S, Pinv,
-                       gamma, lambda, max_iter, tol, status, iters, flag);
+                       gamma, lambda, max_iter, tol, status, iters, flag, eta);
   else
     hipLaunchKernelGGL((k_pcg_res<false, NT>), dim3(n_bins), dim3(NT), smem, st, b, items, bin_off, bin0, S, Pinv,
-                       gamma, lambda, max_iter, tol, status, iters, flag);
+                       gamma, lambda, max_iter, tol, status, iters, flag, eta);
   return hipGetLastError();
 }
 
@@ -62,14 +63,14 @@ uint32_t pcg_res_max_items() { return MAXI; }
 hipError_t launch_pcg_res(const Batch &b, const uint4 *items, const uint32_t *bin_off, uint32_t bin0, uint32_t n_bins,
                           uint32_t nt, const float *S, const float *Pinv, const float *gamma, float *lambda,
                           uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters,
-                          uint32_t *flag, bool stair, hipStream_t st) {
+                          uint32_t *flag, float *eta, bool stair, hipStream_t st) {
   if (n_bins == 0) return hipSuccess;
   if (nt == 512)
     return launch_shape<512>(b, items, bin_off, bin0, n_bins, S, Pinv, gamma, lambda, max_iter, tol, status, iters,
-                             flag, stair, st);
+                             flag, eta, stair, st);
   if (nt == 256)
     return launch_shape<256>(b, items, bin_off, bin0, n_bins, S, Pinv, gamma, lambda, max_iter, tol, status, iters,
-                             flag, stair, st);
+                             flag, eta, stair, st);
   return hipErrorInvalidValue;
 }
 

@@ -29,10 +29,18 @@
 // team per Q workgroups; more teams than fit at once are solved by
 // consecutive launches.  The parts of a team sit 8 workgroup ids apart so
 // they land on the same XCD and share its L2.  Mailbox words carry their
-// epoch, so relaxed (fence-free) atomics suffice.  Every wait is bounded:
-// a wave that gives up sets the error word and the workgroup's fail flag,
-// and the workgroup leaves at its next barrier with the instance's PCG flag
-// set to PCG_FLAG_FAULT; its partner then times out as well.  Under stream
+// epoch, so relaxed (fence-free) atomics suffice; four epoch buffers keep a
+// word until every reader is done with it (TeamMailbox).  Every wait is
+// bounded: a wave that gives up sets the error word and the workgroup's
+// fail flag, and the workgroup leaves at its next barrier with the
+// instance's PCG flag set to PCG_FLAG_FAULT; its partners then time out as
+// well.  The flag is raised with an atomic maximum over a value cleared
+// before the launch (0 converged < 1 max_iter < PCG_FLAG_FAULT), so a
+// fault wins over the result part 0 writes, in either order.
+// `poison` (host test hook; 0 = off) makes part poison - 1 of every team
+// post its words under a wrong epoch tag: its partners' waits then run out
+// and the whole team ends with PCG_FLAG_FAULT, every workgroup exiting
+// normally.  Under stream
 // capture (HIP graphs) the launches are ordinary kernel launches of the same
 // grid, which fits the device at once (at most one workgroup per CU); the
 // bounded waits turn a partner that is not co-resident into that flag
@@ -86,8 +94,10 @@ __device__ __forceinline__ float sum_lanes8(float x, uint32_t l0) {
   return s;
 }
 
-__device__ __forceinline__ void mb_post(TeamMailbox *mb, uint32_t part, uint32_t e, uint32_t word, float v) {
-  __hip_atomic_store(&mb->w[e & 1u][part][word], ((unsigned long long)e << 32) | __float_as_uint(v),
+// tag_xor: 0, or a bit pattern that makes the tag wrong (the poison hook)
+__device__ __forceinline__ void mb_post(TeamMailbox *mb, uint32_t part, uint32_t e, uint32_t word, float v,
+                                        uint32_t tag_xor) {
+  __hip_atomic_store(&mb->w[e & 3u][part][word], ((unsigned long long)(e ^ tag_xor) << 32) | __float_as_uint(v),
                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -127,7 +137,8 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
                                           const float *__restrict__ Pinv, const float *__restrict__ gamma,
                                           float *__restrict__ lambda, uint32_t max_iter, float tol,
                                           const uint32_t *__restrict__ status, uint32_t *__restrict__ iters_out,
-                                          uint32_t *__restrict__ flag_out, uint32_t *__restrict__ err) {
+                                          uint32_t *__restrict__ flag_out, float *__restrict__ eta_out,
+                                          uint32_t *__restrict__ err, uint32_t poison) {
   TeamShared &sh = *reinterpret_cast<TeamShared *>(smem);
   const uint32_t tid = threadIdx.x, g = tid >> 4, i = tid & 15u, w = tid >> 6, lane64 = tid & 63u;
   const uint32_t d = ror_step();
@@ -140,11 +151,13 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
   const uint32_t inst = team_inst[team];
   if (status && !(status[inst] & ST_ACTIVE)) return;  // same decision on every part
   TeamMailbox *mb = mboxes + slot;
+  const uint32_t tag_xor = poison == part + 1u ? 0x80000000u : 0u;
   const bool lane_ok = i < (uint32_t)NX;
   const uint32_t N = inst_len(b, inst), K0 = inst_first(b, inst);
   const uint32_t chunk = (((N + Q - 1) / Q) + 3u) & ~3u;  // <= 128 for N <= 128 Q
   const uint32_t kb = part * chunk, ke = kb + chunk < N ? kb + chunk : N;  // own knots [kb, ke)
   const uint32_t n_own = ke - kb;  // > 0: the host gives Q = 4 teams N > 256
+  const uint32_t cap = pcg_cap(max_iter, N);
   const bool has_prev = part > 0, has_next = ke < N;
 
   {
@@ -187,12 +200,12 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
   // a wait timed out somewhere in the workgroup: the solve is abandoned and
   // the instance flagged (every part writes the same value)
   auto fault = [&]() {
-    if (tid == 0 && flag_out) flag_out[inst] = PCG_FLAG_FAULT;
+    if (tid == 0 && flag_out) atomicMax(flag_out + inst, PCG_FLAG_FAULT);
   };
   // own first / last element to LDS, the rows at the splits to the mailbox
   auto post_rows = [&](const float (&v)[RK]) {
-    if (first_lane) mb_post(mb, part, e, MB_FIRST + i, v[0]);
-    if (last_lane) mb_post(mb, part, e, MB_LAST + i, v[RK - 1]);
+    if (first_lane) mb_post(mb, part, e, MB_FIRST + i, v[0], tag_xor);
+    if (last_lane) mb_post(mb, part, e, MB_LAST + i, v[RK - 1], tag_xor);
   };
   auto stage = [&](const float (&v)[RK]) {
     zb0[0] = v[0];
@@ -205,7 +218,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
   // each polling lane's role and mailbox word (within one epoch's parity):
   // 1 = a partial of another part, 2 = the row before the part (wave 0),
   // 3 = the row after it (wave wl)
-  constexpr uint32_t PART_WORDS = sizeof(TeamMailbox) / sizeof(unsigned long long) / 8u;
+  constexpr uint32_t PART_WORDS = sizeof(TeamMailbox{}.w[0][0]) / sizeof(unsigned long long);  // 40
   uint32_t role = 0, word = 0;
   if (lane64 < 8u * (Q - 1)) {
     const uint32_t q = lane64 >> 3;
@@ -223,7 +236,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     const bool want = (role == 1u && partials) || (role == 2u && rows) || (role == 3u && rows);
     float x = 0.f;
     if (partials || rm || rp) {
-      x = mb_wait(&mb->w[e & 1u][0][0] + word, want, e, ok);
+      x = mb_wait(&mb->w[e & 3u][0][0] + word, want, e, ok);
       if (want && role == 2u) sh.recv[0][lane64 - LANE_M] = x;
       if (want && role == 3u) sh.recv[1][lane64 - LANE_P] = x;
       if (rm || rp) wave_sync();
@@ -300,7 +313,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
   ++e;
   if (lane64 == 0) {
     sh.red[1][w] = part_sum;
-    mb_post(mb, part, e, w, part_sum);
+    mb_post(mb, part, e, w, part_sum, tag_xor);
   }
   stage(z);
   __syncthreads();
@@ -313,8 +326,9 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
   // tol < 0: exit relative to the first residual, |eta| < -tol |eta_0|
   const float tol_i = tol < 0.f ? fmaxf(-tol * fabsf(eta), FLT_MIN) : tol;
   bool conv = fabsf(eta) < tol_i;
+  float eta_last = eta;  // of the last exit test
 
-  while (!conv && it < max_iter) {
+  while (!conv && it < cap) {
     smv(p, pm, pp);
     float pyp = 0.f;
 #pragma unroll
@@ -323,7 +337,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     ++e;
     if (lane64 == 0) {
       sh.red[0][w] = pyp;
-      mb_post(mb, part, e, w, pyp);
+      mb_post(mb, part, e, w, pyp, tag_xor);
     }
     __syncthreads();
     if (sh.fail) return fault();
@@ -358,13 +372,14 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     ++e;
     if (lane64 == 0) {
       sh.red[1][w] = rz;
-      mb_post(mb, part, e, w, rz);
+      mb_post(mb, part, e, w, rz, tag_xor);
     }
     stage(z);
     __syncthreads();
     if (sh.fail) return fault();
     float zm = own_m ? zb0[-16] : 0.f, zp = own_p ? zb0[RK * 16] : 0.f;  // issued before the wait
     const float eta_new = total(sum8(sh.red[1]), collect(true, true));
+    eta_last = eta_new;
     if (far_m && lane_ok) zm = sh.recv[0][i];
     if (far_p && lane_ok) zp = sh.recv[1][i];
     ++it;
@@ -388,7 +403,8 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
   }
   if (part == 0 && tid == 0) {
     if (iters_out) iters_out[inst] = it;
-    if (flag_out) flag_out[inst] = conv ? 0u : 1u;
+    if (eta_out) eta_out[inst] = fabsf(eta_last);
+    if (flag_out) atomicMax(flag_out + inst, conv ? 0u : 1u);
   }
 }
 
@@ -399,10 +415,11 @@ __global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__rest
                                                  const float *__restrict__ gamma, float *__restrict__ lambda,
                                                  uint32_t max_iter, float tol, const uint32_t *__restrict__ status,
                                                  uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out,
-                                                 uint32_t *__restrict__ err) {
+                                                 float *__restrict__ eta_out, uint32_t *__restrict__ err,
+                                                 uint32_t poison) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   team_body<STAIR, Q>(blockIdx.x, smem_raw, b, team_inst, n_teams, team0, mboxes, S, Pinv, gamma, lambda, max_iter,
-                      tol, status, iters_out, flag_out, err);
+                      tol, status, iters_out, flag_out, eta_out, err, poison);
 }
 
 // The last, partly filled launch of two-workgroup teams with resident bins
@@ -418,20 +435,30 @@ __global__ void __launch_bounds__(NT) k_pcg_mixed(Batch b, const uint32_t *__res
                                                   const float *__restrict__ gamma, float *__restrict__ lambda,
                                                   uint32_t max_iter, float tol, const uint32_t *__restrict__ status,
                                                   uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out,
-                                                  uint32_t *__restrict__ err) {
+                                                  float *__restrict__ eta_out, uint32_t *__restrict__ err,
+                                                  uint32_t poison) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   if (blockIdx.x < team_grid)
     team_body<STAIR, 2>(blockIdx.x, smem_raw, b, team_inst, n_teams, team0, mboxes, S, Pinv, gamma, lambda,
-                        max_iter, tol, status, iters_out, flag_out, err);
+                        max_iter, tol, status, iters_out, flag_out, eta_out, err, poison);
   else
     resident::body<STAIR, 512>(bin0 + (blockIdx.x - team_grid), smem_raw, b, items, bin_off, S, Pinv, gamma, lambda,
-                          max_iter, tol, status, iters_out, flag_out);
+                               max_iter, tol, status, iters_out, flag_out, eta_out);
 }
 
-__global__ void k_reset_mailboxes(TeamMailbox *mb, uint32_t n) {
+// Before a team launch: its n mailboxes cleared, and the PCG flag of every
+// active instance among its teams team_inst[team0 .. team0 + n_launch)
+// set to 0, the floor of the atomic maximum the team raises it to.
+__global__ void k_reset_mailboxes(TeamMailbox *mb, uint32_t n, const uint32_t *__restrict__ team_inst,
+                                  uint32_t team0, uint32_t n_launch, const uint32_t *__restrict__ status,
+                                  uint32_t *__restrict__ flag_out) {
   constexpr uint32_t W = sizeof(TeamMailbox) / sizeof(unsigned long long);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n * W) (&mb[0].w[0][0][0])[i] = 0ull;
+  if (flag_out && i < n_launch) {
+    const uint32_t inst = team_inst[team0 + i];
+    if (!status || (status[inst] & ST_ACTIVE)) flag_out[inst] = 0u;
+  }
 }
 
 }  // namespace
@@ -495,7 +522,8 @@ hipError_t launch_pcg_team(const Batch &b, uint32_t q, const uint32_t *team_inst
                            TeamMailbox *mb, uint32_t n_mb, const uint4 *items, const uint32_t *bin_off,
                            uint32_t n_bins, uint32_t *bins_taken, const float *S, const float *Pinv,
                            const float *gamma, float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
-                           uint32_t *iters, uint32_t *flag, uint32_t *err, bool stair, hipStream_t st) {
+                           uint32_t *iters, uint32_t *flag, float *eta, uint32_t *err, bool stair, uint32_t poison,
+                           hipStream_t st) {
   if (bins_taken) *bins_taken = 0;
   if (n_teams == 0) return hipSuccess;
   if (q != 2 && q != 4) return hipErrorInvalidValue;
@@ -518,7 +546,8 @@ hipError_t launch_pcg_team(const Batch &b, uint32_t q, const uint32_t *team_inst
     const uint32_t n = n_teams - team0 < per_launch ? n_teams - team0 : per_launch;
     const uint32_t grid = 8u * q * ((n + 7u) / 8u);  // teams in whole groups of 8
     if (grid / q > n_mb) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_reset_mailboxes, dim3((grid / q * W + 255) / 256), dim3(256), 0, st, mb, grid / q);
+    hipLaunchKernelGGL(k_reset_mailboxes, dim3((grid / q * W + 255) / 256), dim3(256), 0, st, mb, grid / q, team_inst,
+                       team0, n, status, flag);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     Batch bb = b;
@@ -531,7 +560,7 @@ hipError_t launch_pcg_team(const Batch &b, uint32_t q, const uint32_t *team_inst
     }
     if (nb == 0) {
       void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, (void *)&S, (void *)&Pinv, (void *)&gamma,
-                      &lambda, &max_iter, &tol, (void *)&status, &iters, &flag, &err};
+                      &lambda, &max_iter, &tol, (void *)&status, &iters, &flag, &eta, &err, &poison};
       const void *fn = q == 4 ? (stair ? (const void *)k_pcg_team<true, 4> : (const void *)k_pcg_team<false, 4>)
                               : (stair ? (const void *)k_pcg_team<true, 2> : (const void *)k_pcg_team<false, 2>);
       e = launch(fn, grid, args, sizeof(TeamShared));
@@ -539,7 +568,7 @@ hipError_t launch_pcg_team(const Batch &b, uint32_t q, const uint32_t *team_inst
       uint32_t tg = grid, bin0 = 0;
       void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, &tg, (void *)&items, (void *)&bin_off, &bin0,
                       (void *)&S, (void *)&Pinv, (void *)&gamma, &lambda, &max_iter, &tol, (void *)&status, &iters,
-                      &flag, &err};
+                      &flag, &eta, &err, &poison};
       const void *fn = stair ? (const void *)k_pcg_mixed<true> : (const void *)k_pcg_mixed<false>;
       e = launch(fn, grid + nb, args, MIXED_SMEM);
       if (e == hipSuccess) *bins_taken = nb;

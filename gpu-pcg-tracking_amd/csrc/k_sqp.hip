@@ -84,6 +84,7 @@ __global__ void __launch_bounds__(NT) k_batch_init(Batch b, const float *__restr
     s.drho[inst] = 1.f;
     s.status[inst] = ST_ACTIVE;
     s.sqp_done[inst] = 0;
+    if (inst == 0 && s.t0) *s.t0 = (unsigned long long)wall_clock64();
   }
 }
 
@@ -106,6 +107,17 @@ __global__ void __launch_bounds__(NT) k_ls_update(Batch b, Params p, const float
     const uint32_t pf = pcg_flag_now ? pcg_flag_now[inst] : 0u;
     if (pf == PCG_FLAG_FAULT) st |= ST_PCG_FAULT;
     else if (pf) st |= ST_PCG_MAXITER;
+    // the time budget, checked where the reference checks it before its
+    // line-search decision (toplevel.cuh:381): past it, the iteration ends
+    // without its step and is not counted
+    if (p.budget_ticks && s.t0 && (unsigned long long)wall_clock64() - *s.t0 > p.budget_ticks)
+      st = (st | ST_TIME_BUDGET) & ~ST_ACTIVE;
+    s.status[inst] = st;
+  }
+  __syncthreads();
+  if (!(s.status[inst] & ST_ACTIVE)) return;  // uniform: written by thread 0 before the barrier
+  if (threadIdx.x == 0) {
+    uint32_t st = s.status[inst];
     const float m0 = s.merit0[inst];
     float minm = m0;
     int step = -1;

@@ -43,6 +43,9 @@ struct mpc_handle {
   float *xs = nullptr;           // [14 B]
   float *drho = nullptr, *merit0 = nullptr, *rho1 = nullptr;
   uint32_t *status_ws = nullptr, *sqp_done = nullptr, *pcg_iters = nullptr, *pcg_flag = nullptr;
+  float *pcg_eta = nullptr;  // [B] |r^T Pinv r| at each instance's last PCG exit test
+  unsigned long long *t0 = nullptr;  // [1] device clock at the start of a batched solve
+  double clock_ticks_per_us = 0.0;   // device wall clock rate
   uint32_t *knot_inst = nullptr, *inst_k0 = nullptr;
   // PCG dispatch of a batch: bins of small instances (resident kernel) and
   // the list of instances beyond the teams (streaming kernel)
@@ -78,6 +81,7 @@ struct mpc_handle {
     }
   };
   uint32_t team_launches = 0;  // cooperative team launches per PCG phase
+  uint32_t fault_inject = 0;   // mpc_set_fault_injection (test hook)
   bool graph_on = false;
   uint64_t graph_epoch = 1;
   GraphKey graph_key;
@@ -122,10 +126,19 @@ static Params make_params(const mpc_handle *h) {
   p.pcg_tol = c.pcg_exit_tol;
   p.pcg_max_iter = c.pcg_max_iter;
   p.flags = c.flags;
+  p.budget_ticks = c.sqp_max_time_us > 0.f
+                       ? (unsigned long long)std::ceil((double)c.sqp_max_time_us * h->clock_ticks_per_us)
+                       : 0ull;
   return p;
 }
 
 static Batch single(uint32_t N) { return Batch{nullptr, nullptr, 1u, N}; }
+
+// PCG cap of the SQP solves: the configured one, or with
+// MPC_FLAG_PCG_CAP_BY_N the reference experiments' cap per horizon
+static uint32_t sqp_pcg_cap(const mpc_config &c) {
+  return (c.flags & FLAG_PCG_CAP_BY_N) ? PCG_CAP_BY_N : c.pcg_max_iter;
+}
 
 
 
@@ -142,14 +155,15 @@ static hipError_t pcg_single(mpc_handle *h, uint32_t N, const float *S, const fl
   if (!(flags & FLAG_PCG_STREAM)) {
     if (N <= pcg_res_max_knots(512))
       return launch_pcg_res(single(N), nullptr, nullptr, 0, 1, 512, S, P, gamma, lambda, max_iter, tol, nullptr,
-                            iters, flag, stair, st);
+                            iters, flag, h->pcg_eta, stair, st);
     for (uint32_t q : {2u, 4u})
       if (N <= pcg_team_max_knots(q) && pcg_team_per_launch(q) > 0)
         return launch_pcg_team(single(N), q, h->team_one, 1, h->mbox, h->n_mbox, nullptr, nullptr, 0, nullptr, S, P,
-                               gamma, lambda, max_iter, tol, nullptr, iters, flag, h->team_err, stair, st);
+                               gamma, lambda, max_iter, tol, nullptr, iters, flag, h->pcg_eta, h->team_err, stair,
+                               h->fault_inject, st);
   }
   return launch_pcg_stream(single(N), N, S, P, gamma, lambda, max_iter, tol, nullptr, nullptr, 0, iters, flag,
-                           stair, st);
+                           h->pcg_eta, stair, st);
 }
 
 static Batch batch_of(const mpc_handle *h) {
@@ -200,7 +214,8 @@ static hipError_t refine_kkt(mpc_handle *h, const float *d_rho, const float *g, 
 static bool valid_numbers(const mpc_config &c) {
   return c.dt > 0 && std::isfinite(c.dt) && c.q_ee >= 0 && c.qd_cost >= 0 && c.r_cost >= 0 &&
          c.mu >= 0 && c.rho_factor > 1 && c.rho_max > 0 && c.rho_min > 0 && c.pcg_exit_tol >= 0 &&
-         c.sqp_max_iter <= MPC_MAX_SQP_ITERS && c.direct_refine <= 16 && c.direct_kkt_refine <= 16;
+         c.sqp_max_iter <= MPC_MAX_SQP_ITERS && c.direct_refine <= 16 && c.direct_kkt_refine <= 16 &&
+         c.sqp_max_time_us >= 0 && std::isfinite(c.sqp_max_time_us);
 }
 
 extern "C" {
@@ -243,7 +258,7 @@ const char *mpc_status_string(int s) {
 static void free_all(mpc_handle *h) {
   void *ptrs[] = {h->G, h->Ginv, h->C, h->g, h->c, h->S, h->P, h->gamma, h->dz, h->knot_merits,
                   h->merits, h->xs, h->drho, h->merit0, h->rho1, h->status_ws, h->sqp_done,
-                  h->pcg_iters, h->pcg_flag, h->knot_inst, h->inst_k0, h->bin_items,
+                  h->pcg_iters, h->pcg_flag, h->pcg_eta, h->t0, h->knot_inst, h->inst_k0, h->bin_items,
                   h->bin_off, h->big_list, h->team_list, h->team_one, h->team_err, h->mbox, h->dir_order,
                   h->adj_mu, h->adj_knot, h->loss_knot, h->ref_g, h->ref_c, h->ref_dz, h->ref_lam,
                   h->ref_scale, h->cl_win, h->cl_xp, h->cl_rho, h->cl_ctr, h->cl_status, h->cl_pcg,
@@ -272,10 +287,14 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
   if (!h) return MPC_ERR_ARG;
   h->cfg = *cfg;
   weights_from_config(h);
-  if (hipGetDevice(&h->device) != hipSuccess) {
+  int clock_khz = 0;
+  if (hipGetDevice(&h->device) != hipSuccess ||
+      hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, h->device) != hipSuccess ||
+      clock_khz <= 0) {
     delete h;
     return MPC_ERR_HIP;
   }
+  h->clock_ticks_per_us = clock_khz * 1e-3;
   const size_t T = cfg->max_total_knots, B = cfg->max_instances;
   auto alloc = [&](void **p, size_t bytes) { return hipMalloc(p, bytes) == hipSuccess; };
   bool ok = alloc((void **)&h->G, sizeof(float) * G_STRIDE * T) &&
@@ -297,6 +316,8 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
             alloc((void **)&h->sqp_done, sizeof(uint32_t) * B) &&
             alloc((void **)&h->pcg_iters, sizeof(uint32_t) * B) &&
             alloc((void **)&h->pcg_flag, sizeof(uint32_t) * B) &&
+            alloc((void **)&h->pcg_eta, sizeof(float) * B) &&
+            alloc((void **)&h->t0, sizeof(unsigned long long)) &&
             alloc((void **)&h->knot_inst, sizeof(uint32_t) * T) &&
             alloc((void **)&h->inst_k0, sizeof(uint32_t) * (B + 1)) &&
             alloc((void **)&h->bin_items, sizeof(uint4) * B) &&
@@ -373,6 +394,13 @@ int mpc_set_cost_weights(mpc_handle *h, const float *wx14, const float *wu7) {
   std::memcpy(h->wx, wx14, sizeof(h->wx));
   std::memcpy(h->wu, wu7, sizeof(h->wu));
   ++h->graph_epoch;
+  return MPC_OK;
+}
+
+int mpc_set_fault_injection(mpc_handle *h, uint32_t part_plus_one) {
+  if (!h || part_plus_one > 4) return MPC_ERR_ARG;
+  h->fault_inject = part_plus_one;
+  ++h->graph_epoch;  // captured team launches carry the value
   return MPC_OK;
 }
 
@@ -511,6 +539,13 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
   std::memset(&s, 0, sizeof(s));
   const auto t0 = std::chrono::steady_clock::now();
   const uint32_t len = NXU * N - NU;
+  // sqpTimecheck (toplevel.cuh:223-231): the host clock since the start
+  // against SQP_MAX_TIME_US
+  auto over_budget = [&]() {
+    return cfg.sqp_max_time_us > 0.f &&
+           std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() >
+               (double)cfg.sqp_max_time_us;
+  };
 
   HIP_TRY(hipMemcpyAsync(h->xs, d_xu, sizeof(float) * NX, hipMemcpyDeviceToDevice, st));
   HIP_TRY(launch_merit(b, prm, d_xu, nullptr, d_ee_ref, h->xs, nullptr, 8, 1, h->knot_merits, st));
@@ -518,16 +553,23 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
   HIP_TRY(hipMemcpyAsync(h->h_merits, h->merits, sizeof(float), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   float merit0 = h->h_merits[0];
+  if (!std::isfinite(merit0)) {  // nothing a line search could improve on
+    s.sqp_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+    return MPC_ERR_NONFINITE;
+  }
   float drho = 1.f;
   const float rf = cfg.rho_factor;
   uint32_t sqp_iter = 0;
   bool not_aborted = true;
+  bool time_exit = false;
   for (uint32_t it = 0; it < cfg.sqp_max_iter; ++it) {
     HIP_TRY(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, nullptr, h->G, h->C, h->g, h->c, st));
+    if ((time_exit = over_budget())) break;  // toplevel.cuh:271
     HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, h->c, nullptr, *rho, nullptr, h->S, h->P, h->gamma,
                               h->Ginv, 0, st));
+    if ((time_exit = over_budget())) break;  // toplevel.cuh:283
     HIP_TRY(hipEventRecord(h->ev[0], st));
-    HIP_TRY(pcg_single(h, N, h->S, h->P, h->gamma, d_lambda, cfg.pcg_max_iter, cfg.pcg_exit_tol, h->pcg_iters,
+    HIP_TRY(pcg_single(h, N, h->S, h->P, h->gamma, d_lambda, sqp_pcg_cap(cfg), cfg.pcg_exit_tol, h->pcg_iters,
                        h->pcg_flag, st));
     HIP_TRY(hipEventRecord(h->ev[1], st));
     HIP_TRY(hipMemcpyAsync(h->h_pcg, h->pcg_iters, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -553,6 +595,7 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
     s.pcg_iters[it] = h->h_pcg[0];
     s.pcg_maxiter[it] = h->h_pcg[1];
     s.linsys_us[it] = 1e3 * ms;
+    if ((time_exit = over_budget())) break;  // toplevel.cuh:381, before the line-search decision
     int step = -1;
     float minm = merit0;
     for (int i = 0; i < 8; ++i)
@@ -561,6 +604,12 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
         step = i;
       }
     s.ls_step[it] = step;
+    if (step < 0 && !std::isfinite(h->h_merits[0])) {  // the step itself is not finite: stop, not taken
+      s.sqp_iters = sqp_iter + 1;
+      s.not_aborted = 1u;
+      s.sqp_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+      return MPC_ERR_NONFINITE;
+    }
     if (step < 0) {
       drho = std::fmax(drho * rf, rf);
       *rho = std::fmax(*rho * drho, cfg.rho_min);
@@ -578,10 +627,12 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
     HIP_TRY(launch_axpy(len, alpha, h->dz, d_xu, st));
     sqp_iter++;
     merit0 = minm;
+    if ((time_exit = over_budget())) break;  // toplevel.cuh:437
   }
   HIP_TRY(hipStreamSynchronize(st));
   s.sqp_iters = sqp_iter;
   s.not_aborted = not_aborted ? 1u : 0u;
+  s.time_exit = time_exit ? 1u : 0u;
   s.sqp_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
   return MPC_OK;
 }
@@ -908,6 +959,25 @@ int mpc_batch_set_timing(mpc_handle *h, int enable) {
   return MPC_OK;
 }
 
+int mpc_batch_pcg_exit(mpc_handle *h, uint32_t *d_iters, uint32_t *d_flags, float *d_eta, mpc_stream_t stream) {
+  if (!h) return MPC_ERR_ARG;
+  if (!h->batch_ready) return MPC_ERR_NOT_READY;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t n = h->n_inst;
+  if (d_iters) HIP_TRY(hipMemcpyAsync(d_iters, h->pcg_iters, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+  if (d_flags) HIP_TRY(hipMemcpyAsync(d_flags, h->pcg_flag, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+  if (d_eta) HIP_TRY(hipMemcpyAsync(d_eta, h->pcg_eta, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  return MPC_OK;
+}
+
+int mpc_batch_direct_resolve(mpc_handle *h, float *d_lambda_out, uint32_t refine_steps, mpc_stream_t stream) {
+  if (!h || !d_lambda_out || refine_steps > 16) return MPC_ERR_ARG;
+  if (!h->batch_ready) return MPC_ERR_NOT_READY;
+  HIP_TRY(launch_direct_solve(batch_of(h), h->dir_order, h->n_inst, h->S, h->gamma, nullptr, h->P, d_lambda_out,
+                              refine_steps, true, nullptr, nullptr, (hipStream_t)stream));
+  return MPC_OK;
+}
+
 int mpc_batch_phase_times(mpc_handle *h, double *us7) {
   if (!h || !us7) return MPC_ERR_ARG;
   for (int i = 0; i < 7; ++i) us7[i] = h->phase_us[i];
@@ -933,28 +1003,28 @@ static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_st
   const bool stair = h->cfg.flags & FLAG_PCG_STAIR;
   if (h->cfg.flags & FLAG_PCG_STREAM) {
     e = launch_pcg_stream(b, h->max_n, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, nullptr, 0,
-                          h->pcg_iters, h->pcg_flag, stair, st);
+                          h->pcg_iters, h->pcg_flag, h->pcg_eta, stair, st);
     if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
     return e;
   }
   uint32_t taken = 0;
   e = launch_pcg_team(b, 4, h->team_list + h->n_teams, h->n_quads, h->mbox, h->n_mbox, nullptr, nullptr, 0, nullptr,
                       h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, h->pcg_iters, h->pcg_flag,
-                      h->team_err, stair, st);
+                      h->pcg_eta, h->team_err, stair, h->fault_inject, st);
   if (e == hipSuccess)
     e = launch_pcg_team(b, 2, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->plan.mixed ? h->bin_items : nullptr,
                       h->bin_off, h->n_bins, &taken, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status,
-                      h->pcg_iters, h->pcg_flag, h->team_err, stair, st);
+                      h->pcg_iters, h->pcg_flag, h->pcg_eta, h->team_err, stair, h->fault_inject, st);
   if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
   if (e == hipSuccess)
     e = launch_pcg_stream(b, h->max_big_n, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, h->big_list,
-                          h->n_big, h->pcg_iters, h->pcg_flag, stair, st);
+                          h->n_big, h->pcg_iters, h->pcg_flag, h->pcg_eta, stair, st);
   if (e == hipSuccess)
     e = launch_pcg_res(b, h->bin_items, h->bin_off, taken, h->n_bins - taken, 512, h->S, h->P, h->gamma, d_lambda,
-                       max_iter, tol, d_status, h->pcg_iters, h->pcg_flag, stair, st);
+                       max_iter, tol, d_status, h->pcg_iters, h->pcg_flag, h->pcg_eta, stair, st);
   if (e == hipSuccess)
     e = launch_pcg_res(b, h->bin_items, h->bin_off, h->n_bins, h->n_bins_small, 256, h->S, h->P, h->gamma,
-                       d_lambda, max_iter, tol, d_status, h->pcg_iters, h->pcg_flag, stair, st);
+                       d_lambda, max_iter, tol, d_status, h->pcg_iters, h->pcg_flag, h->pcg_eta, stair, st);
   return e;
 }
 
@@ -965,7 +1035,7 @@ static hipError_t batch_sqp_run(mpc_handle *h, float *d_xu, float *d_lambda, con
                                 uint32_t sqp_iters, uint32_t *d_status, uint32_t *d_pcg_iters, hipStream_t st) {
   const Batch b = batch_of(h);
   const Params prm = make_params(h);
-  InstState s{d_rho, h->drho, h->merit0, d_status, h->sqp_done, h->xs};
+  InstState s{d_rho, h->drho, h->merit0, d_status, h->sqp_done, h->xs, h->t0};
   const bool tm = h->timing;
   hipError_t e;
 #define MPC_RUN(expr)           \
@@ -982,7 +1052,7 @@ static hipError_t batch_sqp_run(mpc_handle *h, float *d_xu, float *d_lambda, con
     MPC_RUN(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma, h->Ginv, 0,
                               st));
     if (tm) MPC_RUN(hipEventRecord(h->ev[2], st));
-    MPC_RUN(pcg_batch(h, d_lambda, d_status, h->cfg.pcg_max_iter, h->cfg.pcg_exit_tol, tm ? h->ev[3] : nullptr, st));
+    MPC_RUN(pcg_batch(h, d_lambda, d_status, sqp_pcg_cap(h->cfg), h->cfg.pcg_exit_tol, tm ? h->ev[3] : nullptr, st));
     if (tm) MPC_RUN(hipEventRecord(h->ev[4], st));
     MPC_RUN(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
     if (h->cfg.flags & FLAG_DIRECT_SOLVE)

@@ -70,7 +70,7 @@ __device__ __forceinline__ void body(uint32_t bin, unsigned char *smem, Batch b,
                                      const float *__restrict__ Pinv, const float *__restrict__ gamma,
                                      float *__restrict__ lambda, uint32_t max_iter, float tol,
                                      const uint32_t *__restrict__ status, uint32_t *__restrict__ iters_out,
-                                     uint32_t *__restrict__ flag_out) {
+                                     uint32_t *__restrict__ flag_out, float *__restrict__ eta_out) {
   constexpr int NGRP = ResShape<NT>::NGRP, NWAV = ResShape<NT>::NWAV, KNOTS = ResShape<NT>::KNOTS;
   ResShared<NT> &sh = *reinterpret_cast<ResShared<NT> *>(smem);
   const uint32_t tid = threadIdx.x, g = tid >> 4, i = tid & 15u, w = tid >> 6;
@@ -104,6 +104,7 @@ __device__ __forceinline__ void body(uint32_t bin, unsigned char *smem, Batch b,
   const uint32_t N = mine ? inst_len(b, inst) : 0u;
   const uint32_t K0 = mine ? inst_first(b, inst) : 0u;
   const bool live = mine && !(status && !(status[inst] & ST_ACTIVE));
+  const uint32_t cap = pcg_cap(max_iter, N);
   const uint32_t lg = g - g0, w0 = g0 >> 2, nw = ng >> 2, lbase = g0 * RK;
   const bool lane_ok = i < (uint32_t)NX;
   __syncthreads();  // LDS cleared
@@ -184,9 +185,10 @@ __device__ __forceinline__ void body(uint32_t bin, unsigned char *smem, Batch b,
   // tol < 0: exit relative to the first residual, |eta| < -tol |eta_0|
   const float tol_i = tol < 0.f ? fmaxf(-tol * fabsf(eta), FLT_MIN) : tol;
   bool conv = !live || fabsf(eta) < tol_i;
+  float eta_last = eta;  // of the last exit test
 
   for (;;) {
-    const bool run = live && !conv && it < max_iter;  // uniform over the instance's waves
+    const bool run = live && !conv && it < cap;  // uniform over the instance's waves
     float pyp = 0.f;
     if (run) {
       smv(p, pm, pp);
@@ -244,6 +246,7 @@ __device__ __forceinline__ void body(uint32_t bin, unsigned char *smem, Batch b,
       float zm, zp;
       neighbours(zm, zp);
       const float eta_new = seg_sum(sh.red[1], w0, nw);
+      eta_last = eta_new;
       ++it;
       if (fabsf(eta_new) < tol_i) {
         conv = true;
@@ -265,6 +268,7 @@ __device__ __forceinline__ void body(uint32_t bin, unsigned char *smem, Batch b,
   if (lg == 0 && i == 0) {
     if (iters_out) iters_out[inst] = it;
     if (flag_out) flag_out[inst] = conv ? 0u : 1u;
+    if (eta_out) eta_out[inst] = fabsf(eta_last);
   }
 }
 

@@ -19,9 +19,11 @@
  *   dz      21N-7
  *
  * All device pointers are hipMalloc'd device memory owned by the caller.
- * All entry points are stream-ordered on the given stream (NULL = default
- * stream); none of them synchronises the device.  A handle is used by one
- * host thread at a time.  Errors are returned as MPC_* status codes; no
+ * The compute entry points are stream-ordered on the given stream (NULL =
+ * default stream) and none of them synchronises the device; the set-up
+ * calls without a stream argument (mpc_create, mpc_destroy, mpc_batch_setup,
+ * mpc_batch_set_plan) copy and synchronise like hipMemcpy.  A handle is
+ * used by one host thread at a time.  Errors are returned as MPC_* status codes; no
  * entry point exits the process.
  */
 #ifndef MPCPCG_H
@@ -52,7 +54,8 @@ enum {
   MPC_ST_PCG_MAXITER = 2,  /* some PCG solve hit max_iter */
   MPC_ST_RHO_ABORT = 4,    /* line search failed with rho > rho_max (toplevel.cuh:396-407) */
   MPC_ST_NONFINITE = 8,    /* a merit or step was not finite */
-  MPC_ST_PCG_FAULT = 16    /* a PCG team exchange timed out: the instance was stopped, its step not taken */
+  MPC_ST_PCG_FAULT = 16,   /* a PCG team exchange timed out: the instance was stopped, its step not taken */
+  MPC_ST_TIME_BUDGET = 32  /* the SQP time budget ran out: the instance was stopped, its last step not taken */
 };
 
 /* Runtime configuration.  Reference equivalents: settings.cuh and the -D
@@ -81,6 +84,15 @@ typedef struct mpc_config {
    * with two to ~1e-6. */
   uint32_t direct_refine;
   uint32_t direct_kkt_refine;
+  /* SQP_MAX_TIME_US (perform_experiment.sh:45, sqpTimecheck at
+   * toplevel.cuh:223-231): wall-clock budget of one solve in microseconds,
+   * 0 = none.  The plain driver checks its host clock where the reference
+   * does (after the linearisation and Schur launches, and after the
+   * line-search merits: the iteration then ends without its step); the
+   * batched solve checks the device clock against the solve's start at
+   * each line search, and an instance past the budget stops there with
+   * MPC_ST_TIME_BUDGET without taking that step. */
+  float sqp_max_time_us;
 } mpc_config;
 
 /* The terminal knot's cost uses x_{N-1} instead of the reference's x_{N-2}
@@ -101,6 +113,11 @@ typedef struct mpc_config {
  * iteration count on the block-tridiagonal Schur system for one more block
  * product (and one more neighbour exchange) per iteration. */
 #define MPC_FLAG_PCG_STAIR 8u
+/* The SQP solves' PCG cap per horizon as in the reference's experiments
+ * (perform_experiment.sh:14-19: 173 at N = 32, 167 at N = 64 and 128, 118 at
+ * N = 256, 67 at N = 512), taken for the tabulated horizon at or above N,
+ * instead of pcg_max_iter for all horizons. */
+#define MPC_FLAG_PCG_CAP_BY_N 16u
 
 typedef struct mpc_handle mpc_handle;
 
@@ -117,6 +134,8 @@ typedef struct mpc_sqp_stats {
   uint32_t pcg_maxiter[MPC_MAX_SQP_ITERS];
   double linsys_us[MPC_MAX_SQP_ITERS];
   int32_t ls_step[MPC_MAX_SQP_ITERS];   /* accepted alpha index, -1 = failure */
+  uint32_t time_exit;                   /* 1: the solve ended on sqp_max_time_us */
+  uint32_t reserved;
 } mpc_sqp_stats;
 
 /* Fill *cfg with the defaults of the published workload: dt = 0.01,
@@ -220,7 +239,11 @@ int mpc_line_search_merits(mpc_handle *h, uint32_t N, const float *d_xs, const f
 /* One SQP solve of one instance; drop-in for sqpSolve (toplevel.cuh:24-503).
  * d_xu (in/out; x_0 is the current state), d_lambda (in/out warm start),
  * *rho (in/out).  Host-synchronising once per SQP iteration, like the
- * reference, to report per-iteration statistics. */
+ * reference, to report per-iteration statistics.  Returns
+ * MPC_ERR_NONFINITE (the iterate left as it was, stats up to that point)
+ * when the initial merit, or the merit of a step the line search rejected,
+ * is not finite, and MPC_ERR_PCG_FAULT when a PCG team exchange timed
+ * out. */
 int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lambda,
                   float *d_xu, float *rho, float rho_reset, mpc_sqp_stats *stats,
                   mpc_stream_t stream);
@@ -229,7 +252,10 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
  * counterpart: the reference solves one instance per call).  Instance i has
  * horizon horizons[i]; its buffers are packed back to back in the
  * reference per-instance layouts: xu at 21*K_i - 7*i, lambda at 14*K_i,
- * ee_ref at 6*K_i, with K_i = sum_{j<i} horizons[j]. */
+ * ee_ref at 6*K_i, with K_i = sum_{j<i} horizons[j].  A set-up call: it
+ * uploads the layout tables with synchronous copies (hipMemcpy), so it
+ * waits for the device's earlier work; the solves that follow are
+ * stream-ordered. */
 int mpc_batch_setup(mpc_handle *h, uint32_t n_inst, const uint32_t *horizons);
 
 /* Run sqp_iters SQP iterations of every instance, entirely stream-ordered
@@ -303,6 +329,25 @@ int mpc_batch_set_plan(mpc_handle *h, const mpc_pcg_plan *plan);
 int mpc_batch_autotune(mpc_handle *h, uint32_t iters, uint32_t reps, mpc_pcg_plan *results, uint32_t max_results,
                        uint32_t *n_results, mpc_stream_t stream);
 int mpc_batch_phase_times(mpc_handle *h, double *us7);
+
+/* Exit state of each instance's last PCG solve in the batch layout (the
+ * last SQP iteration's after mpc_batch_sqp): iterations (S products), the
+ * max-iter flag (0 converged, 1 max_iter reached, 2 team exchange timed
+ * out) and |r^T Pinv r| of its last exit test, n_inst values each (any
+ * pointer may be NULL; stream-ordered device copies).  The reference
+ * returns the first two per solve (toplevel.cuh:292-294, :502); the third
+ * has no counterpart.  Not written in MPC_FLAG_DIRECT_SOLVE mode. */
+int mpc_batch_pcg_exit(mpc_handle *h, uint32_t *d_iters, uint32_t *d_flags, float *d_eta, mpc_stream_t stream);
+
+/* The Schur systems of the last batched SQP iteration (the workspace's S
+ * and gamma) solved directly, block LDL^T of -S with refine_steps (<= 16)
+ * steps of fp64-residual refinement, into d_lambda_out (14 per knot, the
+ * batch's lambda layout): the accuracy reference for the PCG multipliers
+ * that solve returned (the reference's QDLDL solve of the same system,
+ * toplevel.cuh:314-335).  Overwrites the preconditioner workspace, so it
+ * is for use after a solve, before the next one; instances that were no
+ * longer active in that iteration get the solve of a stale system. */
+int mpc_batch_direct_resolve(mpc_handle *h, float *d_lambda_out, uint32_t refine_steps, mpc_stream_t stream);
 
 /* ---- closed-loop tracking ----------------------------------------------------
  * Receding-horizon tracking of one instance entirely on the device: per
@@ -390,6 +435,17 @@ int mpc_batch_qp_loss(mpc_handle *h, const float *d_xu, const float *d_ee_ref, f
 int mpc_batch_qp_adjoint(mpc_handle *h, const float *d_xu, const float *d_ee_ref, const float *d_rho,
                          const float *d_dz, const float *d_a, float pcg_tol, uint32_t refine_steps,
                          float refine_tol, float *d_grad_w, float *d_grad_ref, mpc_stream_t stream);
+
+/* ---- test hook ---------------------------------------------------------------
+ * No counterpart in the reference (its failures exit the process,
+ * include/gpuassert.cuh:5-14).  With part_plus_one = q + 1 (q < 4), part q
+ * of every PCG team (the multi-workgroup solves of 128 < N <= 512) posts its
+ * exchange words under a wrong epoch tag, so its partners' bounded waits
+ * run out: every team solve ends with the max-iter flag 2, the plain driver
+ * returns MPC_ERR_PCG_FAULT and the batched solve marks the instance
+ * MPC_ST_PCG_FAULT without taking its step, while every kernel exits
+ * normally.  0 (the default) turns it off.  For failure-path tests only. */
+int mpc_set_fault_injection(mpc_handle *h, uint32_t part_plus_one);
 
 /* ---- bandwidth probes --------------------------------------------------------
  * No counterpart in the reference: the device's own HBM and LDS bandwidth,

@@ -55,10 +55,27 @@ def test_phase_roofline_picks_the_largest_limit(bench):
     assert rows["dz"]["bound"] == "fp32"  # no LDS bytes, HBM nearly free
 
 
-def test_cpu_baseline_threads_counts_solves(bench, orc):
-    import helpers
+def test_phase_roofline_uses_counted_flops(bench):
+    bw = {"hbm_gbs": 5000.0, "lds_gbs": 100000.0, "method": "test"}
+    phases = dict.fromkeys(("linearize", "schur", "pcg_team", "pcg_rest", "dz", "merit", "update"), 100.0)
+    counters = {"flops_per_knot": {"linearize": 6.0e4, "merit": 2.0e4}}
+    out = bench.phase_roofline(phases, 1.0e5, 1.0e6, bw, counters)
+    rows = out["phases"]
+    assert rows["linearize"]["flops"] == int(6.0e4 * 1.0e5) and rows["merit"]["flops"] == int(2.0e4 * 1.0e5)
+    assert out["flops_source"] == {"linearize": "counted", "merit": "counted"}
+    assert bench.phase_roofline(phases, 1.0e5, 1.0e6, bw)["flops_source"]["linearize"] == "estimate"
+
+
+def test_cpu_baseline_threads_counts_solves(bench, orc, pkg):
     horizons = np.array([8, 16] * 4)
-    one, _, t_per, per_n = bench.cpu_baseline(orc, helpers, horizons, 0.2, 5)
-    assert one > 0 and set(t_per) == {8, 16}
+    rates, desc, t_per, per_n = bench.cpu_baseline(orc, pkg, None, horizons, 0.2, 5)
+    assert rates["ldl"] > 0 and rates["schur"] > 0 and rates["ldl_with_copies"] is None
+    assert set(t_per) == {8, 16} and "Schur formation" in desc
     two = bench.cpu_baseline_threads(orc, per_n, t_per, horizons, 0.2, 2)
     assert two > 0
+
+
+def test_host_info_reports_cores_and_model(bench):
+    h = bench.host_info()
+    assert h["nproc"] >= 1 and 1 <= h["usable_cores"] <= h["nproc"]
+    assert h["model"] is None or isinstance(h["model"], str)

@@ -1,0 +1,212 @@
+"""The batched solve at the benchmark's full shapes against the oracle.
+
+Batched workload (1024 instances, N ~ U{32, 64, 128, 256}, seed 2024 as in
+bench.py, 3 SQP iterations, PCG tol 1e-5, cap 173, replayed as a HIP graph):
+- every instance: finite iterate and multipliers, only the ACTIVE / MAXITER
+  status bits, PCG counts within the cap, bitwise-equal graph replays;
+- 16 sampled instances (4 per horizon) against the oracle's whole fp64 SQP
+  solve (toplevel.cuh:24-503 restated): the same accepted line-search steps
+  (the same rho after each), PCG counts within 10 % (after the first SQP
+  iteration the systems differ by the rounding of the earlier steps; on the
+  GPU's own systems the counts agree within 2 %, below), final xu within 1e-3 or
+  4 x the distance of the oracle's own fp32 twin from its fp64 run (a PCG
+  capped at 173 iterations leaves lambda at the fp32 scatter level, and dz
+  amplifies that by about 1 / rho);
+- every SQP iteration of the sampled instances: the batched PCG's
+  multipliers against the fp64 oracle PCG on the GPU's own Schur system
+  (formed by the stage entry points at the batch's iterate and rho, warm
+  started from the batch's previous multipliers) within 1e-4 relative, or
+  4 x the fp32 floor of that solve where it exceeds 2.5e-5; iteration
+  counts within 2 %.
+
+Stress workload (4096 instances, N ~ U{128, 256, 512}, Q/R = 1e6):
+every instance finite with only ACTIVE / MAXITER (/ PCG-fault free) bits
+for block-Jacobi PCG, stair PCG and the direct solve; sampled N = 512
+instances against the oracle's SQP in the same mode (direct: within 1e-4;
+PCG modes: the same steps, xu within 1e-3)."""
+import numpy as np
+import pytest
+
+import helpers
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+SEED = 2024
+CAP = 173
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+class Workload:
+    def __init__(self, pkg, n, seed, choices, seed_base, r_cost=1e-4, max_knots=256):
+        self.horizons = pkg.synth_horizons(seed, n, choices).astype(np.int64)
+        self.n, self.T = n, int(self.horizons.sum())
+        self.cfg = pkg.default_config(max_knots=max_knots, max_instances=n, max_total_knots=self.T,
+                                      r_cost=r_cost)
+        self.solver = pkg.Solver(self.cfg)
+        xus, qrefs = [], []
+        for i, N in enumerate(self.horizons):
+            x0, qref = pkg.synth_instance(seed_base + i, int(N), 0.01)
+            xu = np.zeros(21 * N - 7, np.float32)
+            for k in range(N):
+                xu[21 * k:21 * k + 14] = x0
+            xus.append(xu)
+            qrefs.append(qref)
+        self.xu0 = np.concatenate(xus)
+        self.ee_dev = self.solver.fk(torch.from_numpy(np.concatenate(qrefs)).cuda()).reshape(-1).contiguous()
+        self.ee = host(self.ee_dev)
+        self.K0 = np.r_[0, np.cumsum(self.horizons)]
+        self.solver.batch_setup(self.horizons.tolist())
+
+    def xu_slice(self, i):
+        a = 21 * self.K0[i] - 7 * i
+        return slice(a, a + 21 * int(self.horizons[i]) - 7)
+
+    def lam_slice(self, i):
+        return slice(14 * self.K0[i], 14 * self.K0[i + 1])
+
+    def ee_slice(self, i):
+        return slice(6 * self.K0[i], 6 * self.K0[i + 1])
+
+    def run(self, iters, graph=True):
+        n, T = self.n, self.T
+        xu = torch.from_numpy(self.xu0).cuda()
+        lam = torch.zeros(14 * T, device="cuda")
+        rho = torch.full((n,), 1e-3, device="cuda")
+        st = torch.zeros(n, dtype=torch.int32, device="cuda")
+        pit = torch.zeros(n * iters, dtype=torch.int32, device="cuda")
+        self.solver.batch_set_graph(graph)
+        self.solver.batch_sqp(xu, lam, self.ee_dev, rho, iters, st, pit)
+        return (host(xu), host(lam), host(rho), host(st).astype(np.int64), host(pit).reshape(n, iters))
+
+    def close(self):
+        self.solver.close()
+
+
+@pytest.fixture(scope="module")
+def batched(pkg):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    w = Workload(pkg, 1024, SEED, (32, 64, 128, 256), SEED * 7919)
+    yield w
+    w.close()
+
+
+def _sample(horizons, per=4):
+    return [int(i) for N in np.unique(horizons) for i in np.flatnonzero(horizons == N)[:per]]
+
+
+def _check_batch_properties(pkg, w, out, cap=CAP, allowed=None):
+    xu, lam, rho, st, pit = out
+    allowed = pkg.ST_ACTIVE | pkg.ST_PCG_MAXITER if allowed is None else allowed
+    assert np.all(np.isfinite(xu)) and np.all(np.isfinite(lam)) and np.all(np.isfinite(rho))
+    assert np.all((st & ~np.int64(allowed)) == 0), np.unique(st)
+    assert pit.min() >= 0 and pit.max() <= cap
+
+
+def test_batched_full_shape_properties_and_replay(batched, pkg):
+    a = batched.run(3, graph=True)
+    b = batched.run(3, graph=True)  # a replay of the captured graph
+    e = batched.run(3, graph=False)  # the same launches issued eagerly
+    _check_batch_properties(pkg, batched, a)
+    for x, y, z in zip(a, b, e):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
+    pit = a[4]
+    print(f"1024 instances: mean PCG {pit.mean():.1f}, at the cap {(pit >= CAP).mean():.3f}, "
+          f"status {np.unique(a[3]).tolist()}")
+
+
+def test_batched_sampled_instances_vs_oracle_sqp(batched, orc):
+    xu, lam, rho, st, pit = batched.run(3)
+    cfg = orc.make_cfg(sqp_max_iter=3)
+    worst = 0.0
+    for i in _sample(batched.horizons):
+        N = int(batched.horizons[i])
+        r = orc.sqp_solve(cfg, N, batched.ee[batched.ee_slice(i)].astype(np.float64), np.zeros(14 * N),
+                          batched.xu0[batched.xu_slice(i)].astype(np.float64), 1e-3)
+        r32 = orc.sqp_solve(cfg, N, batched.ee[batched.ee_slice(i)].astype(np.float64), np.zeros(14 * N),
+                            batched.xu0[batched.xu_slice(i)].astype(np.float64), 1e-3, dtype=np.float32)
+        err = helpers.rel_err(xu[batched.xu_slice(i)], r["xu"])
+        floor = helpers.rel_err(r32["xu"], r["xu"])
+        worst = max(worst, err)
+        print(f"inst {i} N={N}: pcg gpu={pit[i].tolist()} oracle={r['pcg_iters']} steps oracle={r['ls_steps']} "
+              f"rel(xu)={err:.2e} fp32 twin={floor:.2e} rho gpu={rho[i]:.3e} oracle={r['rho']:.3e}")
+        assert r["sqp_iters"] == 3
+        assert all(abs(int(a) - b) <= max(2, b // 10) for a, b in zip(pit[i], r["pcg_iters"])), i
+        assert rho[i] == pytest.approx(r["rho"], rel=1e-6), i  # the same accepted / failed steps
+        assert err < max(1e-3, 4 * floor), i
+    print(f"worst rel(xu) {worst:.2e}")
+
+
+def test_batched_every_iteration_lambda_vs_oracle_pcg_on_gpu_schur(batched, orc, pkg):
+    solver = batched.solver
+    states = [(batched.xu0, np.zeros(14 * batched.T, np.float32), np.full(batched.n, 1e-3, np.float32))]
+    outs = []
+    for k in (1, 2, 3):
+        xu, lam, rho, st, pit = batched.run(k)
+        states.append((xu, lam, rho))
+        outs.append(pit)
+    pit3 = outs[-1]
+    worst = 0.0
+    for i in _sample(batched.horizons):
+        N = int(batched.horizons[i])
+        G = torch.zeros(245 * N - 49, device="cuda")
+        Cm = torch.zeros(294 * (N - 1), device="cuda")
+        g = torch.zeros(21 * N - 7, device="cuda")
+        c = torch.zeros(14 * N, device="cuda")
+        S = torch.zeros(588 * N, device="cuda")
+        P = torch.zeros(588 * N, device="cuda")
+        gam = torch.zeros(14 * N, device="cuda")
+        xs = torch.from_numpy(batched.xu0[batched.xu_slice(i)][:14].copy()).cuda()
+        ee = batched.ee_dev[6 * int(batched.K0[i]):6 * int(batched.K0[i + 1])]
+        for k in range(3):
+            xu_k, lam_k, rho_k = states[k]
+            solver.form_kkt(N, torch.from_numpy(xu_k[batched.xu_slice(i)].copy()).cuda(), ee, xs, G, Cm, g, c)
+            solver.form_schur(N, G, Cm, g, c, S, P, gam, float(rho_k[i]))
+            S64, P64, g64 = (host(t).astype(np.float64) for t in (S, P, gam))
+            warm = lam_k[batched.lam_slice(i)].astype(np.float64)
+            lo, it_o, fl_o = orc.pcg(N, S64, P64, g64, warm, CAP, 1e-5)
+            lt, _, _ = orc.pcg(N, S64, P64, g64, warm, CAP, 1e-5, dtype=np.float32)
+            lam_gpu = states[k + 1][1][batched.lam_slice(i)]
+            err, floor = helpers.rel_err(lam_gpu, lo), helpers.rel_err(lt, lo)
+            worst = max(worst, err)
+            it_g = int(pit3[i, k])
+            print(f"inst {i} N={N} SQP iteration {k}: PCG gpu={it_g} oracle={it_o} (cap {CAP}) rel(lambda)={err:.2e} "
+                  f"fp32 floor={floor:.2e}")
+            assert abs(it_g - it_o) <= max(2, it_o // 50), (i, k)
+            assert err < max(1e-4, 4 * floor), (i, k)
+    print(f"worst rel(lambda) {worst:.2e}")
+
+
+@pytest.fixture(scope="module")
+def stress(pkg):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    w = Workload(pkg, 4096, SEED + 5, (128, 256, 512), SEED * 7919 + 200000, r_cost=1e-6, max_knots=512)
+    yield w
+    w.close()
+
+
+@pytest.mark.parametrize("mode", ["pcg", "stair", "direct"])
+def test_stress_full_shape_vs_oracle(stress, orc, pkg, mode):
+    flags = {"pcg": 0, "stair": pkg.FLAG_PCG_STAIR, "direct": pkg.FLAG_DIRECT_SOLVE}[mode]
+    stress.solver.set_config(r_cost=1e-6, flags=flags)
+    out = stress.run(3, graph=True)
+    allowed = pkg.ST_ACTIVE | pkg.ST_PCG_MAXITER | pkg.ST_RHO_ABORT
+    _check_batch_properties(pkg, stress, out, allowed=allowed)
+    xu, lam, rho, st, pit = out
+    print(f"stress {mode}: mean PCG {pit.mean():.1f}, at the cap {(pit >= CAP).mean():.3f}, "
+          f"aborted {(st & pkg.ST_RHO_ABORT != 0).sum()}")
+    cfg = orc.make_cfg(sqp_max_iter=3, r_cost=1e-6, pcg_stair=int(mode == "stair"))
+    for i in [int(j) for j in np.flatnonzero(stress.horizons == 512)[:2]]:
+        r = orc.sqp_solve(cfg, 512, stress.ee[stress.ee_slice(i)].astype(np.float64), np.zeros(14 * 512),
+                          stress.xu0[stress.xu_slice(i)].astype(np.float64), 1e-3, use_direct=(mode == "direct"))
+        err = helpers.rel_err(xu[stress.xu_slice(i)], r["xu"])
+        print(f"  inst {i}: pcg gpu={pit[i].tolist()} oracle={r['pcg_iters']} rel(xu)={err:.2e} "
+              f"rho gpu={rho[i]:.3e} oracle={r['rho']:.3e}")
+        assert rho[i] == pytest.approx(r["rho"], rel=1e-6), i
+        assert err < (1e-4 if mode == "direct" else 1e-3), i

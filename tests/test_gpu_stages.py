@@ -136,8 +136,11 @@ def _fp32_floor(orc, N, S32, P32, gam32, tol, ref):
 
 
 # Same exit rule on both sides (|r^T Pinv r| < 1e-5): fp32 GPU vs the fp64
-# oracle PCG.  Bound: max(2e-4, 4 x the fp32 rounding floor); measured values
-# are printed (typically 1e-6 .. 1e-4, up to ~1e-3 after ~450 iterations).
+# oracle PCG.  Bound: 1e-4, or 4 x the fp32 rounding floor where that floor
+# (the oracle's fp32 twin against its fp64 run) is above 2.5e-5: past ~250
+# iterations any fp32 PCG scatters beyond 1e-4 (measured on MI355X: 1.3e-6
+# .. 3.3e-5 up to N = 128, 1.3e-4 at N = 256 after 851 iterations, where the
+# twin's own floor is 3.7e-4).
 @pytest.mark.parametrize("N", [2, 16, 32, 64, 128, 256])
 def test_pcg_same_rule_parity(solver, orc, N):
     S32, P32, gam32 = _natural_schur(orc, 30 + N, N)
@@ -150,16 +153,19 @@ def test_pcg_same_rule_parity(solver, orc, N):
           f"rel(gpu,direct)={helpers.rel_err(lam_g, lam_d):.2e} rel(oracle,direct)={helpers.rel_err(lam_o, lam_d):.2e}")
     assert fl_g == 0 and fl_o == 0
     assert abs(it_g - it_o) <= max(1, it_o // 50)
-    assert err < max(2e-4, 4 * floor)
+    assert err < max(1e-4, 4 * floor)
 
 
 # Convergence to the direct solve: run to |r^T Pinv r| < 1e-9 (hundreds to
 # ~1100 iterations).  At that depth every fp32 PCG scatters by rounding
 # alone: on these systems the GPU kernels and the fp32 oracle twin each land
-# 2e-5 .. 5e-4 from the direct solution, in no fixed order.  So the checks
+# 5e-6 .. 2e-4 from the direct solution, in no fixed order.  So the checks
 # are (a) the true residual |gamma - S lambda| / |gamma| of the GPU solution,
 # evaluated in fp64, is within 2x the twin's -- an error in the arithmetic
-# shows up there first -- and (b) the error stays below 1e-3.
+# shows up there first -- and (b) the error stays below 2e-4 (measured on
+# MI355X: 5.6e-6 .. 1.0e-4).  At the SQP's tolerance 1e-5 (absolute) no PCG
+# is within 1e-4 of the direct solution (the fp64 oracle PCG itself is 2e-4
+# .. 6e-3 away): that comparison is made at the same rule, above.
 @pytest.mark.parametrize("N", [2, 16, 32, 64, 128, 200, 256])
 def test_pcg_converges_to_direct_solve(solver, orc, N):
     S32, P32, gam32 = _natural_schur(orc, 30 + N, N)
@@ -178,7 +184,7 @@ def test_pcg_converges_to_direct_solve(solver, orc, N):
           f"fp32 twin: rel={err_t:.2e} residual={res(lam_t):.2e}")
     assert fl_g == 0
     assert res(lam_g) <= 2 * res(lam_t) + 1e-7
-    assert err < 1e-3
+    assert err < 2e-4
 
 
 # The on-chip PCG kernels (resident for N <= 128, teams of two workgroups
@@ -201,11 +207,13 @@ def test_pcg_kernels_agree(solver, orc, pkg, N):
                 solver.set_config(flags=0)
         for lam_g, it_g, fl_g in out:
             assert fl_g == 0 and (it_g == it_o if N <= 128 else abs(it_g - it_o) <= max(2, it_o // 50))
-            assert helpers.rel_err(lam_g, lam_o) < max(2e-4, 4 * floor)
+            assert helpers.rel_err(lam_g, lam_o) < max(1e-4, 4 * floor)
 
 
 # Ill-conditioned stress: perturbed trajectories with large controls need
-# ~10x more iterations; GPU and oracle still follow the same iteration.
+# ~10x more iterations; GPU and oracle still follow the same iteration, and
+# agree to 1e-4 or 4 x the fp32 floor (measured 6.5e-6 at N = 16, 2.7e-4 at
+# N = 128 after 1325 iterations).
 @pytest.mark.parametrize("N", [16, 128])
 def test_pcg_stress_same_rule(solver, orc, N):
     G, Cm, g, c = _schur_inputs(orc, 30 + N, N)
@@ -213,9 +221,11 @@ def test_pcg_stress_same_rule(solver, orc, N):
     S32, P32, gam32 = (np.asarray(a, np.float32) for a in (S, P, gam))
     lam_g, it_g, fl_g = _gpu_pcg(solver, N, S32, P32, gam32, 5000, 1e-5)
     lam_o, it_o, fl_o = orc.pcg(N, S32, P32, gam32, np.zeros(14 * N), 5000, 1e-5)
-    print(f"stress N={N} iters gpu={it_g} oracle={it_o} rel={helpers.rel_err(lam_g, lam_o):.2e}")
+    floor = _fp32_floor(orc, N, S32, P32, gam32, 1e-5, lam_o)
+    err = helpers.rel_err(lam_g, lam_o)
+    print(f"stress N={N} iters gpu={it_g} oracle={it_o} rel={err:.2e} fp32-floor={floor:.2e}")
     assert fl_g == fl_o == 0 and abs(it_g - it_o) <= max(2, it_o // 50)
-    assert helpers.rel_err(lam_g, lam_o) < 1e-3
+    assert err < max(1e-4, 4 * floor)
 
 
 def test_pcg_maxiter_and_warm_start(solver, orc):

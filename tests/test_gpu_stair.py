@@ -59,7 +59,10 @@ def test_stair_pcg_matches_oracle(solver, orc, stair, N, perturbed):
           f"rel(gpu, oracle)={err:.2e} fp32 floor={floor:.2e}")
     assert fl_g == 0 and fl_o == 0
     assert abs(it_g - it_o) <= max(1, it_o // 50)
-    assert err < max(2e-4, 4 * floor)
+    # 1e-4, or 4 x the fp32 floor where past a few hundred iterations the
+    # floor itself exceeds 2.5e-5 (measured: 2.4e-6 .. 3.6e-4, the largest
+    # at N = 128 after 608 iterations with a floor of 3.0e-4)
+    assert err < max(1e-4, 4 * floor)
 
 
 @pytest.mark.parametrize("N", [8, 32, 128])
