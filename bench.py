@@ -696,20 +696,21 @@ def main():
     kkt_per_step = int(executed.sum())
     value = kkt_per_step / (ms_step * 1e-3)
 
-    # per-phase split and roofline of the dominant kernel, k_pcg_team (the
-    # PCG of the horizons above 128 on teams of workgroups), timed with HIP
-    # events on the launch stream
+    # per-phase split and roofline of the dominant kernel, the PCG pool (the
+    # PCG of the horizons above 128 on teams of workgroups and of the
+    # resident bins, one persistent launch), timed with HIP events on the
+    # launch stream
     solver.batch_set_timing(True)
     step()
     torch.cuda.synchronize()
     ph = solver.batch_phase_times()
     solver.batch_set_timing(False)
     log_t = pcg_log.cpu().numpy().reshape(n, K)
-    # the team PCG of one SQP iteration is `launches` cooperative launches
-    # (at most one workgroup per CU, two per team; the last launch also
-    # solves the first resident bins in its spare workgroups); per launch:
-    # the knot-iterations of every instance those launches solve and the
-    # phase time, divided by that count
+    # the PCG phase before its first non-cooperative launch is `launches`
+    # cooperative launches (one workgroup per CU: the pool of teams and
+    # bins, or with the static plan the team launches); per launch: the
+    # knot-iterations of every instance those launches solve and the phase
+    # time, divided by that count
     layout = solver.batch_pcg_layout()
     launches = max(layout["team_launches"], 1)
     in_team_phase = solver.batch_pcg_launch_map() < layout["team_launches"]
@@ -857,7 +858,8 @@ def main():
                   "eager_kkt_solves_per_sec": round(kkt_per_step / (ms_eager * 1e-3), 1),
                   "note": "eager = the same launch sequence without capture, PCG teams as cooperative launches; "
                           "in the graph they are ordinary kernel nodes (bounded waits, fault status)"},
-        "roofline": {"kernel": "k_pcg_team", "bound": "lds", "achieved": round(achieved, 1),
+        "roofline": {"kernel": "k_pcg_pool" if solver.batch_get_plan()["mixed"] else "k_pcg_team",
+                     "bound": "lds", "achieved": round(achieved, 1),
                      "peak": lds_peak, "unit": "GB/s", "frac": round(achieved / lds_peak, 4),
                      "peak_source": "LDS bandwidth measured on this device in this run (mpc_bw_lds, 128-bit reads)",
                      "traffic": traffic, "algorithmic_bytes_per_launch": int(PCG_BYTES_PER_KNOT_ITER * units_launch),
@@ -865,8 +867,9 @@ def main():
                      "launches_per_sqp_iter": launches, "pcg_layout": layout,
                      "note": "algorithmic bytes = 2296 B per knot and PCG iteration (S, Pinv and vectors) x the "
                              "knot-iterations of one launch; S and Pinv stay on chip (LDS and registers), so the "
-                             "roof is the LDS rate; traffic = HBM bytes per launch (PMC), the one-time load; the "
-                             "last team launch (k_pcg_mixed) also solves the first resident bins, counted with it",
+                             "roof is the LDS rate; traffic = HBM bytes per launch (PMC), the one-time load; "
+                             "k_pcg_pool is one persistent launch per PCG phase for the two-workgroup teams and the "
+                             "512-lane resident bins (work queues), k_pcg_team the static team launches",
                      "hbm": None if traffic is None else
                      {"achieved": round(traffic / (t_launch * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": round(traffic / (t_launch * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},

@@ -68,17 +68,29 @@ struct Params {
   unsigned long long budget_ticks;
 };
 
-// mailbox of a PCG team of up to 4 workgroups: per epoch (mod 4) and part,
-// 40 self-validating words {epoch (high 32 bits), float bits (low 32 bits)}:
+// mailbox of a PCG team of up to 4 workgroups: per buffer and part, 40
+// self-validating words {epoch (high 32 bits), float bits (low 32 bits)}:
 // words 0..7 the partial sums of the part's 8 waves, 8..21 the vector
 // elements of its first row (for the part before it), 24..37 of its last
-// row (for the part after it).  Four epoch buffers: a part overwrites the
-// words of epoch e only when it posts epoch e + 4, and between the two it
-// has waited for the partials of every wave of every other part at an
+// row (for the part after it).  Epoch e of a solve uses buffer
+// 4 set + (e mod 4).  Four buffers per set: a part overwrites the words of
+// epoch e only when it posts epoch e + 4, and between the two it has
+// waited for the partials of every wave of every other part at an
 // all-to-all epoch, which each wave posts only after its reads of epoch e
-// (no two consecutive epochs are rows-only exchanges)
+// (no two consecutive epochs are rows-only exchanges).  Two sets: the
+// persistent pool alternates them between the consecutive teams of a slot.
+// `next` carries the pool's team index from part 0 to part 1 {round, index}.
 struct TeamMailbox {
-  unsigned long long w[4][4][40];
+  unsigned long long w[8][4][40];
+  unsigned long long next;
+};
+
+// work queues of the persistent PCG pool (k_pcg_pool), one cache line each
+struct PoolCounters {
+  uint32_t team_next;
+  uint32_t pad0[31];
+  uint32_t bin_next;
+  uint32_t pad1[31];
 };
 
 // per-instance device state of the batched SQP (struct of arrays)
@@ -128,9 +140,15 @@ hipError_t launch_form_kkt(const Batch &b, const Params &p, const float *xu, con
 hipError_t launch_merit(const Batch &b, const Params &p, const float *xu, const float *dz,
                         const float *ee_ref, const float *xs, const uint32_t *status, int p0, int np,
                         float *out, hipStream_t st);
+// out_mode: SCHUR_RIGHT_BLOCK also writes S(b-1,b) = S(b,b-1)^T into the
+// right slot of row b-1 (the reference layout; the solvers read the
+// transpose of the next row's left block instead), SCHUR_ZERO_UNUSED
+// writes zeros into the blocks that are structurally zero (S(0,-1), the
+// right block of the last row, Pinv's off-diagonal blocks)
+constexpr int SCHUR_ZERO_UNUSED = 1, SCHUR_RIGHT_BLOCK = 2;
 hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, const float *g,
                              const float *c, const float *rho_dev, float rho, const uint32_t *status,
-                             float *S, float *Pinv, float *gamma, float *Ginv, int zero_unused,
+                             float *S, float *Pinv, float *gamma, float *Ginv, int out_mode,
                              hipStream_t st);
 hipError_t launch_compute_dz(const Batch &b, const float *Ginv, const float *C, const float *g,
                              const float *lambda, const uint32_t *status, float *dz, hipStream_t st);
@@ -154,23 +172,32 @@ hipError_t launch_pcg_res(const Batch &b, const uint4 *items, const uint32_t *bi
                           uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters,
                           uint32_t *flag, float *eta, bool stair, hipStream_t st);
 // PCG of 128 < N <= pcg_team_max_knots(q) on a team of q = 2 or 4
-// workgroups (cooperative launches); err receives 1 if a team wait timed
-// out.  With a bin table (items, bin_off, n_bins) and bins_taken, the spare
-// workgroups of the last two-workgroup team launch solve the first
-// *bins_taken bins (the caller launches the rest with launch_pcg_res from
-// bin *bins_taken).  poison (test hook, 0 = off): part poison - 1 of every
+// workgroups (cooperative launches, pcg_team_launches of them); err
+// receives 1 if a team wait timed out; done (one word per instance) counts
+// the parts that finished each team, and a team short of q gets
+// PCG_FLAG_FAULT.  poison (test hook, 0 = off): part poison - 1 of every
 // team posts its exchange words under a wrong epoch, so every team ends
 // with PCG_FLAG_FAULT.
 hipError_t launch_pcg_team(const Batch &b, uint32_t q, const uint32_t *team_inst, uint32_t n_teams,
-                           TeamMailbox *mb, uint32_t n_mb, const uint4 *items, const uint32_t *bin_off,
-                           uint32_t n_bins, uint32_t *bins_taken, const float *S, const float *Pinv,
-                           const float *gamma, float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
-                           uint32_t *iters, uint32_t *flag, float *eta, uint32_t *err, bool stair, uint32_t poison,
+                           TeamMailbox *mb, uint32_t n_mb, const float *S, const float *Pinv, const float *gamma,
+                           float *lambda, uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters,
+                           uint32_t *flag, float *eta, uint32_t *done, uint32_t *err, bool stair, uint32_t poison,
                            hipStream_t st);
+// One persistent cooperative launch (one workgroup per CU) for the
+// two-workgroup teams team_inst[0 .. n_teams) and the 512-lane bins
+// 0 .. n_bins - 1 of the bin table, taken from work queues (ctr): teams
+// first, then bins.  Same outputs and fault handling as launch_pcg_team.
+// Needs pcg_pool_ok(n_teams, max_iter).
+hipError_t launch_pcg_pool(const Batch &b, const uint32_t *team_inst, uint32_t n_teams, TeamMailbox *mb,
+                           uint32_t n_mb, PoolCounters *ctr, const uint4 *items, const uint32_t *bin_off,
+                           uint32_t n_bins, const float *S, const float *Pinv, const float *gamma, float *lambda,
+                           uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters, uint32_t *flag,
+                           float *eta, uint32_t *done, uint32_t *err, bool stair, uint32_t poison, hipStream_t st);
 uint32_t pcg_team_max_knots(uint32_t q);
 uint32_t pcg_team_max_grid();
 uint32_t pcg_team_per_launch(uint32_t q);
-void pcg_team_plan(uint32_t n_teams, uint32_t q, uint32_t n_bins, uint32_t *launches, uint32_t *bins_taken);
+uint32_t pcg_team_launches(uint32_t n_teams, uint32_t q);
+bool pcg_pool_ok(uint32_t n_teams, uint32_t max_iter);
 uint32_t pcg_res_groups(uint32_t N);
 uint32_t pcg_res_bin_groups(uint32_t nt);  // 16-lane groups per workgroup of nt lanes
 uint32_t pcg_res_max_knots(uint32_t nt);

@@ -454,9 +454,9 @@ __global__ void __launch_bounds__(WG) k_schur_residual(Batch b, const float *__r
 #pragma unroll
     for (int m = 0; m < NX; ++m) res += (double)Sr[lane + NX * m] * (double)l[m - NX];
   }
-  if (kr.k + 1 < kr.N) {
+  if (kr.k + 1 < kr.N) {  // S(b,b+1) = S(b+1,b)^T: the next block row's left block
 #pragma unroll
-    for (int m = 0; m < NX; ++m) res += (double)Sr[2 * NXX + lane + NX * m] * (double)l[m + NX];
+    for (int m = 0; m < NX; ++m) res += (double)Sr[S_STRIDE + m + NX * lane] * (double)l[m + NX];
   }
   slot_z(work, gk)[lane] = (float)res;
 }

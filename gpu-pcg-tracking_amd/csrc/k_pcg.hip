@@ -51,6 +51,14 @@ __device__ __forceinline__ float blk_row_dot(const float *__restrict__ blk, int 
   for (int j = 0; j < NX; ++j) acc += blk[i + NX * j] * x[j];
   return acc;
 }
+// row i of (B^T x): column i of the column-major block (S(b,b+1) = S(b+1,b)^T
+// is read from the next block row's left block)
+__device__ __forceinline__ float blk_col_dot(const float *__restrict__ blk, int i, const float *x) {
+  float acc = 0.f;
+#pragma unroll
+  for (int j = 0; j < NX; ++j) acc += blk[j + NX * i] * x[j];
+  return acc;
+}
 
 template <int NG, int R, bool STAIR>
 __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restrict__ S,
@@ -88,7 +96,7 @@ __global__ void __launch_bounds__(NG * 16) k_pcg(Batch b, const float *__restric
         const float *row = Sg + (size_t)S_STRIDE * bk;
         acc = blk_row_dot(row + NXX, i, pv + NX * bk);
         if (bk > 0) acc += blk_row_dot(row, i, pv + NX * (bk - 1));
-        if (bk + 1 < N) acc += blk_row_dot(row + 2 * NXX, i, pv + NX * (bk + 1));
+        if (bk + 1 < N) acc += blk_col_dot(row + S_STRIDE, i, pv + NX * (bk + 1));
       }
       out[t] = acc;
     }

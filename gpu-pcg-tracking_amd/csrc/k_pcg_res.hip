@@ -19,8 +19,8 @@ __global__ void __launch_bounds__(NT) k_pcg_res(Batch b, const uint4 *__restrict
                                                 uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out,
                                                 float *__restrict__ eta_out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  body<STAIR, NT>(bin0 + blockIdx.x, smem_raw, b, items, bin_off, S, Pinv, gamma, lambda, max_iter, tol, status,
-                  iters_out, flag_out, eta_out);
+  body<STAIR, NT>(threadIdx.x, bin0 + blockIdx.x, smem_raw, b, items, bin_off, S, Pinv, gamma, lambda, max_iter,
+                  tol, status, iters_out, flag_out, eta_out);
 }
 
 template <int NT>

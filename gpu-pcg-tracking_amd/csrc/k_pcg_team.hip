@@ -25,21 +25,33 @@
 // algorithm and exit rule as k_pcg.hip (GPU-PCG pcg<T,14,N>,
 // toplevel.cuh:143-157).
 //
-// Each launch is cooperative (all workgroups co-resident) and solves one
-// team per Q workgroups; more teams than fit at once are solved by
-// consecutive launches.  The parts of a team sit 8 workgroup ids apart so
-// they land on the same XCD and share its L2.  Mailbox words carry their
-// epoch, so relaxed (fence-free) atomics suffice; four epoch buffers keep a
-// word until every reader is done with it (TeamMailbox).  Every wait is
-// bounded: a wave that gives up sets the error word and the workgroup's
-// fail flag, and the workgroup leaves at its next barrier with the
-// instance's PCG flag set to PCG_FLAG_FAULT; its partners then time out as
-// well.  The flag is raised with an atomic maximum over a value cleared
-// before the launch (0 converged < 1 max_iter < PCG_FLAG_FAULT), so a
-// fault wins over the result part 0 writes, in either order.
+// Two launch forms, both cooperative (all workgroups co-resident):
+// - k_pcg_team: one team per Q workgroups; more teams than fit at once are
+//   solved by consecutive launches (four-workgroup teams, single instances);
+// - k_pcg_pool: one persistent launch of one workgroup per CU for a PCG
+//   phase's two-workgroup teams and 512-lane resident bins.  Workgroups are
+//   paired into team slots; the slot's part 0 takes the next team from a
+//   queue (a device counter) and passes its index to part 1 through the
+//   slot's mailbox; when the team queue is empty every workgroup takes
+//   bins from a second queue (resident::body).  A CU whose team converged
+//   early takes the next item instead of idling until the slowest team of a
+//   static launch is done.  Teams are queued before bins and the bins
+//   largest first.
+// The parts of a team sit 8 workgroup ids apart so they land on the same
+// XCD and share its L2.  Mailbox words carry their epoch, so relaxed
+// (fence-free) atomics suffice; four epoch buffers keep a word until every
+// reader is done with it, and a pool slot alternates between two sets of
+// them from one team to the next (TeamMailbox).  Every wait is bounded: a
+// wave that gives up sets the error word and the workgroup's fail flag, and
+// the workgroup leaves the solve at its next barrier; its partners then
+// time out as well.  A part counts itself in the instance's `done` word
+// only after a solve without a fault, and the finalising kernel after the
+// launch sets PCG_FLAG_FAULT for every team instance whose count is short of
+// Q (an abandoned solve, or a team the pool never reached), so a fault wins
+// over the result part 0 writes, whatever the order.
 // `poison` (host test hook; 0 = off) makes part poison - 1 of every team
 // post its words under a wrong epoch tag: its partners' waits then run out
-// and the whole team ends with PCG_FLAG_FAULT, every workgroup exiting
+// and every team ends with PCG_FLAG_FAULT, every workgroup exiting
 // normally.  Under stream
 // capture (HIP graphs) the launches are ordinary kernel launches of the same
 // grid, which fits the device at once (at most one workgroup per CU); the
@@ -95,10 +107,14 @@ __device__ __forceinline__ float sum_lanes8(float x, uint32_t l0) {
 }
 
 // tag_xor: 0, or a bit pattern that makes the tag wrong (the poison hook)
-__device__ __forceinline__ void mb_post(TeamMailbox *mb, uint32_t part, uint32_t e, uint32_t word, float v,
-                                        uint32_t tag_xor) {
-  __hip_atomic_store(&mb->w[e & 3u][part][word], ((unsigned long long)(e ^ tag_xor) << 32) | __float_as_uint(v),
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// buffer of epoch e in the set `set` (0 or 1) of the mailbox
+__device__ __forceinline__ uint32_t mb_buf(uint32_t set, uint32_t e) { return (set << 2) | (e & 3u); }
+
+__device__ __forceinline__ void mb_post(TeamMailbox *mb, uint32_t set, uint32_t part, uint32_t e, uint32_t word,
+                                        float v, uint32_t tag_xor) {
+  __hip_atomic_store(&mb->w[mb_buf(set, e)][part][word],
+                     ((unsigned long long)(e ^ tag_xor) << 32) | __float_as_uint(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Lanes with `want` wait for the mailbox word at src to carry epoch e and
@@ -127,30 +143,26 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Part `(blk >> 3) & (Q-1)` of team team0 + slot(blk) on a 512-lane
-// workgroup with the TeamShared block at smem; blk is the workgroup's id
-// among the team workgroups of the launch.
+// Part `part` of the PCG of team `team` (instance team_inst[team]) on a
+// 512-lane workgroup with the TeamShared block at smem, exchanging through
+// mailbox mb, buffer set `set`, epochs e0 + 1, e0 + 2, ...  Returns false
+// when a wait ran out (the solve was abandoned; `done` not counted).
 template <bool STAIR, int Q>
-__device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Batch b,
-                                          const uint32_t *__restrict__ team_inst, uint32_t n_teams, uint32_t team0,
-                                          TeamMailbox *mboxes, const float *__restrict__ S,
-                                          const float *__restrict__ Pinv, const float *__restrict__ gamma,
-                                          float *__restrict__ lambda, uint32_t max_iter, float tol,
-                                          const uint32_t *__restrict__ status, uint32_t *__restrict__ iters_out,
-                                          uint32_t *__restrict__ flag_out, float *__restrict__ eta_out,
-                                          uint32_t *__restrict__ err, uint32_t poison) {
+__device__ __forceinline__ bool team_solve(uint32_t tid, uint32_t part, uint32_t team, uint32_t set, uint32_t e0,
+                                           TeamMailbox *mb,
+                                           unsigned char *smem, Batch b, const uint32_t *__restrict__ team_inst,
+                                           const float *__restrict__ S, const float *__restrict__ Pinv,
+                                           const float *__restrict__ gamma, float *__restrict__ lambda,
+                                           uint32_t max_iter, float tol, const uint32_t *__restrict__ status,
+                                           uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out,
+                                           float *__restrict__ eta_out, uint32_t *__restrict__ done,
+                                           uint32_t *__restrict__ err, uint32_t poison) {
   TeamShared &sh = *reinterpret_cast<TeamShared *>(smem);
-  const uint32_t tid = threadIdx.x, g = tid >> 4, i = tid & 15u, w = tid >> 6, lane64 = tid & 63u;
-  const uint32_t d = ror_step();
+  const uint32_t g = tid >> 4, i = tid & 15u, w = tid >> 6, lane64 = tid & 63u;
+  const uint32_t d = ror_step(tid);
   static_assert(Q == 2 || Q == 4, "teams of 2 or 4 workgroups");
-  constexpr uint32_t QS = Q == 4 ? 2u : 1u;  // log2 Q
-  const uint32_t part = (blk >> 3) & (uint32_t)(Q - 1);
-  const uint32_t slot = (blk >> (3u + QS)) * 8u + (blk & 7u);
-  const uint32_t team = team0 + slot;
-  if (team >= n_teams) return;
   const uint32_t inst = team_inst[team];
-  if (status && !(status[inst] & ST_ACTIVE)) return;  // same decision on every part
-  TeamMailbox *mb = mboxes + slot;
+  if (status && !(status[inst] & ST_ACTIVE)) return true;  // same decision on every part
   const uint32_t tag_xor = poison == part + 1u ? 0x80000000u : 0u;
   const bool lane_ok = i < (uint32_t)NX;
   const uint32_t N = inst_len(b, inst), K0 = inst_first(b, inst);
@@ -195,17 +207,15 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
   const uint32_t gl = n_own / RK - 1u, wl = gl >> 2;  // last group / its wave (n_own % RK == 0 with has_next)
   const bool first_lane = has_prev && g == 0 && lane_ok, last_lane = has_next && g == gl && lane_ok;
   float *zb0 = sh.zb + b0 * 16 + i;
-  uint32_t e = 0;  // mailbox epoch
+  uint32_t e = e0;  // mailbox epoch
   bool ok = true;
-  // a wait timed out somewhere in the workgroup: the solve is abandoned and
-  // the instance flagged (every part writes the same value)
-  auto fault = [&]() {
-    if (tid == 0 && flag_out) atomicMax(flag_out + inst, PCG_FLAG_FAULT);
-  };
+  // a wait timed out somewhere in the workgroup: the solve is abandoned
+  // (this part is not counted in `done`, so the instance gets PCG_FLAG_FAULT)
+  auto fault = [&]() { return false; };
   // own first / last element to LDS, the rows at the splits to the mailbox
   auto post_rows = [&](const float (&v)[RK]) {
-    if (first_lane) mb_post(mb, part, e, MB_FIRST + i, v[0], tag_xor);
-    if (last_lane) mb_post(mb, part, e, MB_LAST + i, v[RK - 1], tag_xor);
+    if (first_lane) mb_post(mb, set, part, e, MB_FIRST + i, v[0], tag_xor);
+    if (last_lane) mb_post(mb, set, part, e, MB_LAST + i, v[RK - 1], tag_xor);
   };
   auto stage = [&](const float (&v)[RK]) {
     zb0[0] = v[0];
@@ -236,7 +246,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     const bool want = (role == 1u && partials) || (role == 2u && rows) || (role == 3u && rows);
     float x = 0.f;
     if (partials || rm || rp) {
-      x = mb_wait(&mb->w[e & 3u][0][0] + word, want, e, ok);
+      x = mb_wait(&mb->w[mb_buf(set, e)][0][0] + word, want, e, ok);
       if (want && role == 2u) sh.recv[0][lane64 - LANE_M] = x;
       if (want && role == 3u) sh.recv[1][lane64 - LANE_P] = x;
       if (rm || rp) wave_sync();
@@ -313,7 +323,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
   ++e;
   if (lane64 == 0) {
     sh.red[1][w] = part_sum;
-    mb_post(mb, part, e, w, part_sum, tag_xor);
+    mb_post(mb, set, part, e, w, part_sum, tag_xor);
   }
   stage(z);
   __syncthreads();
@@ -337,7 +347,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     ++e;
     if (lane64 == 0) {
       sh.red[0][w] = pyp;
-      mb_post(mb, part, e, w, pyp, tag_xor);
+      mb_post(mb, set, part, e, w, pyp, tag_xor);
     }
     __syncthreads();
     if (sh.fail) return fault();
@@ -372,7 +382,7 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     ++e;
     if (lane64 == 0) {
       sh.red[1][w] = rz;
-      mb_post(mb, part, e, w, rz, tag_xor);
+      mb_post(mb, set, part, e, w, rz, tag_xor);
     }
     stage(z);
     __syncthreads();
@@ -401,11 +411,15 @@ __device__ __forceinline__ void team_body(uint32_t blk, unsigned char *smem, Bat
     const uint32_t lk = b0 + (uint32_t)t;
     if (lk < n_own && lane_ok) lambda[(size_t)NX * (K0 + kb + lk) + i] = lam[t];
   }
-  if (part == 0 && tid == 0) {
-    if (iters_out) iters_out[inst] = it;
-    if (eta_out) eta_out[inst] = fabsf(eta_last);
-    if (flag_out) atomicMax(flag_out + inst, conv ? 0u : 1u);
+  if (tid == 0) {
+    if (part == 0) {
+      if (iters_out) iters_out[inst] = it;
+      if (eta_out) eta_out[inst] = fabsf(eta_last);
+      if (flag_out) flag_out[inst] = conv ? 0u : 1u;
+    }
+    if (done) atomicAdd(done + inst, 1u);
   }
+  return true;
 }
 
 template <bool STAIR, int Q>
@@ -415,61 +429,145 @@ __global__ void __launch_bounds__(NT) k_pcg_team(Batch b, const uint32_t *__rest
                                                  const float *__restrict__ gamma, float *__restrict__ lambda,
                                                  uint32_t max_iter, float tol, const uint32_t *__restrict__ status,
                                                  uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out,
-                                                 float *__restrict__ eta_out, uint32_t *__restrict__ err,
+                                                 float *__restrict__ eta_out, uint32_t *__restrict__ done,
+                                                 uint32_t *__restrict__ err, uint32_t poison) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr uint32_t QS = Q == 4 ? 2u : 1u;  // log2 Q
+  const uint32_t blk = blockIdx.x;
+  const uint32_t part = (blk >> 3) & (uint32_t)(Q - 1);
+  const uint32_t slot = (blk >> (3u + QS)) * 8u + (blk & 7u);
+  if (team0 + slot >= n_teams) return;
+  team_solve<STAIR, Q>(threadIdx.x, part, team0 + slot, 0u, 0u, mboxes + slot, smem_raw, b, team_inst, S, Pinv, gamma,
+                       lambda, max_iter, tol, status, iters_out, flag_out, eta_out, done, err, poison);
+}
+
+// Largest pool round: its epochs start at round << POOL_ROUND_SHIFT, so a
+// team solve may use 2^POOL_ROUND_SHIFT epochs (about 2^19 PCG iterations)
+// and a slot 2^(32 - POOL_ROUND_SHIFT) - 1 teams (the host checks both).
+constexpr uint32_t POOL_ROUND_SHIFT = 20;
+
+// The persistent PCG phase: team slot (blk >> 4) * 8 + (blk & 7) with part
+// (blk >> 3) & 1 takes two-workgroup teams from ctr->team_next while there
+// are any, then every workgroup takes bins from ctr->bin_next.
+template <bool STAIR>
+__global__ void __launch_bounds__(NT) k_pcg_pool(Batch b, const uint32_t *__restrict__ team_inst, uint32_t n_teams,
+                                                 TeamMailbox *mboxes, PoolCounters *ctr,
+                                                 const uint4 *__restrict__ items, const uint32_t *__restrict__ bin_off,
+                                                 uint32_t n_bins, const float *__restrict__ S,
+                                                 const float *__restrict__ Pinv, const float *__restrict__ gamma,
+                                                 float *__restrict__ lambda, uint32_t max_iter, float tol,
+                                                 const uint32_t *__restrict__ status, uint32_t *__restrict__ iters_out,
+                                                 uint32_t *__restrict__ flag_out, float *__restrict__ eta_out,
+                                                 uint32_t *__restrict__ done, uint32_t *__restrict__ err,
                                                  uint32_t poison) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  team_body<STAIR, Q>(blockIdx.x, smem_raw, b, team_inst, n_teams, team0, mboxes, S, Pinv, gamma, lambda, max_iter,
-                      tol, status, iters_out, flag_out, eta_out, err, poison);
+  __shared__ uint32_t s_item, s_round, s_team;
+  const uint32_t blk = blockIdx.x;
+  const uint32_t part = (blk >> 3) & 1u, slot = (blk >> 4) * 8u + (blk & 7u);
+  TeamMailbox *mb = mboxes + slot;
+  // One loop, one item per pass, a team or a bin in exclusive branches (as
+  // two loops one after the other the kernel needs more registers than it
+  // has and spills in the team iteration).
+  bool teams = n_teams > 0;  // uniform: this workgroup still takes teams
+  for (uint32_t r = 1;; ++r) {
+    if (threadIdx.x == 0) {
+      uint32_t t = 0xFFFFFFFFu, rr = r;
+      if (teams) {
+        // part 0 takes the next team and posts its index under the round's
+        // tag; part 1 waits for that tag or a later one (part 0 runs ahead
+        // past teams that are no longer active, which need no partner) and
+        // adopts its round; no tag in time means no team
+        if (part == 0) {
+          t = atomicAdd(&ctr->team_next, 1u);
+          __hip_atomic_store(&mb->next, ((unsigned long long)r << 32) | t, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          for (uint32_t spins = 0; spins <= SPIN_LIMIT; ++spins) {
+            const unsigned long long v = __hip_atomic_load(&mb->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint32_t)(v >> 32) >= r) {
+              t = (uint32_t)v;
+              rr = (uint32_t)(v >> 32);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+        }
+      }
+      const bool is_team = teams && t < n_teams;
+      if (!is_team) t = atomicAdd(&ctr->bin_next, 1u);  // the team queue is done for this workgroup
+      s_item = t;
+      s_round = rr;
+      s_team = is_team ? 1u : 0u;
+    }
+    __syncthreads();
+    // read through readfirstlane: the item is uniform, and the compiler keeps
+    // what it derives from it (instance, offsets, addresses) in scalar
+    // registers, as for the static team launch's block-index-derived team
+    const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_item);
+    r = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_round);
+    const bool is_team = __builtin_amdgcn_readfirstlane((int)s_team) != 0;
+    __syncthreads();  // every lane has read the item before it is written again
+    // the thread index made opaque per pass: what the bodies derive from it
+    // is recomputed inside the loop instead of held in registers across it
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    if (is_team) {
+      // consecutive teams of a slot use the two buffer sets in turn, so a late
+      // reader of one team's last words is not overwritten by the next team
+      if (!team_solve<STAIR, 2>(tid, part, t, r & 1u, r << POOL_ROUND_SHIFT, mb, smem_raw, b, team_inst, S, Pinv,
+                                gamma, lambda, max_iter, tol, status, iters_out, flag_out, eta_out, done, err,
+                                poison))
+        teams = false;  // the partner is out of step: this workgroup takes no further team
+    } else {
+      teams = false;
+      if (t >= n_bins) break;
+      resident::body<STAIR, 512>(tid, t, smem_raw, b, items, bin_off, S, Pinv, gamma, lambda, max_iter, tol, status,
+                                 iters_out, flag_out, eta_out);
+    }
+    __syncthreads();  // LDS of this item read by every wave before the next one clears it
+  }
 }
 
-// The last, partly filled launch of two-workgroup teams with resident bins
-// in its spare workgroups: ids below team_grid run the team body, the others bins
-// bin0 + (id - team_grid).  Bins need no co-residency; they only fill
-// workgroup slots that would idle until the launch ends.
-template <bool STAIR>
-__global__ void __launch_bounds__(NT) k_pcg_mixed(Batch b, const uint32_t *__restrict__ team_inst,
-                                                  uint32_t n_teams, uint32_t team0, TeamMailbox *mboxes,
-                                                  uint32_t team_grid, const uint4 *__restrict__ items,
-                                                  const uint32_t *__restrict__ bin_off, uint32_t bin0,
-                                                  const float *__restrict__ S, const float *__restrict__ Pinv,
-                                                  const float *__restrict__ gamma, float *__restrict__ lambda,
-                                                  uint32_t max_iter, float tol, const uint32_t *__restrict__ status,
-                                                  uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out,
-                                                  float *__restrict__ eta_out, uint32_t *__restrict__ err,
-                                                  uint32_t poison) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  if (blockIdx.x < team_grid)
-    team_body<STAIR, 2>(blockIdx.x, smem_raw, b, team_inst, n_teams, team0, mboxes, S, Pinv, gamma, lambda,
-                        max_iter, tol, status, iters_out, flag_out, eta_out, err, poison);
-  else
-    resident::body<STAIR, 512>(bin0 + (blockIdx.x - team_grid), smem_raw, b, items, bin_off, S, Pinv, gamma, lambda,
-                               max_iter, tol, status, iters_out, flag_out, eta_out);
-}
-
-// Before a team launch: its n mailboxes cleared, and the PCG flag of every
-// active instance among its teams team_inst[team0 .. team0 + n_launch)
-// set to 0, the floor of the atomic maximum the team raises it to.
-__global__ void k_reset_mailboxes(TeamMailbox *mb, uint32_t n, const uint32_t *__restrict__ team_inst,
-                                  uint32_t team0, uint32_t n_launch, const uint32_t *__restrict__ status,
-                                  uint32_t *__restrict__ flag_out) {
+// Before a team launch: its n_mb mailboxes cleared, `done` of every active
+// instance among teams team_inst[team0 .. team0 + n) set to 0, and the pool
+// counters (ctr, may be null) reset.
+__global__ void k_team_reset(TeamMailbox *mb, uint32_t n_mb, const uint32_t *__restrict__ team_inst, uint32_t team0,
+                             uint32_t n, const uint32_t *__restrict__ status, uint32_t *__restrict__ done,
+                             PoolCounters *ctr) {
   constexpr uint32_t W = sizeof(TeamMailbox) / sizeof(unsigned long long);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n * W) (&mb[0].w[0][0][0])[i] = 0ull;
-  if (flag_out && i < n_launch) {
+  if (i < n_mb * W) (&mb[0].w[0][0][0])[i] = 0ull;
+  if (i < n) {
     const uint32_t inst = team_inst[team0 + i];
-    if (!status || (status[inst] & ST_ACTIVE)) flag_out[inst] = 0u;
+    if (!status || (status[inst] & ST_ACTIVE)) done[inst] = 0u;
   }
+  if (ctr && i == 0) {
+    ctr->team_next = 0u;
+    ctr->bin_next = 0u;
+  }
+}
+
+// After it: PCG_FLAG_FAULT for every active team instance that fewer than q
+// parts finished (an abandoned solve, or one never started)
+__global__ void k_team_finalize(const uint32_t *__restrict__ team_inst, uint32_t team0, uint32_t n, uint32_t q,
+                                const uint32_t *__restrict__ status, const uint32_t *__restrict__ done,
+                                uint32_t *__restrict__ flag_out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t inst = team_inst[team0 + i];
+  if (status && !(status[inst] & ST_ACTIVE)) return;
+  if (done[inst] != q && flag_out) flag_out[inst] = PCG_FLAG_FAULT;
 }
 
 }  // namespace
 
 uint32_t pcg_team_max_knots(uint32_t q) { return q * (uint32_t)KNOTS; }
 
-constexpr size_t MIXED_SMEM =
+constexpr size_t POOL_SMEM =
     sizeof(TeamShared) > sizeof(resident::ResShared<512>) ? sizeof(TeamShared) : sizeof(resident::ResShared<512>);
 
-// Largest cooperative grid of k_pcg_team (and k_pcg_mixed) on this device
-// (0: not supported).
+// Largest cooperative grid of k_pcg_team and k_pcg_pool on this device (0:
+// not supported).
 uint32_t pcg_team_max_grid() {
   static int cached = -1;
   if (cached >= 0) return (uint32_t)cached;
@@ -480,13 +578,13 @@ uint32_t pcg_team_max_grid() {
     cached = 0;
     return 0;
   }
-  // every variant (block Jacobi, stair; two or four parts; mixed) must fit;
+  // every variant (block Jacobi, stair; two or four parts; pool) must fit;
   // the launches use the smallest
   const void *fns[6] = {(const void *)k_pcg_team<false, 2>, (const void *)k_pcg_team<true, 2>,
                         (const void *)k_pcg_team<false, 4>, (const void *)k_pcg_team<true, 4>,
-                        (const void *)k_pcg_mixed<false>,   (const void *)k_pcg_mixed<true>};
+                        (const void *)k_pcg_pool<false>,    (const void *)k_pcg_pool<true>};
   const size_t smem[6] = {sizeof(TeamShared), sizeof(TeamShared), sizeof(TeamShared),
-                          sizeof(TeamShared), MIXED_SMEM,         MIXED_SMEM};
+                          sizeof(TeamShared), POOL_SMEM,          POOL_SMEM};
   int per_cu = 1 << 30;
   for (int k = 0; k < 6; ++k) {
     int n = 0;
@@ -503,79 +601,99 @@ uint32_t pcg_team_max_grid() {
 // per XCD slot, each team's parts on one XCD) within the grid limit.
 uint32_t pcg_team_per_launch(uint32_t q) { return pcg_team_max_grid() / (8u * q) * 8u; }
 
-// Launch plan of n_teams teams of q workgroups: the number of cooperative
-// launches and how many of n_bins resident bins the last one solves in its
-// spare workgroups (two-workgroup teams only).
-void pcg_team_plan(uint32_t n_teams, uint32_t q, uint32_t n_bins, uint32_t *launches, uint32_t *bins_taken) {
-  *launches = 0;
-  *bins_taken = 0;
+// Cooperative team launches of n_teams teams of q workgroups.
+uint32_t pcg_team_launches(uint32_t n_teams, uint32_t q) {
   const uint32_t per_launch = pcg_team_per_launch(q);
-  if (n_teams == 0 || per_launch == 0) return;
-  *launches = (n_teams + per_launch - 1) / per_launch;
-  if (q != 2) return;
-  const uint32_t n_last = n_teams - (*launches - 1) * per_launch;
-  const uint32_t spare = pcg_team_max_grid() - 16u * ((n_last + 7u) / 8u);
-  *bins_taken = spare < n_bins ? spare : n_bins;
+  return n_teams == 0 || per_launch == 0 ? 0u : (n_teams + per_launch - 1) / per_launch;
 }
 
+// Whether the pool can take n_teams teams with max_iter iterations each (its
+// epoch and round ranges) on this device.
+bool pcg_pool_ok(uint32_t n_teams, uint32_t max_iter) {
+  return pcg_team_max_grid() >= 16u && n_teams < (1u << (32u - POOL_ROUND_SHIFT)) - 1u &&
+         (max_iter == PCG_CAP_BY_N || max_iter < (1u << (POOL_ROUND_SHIFT - 2u)));
+}
+
+namespace {
+
+// launch fn cooperatively, or as an ordinary launch of the same grid under
+// stream capture (a graph node carries no co-residency guarantee; the grid
+// fits the device at once, and the bounded waits turn a partner that is
+// not resident into PCG_FLAG_FAULT instead of a hang)
+hipError_t launch_coop(const void *fn, uint32_t grid, void **args, size_t smem, hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  const hipError_t ec = hipStreamIsCapturing(st, &cs);
+  if (ec != hipSuccess) return ec;
+  return cs == hipStreamCaptureStatusActive
+             ? hipLaunchKernel(fn, dim3(grid), dim3(NT), args, smem, st)
+             : hipLaunchCooperativeKernel(fn, dim3(grid), dim3(NT), args, (unsigned)smem, st);
+}
+
+hipError_t team_reset(TeamMailbox *mb, uint32_t n_mb, const uint32_t *team_inst, uint32_t team0, uint32_t n,
+                      const uint32_t *status, uint32_t *done, PoolCounters *ctr, hipStream_t st) {
+  constexpr uint32_t W = sizeof(TeamMailbox) / sizeof(unsigned long long);
+  const uint32_t m = n_mb * W > n ? n_mb * W : n;
+  hipLaunchKernelGGL(k_team_reset, dim3((m + 255) / 256), dim3(256), 0, st, mb, n_mb, team_inst, team0, n, status,
+                     done, ctr);
+  return hipGetLastError();
+}
+
+hipError_t team_finalize(const uint32_t *team_inst, uint32_t team0, uint32_t n, uint32_t q, const uint32_t *status,
+                         const uint32_t *done, uint32_t *flag, hipStream_t st) {
+  if (n == 0 || !flag) return hipSuccess;
+  hipLaunchKernelGGL(k_team_finalize, dim3((n + 255) / 256), dim3(256), 0, st, team_inst, team0, n, q, status, done,
+                     flag);
+  return hipGetLastError();
+}
+
+}  // namespace
+
 hipError_t launch_pcg_team(const Batch &b, uint32_t q, const uint32_t *team_inst, uint32_t n_teams,
-                           TeamMailbox *mb, uint32_t n_mb, const uint4 *items, const uint32_t *bin_off,
-                           uint32_t n_bins, uint32_t *bins_taken, const float *S, const float *Pinv,
-                           const float *gamma, float *lambda, uint32_t max_iter, float tol, const uint32_t *status,
-                           uint32_t *iters, uint32_t *flag, float *eta, uint32_t *err, bool stair, uint32_t poison,
+                           TeamMailbox *mb, uint32_t n_mb, const float *S, const float *Pinv, const float *gamma,
+                           float *lambda, uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters,
+                           uint32_t *flag, float *eta, uint32_t *done, uint32_t *err, bool stair, uint32_t poison,
                            hipStream_t st) {
-  if (bins_taken) *bins_taken = 0;
   if (n_teams == 0) return hipSuccess;
   if (q != 2 && q != 4) return hipErrorInvalidValue;
   const uint32_t per_launch = pcg_team_per_launch(q);  // teams per launch
   if (per_launch == 0) return hipErrorNotSupported;
-  constexpr uint32_t W = sizeof(TeamMailbox) / sizeof(unsigned long long);
-  // under stream capture: ordinary launches of the same grids (see the
-  // header comment), since a graph node carries no co-residency guarantee
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  {
-    const hipError_t ec = hipStreamIsCapturing(st, &cs);
-    if (ec != hipSuccess) return ec;
-  }
-  const bool capturing = cs == hipStreamCaptureStatusActive;
-  auto launch = [&](const void *fn, uint32_t grid, void **args, size_t smem) {
-    return capturing ? hipLaunchKernel(fn, dim3(grid), dim3(NT), args, smem, st)
-                     : hipLaunchCooperativeKernel(fn, dim3(grid), dim3(NT), args, (unsigned)smem, st);
-  };
   for (uint32_t team0 = 0; team0 < n_teams; team0 += per_launch) {
     const uint32_t n = n_teams - team0 < per_launch ? n_teams - team0 : per_launch;
     const uint32_t grid = 8u * q * ((n + 7u) / 8u);  // teams in whole groups of 8
     if (grid / q > n_mb) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_reset_mailboxes, dim3((grid / q * W + 255) / 256), dim3(256), 0, st, mb, grid / q, team_inst,
-                       team0, n, status, flag);
-    hipError_t e = hipGetLastError();
+    hipError_t e = team_reset(mb, grid / q, team_inst, team0, n, status, done, nullptr, st);
     if (e != hipSuccess) return e;
     Batch bb = b;
     uint32_t t0 = team0;
-    // the last launch takes the first bins into its spare workgroups
-    uint32_t nb = 0;
-    if (team0 + n == n_teams && items && bins_taken) {
-      uint32_t nl = 0;
-      pcg_team_plan(n_teams, q, n_bins, &nl, &nb);
-    }
-    if (nb == 0) {
-      void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, (void *)&S, (void *)&Pinv, (void *)&gamma,
-                      &lambda, &max_iter, &tol, (void *)&status, &iters, &flag, &eta, &err, &poison};
-      const void *fn = q == 4 ? (stair ? (const void *)k_pcg_team<true, 4> : (const void *)k_pcg_team<false, 4>)
-                              : (stair ? (const void *)k_pcg_team<true, 2> : (const void *)k_pcg_team<false, 2>);
-      e = launch(fn, grid, args, sizeof(TeamShared));
-    } else {
-      uint32_t tg = grid, bin0 = 0;
-      void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, &tg, (void *)&items, (void *)&bin_off, &bin0,
-                      (void *)&S, (void *)&Pinv, (void *)&gamma, &lambda, &max_iter, &tol, (void *)&status, &iters,
-                      &flag, &eta, &err, &poison};
-      const void *fn = stair ? (const void *)k_pcg_mixed<true> : (const void *)k_pcg_mixed<false>;
-      e = launch(fn, grid + nb, args, MIXED_SMEM);
-      if (e == hipSuccess) *bins_taken = nb;
-    }
+    void *args[] = {&bb, (void *)&team_inst, &n_teams, &t0, &mb, (void *)&S, (void *)&Pinv, (void *)&gamma,
+                    &lambda, &max_iter, &tol, (void *)&status, &iters, &flag, &eta, &done, &err, &poison};
+    const void *fn = q == 4 ? (stair ? (const void *)k_pcg_team<true, 4> : (const void *)k_pcg_team<false, 4>)
+                            : (stair ? (const void *)k_pcg_team<true, 2> : (const void *)k_pcg_team<false, 2>);
+    e = launch_coop(fn, grid, args, sizeof(TeamShared), st);
+    if (e == hipSuccess) e = team_finalize(team_inst, team0, n, q, status, done, flag, st);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+hipError_t launch_pcg_pool(const Batch &b, const uint32_t *team_inst, uint32_t n_teams, TeamMailbox *mb,
+                           uint32_t n_mb, PoolCounters *ctr, const uint4 *items, const uint32_t *bin_off,
+                           uint32_t n_bins, const float *S, const float *Pinv, const float *gamma, float *lambda,
+                           uint32_t max_iter, float tol, const uint32_t *status, uint32_t *iters, uint32_t *flag,
+                           float *eta, uint32_t *done, uint32_t *err, bool stair, uint32_t poison, hipStream_t st) {
+  if (n_teams == 0 && n_bins == 0) return hipSuccess;
+  const uint32_t grid = pcg_team_max_grid();
+  if (!pcg_pool_ok(n_teams, max_iter) || grid / 2 > n_mb) return hipErrorInvalidValue;
+  hipError_t e = team_reset(mb, grid / 2, team_inst, 0, n_teams, status, done, ctr, st);
+  if (e != hipSuccess) return e;
+  Batch bb = b;
+  void *args[] = {&bb,          (void *)&team_inst, &n_teams, &mb,     &ctr,    (void *)&items, (void *)&bin_off,
+                  &n_bins,      (void *)&S,         (void *)&Pinv, (void *)&gamma, &lambda, &max_iter, &tol,
+                  (void *)&status, &iters,          &flag,    &eta,    &done,   &err,           &poison};
+  e = launch_coop(stair ? (const void *)k_pcg_pool<true> : (const void *)k_pcg_pool<false>, grid, args, POOL_SMEM,
+                  st);
+  if (e == hipSuccess) e = team_finalize(team_inst, 0, n_teams, 2, status, done, flag, st);
+  return e;
 }
 
 }  // namespace mpc

@@ -103,7 +103,8 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
                                                    const uint32_t *__restrict__ status,
                                                    float *__restrict__ S, float *__restrict__ Pinv,
                                                    float *__restrict__ gamma, const float *__restrict__ Ginv,
-                                                   int zero_unused) {
+                                                   int out_mode) {
+  const bool zero_unused = out_mode & SCHUR_ZERO_UNUSED, right = out_mode & SCHUR_RIGHT_BLOCK;
   __shared__ SchurLds lds[GROUPS_PER_WG];
   const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
   const uint32_t gk_raw = blockIdx.x * GROUPS_PER_WG + grp;
@@ -220,8 +221,10 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
   if (valid && !row0) {
     store_block(lane, Srow, [&](int i) { return L.AQ[i]; });
     store_block(lane, Srow + NXX, [&](int i) { return L.A[i]; });
-    // right block of the previous row: S(b-1, b) = AQ^T
-    store_block(lane, S + (size_t)S_STRIDE * kprev + 2 * NXX, [&](int i) { return L.AQ[i / NX + NX * (i % NX)]; });
+    // right block of the previous row: S(b-1, b) = AQ^T (the reference layout;
+    // the solvers read it as the transpose of the next row's left block)
+    if (right)
+      store_block(lane, S + (size_t)S_STRIDE * kprev + 2 * NXX, [&](int i) { return L.AQ[i / NX + NX * (i % NX)]; });
     if (lane < NX) {
       float a1 = 0.f, a2 = 0.f, a3 = 0.f;
 #pragma unroll
@@ -234,11 +237,10 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
       gamma[(size_t)NX * row + lane] = L.cb[lane] - (a1 - a2 - a3);
     }
     if (zero_unused) {
+      if (bk + 1 == kr.N && right)
+        for (int i = lane; i < NXX; i += GROUP) Srow[2 * NXX + i] = 0.f;
       if (bk + 1 == kr.N)
-        for (int i = lane; i < NXX; i += GROUP) {
-          Srow[2 * NXX + i] = 0.f;
-          Prow[2 * NXX + i] = 0.f;
-        }
+        for (int i = lane; i < NXX; i += GROUP) Prow[2 * NXX + i] = 0.f;
       for (int i = lane; i < NXX; i += GROUP) {
         Prow[i] = 0.f;
         if (bk + 1 < kr.N) Prow[2 * NXX + i] = 0.f;
@@ -319,7 +321,7 @@ __global__ void __launch_bounds__(WG) k_compute_dz(Batch b, const float *__restr
 
 hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, const float *g,
                              const float *c, const float *rho_dev, float rho, const uint32_t *status,
-                             float *S, float *Pinv, float *gamma, float *Ginv, int zero_unused,
+                             float *S, float *Pinv, float *gamma, float *Ginv, int out_mode,
                              hipStream_t st) {
   if (b.total_knots == 0) return hipSuccess;
   const dim3 grid((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG);
@@ -327,7 +329,7 @@ hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, con
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_form_schur, grid, dim3(WG), 0, st, b, G, C, g, c, rho_dev, rho, status, S, Pinv, gamma,
-                     (const float *)Ginv, zero_unused);
+                     (const float *)Ginv, out_mode);
   return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ using namespace mpc;
 struct PcgPlan {
   uint32_t small_nt = 512;  // workgroup of the bins of N <= 64: 512 or 256 lanes
   uint32_t max_items = 8;   // instances per bin
-  bool mixed = true;        // the last team launch also solves the first bins
+  bool mixed = true;        // two-workgroup teams and 512-lane bins in one persistent pool launch
   bool teams = true;        // 128 < N <= 512 on teams of two or four workgroups (else streamed)
 };
 
@@ -54,6 +54,8 @@ struct mpc_handle {
   uint32_t n_bins = 0, n_bins_small = 0, n_big = 0, max_big_n = 0;
   // PCG teams: two workgroups for 128 < N <= 256, four for 256 < N <= 512
   uint32_t *team_list = nullptr, *team_one = nullptr, *team_err = nullptr;
+  uint32_t *pcg_done = nullptr;       // [B] parts that finished each team solve
+  PoolCounters *pool_ctr = nullptr;   // work queues of the PCG pool
   uint32_t *dir_order = nullptr;  // direct solve: instances longest first
   TeamMailbox *mbox = nullptr;
   uint32_t n_teams = 0, n_quads = 0, n_mbox = 256;  // team_list: n_teams pairs, then n_quads quads
@@ -158,8 +160,8 @@ static hipError_t pcg_single(mpc_handle *h, uint32_t N, const float *S, const fl
                             iters, flag, h->pcg_eta, stair, st);
     for (uint32_t q : {2u, 4u})
       if (N <= pcg_team_max_knots(q) && pcg_team_per_launch(q) > 0)
-        return launch_pcg_team(single(N), q, h->team_one, 1, h->mbox, h->n_mbox, nullptr, nullptr, 0, nullptr, S, P,
-                               gamma, lambda, max_iter, tol, nullptr, iters, flag, h->pcg_eta, h->team_err, stair,
+        return launch_pcg_team(single(N), q, h->team_one, 1, h->mbox, h->n_mbox, S, P, gamma, lambda, max_iter,
+                               tol, nullptr, iters, flag, h->pcg_eta, h->pcg_done, h->team_err, stair,
                                h->fault_inject, st);
   }
   return launch_pcg_stream(single(N), N, S, P, gamma, lambda, max_iter, tol, nullptr, nullptr, 0, iters, flag,
@@ -260,6 +262,7 @@ static void free_all(mpc_handle *h) {
                   h->merits, h->xs, h->drho, h->merit0, h->rho1, h->status_ws, h->sqp_done,
                   h->pcg_iters, h->pcg_flag, h->pcg_eta, h->t0, h->knot_inst, h->inst_k0, h->bin_items,
                   h->bin_off, h->big_list, h->team_list, h->team_one, h->team_err, h->mbox, h->dir_order,
+                  h->pcg_done, h->pool_ctr,
                   h->adj_mu, h->adj_knot, h->loss_knot, h->ref_g, h->ref_c, h->ref_dz, h->ref_lam,
                   h->ref_scale, h->cl_win, h->cl_xp, h->cl_rho, h->cl_ctr, h->cl_status, h->cl_pcg,
                   h->rho_one};
@@ -327,6 +330,8 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
             alloc((void **)&h->dir_order, sizeof(uint32_t) * B) &&
             alloc((void **)&h->team_one, sizeof(uint32_t)) &&
             alloc((void **)&h->team_err, sizeof(uint32_t)) &&
+            alloc((void **)&h->pcg_done, sizeof(uint32_t) * B) &&
+            alloc((void **)&h->pool_ctr, sizeof(PoolCounters)) &&
             alloc((void **)&h->mbox, sizeof(TeamMailbox) * h->n_mbox) &&
             alloc((void **)&h->adj_mu, sizeof(float) * NX * T) &&
             alloc((void **)&h->adj_knot, sizeof(float) * NXU * T) &&
@@ -471,7 +476,7 @@ int mpc_form_schur(mpc_handle *h, uint32_t N, float *d_G, const float *d_C, cons
     return MPC_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   HIP_TRY(launch_form_schur(single(N), d_G, d_C, d_g, d_c, nullptr, rho, nullptr, d_S, d_Pinv, d_gamma,
-                            h->Ginv, 1, st));
+                            h->Ginv, SCHUR_ZERO_UNUSED | SCHUR_RIGHT_BLOCK, st));
   HIP_TRY(hipMemcpyAsync(d_G, h->Ginv, sizeof(float) * (G_STRIDE * (size_t)N - 49),
                          hipMemcpyDeviceToDevice, st));
   return MPC_OK;
@@ -712,24 +717,27 @@ static int build_pcg_plan(mpc_handle *h) {
   tl.insert(tl.end(), quads.begin(), quads.end());
   if (!tl.empty()) HIP_TRY(hipMemcpy(h->team_list, tl.data(), sizeof(uint32_t) * tl.size(), hipMemcpyHostToDevice));
   // launch of each instance within a PCG phase: four-workgroup team
-  // launches 0 .. nq-1, two-workgroup ones nq .. nq+np-1 (the last also
-  // takes the first bins), then the streaming launch, then the resident
-  // launches (512-lane bins, then 256-lane bins)
-  uint32_t nq = 0, np = 0, taken = 0, unused = 0;
-  pcg_team_plan((uint32_t)quads.size(), 4, 0, &nq, &unused);
-  pcg_team_plan((uint32_t)teams.size(), 2, pl.mixed ? n512 : 0u, &np, &taken);
+  // launches 0 .. nq-1; then with the pool (plan.mixed) one launch nq for
+  // the two-workgroup teams and the 512-lane bins, else two-workgroup team
+  // launches nq .. nq+np-1 and the 512-lane bins after the streaming
+  // launch; the streaming launch; the 256-lane bins last
+  const uint32_t nq = pcg_team_launches((uint32_t)quads.size(), 4);
+  const bool pool = pl.mixed && pcg_pool_ok((uint32_t)teams.size(), h->cfg.pcg_max_iter) &&
+                    (!teams.empty() || n512 > 0);
+  const uint32_t np = pool ? 1u : pcg_team_launches((uint32_t)teams.size(), 2);
   h->pcg_launch.assign(n_inst, 0u);
   for (size_t t = 0; t < quads.size(); ++t) h->pcg_launch[quads[t]] = (uint32_t)(t / pcg_team_per_launch(4));
-  for (size_t t = 0; t < teams.size(); ++t) h->pcg_launch[teams[t]] = nq + (uint32_t)(t / pcg_team_per_launch(2));
+  for (size_t t = 0; t < teams.size(); ++t)
+    h->pcg_launch[teams[t]] = nq + (pool ? 0u : (uint32_t)(t / pcg_team_per_launch(2)));
   const uint32_t nl = nq + np;
   uint32_t next = nl;
   if (!big.empty())
     for (uint32_t i : big) h->pcg_launch[i] = next;
   next += big.empty() ? 0u : 1u;
-  const uint32_t res512 = next;
-  next += n512 > taken ? 1u : 0u;
+  const uint32_t res512 = pool ? nq : next;
+  next += (!pool && n512 > 0) ? 1u : 0u;
   for (size_t w = 0; w < bins.size(); ++w)
-    for (uint32_t i : bins[w].inst) h->pcg_launch[i] = w < taken ? nl - 1 : (w < n512 ? res512 : next);
+    for (uint32_t i : bins[w].inst) h->pcg_launch[i] = w < n512 ? res512 : next;
   h->n_teams = (uint32_t)teams.size();
   h->n_quads = (uint32_t)quads.size();
   h->team_launches = nl;
@@ -883,11 +891,12 @@ int mpc_batch_autotune(mpc_handle *h, uint32_t iters, uint32_t reps, mpc_pcg_pla
   if (h->cfg.flags & (FLAG_PCG_STREAM | FLAG_DIRECT_SOLVE)) return MPC_ERR_ARG;  // no launch plan in these modes
   hipStream_t st = (hipStream_t)stream;
   const Batch b = batch_of(h);
-  bool has_small = false, has_bins = false, has_team = false;
+  bool has_small = false, has_bins = false, has_team = false, has_pair = false;
   for (uint32_t n : h->horizons) {
     has_small |= n <= pcg_res_max_knots(256);
     has_bins |= n <= pcg_res_max_knots(512);
     has_team |= n > pcg_res_max_knots(512) && n <= pcg_team_max_knots(4);
+    has_pair |= n > pcg_res_max_knots(512) && n <= pcg_team_max_knots(2);
   }
   has_team = has_team && pcg_team_per_launch(4) > 0;
   std::vector<PcgPlan> cands;
@@ -897,8 +906,8 @@ int mpc_batch_autotune(mpc_handle *h, uint32_t iters, uint32_t reps, mpc_pcg_pla
       if (mi != 8 && !has_bins) continue;
       for (int teams = 1; teams >= 0; --teams) {
         if (!teams && !has_team) continue;
-        for (int mixed = 1; mixed >= 0; --mixed) {
-          if (!mixed && !(teams && has_team && has_bins)) continue;
+        for (int mixed = 1; mixed >= 0; --mixed) {  // the pool, or static launches
+          if (!mixed && !((teams && has_pair) || has_bins)) continue;
           PcgPlan c;
           c.small_nt = nt;
           c.max_items = mi;
@@ -1007,21 +1016,28 @@ static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_st
     if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
     return e;
   }
-  uint32_t taken = 0;
-  e = launch_pcg_team(b, 4, h->team_list + h->n_teams, h->n_quads, h->mbox, h->n_mbox, nullptr, nullptr, 0, nullptr,
-                      h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, h->pcg_iters, h->pcg_flag,
-                      h->pcg_eta, h->team_err, stair, h->fault_inject, st);
-  if (e == hipSuccess)
-    e = launch_pcg_team(b, 2, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->plan.mixed ? h->bin_items : nullptr,
-                      h->bin_off, h->n_bins, &taken, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status,
-                      h->pcg_iters, h->pcg_flag, h->pcg_eta, h->team_err, stair, h->fault_inject, st);
+  // four-workgroup teams, then the two-workgroup teams and the 512-lane bins
+  // (one persistent pool launch, or team launches and a bin launch), then
+  // the streamed instances and the 256-lane bins
+  e = launch_pcg_team(b, 4, h->team_list + h->n_teams, h->n_quads, h->mbox, h->n_mbox, h->S, h->P, h->gamma,
+                      d_lambda, max_iter, tol, d_status, h->pcg_iters, h->pcg_flag, h->pcg_eta, h->pcg_done,
+                      h->team_err, stair, h->fault_inject, st);
+  const bool pool = h->plan.mixed && pcg_pool_ok(h->n_teams, max_iter);
+  if (e == hipSuccess && pool)
+    e = launch_pcg_pool(b, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->pool_ctr, h->bin_items, h->bin_off,
+                        h->n_bins, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, h->pcg_iters, h->pcg_flag,
+                        h->pcg_eta, h->pcg_done, h->team_err, stair, h->fault_inject, st);
+  if (e == hipSuccess && !pool)
+    e = launch_pcg_team(b, 2, h->team_list, h->n_teams, h->mbox, h->n_mbox, h->S, h->P, h->gamma, d_lambda,
+                        max_iter, tol, d_status, h->pcg_iters, h->pcg_flag, h->pcg_eta, h->pcg_done, h->team_err,
+                        stair, h->fault_inject, st);
   if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
   if (e == hipSuccess)
     e = launch_pcg_stream(b, h->max_big_n, h->S, h->P, h->gamma, d_lambda, max_iter, tol, d_status, h->big_list,
                           h->n_big, h->pcg_iters, h->pcg_flag, h->pcg_eta, stair, st);
-  if (e == hipSuccess)
-    e = launch_pcg_res(b, h->bin_items, h->bin_off, taken, h->n_bins - taken, 512, h->S, h->P, h->gamma, d_lambda,
-                       max_iter, tol, d_status, h->pcg_iters, h->pcg_flag, h->pcg_eta, stair, st);
+  if (e == hipSuccess && !pool)
+    e = launch_pcg_res(b, h->bin_items, h->bin_off, 0, h->n_bins, 512, h->S, h->P, h->gamma, d_lambda, max_iter,
+                       tol, d_status, h->pcg_iters, h->pcg_flag, h->pcg_eta, stair, st);
   if (e == hipSuccess)
     e = launch_pcg_res(b, h->bin_items, h->bin_off, h->n_bins, h->n_bins_small, 256, h->S, h->P, h->gamma,
                        d_lambda, max_iter, tol, d_status, h->pcg_iters, h->pcg_flag, h->pcg_eta, stair, st);

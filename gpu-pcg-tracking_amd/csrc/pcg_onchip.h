@@ -34,9 +34,9 @@ constexpr int L_SLOT = NX * L_LANE;     // 252 floats per knot slot (lanes 14, 1
 constexpr int L_GUARD = 2 * L_LANE;     // finite tail read by lanes 14, 15 past the last slot
 
 // direction of row_ror on this device: src(i, 1) - i (mod 16), 1 or 15.
-// Must run with every lane of the row active.
-__device__ __forceinline__ uint32_t ror_step() {
-  const int i = (int)(threadIdx.x & 15u);
+// Must run with every lane of the row active; tid = the lane's thread index.
+__device__ __forceinline__ uint32_t ror_step(uint32_t tid) {
+  const int i = (int)(tid & 15u);
   const int s = __builtin_amdgcn_mov_dpp(i, 0x121, 0xF, 0xF, true);
   return (uint32_t)(s - i) & 15u;
 }

@@ -63,9 +63,12 @@ __device__ __forceinline__ float seg_sum(const float *red, uint32_t w0, uint32_t
 }
 
 // The PCG of bin `bin` (its instances: items[bin_off[bin] .. bin_off[bin+1]))
-// on an NT-lane workgroup with the ResShared<NT> block at smem.
+// on an NT-lane workgroup with the ResShared<NT> block at smem; tid is
+// threadIdx.x (passed in so that a caller's loop can keep everything
+// derived from it inside the loop).
 template <bool STAIR, int NT>
-__device__ __forceinline__ void body(uint32_t bin, unsigned char *smem, Batch b, const uint4 *__restrict__ items,
+__device__ __forceinline__ void body(uint32_t tid, uint32_t bin, unsigned char *smem, Batch b,
+                                     const uint4 *__restrict__ items,
                                      const uint32_t *__restrict__ bin_off, const float *__restrict__ S,
                                      const float *__restrict__ Pinv, const float *__restrict__ gamma,
                                      float *__restrict__ lambda, uint32_t max_iter, float tol,
@@ -73,8 +76,8 @@ __device__ __forceinline__ void body(uint32_t bin, unsigned char *smem, Batch b,
                                      uint32_t *__restrict__ flag_out, float *__restrict__ eta_out) {
   constexpr int NGRP = ResShape<NT>::NGRP, NWAV = ResShape<NT>::NWAV, KNOTS = ResShape<NT>::KNOTS;
   ResShared<NT> &sh = *reinterpret_cast<ResShared<NT> *>(smem);
-  const uint32_t tid = threadIdx.x, g = tid >> 4, i = tid & 15u, w = tid >> 6;
-  const uint32_t d = ror_step();
+  const uint32_t g = tid >> 4, i = tid & 15u, w = tid >> 6;
+  const uint32_t d = ror_step(tid);
   {
     float4 *z4 = reinterpret_cast<float4 *>(sh.L);
     constexpr uint32_t n4 = ((KNOTS + 1) * L_SLOT + L_GUARD + (KNOTS + 1) * 16) / 4;

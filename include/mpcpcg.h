@@ -284,12 +284,14 @@ int mpc_batch_set_timing(mpc_handle *h, int enable);
 int mpc_batch_set_graph(mpc_handle *h, int enable);
 /* PCG packing of the batch layout: out4 = {resident-kernel bins, teams
  * (two workgroups for N <= 256, four for N <= 512), instances on the
- * streaming kernel, cooperative team launches per PCG phase}. */
+ * streaming kernel, cooperative launches per PCG phase before the first
+ * non-cooperative one (four-workgroup team launches, then the pool or the
+ * two-workgroup team launches)}. */
 int mpc_batch_pcg_layout(const mpc_handle *h, uint32_t *out4);
 /* The PCG launch that solves each instance (n_inst values) within one PCG
- * phase: 0 .. L-1 the cooperative team launches (four-workgroup teams
- * first; the last two-workgroup launch also solves the first resident bins
- * in its spare workgroups), L the streaming launch
+ * phase: 0 .. L-1 the cooperative launches (four-workgroup teams first,
+ * then the pool of two-workgroup teams and 512-lane bins, or the
+ * two-workgroup team launches), L the streaming launch
  * when there are instances beyond the teams' reach, then the launches of the
  * remaining resident bins (512-lane bins, then 256-lane bins); L = out4[3]
  * of mpc_batch_pcg_layout.  All 0 with
@@ -303,11 +305,14 @@ int mpc_batch_pcg_launch_map(const mpc_handle *h, uint32_t *launch_of_inst);
  * CU; with small_nt = 256, instances with N <= 64 go to bins of 256 lanes =
  * 64 knots, two per CU), at most max_items instances per bin; 128 < N <= 256
  * run on two-workgroup and 256 < N <= 512 on four-workgroup teams
- * (teams = 1) or on the streaming kernel; with mixed = 1 the last
- * two-workgroup team launch also solves the first bins in its spare
- * workgroups.  Every plan gives the same arithmetic per instance (bins and
- * teams: bitwise), only the packing changes.  Default {512, 8, 1, 1}.  No
- * counterpart in the reference (one instance per solve). */
+ * (teams = 1) or on the streaming kernel.  mixed = 1: the two-workgroup
+ * teams and the 512-lane bins share one persistent launch of one workgroup
+ * per CU that takes them from work queues (teams first, bins largest
+ * first), so a CU whose item converged early takes the next one; mixed = 0:
+ * static team launches, then one bin launch.  Every plan gives the same
+ * arithmetic per instance (bins and teams: bitwise), only the packing and
+ * the schedule change.  Default {512, 8, 1, 1}.  No counterpart in the
+ * reference (one instance per solve). */
 typedef struct mpc_pcg_plan {
   uint32_t small_nt;
   uint32_t max_items;

@@ -83,7 +83,8 @@ def test_autotune_picks_the_fastest_plan(pkg, orc, tsolver):
     s.batch_set_plan(**DEFAULT)
     res = s.batch_autotune(iters=8, reps=2)
     print("\n".join(str(r) for r in res))
-    assert len(res) == 2 * 3 * 3  # bin shape x instances per bin x (teams+mixed, teams, streamed)
+    # bin shape x instances per bin x (pool or static launches) x (teams or streamed)
+    assert len(res) == 2 * 3 * 4
     assert all(r["us"] > 0 for r in res)
     best = min(res, key=lambda r: r["us"])
     assert s.batch_get_plan() == {k: best[k] for k in ("small_nt", "max_items", "mixed", "teams")}

@@ -11,12 +11,12 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 
-def test_launch_map_and_every_instance_solved(pkg, orc):
+@pytest.mark.parametrize("pool", [1, 0])
+def test_launch_map_and_every_instance_solved(pkg, orc, pool):
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    # 140 two-workgroup teams (two launches at 128 teams per launch), two
-    # four-workgroup teams, bins of every packing (one 128, two 64s,
-    # several short instances)
+    # 140 two-workgroup teams (more than one launch's 128), two four-workgroup
+    # teams, bins of every packing (one 128, two 64s, several short instances)
     horizons = [200] * 140 + [64] * 20 + [128] * 10 + [300] * 2 + [3] * 5
     rng = np.random.default_rng(7)
     horizons = [int(h) for h in rng.permutation(horizons)]
@@ -25,25 +25,27 @@ def test_launch_map_and_every_instance_solved(pkg, orc):
     s = pkg.Solver(cfg)
     try:
         s.batch_setup(horizons)
+        s.batch_set_plan(mixed=pool)
         lay = s.batch_pcg_layout()
         L = lay["team_launches"]
         m = s.batch_pcg_launch_map()
         h = np.asarray(horizons)
         print("layout", lay, "launch counts", np.bincount(m))
         team, quad, res = (h > 128) & (h <= 256), (h > 256) & (h <= 512), h <= 128
-        assert lay["teams"] == team.sum() + quad.sum() and lay["streamed"] == 0 and L >= 2
-        # four-workgroup teams first, then the two-workgroup ones
-        assert (m[quad] == 0).all() and (m[team] >= 1).all() and (m[team] < L).all()
-        # teams fill the launches in instance order, at most one launch's share each
-        per = np.bincount(m[team] - 1, minlength=L - 1)
-        assert per[:-1].min() == per.max() and per[-1] <= per[0]
-        assert np.all(np.diff(m[team].astype(np.int64)) >= 0)
-        rest = L
-        assert np.isin(m[res], [L - 1, rest]).all()
-        taken, left = h[res & (m == L - 1)], h[res & (m == rest)]
-        assert taken.size > 0
-        if left.size:
-            assert taken.min() >= left.max()  # the longest bins go into the team launch
+        assert lay["teams"] == team.sum() + quad.sum() and lay["streamed"] == 0
+        # four-workgroup teams first
+        assert (m[quad] == 0).all()
+        if pool:
+            # one persistent launch for the two-workgroup teams and the bins
+            assert L == 2 and (m[team] == 1).all() and (m[res] == 1).all()
+        else:
+            # team launches in instance order, at most one launch's share each,
+            # then one launch of the bins
+            assert L == 3 and (m[team] >= 1).all() and (m[team] < L).all()
+            per = np.bincount(m[team] - 1, minlength=L - 1)
+            assert per[:-1].min() == per.max() and per[-1] <= per[0]
+            assert np.all(np.diff(m[team].astype(np.int64)) >= 0)
+            assert (m[res] == L).all()
         # a batched solve touches every instance: each one's PCG ran (or it
         # started converged) and its multipliers were written
         xu = torch.zeros(21 * T - 7 * n, device="cuda")
