@@ -712,8 +712,17 @@ def main():
     # knot-iterations of every instance those launches solve and the phase
     # time, divided by that count
     layout = solver.batch_pcg_layout()
-    launches = max(layout["team_launches"], 1)
-    in_team_phase = solver.batch_pcg_launch_map() < layout["team_launches"]
+    plan = solver.batch_get_plan()
+    # block Jacobi with every N <= 256: one split-form launch solves every
+    # instance (its phase ends at the same event)
+    split_on = bool(plan["split"]) and int(horizons.max()) <= 256
+    if split_on:
+        kernel, launches = "k_pcg_split", 1
+        in_team_phase = np.ones(n, bool)
+    else:
+        kernel = "k_pcg_pool" if plan["mixed"] else "k_pcg_team"
+        launches = max(layout["team_launches"], 1)
+        in_team_phase = solver.batch_pcg_launch_map() < layout["team_launches"]
     units = float(sum(int(horizons[i]) * int(log_t[i, j]) for i in range(n) for j in range(K)
                       if in_team_phase[i] and log_t[i, j] >= 0)) / K
     t_phase = ph[2] / K
@@ -721,10 +730,11 @@ def main():
     units_launch = units / launches
     achieved = PCG_BYTES_PER_KNOT_ITER * units / (t_phase * 1e-6) / 1e9
     tflops = PCG_FLOPS_PER_KNOT_ITER * units / (t_phase * 1e-6) / 1e12
+    # the PMC figures of the same kernel (profiles/), if they are of it
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("team_phase", {}).get("hbm_bytes_per_launch")
+            traffic = json.load(f).get("kernels", {}).get(kernel, {}).get("hbm_bytes_per_launch")
     counters = None
     if os.path.exists(COUNTERS_FILE):
         with open(COUNTERS_FILE) as f:
@@ -740,8 +750,8 @@ def main():
     proof = phase_roofline(phases, knots_it, pcg_it, measure_bandwidth(pkg, torch), counters)
     lds_peak = proof["bandwidth"]["lds_gbs"]
     lds_counted = None
-    if counters and "team_phase" in counters:
-        tp = counters["team_phase"]
+    if counters and counters.get("pcg_kernel", {}).get("kernel") == kernel:
+        tp = counters["pcg_kernel"]
         b_launch = tp["lds_wave_instructions_per_launch"] * tp["lds_bytes_per_wave_instruction"]
         lds_counted = {"bytes_per_launch": int(b_launch), "achieved": round(b_launch / (t_launch * 1e-6) / 1e9, 1),
                        "peak": lds_peak, "unit": "GB/s", "frac": round(b_launch / (t_launch * 1e-6) / 1e9 / lds_peak, 4),
@@ -858,7 +868,7 @@ def main():
                   "eager_kkt_solves_per_sec": round(kkt_per_step / (ms_eager * 1e-3), 1),
                   "note": "eager = the same launch sequence without capture, PCG teams as cooperative launches; "
                           "in the graph they are ordinary kernel nodes (bounded waits, fault status)"},
-        "roofline": {"kernel": "k_pcg_pool" if solver.batch_get_plan()["mixed"] else "k_pcg_team",
+        "roofline": {"kernel": kernel,
                      "bound": "lds", "achieved": round(achieved, 1),
                      "peak": lds_peak, "unit": "GB/s", "frac": round(achieved / lds_peak, 4),
                      "peak_source": "LDS bandwidth measured on this device in this run (mpc_bw_lds, 128-bit reads)",
@@ -866,10 +876,12 @@ def main():
                      "knot_iterations_per_launch": int(units_launch), "avg_launch_us": round(t_launch, 2),
                      "launches_per_sqp_iter": launches, "pcg_layout": layout,
                      "note": "algorithmic bytes = 2296 B per knot and PCG iteration (S, Pinv and vectors) x the "
-                             "knot-iterations of one launch; S and Pinv stay on chip (LDS and registers), so the "
+                             "knot-iterations of one launch; the system stays on chip (LDS and registers), so the "
                              "roof is the LDS rate; traffic = HBM bytes per launch (PMC), the one-time load; "
-                             "k_pcg_pool is one persistent launch per PCG phase for the two-workgroup teams and the "
-                             "512-lane resident bins (work queues), k_pcg_team the static team launches",
+                             "k_pcg_split: block-Jacobi PCG as CG on W^T S W (one 14x14 block per knot on chip, "
+                             "two block products per iteration: fewer bytes and flops than the algorithmic count), "
+                             "one launch over bins of 1024-lane workgroups; k_pcg_pool / k_pcg_team: the reference "
+                             "form (S, Pinv on chip) on teams and bins",
                      "hbm": None if traffic is None else
                      {"achieved": round(traffic / (t_launch * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": round(traffic / (t_launch * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},

@@ -102,11 +102,11 @@ class PcgPlan(C.Structure):
     """mpc_pcg_plan: launch plan of the batched PCG."""
     _fields_ = [
         ("small_nt", C.c_uint32), ("max_items", C.c_uint32), ("mixed", C.c_uint32), ("teams", C.c_uint32),
-        ("us", C.c_float), ("reserved", C.c_uint32),
+        ("us", C.c_float), ("split", C.c_uint32),
     ]
 
 
-_PLAN_KEYS = ("small_nt", "max_items", "mixed", "teams")
+_PLAN_KEYS = ("small_nt", "max_items", "mixed", "teams", "split")
 
 
 _lib = None
@@ -531,7 +531,7 @@ class Solver:
         return out
 
     def batch_get_plan(self):
-        """The current PCG launch plan {small_nt, max_items, mixed, teams}."""
+        """The current PCG launch plan {small_nt, max_items, mixed, teams, split}."""
         p = PcgPlan()
         _check(lib().mpc_batch_get_plan(self.h, C.byref(p)), "mpc_batch_get_plan")
         return {k: int(getattr(p, k)) for k in _PLAN_KEYS}

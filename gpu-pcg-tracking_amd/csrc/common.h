@@ -144,8 +144,9 @@ hipError_t launch_merit(const Batch &b, const Params &p, const float *xu, const 
 // right slot of row b-1 (the reference layout; the solvers read the
 // transpose of the next row's left block instead), SCHUR_ZERO_UNUSED
 // writes zeros into the blocks that are structurally zero (S(0,-1), the
-// right block of the last row, Pinv's off-diagonal blocks)
-constexpr int SCHUR_ZERO_UNUSED = 1, SCHUR_RIGHT_BLOCK = 2;
+// right block of the last row, Pinv's off-diagonal blocks), SCHUR_NO_PINV
+// skips Pinv (the split-form PCG has no use for it)
+constexpr int SCHUR_ZERO_UNUSED = 1, SCHUR_RIGHT_BLOCK = 2, SCHUR_NO_PINV = 4;
 hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, const float *g,
                              const float *c, const float *rho_dev, float rho, const uint32_t *status,
                              float *S, float *Pinv, float *gamma, float *Ginv, int out_mode,
@@ -198,6 +199,20 @@ uint32_t pcg_team_max_grid();
 uint32_t pcg_team_per_launch(uint32_t q);
 uint32_t pcg_team_launches(uint32_t n_teams, uint32_t q);
 bool pcg_pool_ok(uint32_t n_teams, uint32_t max_iter);
+// Block-Jacobi PCG in split-preconditioned form (k_pcg_split.hip), N <= 256:
+// launch_split_prep turns S (left and diagonal blocks) and gamma into the
+// split system (rows of Ps: L^_b at 0, W_b at NXX; ghat = W^T gamma), then
+// launch_pcg_split solves bins of instances (one 1024-lane workgroup per
+// bin; items / bin_off as for the resident kernel) and writes lambda = W
+// lambda^.  Same outputs as launch_pcg_res.
+hipError_t launch_split_prep(const Batch &b, const float *S, const float *gamma, const uint32_t *status, float *Ps,
+                             float *ghat, hipStream_t st);
+hipError_t launch_pcg_split(const Batch &b, const uint4 *items, const uint32_t *bin_off, uint32_t n_bins,
+                            const float *Ps, const float *ghat, float *lambda, uint32_t max_iter, float tol,
+                            const uint32_t *status, uint32_t *iters, uint32_t *flag, float *eta, hipStream_t st);
+uint32_t pcg_split_max_knots();
+uint32_t pcg_split_bin_groups();
+uint32_t pcg_split_max_items();
 uint32_t pcg_res_groups(uint32_t N);
 uint32_t pcg_res_bin_groups(uint32_t nt);  // 16-lane groups per workgroup of nt lanes
 uint32_t pcg_res_max_knots(uint32_t nt);

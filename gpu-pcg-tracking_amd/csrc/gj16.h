@@ -15,6 +15,29 @@ __device__ __forceinline__ float bcast16(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x10 | (P << 5)));
 }
 
+// bcast16 with the lane p given as a value that is a constant after loop
+// unrolling (ds_swizzle takes its pattern as an immediate)
+__device__ __forceinline__ float bcast_dyn16(float v, int p) {
+  switch (p) {
+    case 0: return bcast16<0>(v);
+    case 1: return bcast16<1>(v);
+    case 2: return bcast16<2>(v);
+    case 3: return bcast16<3>(v);
+    case 4: return bcast16<4>(v);
+    case 5: return bcast16<5>(v);
+    case 6: return bcast16<6>(v);
+    case 7: return bcast16<7>(v);
+    case 8: return bcast16<8>(v);
+    case 9: return bcast16<9>(v);
+    case 10: return bcast16<10>(v);
+    case 11: return bcast16<11>(v);
+    case 12: return bcast16<12>(v);
+    case 13: return bcast16<13>(v);
+    case 14: return bcast16<14>(v);
+    default: return bcast16<15>(v);
+  }
+}
+
 // Materialise values at this point of the program: keeps the compiler from
 // sinking their computation to a later (conditional) use, which would keep
 // all of its inputs alive in registers.

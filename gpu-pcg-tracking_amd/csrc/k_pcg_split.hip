@@ -1,0 +1,424 @@
+// k_pcg_split.hip -- block-Jacobi PCG in split-preconditioned form for
+// gfx950: one 1024-lane workgroup per bin of instances (N <= 256 each).
+//
+// Block-Jacobi PCG on S lambda = gamma (preconditioner Pinv = S(b,b)^-1) is,
+// in exact arithmetic, plain CG on the transformed system
+//     S^ lambda^ = gamma^,  S^ = W^T S W,  gamma^ = W^T gamma,  lambda = W lambda^,
+// with W = diag(W_b), W_b = U_b^-1 and -S(b,b) = theta_b = U_b^T U_b
+// (Cholesky).  S^ has -I on its diagonal and L^_b = W_b^T S(b,b-1) W_{b-1}
+// below it; the preconditioner becomes -I, and the residuals satisfy
+// r^ = W^T r, so r^T z^ = r^T Pinv r: the exit test |r^T z| < tol and every
+// alpha, beta are those of the reference's PCG (GPU-PCG pcg<T,14,N>, call site
+// toplevel.cuh:143-157), and the iterates map to its iterates.  Only L^ is
+// kept on chip (one 14 x 14 block per knot instead of S(b,b), Pinv(b,b) and
+// S(b,b-1)), and an iteration takes two block products instead of four.
+//
+// k_split_prep forms L^_b, W_b and gamma^_b from S and gamma (one 16-lane
+// group per knot, W_b by Gauss elimination of theta_b without pivoting:
+// theta is symmetric positive definite); k_pcg_split runs the iteration.
+//
+// Layout of the split system (the Pinv buffer's rows): L^_b at offset 0 of
+// row b, W_b at offset NXX (both 14 x 14, column-major), gamma^ in its own
+// buffer.
+//
+// Workgroup: 64 groups of 16 lanes (16 waves), group g serving knots
+// 4g .. 4g+3 of its instance, lane i element i; up to 256 knots, bins of
+// up to 8 instances, each on whole waves (as in pcg_res_body.h).  Per
+// group: L^ rows of its last three knots in registers (pre-rotated for the
+// DPP products of pcg_onchip.h), those of its first knot in an LDS slot,
+// read by the group itself and by the group before it (that group's
+// S^(b,b+1) = L^_{b+1}^T term).
+#include "common.h"
+#include "gj16.h"
+#include "pcg_onchip.h"
+
+namespace mpc {
+
+namespace {
+
+using namespace onchip;
+
+constexpr int GROUP = 16;
+
+// ---------------------------------------------------------------- prep ----
+// Workgroups of PG groups overlap by one knot: group g of workgroup w serves
+// row kg = (PG - 1) w + g - 1.  Every group forms W of its row; groups
+// g >= 1 then take W of the row before from group g - 1 through LDS.
+constexpr int PG = 16;
+constexpr int PWG = PG * GROUP;
+
+struct alignas(16) PrepLds {
+  float W[NXX];  // W_kg, column-major
+  float L[NXX];  // S(kg, kg-1), column-major
+  float gm[16];
+};
+
+__global__ void __launch_bounds__(PWG) k_split_prep(Batch b, const float *__restrict__ S,
+                                                    const float *__restrict__ gamma,
+                                                    const uint32_t *__restrict__ status, float *__restrict__ Ps,
+                                                    float *__restrict__ ghat) {
+  __shared__ PrepLds lds[PG];
+  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
+  const int64_t kg_raw = (int64_t)blockIdx.x * (PG - 1) + grp - 1;
+  const bool in_range = kg_raw >= 0 && kg_raw < (int64_t)b.total_knots;
+  const uint32_t kg = in_range ? (uint32_t)kg_raw : 0u;
+  const KnotRef kr = locate(b, kg);
+  const bool active = in_range && !(status && !(status[kr.inst] & ST_ACTIVE));
+  const bool owner = grp > 0 && active;
+  PrepLds &L = lds[grp];
+  const int j = lane < NX ? lane : NX - 1;
+  const float *Sr = S + (size_t)S_STRIDE * kg;
+  // 1. theta = -S(kg,kg) by columns; forward elimination of [theta | I]:
+  //    after step p, row p of [theta | I] scaled by theta_p(p,p)^-1/2 is row
+  //    p of [U | W^T] (U^T U = theta, W = U^-1); lane j keeps column j.
+  float a[NX], x[NX];
+#pragma unroll
+  for (int r = 0; r < NX; ++r) {
+    a[r] = -Sr[NXX + r + NX * j];
+    x[r] = (r == j) ? 1.f : 0.f;
+  }
+#pragma unroll
+  for (int p = 0; p < NX; ++p) {
+    float pc[NX];
+#pragma unroll
+    for (int r = p; r < NX; ++r) pc[r] = bcast_dyn16(a[r], p);
+    const float inv = 1.f / pc[p];
+    const float s = 1.f / sqrtf(pc[p]);
+    const float ap = a[p], xp = x[p];
+#pragma unroll
+    for (int r = p + 1; r < NX; ++r) {
+      const float m = pc[r] * inv;
+      a[r] -= m * ap;
+      x[r] -= m * xp;
+    }
+    a[p] = ap * s;
+    x[p] = xp * s;
+    pin(a);
+    pin(x);
+  }
+  // x[c] = W^T(c, j) = W(j, c): lane j holds row j of W
+  if (lane < NX) {
+#pragma unroll
+    for (int c = 0; c < NX; ++c) L.W[lane + NX * c] = x[c];
+  }
+  const bool row0 = kr.k == 0;
+  for (int i = lane; i < NXX; i += GROUP) L.L[i] = row0 ? 0.f : Sr[i];
+  if (lane < NX) L.gm[lane] = gamma[(size_t)NX * kg + lane];
+  __syncthreads();
+  // 2. row kg: M = S(kg,kg-1) W_{kg-1} (column j per lane), L^ = W_kg^T M,
+  //    gamma^ = W_kg^T gamma_kg
+  float *Pr = Ps + (size_t)S_STRIDE * kg;
+  if (owner) {
+    const PrepLds &Pv = lds[grp - 1];
+    float m[NX];
+#pragma unroll
+    for (int r = 0; r < NX; ++r) m[r] = 0.f;
+    if (!row0) {
+#pragma unroll
+      for (int k = 0; k < NX; ++k) {
+        const float wkj = Pv.W[k + NX * j];
+#pragma unroll
+        for (int r = 0; r < NX; ++r) m[r] += L.L[r + NX * k] * wkj;
+      }
+    }
+    float lh[NX];
+#pragma unroll
+    for (int r = 0; r < NX; ++r) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < NX; ++k) acc += L.W[k + NX * r] * m[k];
+      lh[r] = acc;
+    }
+    if (lane < NX) {
+#pragma unroll
+      for (int r = 0; r < NX; ++r) Pr[r + NX * lane] = lh[r];
+      float gacc = 0.f;
+#pragma unroll
+      for (int k = 0; k < NX; ++k) gacc += L.W[k + NX * lane] * L.gm[k];
+      ghat[(size_t)NX * kg + lane] = gacc;
+    }
+    for (int i = lane; i < NXX; i += GROUP) Pr[NXX + i] = L.W[i];
+  }
+}
+
+// --------------------------------------------------------------- solve ----
+constexpr int NT = 1024;
+constexpr int NGRP = NT / 16;  // 64 groups
+constexpr int NWAV = NT / 64;  // 16 waves
+constexpr int RK = 4;
+constexpr int KNOTS = NGRP * RK;  // 256
+constexpr int MAXI = 8;
+
+struct SplitShared {
+  float L[(NGRP + 1) * L_SLOT + L_GUARD];  // L^ rows of each group's first knot (+ a zero slot)
+  float zb[(KNOTS + 1) * 16];               // staged vector element per knot slot and lane
+  float red[2][NWAV];
+  uint32_t act[NWAV];
+};
+static_assert(((NGRP + 1) * L_SLOT + L_GUARD) % 4 == 0, "L and zb are cleared as one float4 range");
+
+__device__ __forceinline__ float seg_sum(const float *red, uint32_t w0, uint32_t nw) {
+  float s = 0.f;
+  for (uint32_t v = 0; v < nw; ++v) s += red[w0 + v];
+  return s;
+}
+
+__device__ __forceinline__ void load_lrows(float (&Lr)[16], bool has, uint32_t i, uint32_t d,
+                                           const float *__restrict__ Lh) {
+  const bool row = has && i < (uint32_t)NX;
+#pragma unroll
+  for (int n = 0; n < 16; ++n) {
+    const uint32_t j = (i + (uint32_t)n * d) & 15u;
+    Lr[n] = row && j < (uint32_t)NX ? Lh[i + NX * j] : 0.f;
+  }
+}
+
+__global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restrict__ items,
+                                                  const uint32_t *__restrict__ bin_off,
+                                                  const float *__restrict__ Ps, const float *__restrict__ ghat,
+                                                  float *__restrict__ lambda, uint32_t max_iter, float tol,
+                                                  const uint32_t *__restrict__ status,
+                                                  uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out,
+                                                  float *__restrict__ eta_out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  SplitShared &sh = *reinterpret_cast<SplitShared *>(smem_raw);
+  const uint32_t tid = threadIdx.x, bin = blockIdx.x;
+  const uint32_t g = tid >> 4, i = tid & 15u, w = tid >> 6;
+  const uint32_t d = ror_step(tid);
+  {
+    float4 *z4 = reinterpret_cast<float4 *>(sh.L);
+    constexpr uint32_t n4 = ((NGRP + 1) * L_SLOT + L_GUARD + (KNOTS + 1) * 16) / 4;
+    for (uint32_t idx = tid; idx < n4; idx += NT) z4[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  // my instance in this bin (items: {inst, first group, groups, -})
+  const uint32_t i0 = bin_off[bin], ni = bin_off[bin + 1] - i0;
+  int slot = -1;
+  uint32_t inst = 0, g0 = 0, ng = NGRP;
+  for (uint32_t jx = 0; jx < ni; ++jx) {
+    const uint4 it = items[i0 + jx];
+    if (g >= it.y && g < it.y + it.z) {
+      slot = (int)jx;
+      inst = it.x;
+      g0 = it.y;
+      ng = it.z;
+    }
+  }
+  const bool mine = slot >= 0;
+  const uint32_t N = mine ? inst_len(b, inst) : 0u;
+  const uint32_t K0 = mine ? inst_first(b, inst) : 0u;
+  const bool live = mine && !(status && !(status[inst] & ST_ACTIVE));
+  const uint32_t cap = pcg_cap(max_iter, N);
+  const uint32_t lg = g - g0, w0 = g0 >> 2, nw = ng >> 2, lbase = g0 * RK;
+  const bool lane_ok = i < (uint32_t)NX;
+  __syncthreads();  // LDS cleared
+
+  // ---- load: L^ rows (registers; the first knot's also to LDS slot g),
+  // lambda^ = W^-1 lambda by back substitution (W upper triangular), gamma^
+  const uint32_t b0 = lg * RK, s0 = lbase + b0;
+  // Lr[t - 1]: rows of knot t = 1..3; knot 0's rows stay in LDS slot g only
+  float Lr[RK - 1][16];
+  float lam[RK], r[RK], p[RK], y[RK], gh[RK];  // z = -r is not stored
+#pragma unroll
+  for (int t = 0; t < RK; ++t) {
+    const uint32_t bk = b0 + (uint32_t)t;
+    const bool has = live && bk < N;
+    const float *Pr = Ps + (size_t)S_STRIDE * (K0 + (has ? bk : 0u));
+    if (t == 0) {
+      float L0[16];
+      load_lrows(L0, has && bk > 0, i, d, Pr);
+      if (lane_ok) {
+#pragma unroll
+        for (int n = 0; n < 16; ++n) sh.L[g * L_SLOT + i * L_LANE + n] = L0[n];
+      }
+    } else {
+      load_lrows(Lr[t - 1], has, i, d, Pr);
+    }
+    float wrow[NX];
+#pragma unroll
+    for (int c = 0; c < NX; ++c) wrow[c] = has && lane_ok ? Pr[NXX + i + NX * c] : 0.f;
+    float acc = has && lane_ok ? lambda[(size_t)NX * (K0 + bk) + i] : 0.f;
+    float lh = 0.f;
+#pragma unroll
+    for (int k = NX - 1; k >= 0; --k) {
+      const float xk = bcast_dyn16(acc / wrow[k], k);
+      if ((int)i == k) lh = xk;
+      acc -= wrow[k] * xk;
+    }
+    lam[t] = has && lane_ok ? lh : 0.f;
+    gh[t] = has && lane_ok ? ghat[(size_t)NX * (K0 + bk) + i] : 0.f;
+  }
+  const bool in_m = lg > 0, in_p = b0 + RK < N;
+  float *zb0 = sh.zb + s0 * 16 + i;
+  auto stage = [&](const float (&v)[RK]) {
+    zb0[0] = v[0];
+    zb0[(RK - 1) * 16] = v[RK - 1];
+  };
+  auto stage_neg = [&](const float (&v)[RK]) {
+    zb0[0] = -v[0];
+    zb0[(RK - 1) * 16] = -v[RK - 1];
+  };
+  auto neighbours = [&](float &vm, float &vp) {
+    vm = in_m ? zb0[-16] : 0.f;
+    vp = in_p ? zb0[RK * 16] : 0.f;
+  };
+  // y = S^ x: y_t = -x_t + L^_t x_{t-1} + L^_{t+1}^T x_{t+1}; xm / xp: the
+  // elements of knots 4lg - 1 and 4lg + 4 (the next group's L^ from LDS)
+  const float *Lown = sh.L + g * L_SLOT;
+  auto smv = [&](const float (&x)[RK], float xm, float xp) {
+    float a0, a1, a2, a3;
+    {
+      float L0[16];
+      read_lrow(L0, Lown, i);
+      rot_mv2(L0, xm, a0, Lr[0], x[0], a1);
+    }
+    rot_mv2(Lr[1], x[1], a2, Lr[2], x[2], a3);
+    const float c1 = rot_mtv(Lr[0], x[1]), c2 = rot_mtv(Lr[1], x[2]), c3 = rot_mtv(Lr[2], x[3]);
+    float c4;
+    {
+      float Ln[16];
+      read_lrow(Ln, Lown + L_SLOT, i);
+      c4 = rot_mtv(Ln, xp);
+    }
+    y[0] = (a0 - x[0]) + c1;
+    y[1] = (a1 - x[1]) + c2;
+    y[2] = (a2 - x[2]) + c3;
+    y[3] = (a3 - x[3]) + c4;
+  };
+  stage(lam);
+  __syncthreads();
+  float pm, pp;
+  neighbours(pm, pp);
+
+  // ---- prologue: r = gamma^ - S^ lambda^, z = -r, p = z, eta = r.z
+  smv(lam, pm, pp);
+  float part = 0.f;
+#pragma unroll
+  for (int t = 0; t < RK; ++t) {
+    r[t] = lane_ok ? gh[t] - y[t] : 0.f;
+    part -= r[t] * r[t];
+  }
+  part = wave_sum(part);
+  __syncthreads();  // every neighbour lambda read before zb is reused
+  if ((tid & 63u) == 0) sh.red[1][w] = part;
+  stage_neg(r);
+  __syncthreads();
+  float eta = seg_sum(sh.red[1], w0, nw);
+  neighbours(pm, pp);
+#pragma unroll
+  for (int t = 0; t < RK; ++t) p[t] = -r[t];
+  uint32_t it = 0;
+  // tol < 0: exit relative to the first residual, |eta| < -tol |eta_0|
+  const float tol_i = tol < 0.f ? fmaxf(-tol * fabsf(eta), FLT_MIN) : tol;
+  bool conv = !live || fabsf(eta) < tol_i;
+  float eta_last = eta;
+
+  for (;;) {
+    const bool run = live && !conv && it < cap;  // uniform over the instance's waves
+    float pyp = 0.f;
+    if (run) {
+      smv(p, pm, pp);
+#pragma unroll
+      for (int t = 0; t < RK; ++t) pyp += p[t] * y[t];
+      pyp = wave_sum(pyp);
+    }
+    if ((tid & 63u) == 0) {
+      sh.red[0][w] = pyp;
+      sh.act[w] = run ? 1u : 0u;
+    }
+    __syncthreads();
+    uint32_t any = 0;
+#pragma unroll
+    for (int v = 0; v < NWAV; ++v) any |= sh.act[v];
+    if (!any) break;
+    float rz = 0.f;
+    if (run) {
+      const float alpha = eta / seg_sum(sh.red[0], w0, nw);
+#pragma unroll
+      for (int t = 0; t < RK; ++t) {
+        lam[t] += alpha * p[t];
+        r[t] = lane_ok ? r[t] - alpha * y[t] : 0.f;
+        rz -= r[t] * r[t];
+      }
+      stage_neg(r);
+      rz = wave_sum(rz);
+    }
+    if ((tid & 63u) == 0) sh.red[1][w] = rz;
+    __syncthreads();
+    if (run) {
+      float zm, zp;
+      neighbours(zm, zp);
+      const float eta_new = seg_sum(sh.red[1], w0, nw);
+      eta_last = eta_new;
+      ++it;
+      if (fabsf(eta_new) < tol_i) {
+        conv = true;
+      } else {
+        const float beta = eta_new / eta;
+        eta = eta_new;
+#pragma unroll
+        for (int t = 0; t < RK; ++t) p[t] = fmaf(beta, p[t], -r[t]);
+        pm = fmaf(beta, pm, zm);
+        pp = fmaf(beta, pp, zp);
+      }
+    }
+  }
+
+  if (!live) return;
+  // lambda = W lambda^, two knots at a time
+#pragma unroll
+  for (int t = 0; t < RK; t += 2) {
+    float Wr[2][16];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const uint32_t bk = b0 + (uint32_t)(t + u);
+      const bool has = bk < N && lane_ok;
+      const float *Wb = Ps + (size_t)S_STRIDE * (K0 + (bk < N ? bk : 0u)) + NXX;
+#pragma unroll
+      for (int n = 0; n < 16; ++n) {
+        const uint32_t j = (i + (uint32_t)n * d) & 15u;
+        Wr[u][n] = has && j < (uint32_t)NX ? Wb[i + NX * j] : 0.f;
+      }
+    }
+    float va, vb;
+    rot_mv2(Wr[0], lam[t], va, Wr[1], lam[t + 1], vb);
+    if (b0 + t < N && lane_ok) lambda[(size_t)NX * (K0 + b0 + t) + i] = va;
+    if (b0 + t + 1 < N && lane_ok) lambda[(size_t)NX * (K0 + b0 + t + 1) + i] = vb;
+  }
+  if (lg == 0 && i == 0) {
+    if (iters_out) iters_out[inst] = it;
+    if (flag_out) flag_out[inst] = conv ? 0u : 1u;
+    if (eta_out) eta_out[inst] = fabsf(eta_last);
+  }
+}
+
+}  // namespace
+
+uint32_t pcg_split_max_knots() { return (uint32_t)KNOTS; }
+uint32_t pcg_split_bin_groups() { return (uint32_t)NGRP; }
+uint32_t pcg_split_max_items() { return (uint32_t)MAXI; }
+
+hipError_t launch_split_prep(const Batch &b, const float *S, const float *gamma, const uint32_t *status, float *Ps,
+                             float *ghat, hipStream_t st) {
+  if (b.total_knots == 0) return hipSuccess;
+  const dim3 grid((b.total_knots + PG - 2) / (PG - 1));
+  hipLaunchKernelGGL(k_split_prep, grid, dim3(PWG), 0, st, b, S, gamma, status, Ps, ghat);
+  return hipGetLastError();
+}
+
+hipError_t launch_pcg_split(const Batch &b, const uint4 *items, const uint32_t *bin_off, uint32_t n_bins,
+                            const float *Ps, const float *ghat, float *lambda, uint32_t max_iter, float tol,
+                            const uint32_t *status, uint32_t *iters, uint32_t *flag, float *eta, hipStream_t st) {
+  if (n_bins == 0) return hipSuccess;
+  static bool attr_set = false;
+  const size_t smem = sizeof(SplitShared);
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void *)k_pcg_split, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    (void)hipGetLastError();
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_pcg_split, dim3(n_bins), dim3(NT), smem, st, b, items, bin_off, Ps, ghat, lambda, max_iter,
+                     tol, status, iters, flag, eta);
+  return hipGetLastError();
+}
+
+}  // namespace mpc

@@ -105,6 +105,7 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
                                                    float *__restrict__ gamma, const float *__restrict__ Ginv,
                                                    int out_mode) {
   const bool zero_unused = out_mode & SCHUR_ZERO_UNUSED, right = out_mode & SCHUR_RIGHT_BLOCK;
+  const bool pinv = !(out_mode & SCHUR_NO_PINV);
   __shared__ SchurLds lds[GROUPS_PER_WG];
   const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
   const uint32_t gk_raw = blockIdx.x * GROUPS_PER_WG + grp;
@@ -203,7 +204,7 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
     const float *Q0 = G + (size_t)G_STRIDE * row;
     for (int i = lane; i < NXX; i += GROUP) {
       Srow[NXX + i] = -Qi[i];
-      Prow[NXX + i] = -(Q0[i] + ((i % NX == i / NX) ? rho : 0.f));
+      if (pinv) Prow[NXX + i] = -(Q0[i] + ((i % NX == i / NX) ? rho : 0.f));
     }
     if (lane < NX) {
       float acc = 0.f;
@@ -214,8 +215,10 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
     if (zero_unused)
       for (int i = lane; i < NXX; i += GROUP) {
         Srow[i] = 0.f;
-        Prow[i] = 0.f;
-        Prow[2 * NXX + i] = 0.f;
+        if (pinv) {
+          Prow[i] = 0.f;
+          Prow[2 * NXX + i] = 0.f;
+        }
       }
   }
   if (valid && !row0) {
@@ -239,15 +242,15 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
     if (zero_unused) {
       if (bk + 1 == kr.N && right)
         for (int i = lane; i < NXX; i += GROUP) Srow[2 * NXX + i] = 0.f;
-      if (bk + 1 == kr.N)
-        for (int i = lane; i < NXX; i += GROUP) Prow[2 * NXX + i] = 0.f;
-      for (int i = lane; i < NXX; i += GROUP) {
-        Prow[i] = 0.f;
-        if (bk + 1 < kr.N) Prow[2 * NXX + i] = 0.f;
-      }
+      if (pinv)
+        for (int i = lane; i < NXX; i += GROUP) {
+          Prow[i] = 0.f;
+          Prow[2 * NXX + i] = 0.f;
+        }
     }
   }
   // Pinv(b,b) = -theta^-1
+  if (!pinv) return;
   float x[NX];
   gj_inverse<NX>(lane, th, x);
   __syncthreads();  // every lane is done reading S(b,b) from A

@@ -30,6 +30,7 @@ struct PcgPlan {
   uint32_t max_items = 8;   // instances per bin
   bool mixed = true;        // two-workgroup teams and 512-lane bins in one persistent pool launch
   bool teams = true;        // 128 < N <= 512 on teams of two or four workgroups (else streamed)
+  bool split = true;        // block Jacobi with every N <= 256: the split-form kernel (k_pcg_split.hip)
 };
 
 struct mpc_handle {
@@ -52,6 +53,13 @@ struct mpc_handle {
   uint4 *bin_items = nullptr;
   uint32_t *bin_off = nullptr, *big_list = nullptr;
   uint32_t n_bins = 0, n_bins_small = 0, n_big = 0, max_big_n = 0;
+  // split-form PCG: its bins (1024-lane workgroups) and gamma^ = W^T gamma
+  // (the split system itself lives in P)
+  uint4 *split_items = nullptr;
+  uint32_t *split_off = nullptr;
+  uint32_t n_split_bins = 0;
+  bool split_ok = false;  // every horizon of the layout fits the split-form kernel
+  float *ghat = nullptr;
   // PCG teams: two workgroups for 128 < N <= 256, four for 256 < N <= 512
   uint32_t *team_list = nullptr, *team_one = nullptr, *team_err = nullptr;
   uint32_t *pcg_done = nullptr;       // [B] parts that finished each team solve
@@ -173,7 +181,12 @@ static Batch batch_of(const mpc_handle *h) {
 }
 
 static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_status, uint32_t max_iter, float tol,
-                            hipEvent_t mid, hipStream_t st);
+                            hipEvent_t mid, hipStream_t st, bool prepped = false);
+
+// the batch's block-Jacobi PCG runs in split form (k_pcg_split.hip)
+static bool split_active(const mpc_handle *h) {
+  return h->plan.split && h->split_ok && !(h->cfg.flags & (FLAG_PCG_STAIR | FLAG_PCG_STREAM | FLAG_DIRECT_SOLVE));
+}
 
 // `steps` steps of iterative refinement of K [dz; lam] = [g; c] (constraint
 // row 0: dx_0 = 0; c == nullptr: c = 0) for the instances of layout b
@@ -262,7 +275,7 @@ static void free_all(mpc_handle *h) {
                   h->merits, h->xs, h->drho, h->merit0, h->rho1, h->status_ws, h->sqp_done,
                   h->pcg_iters, h->pcg_flag, h->pcg_eta, h->t0, h->knot_inst, h->inst_k0, h->bin_items,
                   h->bin_off, h->big_list, h->team_list, h->team_one, h->team_err, h->mbox, h->dir_order,
-                  h->pcg_done, h->pool_ctr,
+                  h->pcg_done, h->pool_ctr, h->split_items, h->split_off, h->ghat,
                   h->adj_mu, h->adj_knot, h->loss_knot, h->ref_g, h->ref_c, h->ref_dz, h->ref_lam,
                   h->ref_scale, h->cl_win, h->cl_xp, h->cl_rho, h->cl_ctr, h->cl_status, h->cl_pcg,
                   h->rho_one};
@@ -325,6 +338,9 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
             alloc((void **)&h->inst_k0, sizeof(uint32_t) * (B + 1)) &&
             alloc((void **)&h->bin_items, sizeof(uint4) * B) &&
             alloc((void **)&h->bin_off, sizeof(uint32_t) * (B + 1)) &&
+            alloc((void **)&h->split_items, sizeof(uint4) * B) &&
+            alloc((void **)&h->split_off, sizeof(uint32_t) * (B + 1)) &&
+            alloc((void **)&h->ghat, sizeof(float) * NX * T) &&
             alloc((void **)&h->big_list, sizeof(uint32_t) * B) &&
             alloc((void **)&h->team_list, sizeof(uint32_t) * B) &&
             alloc((void **)&h->dir_order, sizeof(uint32_t) * B) &&
@@ -693,6 +709,29 @@ static int build_pcg_plan(mpc_handle *h) {
     }
     return bins;
   };
+  // split form: every instance in bins of the 1024-lane kernel
+  h->split_ok = n_inst > 0;
+  for (uint32_t i = 0; i < n_inst; ++i) h->split_ok = h->split_ok && horizons[i] <= pcg_split_max_knots();
+  h->n_split_bins = 0;
+  if (h->split_ok) {
+    std::vector<uint32_t> all(n_inst);
+    for (uint32_t i = 0; i < n_inst; ++i) all[i] = i;
+    const std::vector<BinB> sb = pack(all, pcg_split_bin_groups());
+    std::vector<uint4> sitems;
+    std::vector<uint32_t> soff(1, 0);
+    for (auto &bb : sb) {
+      uint32_t g0 = 0;
+      for (uint32_t i : bb.inst) {
+        const uint32_t ng = pcg_res_groups(horizons[i]);
+        sitems.push_back(make_uint4(i, g0, ng, 0u));
+        g0 += ng;
+      }
+      soff.push_back((uint32_t)sitems.size());
+    }
+    HIP_TRY(hipMemcpy(h->split_items, sitems.data(), sizeof(uint4) * sitems.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->split_off, soff.data(), sizeof(uint32_t) * soff.size(), hipMemcpyHostToDevice));
+    h->n_split_bins = (uint32_t)sb.size();
+  }
   std::vector<BinB> bins = pack(order, pcg_res_bin_groups(512));
   const std::vector<BinB> bins_s = pack(order_s, pcg_res_bin_groups(256));
   const uint32_t n512 = (uint32_t)bins.size();
@@ -848,10 +887,11 @@ int mpc_batch_qp_adjoint(mpc_handle *h, const float *d_xu, const float *d_ee_ref
 int mpc_batch_pcg_layout(const mpc_handle *h, uint32_t *out4) {
   if (!h || !out4) return MPC_ERR_ARG;
   if (!h->batch_ready) return MPC_ERR_NOT_READY;
-  out4[0] = h->n_bins + h->n_bins_small;
-  out4[1] = h->n_teams + h->n_quads;
-  out4[2] = h->n_big;
-  out4[3] = h->cfg.flags & (FLAG_PCG_STREAM | FLAG_DIRECT_SOLVE) ? 0u : h->team_launches;
+  const bool split = split_active(h);
+  out4[0] = split ? h->n_split_bins : h->n_bins + h->n_bins_small;
+  out4[1] = split ? 0u : h->n_teams + h->n_quads;
+  out4[2] = split ? 0u : h->n_big;
+  out4[3] = split || (h->cfg.flags & (FLAG_PCG_STREAM | FLAG_DIRECT_SOLVE)) ? 0u : h->team_launches;
   return MPC_OK;
 }
 
@@ -861,7 +901,7 @@ static void plan_to_c(const PcgPlan &p, float us, mpc_pcg_plan *o) {
   o->mixed = p.mixed ? 1u : 0u;
   o->teams = p.teams ? 1u : 0u;
   o->us = us;
-  o->reserved = 0;
+  o->split = p.split ? 1u : 0u;
 }
 
 int mpc_batch_get_plan(const mpc_handle *h, mpc_pcg_plan *plan) {
@@ -872,12 +912,13 @@ int mpc_batch_get_plan(const mpc_handle *h, mpc_pcg_plan *plan) {
 
 int mpc_batch_set_plan(mpc_handle *h, const mpc_pcg_plan *plan) {
   if (!h || !plan || (plan->small_nt != 256 && plan->small_nt != 512) || plan->max_items < 1 ||
-      plan->max_items > pcg_res_max_items() || plan->mixed > 1 || plan->teams > 1)
+      plan->max_items > pcg_res_max_items() || plan->mixed > 1 || plan->teams > 1 || plan->split > 1)
     return MPC_ERR_ARG;
   h->plan.small_nt = plan->small_nt;
   h->plan.max_items = plan->max_items;
   h->plan.mixed = plan->mixed != 0;
   h->plan.teams = plan->teams != 0;
+  h->plan.split = plan->split != 0;
   return h->batch_ready ? build_pcg_plan(h) : MPC_OK;
 }
 
@@ -900,6 +941,14 @@ int mpc_batch_autotune(mpc_handle *h, uint32_t iters, uint32_t reps, mpc_pcg_pla
   }
   has_team = has_team && pcg_team_per_launch(4) > 0;
   std::vector<PcgPlan> cands;
+  const bool split_cand = h->split_ok && !(h->cfg.flags & FLAG_PCG_STAIR);
+  if (split_cand)
+    for (uint32_t mi : {8u, 4u, 2u}) {
+      PcgPlan c;
+      c.max_items = mi;
+      c.split = true;
+      cands.push_back(c);
+    }
   for (uint32_t nt : {512u, 256u}) {
     if (nt == 256 && !has_small) continue;
     for (uint32_t mi : {8u, 4u, 2u}) {
@@ -913,6 +962,7 @@ int mpc_batch_autotune(mpc_handle *h, uint32_t iters, uint32_t reps, mpc_pcg_pla
           c.max_items = mi;
           c.teams = teams != 0;
           c.mixed = mixed != 0;
+          c.split = false;
           cands.push_back(c);
         }
       }
@@ -932,8 +982,10 @@ int mpc_batch_autotune(mpc_handle *h, uint32_t iters, uint32_t reps, mpc_pcg_pla
     float tmin = INFINITY;
     for (uint32_t r = 0; e == hipSuccess && r <= reps; ++r) {
       e = launch_tune_system(b, h->S, h->P, h->gamma, h->adj_mu, st);
+      // the split system is formed in the Schur phase of a solve, outside the timed PCG
+      if (e == hipSuccess && split_active(h)) e = launch_split_prep(b, h->S, h->gamma, nullptr, h->P, h->ghat, st);
       if (e == hipSuccess) e = hipEventRecord(h->ev[0], st);
-      if (e == hipSuccess) e = pcg_batch(h, h->adj_mu, nullptr, iters, 0.f, nullptr, st);
+      if (e == hipSuccess) e = pcg_batch(h, h->adj_mu, nullptr, iters, 0.f, nullptr, st, true);
       if (e == hipSuccess) e = hipEventRecord(h->ev[1], st);
       if (e == hipSuccess) e = hipEventSynchronize(h->ev[1]);
       float ms = 0.f;
@@ -957,7 +1009,7 @@ int mpc_batch_autotune(mpc_handle *h, uint32_t iters, uint32_t reps, mpc_pcg_pla
 int mpc_batch_pcg_launch_map(const mpc_handle *h, uint32_t *launch_of_inst) {
   if (!h || !launch_of_inst) return MPC_ERR_ARG;
   if (!h->batch_ready) return MPC_ERR_NOT_READY;
-  const bool one = h->cfg.flags & (FLAG_PCG_STREAM | FLAG_DIRECT_SOLVE);
+  const bool one = split_active(h) || (h->cfg.flags & (FLAG_PCG_STREAM | FLAG_DIRECT_SOLVE));
   for (uint32_t i = 0; i < h->n_inst; ++i) launch_of_inst[i] = one ? 0u : h->pcg_launch[i];
   return MPC_OK;
 }
@@ -1000,9 +1052,19 @@ int mpc_batch_phase_times(mpc_handle *h, double *us7) {
 // everything streaming with FLAG_PCG_STREAM).  mid (optional) is recorded
 // after the team launches.
 static hipError_t pcg_batch(mpc_handle *h, float *d_lambda, const uint32_t *d_status, uint32_t max_iter, float tol,
-                            hipEvent_t mid, hipStream_t st) {
+                            hipEvent_t mid, hipStream_t st, bool prepped) {
   const Batch b = batch_of(h);
   hipError_t e;
+  if (split_active(h)) {
+    // block Jacobi in split form: the split system from S and gamma (unless
+    // the caller formed it already), then one launch over the bins
+    e = prepped ? hipSuccess : launch_split_prep(b, h->S, h->gamma, d_status, h->P, h->ghat, st);
+    if (e == hipSuccess)
+      e = launch_pcg_split(b, h->split_items, h->split_off, h->n_split_bins, h->P, h->ghat, d_lambda, max_iter, tol,
+                           d_status, h->pcg_iters, h->pcg_flag, h->pcg_eta, st);
+    if (e == hipSuccess && mid) e = hipEventRecord(mid, st);
+    return e;
+  }
   if (h->cfg.flags & FLAG_DIRECT_SOLVE) {
     e = launch_direct_solve(b, h->dir_order, h->n_inst, h->S, h->gamma, d_status, h->P, d_lambda, h->cfg.direct_refine,
                             true, h->pcg_iters, h->pcg_flag, st);
@@ -1065,10 +1127,13 @@ static hipError_t batch_sqp_run(mpc_handle *h, float *d_xu, float *d_lambda, con
     if (tm) MPC_RUN(hipEventRecord(h->ev[0], st));
     MPC_RUN(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, d_status, h->G, h->C, h->g, h->c, st));
     if (tm) MPC_RUN(hipEventRecord(h->ev[1], st));
-    MPC_RUN(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma, h->Ginv, 0,
-                              st));
+    const bool split = split_active(h);
+    MPC_RUN(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma, h->Ginv,
+                              split ? SCHUR_NO_PINV : 0, st));
+    if (split) MPC_RUN(launch_split_prep(b, h->S, h->gamma, d_status, h->P, h->ghat, st));
     if (tm) MPC_RUN(hipEventRecord(h->ev[2], st));
-    MPC_RUN(pcg_batch(h, d_lambda, d_status, sqp_pcg_cap(h->cfg), h->cfg.pcg_exit_tol, tm ? h->ev[3] : nullptr, st));
+    MPC_RUN(pcg_batch(h, d_lambda, d_status, sqp_pcg_cap(h->cfg), h->cfg.pcg_exit_tol, tm ? h->ev[3] : nullptr, st,
+                      split));
     if (tm) MPC_RUN(hipEventRecord(h->ev[4], st));
     MPC_RUN(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
     if (h->cfg.flags & FLAG_DIRECT_SOLVE)

@@ -311,15 +311,21 @@ int mpc_batch_pcg_launch_map(const mpc_handle *h, uint32_t *launch_of_inst);
  * first), so a CU whose item converged early takes the next one; mixed = 0:
  * static team launches, then one bin launch.  Every plan gives the same
  * arithmetic per instance (bins and teams: bitwise), only the packing and
- * the schedule change.  Default {512, 8, 1, 1}.  No counterpart in the
- * reference (one instance per solve). */
+ * the schedule change.  split = 1 (block Jacobi, every N <= 256): all
+ * instances instead run on the split-form kernel, bins of up to max_items
+ * instances on 1024-lane workgroups (256 knots, one per CU); block-Jacobi
+ * PCG there is CG on W^T S W (W = chol(-S(b,b))^-1 blockwise, formed after
+ * the Schur complement), the same iteration in exact arithmetic with its
+ * own fp32 rounding, and the other fields do not apply.  Default
+ * {512, 8, 1, 1, split 1}.  No counterpart in the reference (one instance
+ * per solve). */
 typedef struct mpc_pcg_plan {
   uint32_t small_nt;
   uint32_t max_items;
   uint32_t mixed;
   uint32_t teams;
   float us;          /* mpc_batch_autotune: PCG phase time of this plan, microseconds */
-  uint32_t reserved;
+  uint32_t split;
 } mpc_pcg_plan;
 int mpc_batch_get_plan(const mpc_handle *h, mpc_pcg_plan *plan);
 /* Sets the plan (us ignored) and re-packs the current layout. */

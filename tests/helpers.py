@@ -95,3 +95,27 @@ def rel_err(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+# -- fp32 rounding floors ---------------------------------------------------------------------
+def ulp_perturbed(a, seed):
+    """a with every element scaled by 1 +- 2^-24 (random signs): the same
+    values to fp32 precision, a different rounding history downstream."""
+    rng = np.random.default_rng(seed)
+    a = np.asarray(a, np.float64)
+    return a * (1.0 + 2.0 ** -24 * rng.choice([-1.0, 1.0], a.shape))
+
+
+def pcg_fp32_floor(orc, N, S, P, gamma, lam0, cap, tol, ref, samples=3, stair=False):
+    """The fp32 rounding floor of a PCG solve: the largest relative distance
+    from the fp64 result ref of the oracle's fp32 PCG, run on S as given and
+    on `samples` one-ulp perturbations of S.  A single fp32 run is one draw
+    of this floor; after a capped solve (173 iterations) two draws can differ
+    by an order of magnitude, so a parity bound built on one draw rejects
+    correct fp32 implementations at random."""
+    worst = 0.0
+    for k in range(samples + 1):
+        Sk = S if k == 0 else ulp_perturbed(S, k)
+        lt, _, _ = orc.pcg(N, Sk, P, gamma, lam0, cap, tol, dtype=np.float32, stair=stair)
+        worst = max(worst, rel_err(lt, ref))
+    return worst

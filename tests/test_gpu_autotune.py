@@ -44,7 +44,7 @@ def _solve(pkg, orc, s, K=3):
     return [t.cpu().numpy() for t in (xu, lam, rho, st, pit)]
 
 
-DEFAULT = dict(small_nt=512, max_items=8, mixed=1, teams=1)
+DEFAULT = dict(small_nt=512, max_items=8, mixed=1, teams=1, split=1)  # split: no effect here (N = 300)
 
 
 @pytest.mark.parametrize("plan", [dict(small_nt=256), dict(max_items=1), dict(max_items=2, small_nt=256),
@@ -87,7 +87,7 @@ def test_autotune_picks_the_fastest_plan(pkg, orc, tsolver):
     assert len(res) == 2 * 3 * 4
     assert all(r["us"] > 0 for r in res)
     best = min(res, key=lambda r: r["us"])
-    assert s.batch_get_plan() == {k: best[k] for k in ("small_nt", "max_items", "mixed", "teams")}
+    assert s.batch_get_plan() == {k: best[k] for k in DEFAULT}
     # the tuned plan solves like any other
     xu, lam, rho, st, pit = _solve(pkg, orc, s)
     assert np.all(np.isfinite(xu)) and (st & (8 | 16)).max() == 0

@@ -103,12 +103,14 @@ def test_batched_graph_replays_equal_eager(pkg, orc, gsolver):
         assert np.array_equal(x, y), name
 
 
-def test_closed_loop_graph_with_teams(pkg):
+@pytest.mark.parametrize("split", [0, 1])
+def test_closed_loop_graph_with_teams(pkg, split):
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     N, steps, K = 200, 6, 2
     s = pkg.Solver(pkg.default_config(max_knots=N, max_instances=2, max_total_knots=2 * N, pcg_exit_tol=5e-6))
     try:
+        s.batch_set_plan(split=split)  # N = 200: a two-workgroup team, or one split-form bin
         out = {}
         for graph in (1, 0):
             x0, ee, traj = s.track_inputs(41, steps, N)
@@ -118,7 +120,8 @@ def test_closed_loop_graph_with_teams(pkg):
             cl = pkg.default_closed_loop(steps=steps, sqp_iters=K, use_graph=graph)
             states, err, _, stats = s.closed_loop(N, x0, ee, xu, lam, cl, traj, pcg_iters=pcg)
             assert stats["graph"] == graph
-            assert s.batch_pcg_layout()["teams"] == 1
+            lay = s.batch_pcg_layout()
+            assert (lay["teams"], lay["bins"]) == ((0, 1) if split else (1, 0)), lay
             out[graph] = [host(t) for t in (states, err, xu, lam, pcg)]
         for name, a, b in zip(("states", "err", "xu", "lam", "pcg"), out[1], out[0]):
             assert np.array_equal(a, b), name

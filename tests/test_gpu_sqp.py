@@ -59,6 +59,18 @@ def test_batch_matches_plain_and_is_deterministic(solver, orc, pkg, horizons):
     st = torch.zeros(n, dtype=torch.int32, device="cuda")
     pit = torch.zeros(n * 3, dtype=torch.int32, device="cuda")
     solver.batch_setup(horizons)
+    # the plain driver's PCG is the reference form: compare with the same
+    # form here (the split form, default where every N <= 256, is checked
+    # against the oracle in test_gpu_split.py and test_gpu_batch_oracle.py)
+    solver.batch_set_plan(split=0)
+    try:
+        _batch_vs_plain(solver, xus, ees, horizons, xu, lam, ee, rho, st, pit)
+    finally:
+        solver.batch_set_plan(split=1)
+
+
+def _batch_vs_plain(solver, xus, ees, horizons, xu, lam, ee, rho, st, pit):
+    n, T = len(horizons), sum(horizons)
     solver.batch_sqp(xu, lam, ee, rho, 3, st, pit)
     xb, lb = host(xu), host(lam)
     # determinism: a second run from the same inputs is bitwise identical
