@@ -144,7 +144,6 @@ __global__ void __launch_bounds__(PWG) k_split_prep(Batch b, const float *__rest
 // --------------------------------------------------------------- solve ----
 constexpr int NT = 1024;
 constexpr int NGRP = NT / 16;  // 64 groups
-constexpr int NWAV = NT / 64;  // 16 waves
 constexpr int RK = 4;
 constexpr int KNOTS = NGRP * RK;  // 256
 constexpr int MAXI = 8;
@@ -152,15 +151,54 @@ constexpr int MAXI = 8;
 struct SplitShared {
   float L[(NGRP + 1) * L_SLOT + L_GUARD];  // L^ rows of each group's first knot (+ a zero slot)
   float zb[(KNOTS + 1) * 16];               // staged vector element per knot slot and lane
-  float red[2][NWAV];
-  uint32_t act[NWAV];
+  float red[2][NGRP];                       // dot-product partials, one per group
+  uint32_t nlive;                           // instances of the bin still iterating
 };
 static_assert(((NGRP + 1) * L_SLOT + L_GUARD) % 4 == 0, "L and zb are cleared as one float4 range");
 
-__device__ __forceinline__ float seg_sum(const float *red, uint32_t w0, uint32_t nw) {
-  float s = 0.f;
-  for (uint32_t v = 0; v < nw; ++v) s += red[w0 + v];
-  return s;
+// total of the 16 lanes of each row, in every lane of the row (fixed order;
+// each step adds two values in either order, so the lanes agree bitwise)
+__device__ __forceinline__ float row_total(float v) {
+  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f<0x141>(v);  // row_half_mirror
+  v += dpp_f<0x140>(v);  // row_mirror
+  return v;
+}
+
+__device__ __forceinline__ float ror1(float v) { return dpp_f<0x121>(v); }
+
+// (M_k^T x_k)_i for four blocks with rotated rows M_k: sum_n ror_n(m_n),
+// m_n = M[(16 - n) & 15] x, in Horner form m_0 + ror_1(m_1 + ror_1(m_2 + ...
+// + ror_1(m_15))): 16 products and 15 row_ror:1 additions per block, the
+// four chains interleaved
+__device__ __forceinline__ void rot_mtv4(const float (&M0)[16], float x0, const float (&M1)[16], float x1,
+                                         const float (&M2)[16], float x2, const float (&M3)[16], float x3,
+                                         float (&c)[4]) {
+  // separate products: a fused multiply-add would take the rotated sum as
+  // its addend, which DPP cannot supply (it rotates the first source only)
+#pragma clang fp contract(off)
+  float a0 = M0[1] * x0, a1 = M1[1] * x1, a2 = M2[1] * x2, a3 = M3[1] * x3;
+#pragma unroll
+  for (int n = 14; n >= 0; --n) {
+    const int k = (16 - n) & 15;
+    a0 = ror1(a0) + M0[k] * x0;
+    a1 = ror1(a1) + M1[k] * x1;
+    a2 = ror1(a2) + M2[k] * x2;
+    a3 = ror1(a3) + M3[k] * x3;
+  }
+  c[0] = a0;
+  c[1] = a1;
+  c[2] = a2;
+  c[3] = a3;
+}
+
+// (Ma xa)_i - sa and (Mb xb)_i - sb (the -x term of S^'s -I diagonal)
+__device__ __forceinline__ void rot_mv2_minus(const float (&Ma)[16], float xa, float sa, float &ya,
+                                              const float (&Mb)[16], float xb, float sb, float &yb) {
+  ya = fmaf(Ma[0], xa, -sa);
+  yb = fmaf(Mb[0], xb, -sb);
+  rot_fma15x2(ya, xa, Ma, yb, xb, Mb);
 }
 
 __device__ __forceinline__ void load_lrows(float (&Lr)[16], bool has, uint32_t i, uint32_t d,
@@ -183,12 +221,13 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
   extern __shared__ __align__(16) unsigned char smem_raw[];
   SplitShared &sh = *reinterpret_cast<SplitShared *>(smem_raw);
   const uint32_t tid = threadIdx.x, bin = blockIdx.x;
-  const uint32_t g = tid >> 4, i = tid & 15u, w = tid >> 6;
+  const uint32_t g = tid >> 4, i = tid & 15u;
   const uint32_t d = ror_step(tid);
   {
     float4 *z4 = reinterpret_cast<float4 *>(sh.L);
     constexpr uint32_t n4 = ((NGRP + 1) * L_SLOT + L_GUARD + (KNOTS + 1) * 16) / 4;
     for (uint32_t idx = tid; idx < n4; idx += NT) z4[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (tid == 0) sh.nlive = 0u;
   }
   // my instance in this bin (items: {inst, first group, groups, -})
   const uint32_t i0 = bin_off[bin], ni = bin_off[bin + 1] - i0;
@@ -208,7 +247,8 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
   const uint32_t K0 = mine ? inst_first(b, inst) : 0u;
   const bool live = mine && !(status && !(status[inst] & ST_ACTIVE));
   const uint32_t cap = pcg_cap(max_iter, N);
-  const uint32_t lg = g - g0, w0 = g0 >> 2, nw = ng >> 2, lbase = g0 * RK;
+  const uint32_t lg = g - g0, lbase = g0 * RK;
+  const bool leader = lg == 0 && i == 0 && mine;  // the instance's first lane
   const bool lane_ok = i < (uint32_t)NX;
   __syncthreads();  // LDS cleared
 
@@ -269,20 +309,29 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
     {
       float L0[16];
       read_lrow(L0, Lown, i);
-      rot_mv2(L0, xm, a0, Lr[0], x[0], a1);
+      rot_mv2_minus(L0, xm, x[0], a0, Lr[0], x[0], x[1], a1);
     }
-    rot_mv2(Lr[1], x[1], a2, Lr[2], x[2], a3);
-    const float c1 = rot_mtv(Lr[0], x[1]), c2 = rot_mtv(Lr[1], x[2]), c3 = rot_mtv(Lr[2], x[3]);
-    float c4;
+    rot_mv2_minus(Lr[1], x[1], x[2], a2, Lr[2], x[2], x[3], a3);
+    float c[4];
     {
       float Ln[16];
       read_lrow(Ln, Lown + L_SLOT, i);
-      c4 = rot_mtv(Ln, xp);
+      rot_mtv4(Lr[0], x[1], Lr[1], x[2], Lr[2], x[3], Ln, xp, c);
     }
-    y[0] = (a0 - x[0]) + c1;
-    y[1] = (a1 - x[1]) + c2;
-    y[2] = (a2 - x[2]) + c3;
-    y[3] = (a3 - x[3]) + c4;
+    y[0] = a0 + c[0];
+    y[1] = a1 + c[1];
+    y[2] = a2 + c[2];
+    y[3] = a3 + c[3];
+  };
+  // dot products: a row total per group to LDS, then, after the barrier,
+  // the instance's group totals (lanes 0 .. ng-1 of every wave) summed
+  auto post = [&](int k, float part) {
+    const float v = row_total(part);
+    if (i == 0) sh.red[k][g] = v;
+  };
+  auto total = [&](int k) {
+    const uint32_t l = tid & 63u;
+    return wave_sum(l < ng ? sh.red[k][g0 + l] : 0.f);
   };
   stage(lam);
   __syncthreads();
@@ -297,12 +346,11 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
     r[t] = lane_ok ? gh[t] - y[t] : 0.f;
     part -= r[t] * r[t];
   }
-  part = wave_sum(part);
   __syncthreads();  // every neighbour lambda read before zb is reused
-  if ((tid & 63u) == 0) sh.red[1][w] = part;
+  post(1, part);
   stage_neg(r);
   __syncthreads();
-  float eta = seg_sum(sh.red[1], w0, nw);
+  float eta = total(1);
   neighbours(pm, pp);
 #pragma unroll
   for (int t = 0; t < RK; ++t) p[t] = -r[t];
@@ -311,48 +359,46 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
   const float tol_i = tol < 0.f ? fmaxf(-tol * fabsf(eta), FLT_MIN) : tol;
   bool conv = !live || fabsf(eta) < tol_i;
   float eta_last = eta;
+  // the bin's count of iterating instances: each instance's first lane adds
+  // itself here and leaves when it stops (between the second barrier of its
+  // last iteration and the first of the next), so after the first barrier of
+  // an iteration every wave sees the same count
+  if (leader && !conv && cap > 0) atomicAdd(&sh.nlive, 1u);
 
   for (;;) {
     const bool run = live && !conv && it < cap;  // uniform over the instance's waves
-    float pyp = 0.f;
     if (run) {
       smv(p, pm, pp);
+      float pyp = 0.f;
 #pragma unroll
       for (int t = 0; t < RK; ++t) pyp += p[t] * y[t];
-      pyp = wave_sum(pyp);
-    }
-    if ((tid & 63u) == 0) {
-      sh.red[0][w] = pyp;
-      sh.act[w] = run ? 1u : 0u;
+      post(0, pyp);
     }
     __syncthreads();
-    uint32_t any = 0;
-#pragma unroll
-    for (int v = 0; v < NWAV; ++v) any |= sh.act[v];
-    if (!any) break;
-    float rz = 0.f;
+    if (sh.nlive == 0u) break;
     if (run) {
-      const float alpha = eta / seg_sum(sh.red[0], w0, nw);
+      const float alpha = eta / total(0);
+      float rz = 0.f;
 #pragma unroll
       for (int t = 0; t < RK; ++t) {
         lam[t] += alpha * p[t];
+        // lanes 14, 15: y holds products with the next LDS slot's rows (finite, unused)
         r[t] = lane_ok ? r[t] - alpha * y[t] : 0.f;
         rz -= r[t] * r[t];
       }
       stage_neg(r);
-      rz = wave_sum(rz);
+      post(1, rz);
     }
-    if ((tid & 63u) == 0) sh.red[1][w] = rz;
     __syncthreads();
     if (run) {
       float zm, zp;
       neighbours(zm, zp);
-      const float eta_new = seg_sum(sh.red[1], w0, nw);
+      const float eta_new = total(1);
       eta_last = eta_new;
       ++it;
-      if (fabsf(eta_new) < tol_i) {
-        conv = true;
-      } else {
+      conv = fabsf(eta_new) < tol_i;
+      if (leader && (conv || it >= cap)) atomicSub(&sh.nlive, 1u);
+      if (!conv) {
         const float beta = eta_new / eta;
         eta = eta_new;
 #pragma unroll
