@@ -128,6 +128,8 @@ def parse():
     ap.add_argument("--no-closed-loop", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="issue every solve eagerly (no HIP graph replay), e.g. under counter profiling")
+    ap.add_argument("--no-modes", action="store_true",
+                    help="skip the stair-preconditioner and direct-solve legs (e.g. under tracing)")
     ap.add_argument("--no-stress", action="store_true",
                     help="skip the stress configuration (4096 instances, N in {128, 256, 512}, Q/R swept)")
     return ap.parse_args()
@@ -630,11 +632,42 @@ def phase_roofline(phases, knots, pcg_units, bw, counters=None):
                     "peak is the published packed rate (this build issues unpacked fp32: half of it)"}
 
 
+def other_modes(pkg, solver, torch, timed, step, K):
+    """The same workload with the stair preconditioner (half the PCG
+    iterations for one more block product each) and with the direct Schur
+    solve (block LDL^T on the GPU, one fp64-residual refinement of each KKT
+    step) instead of block-Jacobi PCG."""
+    solver.set_config(flags=pkg.FLAG_PCG_STAIR)
+    ms_stair, _, r = timed(True)
+    lg_s, st_s = r[4], r[3]
+    its_s = lg_s[lg_s >= 0]
+    stair = {"kkt_solves_per_sec": round(int((lg_s >= 0).sum()) / (ms_stair * 1e-3), 1),
+             "ms_per_step": round(ms_stair, 4),
+             "pcg_iters": {"mean": round(float(its_s.mean()), 1), "max": int(its_s.max()),
+                           "hit_max_iter": int((st_s & 2 != 0).sum())}}
+
+    solver.set_config(flags=pkg.FLAG_DIRECT_SOLVE)
+    ms_direct, _, r = timed(True)
+    st_d = r[3]
+    kkt_d = int((r[4] >= 0).sum())
+    solver.batch_set_timing(True)
+    step()
+    torch.cuda.synchronize()
+    ph_d = solver.batch_phase_times()
+    solver.batch_set_timing(False)
+    direct = {"kkt_solves_per_sec": round(kkt_d / (ms_direct * 1e-3), 1), "ms_per_step": round(ms_direct, 4),
+              "phase_us_per_sqp_iter": {name: round(v / K, 2) for name, v in zip(PHASES, ph_d)},
+              "status": {"aborted": int((st_d & 4 != 0).sum()), "nonfinite": int((st_d & 8 != 0).sum())},
+              "note": "phase 'pcg_team' holds the direct solve, 'dz' the dz recovery and its KKT refinement"}
+    solver.set_config(flags=0)
+    return stair, direct
+
+
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: every instance is independent, so N GPUs would "
-                                   "split the instance list (see DESIGN.md); only --gpus 1 is measured"}))
+        print(json.dumps({"error": "one GPU only: the instances are independent and would be split across "
+                                   "GPUs by instance, which is not built here; only --gpus 1 is measured"}))
         sys.exit(2)
     import torch
     pkg = load_package()
@@ -799,33 +832,9 @@ def main():
                          f"device-host copies around its solve (S values and gamma down, lambda up, pageable, "
                          f"timed on this GPU); the Schur formation line is the oracle's fp32 form_schur"}
 
-    # the same workload with the stair preconditioner (half the PCG iterations
-    # for one more block product each, on the same kernels)
-    solver.set_config(flags=pkg.FLAG_PCG_STAIR)
-    ms_stair, _, r = timed(True)
-    lg_s, st_s = r[4], r[3]
-    its_s = lg_s[lg_s >= 0]
-    stair = {"kkt_solves_per_sec": round(int((lg_s >= 0).sum()) / (ms_stair * 1e-3), 1),
-             "ms_per_step": round(ms_stair, 4),
-             "pcg_iters": {"mean": round(float(its_s.mean()), 1), "max": int(its_s.max()),
-                           "hit_max_iter": int((st_s & 2 != 0).sum())}}
-
-    # the same workload with the direct Schur solve (block LDL^T on the GPU,
-    # one fp64-residual refinement of each KKT step) instead of PCG
-    solver.set_config(flags=pkg.FLAG_DIRECT_SOLVE)
-    ms_direct, _, r = timed(True)
-    st_d = r[3]
-    kkt_d = int((r[4] >= 0).sum())
-    solver.batch_set_timing(True)
-    step()
-    torch.cuda.synchronize()
-    ph_d = solver.batch_phase_times()
-    solver.batch_set_timing(False)
-    direct = {"kkt_solves_per_sec": round(kkt_d / (ms_direct * 1e-3), 1), "ms_per_step": round(ms_direct, 4),
-              "phase_us_per_sqp_iter": {name: round(v / K, 2) for name, v in zip(PHASES, ph_d)},
-              "status": {"aborted": int((st_d & 4 != 0).sum()), "nonfinite": int((st_d & 8 != 0).sum())},
-              "note": "phase 'pcg_team' holds the direct solve, 'dz' the dz recovery and its KKT refinement"}
-    solver.set_config(flags=0)
+    stair = direct = None
+    if not args.no_modes:
+        stair, direct = other_modes(pkg, solver, torch, timed, step, K)
 
     c1 = config1(pkg, torch, args.seed)
     diff = None if args.no_diff else differentiation(pkg, torch, args.seed)

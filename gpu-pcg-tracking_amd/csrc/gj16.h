@@ -88,4 +88,37 @@ __device__ __forceinline__ void gj_inverse(int lane, float (&a)[NX], float (&x)[
   gj_steps<n>(a, x);
 }
 
+// Whitening factor of a symmetric positive definite n x n matrix M (n <=
+// 14) held by columns (lane j < n: column j in a; lanes >= n a copy of
+// column n-1): Gauss elimination of [M | I] without pivoting, each pivot row
+// scaled by the pivot's inverse square root once final.  Afterwards lane j
+// holds row j of W = U^-1 in x, where M = U^T U (Cholesky, U upper
+// triangular; the scaled rows of the left half are the rows of U), so
+// W^T M W = I; a is destroyed.  Every lane of the group must be active.
+template <int n>
+__device__ __forceinline__ void chol_inv_rows(int lane, float (&a)[NX], float (&x)[NX]) {
+  const int j = lane < n ? lane : n - 1;
+#pragma unroll
+  for (int r = 0; r < NX; ++r) x[r] = (r == j) ? 1.f : 0.f;
+#pragma unroll
+  for (int p = 0; p < n; ++p) {
+    float pc[NX];
+#pragma unroll
+    for (int r = p; r < n; ++r) pc[r] = bcast_dyn16(a[r], p);
+    const float inv = 1.f / pc[p];
+    const float s = 1.f / sqrtf(pc[p]);
+    const float ap = a[p], xp = x[p];
+#pragma unroll
+    for (int r = p + 1; r < n; ++r) {
+      const float m = pc[r] * inv;
+      a[r] -= m * ap;
+      x[r] -= m * xp;
+    }
+    a[p] = ap * s;
+    x[p] = xp * s;
+    pin(a);
+    pin(x);
+  }
+}
+
 }  // namespace mpc

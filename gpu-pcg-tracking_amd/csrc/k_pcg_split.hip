@@ -68,34 +68,11 @@ __global__ void __launch_bounds__(PWG) k_split_prep(Batch b, const float *__rest
   PrepLds &L = lds[grp];
   const int j = lane < NX ? lane : NX - 1;
   const float *Sr = S + (size_t)S_STRIDE * kg;
-  // 1. theta = -S(kg,kg) by columns; forward elimination of [theta | I]:
-  //    after step p, row p of [theta | I] scaled by theta_p(p,p)^-1/2 is row
-  //    p of [U | W^T] (U^T U = theta, W = U^-1); lane j keeps column j.
+  // 1. theta = -S(kg,kg) by columns; W_kg = chol(theta)^-1 by rows
   float a[NX], x[NX];
 #pragma unroll
-  for (int r = 0; r < NX; ++r) {
-    a[r] = -Sr[NXX + r + NX * j];
-    x[r] = (r == j) ? 1.f : 0.f;
-  }
-#pragma unroll
-  for (int p = 0; p < NX; ++p) {
-    float pc[NX];
-#pragma unroll
-    for (int r = p; r < NX; ++r) pc[r] = bcast_dyn16(a[r], p);
-    const float inv = 1.f / pc[p];
-    const float s = 1.f / sqrtf(pc[p]);
-    const float ap = a[p], xp = x[p];
-#pragma unroll
-    for (int r = p + 1; r < NX; ++r) {
-      const float m = pc[r] * inv;
-      a[r] -= m * ap;
-      x[r] -= m * xp;
-    }
-    a[p] = ap * s;
-    x[p] = xp * s;
-    pin(a);
-    pin(x);
-  }
+  for (int r = 0; r < NX; ++r) a[r] = -Sr[NXX + r + NX * j];
+  chol_inv_rows<NX>(lane, a, x);
   // x[c] = W^T(c, j) = W(j, c): lane j holds row j of W
   if (lane < NX) {
 #pragma unroll
@@ -255,9 +232,30 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
   // ---- load: L^ rows (registers; the first knot's also to LDS slot g),
   // lambda^ = W^-1 lambda by back substitution (W upper triangular), gamma^
   const uint32_t b0 = lg * RK, s0 = lbase + b0;
+  float lam[RK], r[RK], p[RK], y[RK], gh[RK];  // z = -r is not stored
+  // lambda^ and gamma^ first, then the L^ rows (fewer values live at once)
+#pragma unroll
+  for (int t = 0; t < RK; ++t) {
+    const uint32_t bk = b0 + (uint32_t)t;
+    const bool has = live && bk < N && lane_ok;
+    const float *Wr = Ps + (size_t)S_STRIDE * (K0 + (has ? bk : 0u)) + NXX;
+    float wrow[NX];
+#pragma unroll
+    for (int c = 0; c < NX; ++c) wrow[c] = has ? Wr[i + NX * c] : 0.f;
+    float acc = has ? lambda[(size_t)NX * (K0 + bk) + i] : 0.f;
+    float lh = 0.f;
+#pragma unroll
+    for (int k = NX - 1; k >= 0; --k) {
+      const float xk = bcast_dyn16(acc / wrow[k], k);
+      lh = (int)i == k ? xk : lh;
+      asm volatile("" : "+v"(lh));  // select now, not after the last step
+      acc -= wrow[k] * xk;
+    }
+    lam[t] = has ? lh : 0.f;
+    gh[t] = has ? ghat[(size_t)NX * (K0 + bk) + i] : 0.f;
+  }
   // Lr[t - 1]: rows of knot t = 1..3; knot 0's rows stay in LDS slot g only
   float Lr[RK - 1][16];
-  float lam[RK], r[RK], p[RK], y[RK], gh[RK];  // z = -r is not stored
 #pragma unroll
   for (int t = 0; t < RK; ++t) {
     const uint32_t bk = b0 + (uint32_t)t;
@@ -270,22 +268,10 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
 #pragma unroll
         for (int n = 0; n < 16; ++n) sh.L[g * L_SLOT + i * L_LANE + n] = L0[n];
       }
+      asm volatile("" ::: "memory");  // store now: L0 need not stay live across the other loads
     } else {
       load_lrows(Lr[t - 1], has, i, d, Pr);
     }
-    float wrow[NX];
-#pragma unroll
-    for (int c = 0; c < NX; ++c) wrow[c] = has && lane_ok ? Pr[NXX + i + NX * c] : 0.f;
-    float acc = has && lane_ok ? lambda[(size_t)NX * (K0 + bk) + i] : 0.f;
-    float lh = 0.f;
-#pragma unroll
-    for (int k = NX - 1; k >= 0; --k) {
-      const float xk = bcast_dyn16(acc / wrow[k], k);
-      if ((int)i == k) lh = xk;
-      acc -= wrow[k] * xk;
-    }
-    lam[t] = has && lane_ok ? lh : 0.f;
-    gh[t] = has && lane_ok ? ghat[(size_t)NX * (K0 + bk) + i] : 0.f;
   }
   const bool in_m = lg > 0, in_p = b0 + RK < N;
   float *zb0 = sh.zb + s0 * 16 + i;
