@@ -144,13 +144,19 @@ hipError_t launch_merit(const Batch &b, const Params &p, const float *xu, const 
 // right slot of row b-1 (the reference layout; the solvers read the
 // transpose of the next row's left block instead), SCHUR_ZERO_UNUSED
 // writes zeros into the blocks that are structurally zero (S(0,-1), the
-// right block of the last row, Pinv's off-diagonal blocks), SCHUR_NO_PINV
-// skips Pinv (the split-form PCG has no use for it)
-constexpr int SCHUR_ZERO_UNUSED = 1, SCHUR_RIGHT_BLOCK = 2, SCHUR_NO_PINV = 4;
+// right block of the last row, Pinv's off-diagonal blocks)
+constexpr int SCHUR_ZERO_UNUSED = 1, SCHUR_RIGHT_BLOCK = 2;
 hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, const float *g,
                              const float *c, const float *rho_dev, float rho, const uint32_t *status,
                              float *S, float *Pinv, float *gamma, float *Ginv, int out_mode,
                              hipStream_t st);
+// The split system (for launch_pcg_split) from the KKT blocks: G~^-1 into
+// Ginv (for launch_compute_dz), then L^_b and W_b into the rows of Ps and
+// gamma^ into ghat with the Schur block rows (S, Pinv and gamma are not
+// formed); each active instance's rho into rho_used (optional).
+hipError_t launch_form_split(const Batch &b, const float *G, const float *C, const float *g, const float *c,
+                             const float *rho_dev, float rho, const uint32_t *status, float *Ginv, float *Ps,
+                             float *ghat, float *rho_used, hipStream_t st);
 hipError_t launch_compute_dz(const Batch &b, const float *Ginv, const float *C, const float *g,
                              const float *lambda, const uint32_t *status, float *dz, hipStream_t st);
 // PCG of every instance.  Outputs per instance: iters (S products), flag (0

@@ -22,6 +22,8 @@
 // which races between neighbouring block rows).
 // dz (oldschur.cuh:966-1105):
 //   dx_k = Q~_k^-1 (q_k - lambda_k + A_k^T lambda_{k+1}),  du_k = R~_k^-1 (r_k + B_k^T lambda_{k+1})
+#include <cstddef>
+
 #include "common.h"
 #include "gj16.h"
 #include "model.h"
@@ -105,7 +107,6 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
                                                    float *__restrict__ gamma, const float *__restrict__ Ginv,
                                                    int out_mode) {
   const bool zero_unused = out_mode & SCHUR_ZERO_UNUSED, right = out_mode & SCHUR_RIGHT_BLOCK;
-  const bool pinv = !(out_mode & SCHUR_NO_PINV);
   __shared__ SchurLds lds[GROUPS_PER_WG];
   const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
   const uint32_t gk_raw = blockIdx.x * GROUPS_PER_WG + grp;
@@ -204,7 +205,7 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
     const float *Q0 = G + (size_t)G_STRIDE * row;
     for (int i = lane; i < NXX; i += GROUP) {
       Srow[NXX + i] = -Qi[i];
-      if (pinv) Prow[NXX + i] = -(Q0[i] + ((i % NX == i / NX) ? rho : 0.f));
+      Prow[NXX + i] = -(Q0[i] + ((i % NX == i / NX) ? rho : 0.f));
     }
     if (lane < NX) {
       float acc = 0.f;
@@ -215,10 +216,8 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
     if (zero_unused)
       for (int i = lane; i < NXX; i += GROUP) {
         Srow[i] = 0.f;
-        if (pinv) {
-          Prow[i] = 0.f;
-          Prow[2 * NXX + i] = 0.f;
-        }
+        Prow[i] = 0.f;
+        Prow[2 * NXX + i] = 0.f;
       }
   }
   if (valid && !row0) {
@@ -242,15 +241,13 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
     if (zero_unused) {
       if (bk + 1 == kr.N && right)
         for (int i = lane; i < NXX; i += GROUP) Srow[2 * NXX + i] = 0.f;
-      if (pinv)
-        for (int i = lane; i < NXX; i += GROUP) {
-          Prow[i] = 0.f;
-          Prow[2 * NXX + i] = 0.f;
-        }
+      for (int i = lane; i < NXX; i += GROUP) {
+        Prow[i] = 0.f;
+        Prow[2 * NXX + i] = 0.f;
+      }
     }
   }
   // Pinv(b,b) = -theta^-1
-  if (!pinv) return;
   float x[NX];
   gj_inverse<NX>(lane, th, x);
   __syncthreads();  // every lane is done reading S(b,b) from A
@@ -260,6 +257,171 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
   }
   __syncthreads();
   if (valid && !row0) store_block(lane, Prow + NXX, [&](int i) { return L.A[i]; });
+}
+
+// ---- the split system for the split-form PCG (k_pcg_split.hip), formed
+// with the block rows: per block row b, theta_b as in k_form_schur (from
+// G~^-1 of k_invert_g), W_b = chol(theta_b)^-1 (chol_inv_rows), gamma^_b =
+// W_b^T gamma_b and L^_b = W_b^T S(b,b-1) W_{b-1}, S(b,b-1) = A Q~_{b-1}^-1.
+// S, gamma and Pinv are not written.  Workgroups of SG groups overlap by
+// one row: group g of workgroup w forms row (SG - 1) w + g - 1, and groups
+// g >= 1 take W_{b-1} from group g - 1 through LDS and write their row
+// (each row from exactly one group).
+constexpr int SG = 16;
+constexpr int SWG = GROUP * SG;
+constexpr uint32_t SSTEP = SG - 1;
+
+struct alignas(16) SplitRowLds {
+  float AQ[NXX];   // A Q~_{b-1}^-1 = S(b, b-1)
+  float A[NXX];    // A_{b-1} (column-major); then W_b
+  float B[98];     // B_{b-1}; then, with BR, the L^ block on its way out
+  float BR[98];    // B R~_{b-1}^-1
+  float qk[16], q1[16], rk[8], cb[16], gm[16];
+};
+static_assert(offsetof(SplitRowLds, BR) == offsetof(SplitRowLds, B) + 98 * sizeof(float), "B, BR adjacent");
+
+__global__ void __launch_bounds__(SWG) k_form_split(Batch b, const float *__restrict__ C,
+                                                    const float *__restrict__ g, const float *__restrict__ cvec,
+                                                    const float *__restrict__ rho_dev, float rho_s,
+                                                    const uint32_t *__restrict__ status,
+                                                    const float *__restrict__ Ginv, float *__restrict__ Ps,
+                                                    float *__restrict__ ghat, float *__restrict__ rho_used) {
+  __shared__ SplitRowLds lds[SG];
+  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
+  const int64_t row_raw = (int64_t)blockIdx.x * SSTEP + grp - 1;
+  const bool in_range = row_raw >= 0 && row_raw < (int64_t)b.total_knots;
+  const uint32_t row = in_range ? (uint32_t)row_raw : (row_raw < 0 ? 0u : b.total_knots - 1);
+  const KnotRef kr = locate(b, row);
+  const bool owner = grp >= 1 && in_range && !(status && !(status[kr.inst] & ST_ACTIVE));
+  const bool row0 = kr.k == 0;
+  SplitRowLds &L = lds[grp];
+  const size_t kprev = row0 ? row : row - 1;  // row 0 stages its own knot, multiplied by zero below
+  const float *Ck = C + (size_t)C_STRIDE * kprev;
+  for (int i = lane; i < NXX; i += GROUP) L.A[i] = -Ck[i];
+  for (int i = lane; i < 98; i += GROUP) L.B[i] = -Ck[NXX + i];
+  if (lane < NX) {
+    L.qk[lane] = g[(size_t)NXU * kprev + lane];
+    L.q1[lane] = g[(size_t)NXU * row + lane];
+    L.cb[lane] = (row0 || !cvec) ? 0.f : cvec[(size_t)NX * row + lane];  // gamma_0 has no c term
+  }
+  if (lane < NU) L.rk[lane] = g[(size_t)NXU * kprev + NX + lane];
+  const int j = lane < NX ? lane : NX - 1;
+  const float *Qi = Ginv + (size_t)G_STRIDE * kprev;  // Q~_{b-1}^-1
+  const float *Q1i = Ginv + (size_t)G_STRIDE * row;   // Q~_b^-1
+  __syncthreads();
+  // AQ = A Q~_{b-1}^-1 (column j), BR = B R~_{b-1}^-1 (column j < 7); zero
+  // inverses for row 0
+  {
+    float qi[NX];
+#pragma unroll
+    for (int l = 0; l < NX; ++l) qi[l] = row0 ? 0.f : Qi[l + NX * j];
+    float acc[NX];
+#pragma unroll
+    for (int r = 0; r < NX; ++r) acc[r] = 0.f;
+#pragma unroll
+    for (int l = 0; l < NX; ++l)
+#pragma unroll
+      for (int r = 0; r < NX; ++r) acc[r] += L.A[r + NX * l] * qi[l];
+    pin(acc);
+    if (lane < NX) {
+#pragma unroll
+      for (int r = 0; r < NX; ++r) L.AQ[r + NX * lane] = acc[r];
+    }
+  }
+  {
+    const int ju = lane < NU ? lane : NU - 1;
+    float ri[NU];
+#pragma unroll
+    for (int l = 0; l < NU; ++l) ri[l] = row0 ? 0.f : Qi[NXX + l + NU * ju];
+    float bacc[NX];
+#pragma unroll
+    for (int r = 0; r < NX; ++r) bacc[r] = 0.f;
+#pragma unroll
+    for (int l = 0; l < NU; ++l)
+#pragma unroll
+      for (int r = 0; r < NX; ++r) bacc[r] += L.B[r + NX * l] * ri[l];
+    pin(bacc);
+    if (lane < NU) {
+#pragma unroll
+      for (int r = 0; r < NX; ++r) L.BR[r + NX * lane] = bacc[r];
+    }
+  }
+  __syncthreads();
+  // theta(:, j) = sum_l AQ(:, l) A(j, l) + sum_l BR(:, l) B(j, l) + Q~_b^-1(:, j)
+  float th[NX];
+#pragma unroll
+  for (int r = 0; r < NX; ++r) th[r] = Q1i[r + NX * j];
+#pragma unroll
+  for (int l = 0; l < NX; ++l) {
+    const float ajl = L.A[j + NX * l];
+#pragma unroll
+    for (int r = 0; r < NX; ++r) th[r] += L.AQ[r + NX * l] * ajl;
+  }
+#pragma unroll
+  for (int l = 0; l < NU; ++l) {
+    const float bjl = L.B[j + NX * l];
+#pragma unroll
+    for (int r = 0; r < NX; ++r) th[r] += L.BR[r + NX * l] * bjl;
+  }
+  pin(th);
+  // gamma = c - (Q~_b^-1 q_b - A Q~^-1 q_{b-1} - B R~^-1 r_{b-1}) (row 0: -Q~_0^-1 q_0)
+  if (lane < NX) {
+    float a1 = 0.f, a2 = 0.f, a3 = 0.f;
+#pragma unroll
+    for (int l = 0; l < NX; ++l) {
+      a1 += Q1i[lane + NX * l] * L.q1[l];  // row lane, as k_form_schur
+      a2 += L.AQ[lane + NX * l] * L.qk[l];
+    }
+#pragma unroll
+    for (int l = 0; l < NU; ++l) a3 += L.BR[lane + NX * l] * L.rk[l];
+    L.gm[lane] = L.cb[lane] - (a1 - a2 - a3);
+  }
+  float x[NX];
+  chol_inv_rows<NX>(lane, th, x);  // row j of W_b
+  __syncthreads();                 // every group is done reading its A
+  if (lane < NX) {
+#pragma unroll
+    for (int c = 0; c < NX; ++c) L.A[lane + NX * c] = x[c];  // W_b, column-major
+  }
+  __syncthreads();
+  // gamma^ = W_b^T gamma_b; L^ = W_b^T (S(b,b-1) W_{b-1}) by columns
+  float *Pr = Ps + (size_t)S_STRIDE * row;
+  float *Lout = L.B;  // B and BR: one 196-float block
+  if (owner) {
+    if (lane < NX) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < NX; ++k) acc += L.A[k + NX * lane] * L.gm[k];
+      ghat[(size_t)NX * row + lane] = acc;
+    }
+    if (row0 && lane == 0 && rho_used) rho_used[kr.inst] = rho_dev ? rho_dev[kr.inst] : rho_s;
+    if (!row0) {
+      const float *Wp = lds[grp - 1].A;  // W_{b-1}
+      float m[NX];
+#pragma unroll
+      for (int r = 0; r < NX; ++r) m[r] = 0.f;
+#pragma unroll
+      for (int k = 0; k < NX; ++k) {
+        const float wkj = Wp[k + NX * j];
+#pragma unroll
+        for (int r = 0; r < NX; ++r) m[r] += L.AQ[r + NX * k] * wkj;
+      }
+      if (lane < NX) {
+#pragma unroll
+        for (int r = 0; r < NX; ++r) {
+          float acc = 0.f;
+#pragma unroll
+          for (int k = 0; k < NX; ++k) acc += L.A[k + NX * r] * m[k];
+          Lout[r + NX * lane] = acc;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (owner) {
+    store_block(lane, Pr + NXX, [&](int i) { return L.A[i]; });
+    if (!row0) store_block(lane, Pr, [&](int i) { return Lout[i]; });
+  }
 }
 
 struct DzLds {
@@ -333,6 +495,19 @@ hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, con
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_form_schur, grid, dim3(WG), 0, st, b, G, C, g, c, rho_dev, rho, status, S, Pinv, gamma,
                      (const float *)Ginv, out_mode);
+  return hipGetLastError();
+}
+
+hipError_t launch_form_split(const Batch &b, const float *G, const float *C, const float *g, const float *c,
+                             const float *rho_dev, float rho, const uint32_t *status, float *Ginv, float *Ps,
+                             float *ghat, float *rho_used, hipStream_t st) {
+  if (b.total_knots == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_invert_g, dim3((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG), 0, st, b, G,
+                     rho_dev, rho, status, Ginv);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_form_split, dim3((b.total_knots + SSTEP - 1) / SSTEP), dim3(SWG), 0, st, b, C, g, c, rho_dev,
+                     rho, status, (const float *)Ginv, Ps, ghat, rho_used);
   return hipGetLastError();
 }
 

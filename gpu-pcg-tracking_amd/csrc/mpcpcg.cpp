@@ -60,6 +60,10 @@ struct mpc_handle {
   uint32_t n_split_bins = 0;
   bool split_ok = false;  // every horizon of the layout fits the split-form kernel
   float *ghat = nullptr;
+  // the batched SQP in split form forms no S: each instance's rho of its last
+  // formation is kept so that mpc_batch_direct_resolve can form S again
+  float *rho_used = nullptr;
+  bool schur_formed = true;  // the last batched solve left S and gamma in the workspace
   // PCG teams: two workgroups for 128 < N <= 256, four for 256 < N <= 512
   uint32_t *team_list = nullptr, *team_one = nullptr, *team_err = nullptr;
   uint32_t *pcg_done = nullptr;       // [B] parts that finished each team solve
@@ -275,7 +279,7 @@ static void free_all(mpc_handle *h) {
                   h->merits, h->xs, h->drho, h->merit0, h->rho1, h->status_ws, h->sqp_done,
                   h->pcg_iters, h->pcg_flag, h->pcg_eta, h->t0, h->knot_inst, h->inst_k0, h->bin_items,
                   h->bin_off, h->big_list, h->team_list, h->team_one, h->team_err, h->mbox, h->dir_order,
-                  h->pcg_done, h->pool_ctr, h->split_items, h->split_off, h->ghat,
+                  h->pcg_done, h->pool_ctr, h->split_items, h->split_off, h->ghat, h->rho_used,
                   h->adj_mu, h->adj_knot, h->loss_knot, h->ref_g, h->ref_c, h->ref_dz, h->ref_lam,
                   h->ref_scale, h->cl_win, h->cl_xp, h->cl_rho, h->cl_ctr, h->cl_status, h->cl_pcg,
                   h->rho_one};
@@ -341,6 +345,7 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
             alloc((void **)&h->split_items, sizeof(uint4) * B) &&
             alloc((void **)&h->split_off, sizeof(uint32_t) * (B + 1)) &&
             alloc((void **)&h->ghat, sizeof(float) * NX * T) &&
+            alloc((void **)&h->rho_used, sizeof(float) * B) &&
             alloc((void **)&h->big_list, sizeof(uint32_t) * B) &&
             alloc((void **)&h->team_list, sizeof(uint32_t) * B) &&
             alloc((void **)&h->dir_order, sizeof(uint32_t) * B) &&
@@ -1034,6 +1039,9 @@ int mpc_batch_pcg_exit(mpc_handle *h, uint32_t *d_iters, uint32_t *d_flags, floa
 int mpc_batch_direct_resolve(mpc_handle *h, float *d_lambda_out, uint32_t refine_steps, mpc_stream_t stream) {
   if (!h || !d_lambda_out || refine_steps > 16) return MPC_ERR_ARG;
   if (!h->batch_ready) return MPC_ERR_NOT_READY;
+  if (!h->schur_formed)  // the last solve was in split form: S and gamma at its last iterate and rho
+    HIP_TRY(launch_form_schur(batch_of(h), h->G, h->C, h->g, h->c, h->rho_used, 0.f, nullptr, h->S, h->P, h->gamma,
+                              h->Ginv, 0, (hipStream_t)stream));
   HIP_TRY(launch_direct_solve(batch_of(h), h->dir_order, h->n_inst, h->S, h->gamma, nullptr, h->P, d_lambda_out,
                               refine_steps, true, nullptr, nullptr, (hipStream_t)stream));
   return MPC_OK;
@@ -1127,10 +1135,15 @@ static hipError_t batch_sqp_run(mpc_handle *h, float *d_xu, float *d_lambda, con
     if (tm) MPC_RUN(hipEventRecord(h->ev[0], st));
     MPC_RUN(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, d_status, h->G, h->C, h->g, h->c, st));
     if (tm) MPC_RUN(hipEventRecord(h->ev[1], st));
+    // split form: the split system with the Schur block rows (no S)
     const bool split = split_active(h);
-    MPC_RUN(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma, h->Ginv,
-                              split ? SCHUR_NO_PINV : 0, st));
-    if (split) MPC_RUN(launch_split_prep(b, h->S, h->gamma, d_status, h->P, h->ghat, st));
+    h->schur_formed = !split;
+    if (split)
+      MPC_RUN(launch_form_split(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->Ginv, h->P, h->ghat,
+                                h->rho_used, st));
+    else
+      MPC_RUN(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma, h->Ginv, 0,
+                                st));
     if (tm) MPC_RUN(hipEventRecord(h->ev[2], st));
     MPC_RUN(pcg_batch(h, d_lambda, d_status, sqp_pcg_cap(h->cfg), h->cfg.pcg_exit_tol, tm ? h->ev[3] : nullptr, st,
                       split));

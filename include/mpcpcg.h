@@ -355,9 +355,11 @@ int mpc_batch_pcg_exit(mpc_handle *h, uint32_t *d_iters, uint32_t *d_flags, floa
  * steps of fp64-residual refinement, into d_lambda_out (14 per knot, the
  * batch's lambda layout): the accuracy reference for the PCG multipliers
  * that solve returned (the reference's QDLDL solve of the same system,
- * toplevel.cuh:314-335).  Overwrites the preconditioner workspace, so it
- * is for use after a solve, before the next one; instances that were no
- * longer active in that iteration get the solve of a stale system. */
+ * toplevel.cuh:314-335).  After a split-form solve (mpc_pcg_plan.split),
+ * which forms no S, S and gamma are first formed from the last iteration's
+ * KKT blocks and rho.  Overwrites the preconditioner workspace, so it is for
+ * use after a solve, before the next one; instances that were no longer
+ * active in that iteration get the solve of a stale system. */
 int mpc_batch_direct_resolve(mpc_handle *h, float *d_lambda_out, uint32_t refine_steps, mpc_stream_t stream);
 
 /* ---- closed-loop tracking ----------------------------------------------------
