@@ -276,11 +276,11 @@ struct alignas(16) SplitRowLds {
   float A[NXX];    // A_{b-1} (column-major); then W_b
   float B[98];     // B_{b-1}; then, with BR, the L^ block on its way out
   float BR[98];    // B R~_{b-1}^-1
-  float qk[16], q1[16], rk[8], cb[16], gm[16];
-};
+  float gm[16];    // gamma_b
+};  // 2416 B: four 16-group workgroups fit one CU's LDS
 static_assert(offsetof(SplitRowLds, BR) == offsetof(SplitRowLds, B) + 98 * sizeof(float), "B, BR adjacent");
 
-__global__ void __launch_bounds__(SWG) k_form_split(Batch b, const float *__restrict__ C,
+__global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k_form_split(Batch b, const float *__restrict__ C,
                                                     const float *__restrict__ g, const float *__restrict__ cvec,
                                                     const float *__restrict__ rho_dev, float rho_s,
                                                     const uint32_t *__restrict__ status,
@@ -299,13 +299,9 @@ __global__ void __launch_bounds__(SWG) k_form_split(Batch b, const float *__rest
   const float *Ck = C + (size_t)C_STRIDE * kprev;
   for (int i = lane; i < NXX; i += GROUP) L.A[i] = -Ck[i];
   for (int i = lane; i < 98; i += GROUP) L.B[i] = -Ck[NXX + i];
-  if (lane < NX) {
-    L.qk[lane] = g[(size_t)NXU * kprev + lane];
-    L.q1[lane] = g[(size_t)NXU * row + lane];
-    L.cb[lane] = (row0 || !cvec) ? 0.f : cvec[(size_t)NX * row + lane];  // gamma_0 has no c term
-  }
-  if (lane < NU) L.rk[lane] = g[(size_t)NXU * kprev + NX + lane];
   const int j = lane < NX ? lane : NX - 1;
+  // q_{b-1}, r_{b-1}, q_b: read by every lane of the group (cached)
+  const float *qk = g + (size_t)NXU * kprev, *rk = qk + NX, *q1v = g + (size_t)NXU * row;
   const float *Qi = Ginv + (size_t)G_STRIDE * kprev;  // Q~_{b-1}^-1
   const float *Q1i = Ginv + (size_t)G_STRIDE * row;   // Q~_b^-1
   __syncthreads();
@@ -369,12 +365,13 @@ __global__ void __launch_bounds__(SWG) k_form_split(Batch b, const float *__rest
     float a1 = 0.f, a2 = 0.f, a3 = 0.f;
 #pragma unroll
     for (int l = 0; l < NX; ++l) {
-      a1 += Q1i[lane + NX * l] * L.q1[l];  // row lane, as k_form_schur
-      a2 += L.AQ[lane + NX * l] * L.qk[l];
+      a1 += Q1i[lane + NX * l] * q1v[l];  // row lane, as k_form_schur
+      a2 += L.AQ[lane + NX * l] * qk[l];
     }
 #pragma unroll
-    for (int l = 0; l < NU; ++l) a3 += L.BR[lane + NX * l] * L.rk[l];
-    L.gm[lane] = L.cb[lane] - (a1 - a2 - a3);
+    for (int l = 0; l < NU; ++l) a3 += L.BR[lane + NX * l] * rk[l];
+    const float cb = (row0 || !cvec) ? 0.f : cvec[(size_t)NX * row + lane];  // gamma_0 has no c term
+    L.gm[lane] = cb - (a1 - a2 - a3);
   }
   float x[NX];
   chol_inv_rows<NX>(lane, th, x);  // row j of W_b
