@@ -56,6 +56,7 @@ PHASES = ("linearize", "schur", "pcg_team", "pcg_rest", "dz", "merit", "update")
 # dots 56, three axpys 84)
 PCG_BYTES_PER_KNOT_ITER = 4 * (105 + 196 + 105 + 12 * 14)
 PCG_FLOPS_PER_KNOT_ITER = 1708
+ENGINE_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 FP32_PEAK_TFLOPS = 157.3
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batched_1024.json")
 # per-kernel counters of the batched workload (rocprofv3 --pmc: LDS and VALU
@@ -783,8 +784,17 @@ def main():
     proof = phase_roofline(phases, knots_it, pcg_it, measure_bandwidth(pkg, torch), counters)
     lds_peak = proof["bandwidth"]["lds_gbs"]
     lds_counted = None
+    valu = None
     if counters and counters.get("pcg_kernel", {}).get("kernel") == kernel:
         tp = counters["pcg_kernel"]
+        # VALU issue: a wave64 instruction occupies a 16-lane SIMD for 4 cycles
+        simds = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+        peak_rate = simds * ENGINE_CLOCK_HZ / 4.0
+        rate = tp["valu_wave_instructions_per_launch"] / (t_launch * 1e-6)
+        valu = {"achieved": round(rate / 1e9, 2), "peak": round(peak_rate / 1e9, 2), "unit": "G wave-instructions/s",
+                "frac": round(rate / peak_rate, 4),
+                "source": "SQ_INSTS_VALU per launch (PMC) from " + os.path.relpath(COUNTERS_FILE, ROOT) +
+                          f"; peak = {simds} SIMDs x {ENGINE_CLOCK_HZ / 1e9:.1f} GHz / 4 cycles per wave64 instruction"}
         b_launch = tp["lds_wave_instructions_per_launch"] * tp["lds_bytes_per_wave_instruction"]
         lds_counted = {"bytes_per_launch": int(b_launch), "achieved": round(b_launch / (t_launch * 1e-6) / 1e9, 1),
                        "peak": lds_peak, "unit": "GB/s", "frac": round(b_launch / (t_launch * 1e-6) / 1e9 / lds_peak, 4),
@@ -895,6 +905,7 @@ def main():
                      {"achieved": round(traffic / (t_launch * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": round(traffic / (t_launch * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},
                      "lds_counted": lds_counted,
+                     "valu_issue": valu,
                      "fp32": {"achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": round(tflops / FP32_PEAK_TFLOPS, 4)}},
         "phase_roofline": proof,
