@@ -25,9 +25,10 @@
 // 4g .. 4g+3 of its instance, lane i element i; up to 256 knots, bins of
 // up to 8 instances, each on whole waves (as in pcg_res_body.h).  Per
 // group: L^ rows of its last three knots in registers (pre-rotated for the
-// DPP products of pcg_onchip.h), those of its first knot in an LDS slot,
-// read by the group itself and by the group before it (that group's
-// S^(b,b+1) = L^_{b+1}^T term).
+// DPP products of pcg_onchip.h); in two LDS slots of its own the rows of
+// L^ of its first knot and the rows of L^^T of the knot after its last
+// (the S^(b,b+1) term of its last knot, applied as a row product like
+// L^ x instead of the longer transposed form).
 #include "common.h"
 #include "gj16.h"
 #include "pcg_onchip.h"
@@ -125,13 +126,36 @@ constexpr int RK = 4;
 constexpr int KNOTS = NGRP * RK;  // 256
 constexpr int MAXI = 8;
 
+// A block slot: rotated row element n of lane i at ((n >> 1) * 16 + i) * 2 +
+// (n & 1), read as 8 x 64-bit words (each 16-lane read covers the 32 banks
+// once); lanes 14, 15 read zeros (never written).
+constexpr int SLOT = 256;
+
 struct SplitShared {
-  float L[(NGRP + 1) * L_SLOT + L_GUARD];  // L^ rows of each group's first knot (+ a zero slot)
-  float zb[(KNOTS + 1) * 16];               // staged vector element per knot slot and lane
-  float red[2][NGRP];                       // dot-product partials, one per group
-  uint32_t nlive;                           // instances of the bin still iterating
+  float R[NGRP * SLOT];       // rows of L^ of each group's first knot
+  float K[NGRP * SLOT];       // rows of L^^T of the knot after each group's last
+  float zb[NGRP * 2 * 16];    // staged vector element of each group's first and last knot
+  float red[2][MAXI][NGRP];   // dot-product partials: one per group, by instance and group
+  uint32_t nlive;             // instances of the bin still iterating
 };
-static_assert(((NGRP + 1) * L_SLOT + L_GUARD) % 4 == 0, "L and zb are cleared as one float4 range");
+constexpr uint32_t SPLIT_CLEAR4 = (uint32_t)(offsetof(SplitShared, nlive) / 16);
+static_assert(offsetof(SplitShared, nlive) % 16 == 0, "slots, zb and red are cleared as one float4 range");
+
+__device__ __forceinline__ void write_slot(float *Sl, uint32_t i, const float (&M)[16]) {
+  float2 *s2 = reinterpret_cast<float2 *>(Sl) + i;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) s2[16 * c] = make_float2(M[2 * c], M[2 * c + 1]);
+}
+
+__device__ __forceinline__ void read_slot(float (&M)[16], const float *Sl, uint32_t i) {
+  const float2 *s2 = reinterpret_cast<const float2 *>(Sl) + i;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const float2 v = s2[16 * c];
+    M[2 * c] = v.x;
+    M[2 * c + 1] = v.y;
+  }
+}
 
 // total of the 16 lanes of each row, in every lane of the row (fixed order;
 // each step adds two values in either order, so the lanes agree bitwise)
@@ -145,29 +169,55 @@ __device__ __forceinline__ float row_total(float v) {
 
 __device__ __forceinline__ float ror1(float v) { return dpp_f<0x121>(v); }
 
-// (M_k^T x_k)_i for four blocks with rotated rows M_k: sum_n ror_n(m_n),
+// (M_k^T x_k)_i for three blocks with rotated rows M_k: sum_n ror_n(m_n),
 // m_n = M[(16 - n) & 15] x, in Horner form m_0 + ror_1(m_1 + ror_1(m_2 + ...
 // + ror_1(m_15))): 16 products and 15 row_ror:1 additions per block, the
-// four chains interleaved
-__device__ __forceinline__ void rot_mtv4(const float (&M0)[16], float x0, const float (&M1)[16], float x1,
-                                         const float (&M2)[16], float x2, const float (&M3)[16], float x3,
-                                         float (&c)[4]) {
+// three chains interleaved
+__device__ __forceinline__ void rot_mtv3(const float (&M0)[16], float x0, const float (&M1)[16], float x1,
+                                         const float (&M2)[16], float x2, float (&c)[3]) {
   // separate products: a fused multiply-add would take the rotated sum as
   // its addend, which DPP cannot supply (it rotates the first source only)
 #pragma clang fp contract(off)
-  float a0 = M0[1] * x0, a1 = M1[1] * x1, a2 = M2[1] * x2, a3 = M3[1] * x3;
+  float a0 = M0[1] * x0, a1 = M1[1] * x1, a2 = M2[1] * x2;
 #pragma unroll
   for (int n = 14; n >= 0; --n) {
     const int k = (16 - n) & 15;
     a0 = ror1(a0) + M0[k] * x0;
     a1 = ror1(a1) + M1[k] * x1;
     a2 = ror1(a2) + M2[k] * x2;
-    a3 = ror1(a3) + M3[k] * x3;
   }
   c[0] = a0;
   c[1] = a1;
   c[2] = a2;
-  c[3] = a3;
+}
+
+// Three interleaved accumulations a_k += sum_{n=1..15} M_k[n] * ror_n(x_k)
+// (as rot_fma15x2 of pcg_onchip.h, one more chain)
+__device__ __forceinline__ void rot_fma15x3(float &a, float xa, const float (&Ma)[16], float &b, float xb,
+                                            const float (&Mb)[16], float &c, float xc, const float (&Mc)[16]) {
+#define MPC_RF3(n, oa, ob, oc)                                                                    \
+  "v_fmac_f32_dpp %0, %3, %" #oa " row_ror:" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+  "v_fmac_f32_dpp %1, %4, %" #ob " row_ror:" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+  "v_fmac_f32_dpp %2, %5, %" #oc " row_ror:" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+  asm("s_nop 4\n" MPC_RF3(1, 6, 11, 16) MPC_RF3(2, 7, 12, 17) MPC_RF3(3, 8, 13, 18) MPC_RF3(4, 9, 14, 19)
+          MPC_RF3(5, 10, 15, 20)
+      : "+v"(a), "+v"(b), "+v"(c)
+      : "v"(xa), "v"(xb), "v"(xc), "v"(Ma[1]), "v"(Ma[2]), "v"(Ma[3]), "v"(Ma[4]), "v"(Ma[5]), "v"(Mb[1]),
+        "v"(Mb[2]), "v"(Mb[3]), "v"(Mb[4]), "v"(Mb[5]), "v"(Mc[1]), "v"(Mc[2]), "v"(Mc[3]), "v"(Mc[4]),
+        "v"(Mc[5]));
+  asm("s_nop 4\n" MPC_RF3(6, 6, 11, 16) MPC_RF3(7, 7, 12, 17) MPC_RF3(8, 8, 13, 18) MPC_RF3(9, 9, 14, 19)
+          MPC_RF3(10, 10, 15, 20)
+      : "+v"(a), "+v"(b), "+v"(c)
+      : "v"(xa), "v"(xb), "v"(xc), "v"(Ma[6]), "v"(Ma[7]), "v"(Ma[8]), "v"(Ma[9]), "v"(Ma[10]), "v"(Mb[6]),
+        "v"(Mb[7]), "v"(Mb[8]), "v"(Mb[9]), "v"(Mb[10]), "v"(Mc[6]), "v"(Mc[7]), "v"(Mc[8]), "v"(Mc[9]),
+        "v"(Mc[10]));
+  asm("s_nop 4\n" MPC_RF3(11, 6, 11, 16) MPC_RF3(12, 7, 12, 17) MPC_RF3(13, 8, 13, 18) MPC_RF3(14, 9, 14, 19)
+          MPC_RF3(15, 10, 15, 20)
+      : "+v"(a), "+v"(b), "+v"(c)
+      : "v"(xa), "v"(xb), "v"(xc), "v"(Ma[11]), "v"(Ma[12]), "v"(Ma[13]), "v"(Ma[14]), "v"(Ma[15]), "v"(Mb[11]),
+        "v"(Mb[12]), "v"(Mb[13]), "v"(Mb[14]), "v"(Mb[15]), "v"(Mc[11]), "v"(Mc[12]), "v"(Mc[13]), "v"(Mc[14]),
+        "v"(Mc[15]));
+#undef MPC_RF3
 }
 
 // (Ma xa)_i - sa and (Mb xb)_i - sb (the -x term of S^'s -I diagonal)
@@ -178,13 +228,16 @@ __device__ __forceinline__ void rot_mv2_minus(const float (&Ma)[16], float xa, f
   rot_fma15x2(ya, xa, Ma, yb, xb, Mb);
 }
 
+// rotated rows of the column-major 14 x 14 block Lh (T = true: of its
+// transpose); zeros where has is false and in lanes 14, 15
+template <bool T = false>
 __device__ __forceinline__ void load_lrows(float (&Lr)[16], bool has, uint32_t i, uint32_t d,
                                            const float *__restrict__ Lh) {
   const bool row = has && i < (uint32_t)NX;
 #pragma unroll
   for (int n = 0; n < 16; ++n) {
     const uint32_t j = (i + (uint32_t)n * d) & 15u;
-    Lr[n] = row && j < (uint32_t)NX ? Lh[i + NX * j] : 0.f;
+    Lr[n] = row && j < (uint32_t)NX ? Lh[T ? j + NX * i : i + NX * j] : 0.f;
   }
 }
 
@@ -201,22 +254,20 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
   const uint32_t g = tid >> 4, i = tid & 15u;
   const uint32_t d = ror_step(tid);
   {
-    float4 *z4 = reinterpret_cast<float4 *>(sh.L);
-    constexpr uint32_t n4 = ((NGRP + 1) * L_SLOT + L_GUARD + (KNOTS + 1) * 16) / 4;
-    for (uint32_t idx = tid; idx < n4; idx += NT) z4[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 *z4 = reinterpret_cast<float4 *>(smem_raw);
+    for (uint32_t idx = tid; idx < SPLIT_CLEAR4; idx += NT) z4[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (tid == 0) sh.nlive = 0u;
   }
   // my instance in this bin (items: {inst, first group, groups, -})
   const uint32_t i0 = bin_off[bin], ni = bin_off[bin + 1] - i0;
   int slot = -1;
-  uint32_t inst = 0, g0 = 0, ng = NGRP;
+  uint32_t inst = 0, g0 = 0;
   for (uint32_t jx = 0; jx < ni; ++jx) {
     const uint4 it = items[i0 + jx];
     if (g >= it.y && g < it.y + it.z) {
       slot = (int)jx;
       inst = it.x;
       g0 = it.y;
-      ng = it.z;
     }
   }
   const bool mine = slot >= 0;
@@ -224,14 +275,14 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
   const uint32_t K0 = mine ? inst_first(b, inst) : 0u;
   const bool live = mine && !(status && !(status[inst] & ST_ACTIVE));
   const uint32_t cap = pcg_cap(max_iter, N);
-  const uint32_t lg = g - g0, lbase = g0 * RK;
+  const uint32_t lg = g - g0;
   const bool leader = lg == 0 && i == 0 && mine;  // the instance's first lane
   const bool lane_ok = i < (uint32_t)NX;
   __syncthreads();  // LDS cleared
 
   // ---- load: L^ rows (registers; the first knot's also to LDS slot g),
   // lambda^ = W^-1 lambda by back substitution (W upper triangular), gamma^
-  const uint32_t b0 = lg * RK, s0 = lbase + b0;
+  const uint32_t b0 = lg * RK;
   float lam[RK], r[RK], p[RK], y[RK], gh[RK];  // z = -r is not stored
   // lambda^ and gamma^ first, then the L^ rows (fewer values live at once)
 #pragma unroll
@@ -254,70 +305,81 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
     lam[t] = has ? lh : 0.f;
     gh[t] = has ? ghat[(size_t)NX * (K0 + bk) + i] : 0.f;
   }
-  // Lr[t - 1]: rows of knot t = 1..3; knot 0's rows stay in LDS slot g only
+  // Lr[t - 1]: rows of knot t = 1..3 in registers; knot 0's rows to LDS slot
+  // R[g], the transposed rows of knot 4 (the next group's first) to K[g]
   float Lr[RK - 1][16];
+  float *Rg = sh.R + g * SLOT, *Kg = sh.K + g * SLOT;
 #pragma unroll
-  for (int t = 0; t < RK; ++t) {
+  for (int t = 0; t <= RK; t += RK) {
+    const uint32_t bk = b0 + (uint32_t)t;
+    const bool has = live && bk < N && bk > 0;
+    const float *Pr = Ps + (size_t)S_STRIDE * (K0 + (has ? bk : 0u));
+    float Ls[16];
+    if (t == 0)
+      load_lrows(Ls, has, i, d, Pr);
+    else
+      load_lrows<true>(Ls, has, i, d, Pr);
+    if (lane_ok) write_slot(t == 0 ? Rg : Kg, i, Ls);
+    asm volatile("" ::: "memory");  // store now: Ls need not stay live across the other loads
+  }
+#pragma unroll
+  for (int t = 1; t < RK; ++t) {
     const uint32_t bk = b0 + (uint32_t)t;
     const bool has = live && bk < N;
-    const float *Pr = Ps + (size_t)S_STRIDE * (K0 + (has ? bk : 0u));
-    if (t == 0) {
-      float L0[16];
-      load_lrows(L0, has && bk > 0, i, d, Pr);
-      if (lane_ok) {
-#pragma unroll
-        for (int n = 0; n < 16; ++n) sh.L[g * L_SLOT + i * L_LANE + n] = L0[n];
-      }
-      asm volatile("" ::: "memory");  // store now: L0 need not stay live across the other loads
-    } else {
-      load_lrows(Lr[t - 1], has, i, d, Pr);
-    }
+    load_lrows(Lr[t - 1], has, i, d, Ps + (size_t)S_STRIDE * (K0 + (has ? bk : 0u)));
   }
   const bool in_m = lg > 0, in_p = b0 + RK < N;
-  float *zb0 = sh.zb + s0 * 16 + i;
+  // zb: [group][first, last knot][lane]
+  float *zbg = sh.zb + g * 32 + i;
   auto stage = [&](const float (&v)[RK]) {
-    zb0[0] = v[0];
-    zb0[(RK - 1) * 16] = v[RK - 1];
+    zbg[0] = v[0];
+    zbg[16] = v[RK - 1];
   };
   auto stage_neg = [&](const float (&v)[RK]) {
-    zb0[0] = -v[0];
-    zb0[(RK - 1) * 16] = -v[RK - 1];
+    zbg[0] = -v[0];
+    zbg[16] = -v[RK - 1];
   };
   auto neighbours = [&](float &vm, float &vp) {
-    vm = in_m ? zb0[-16] : 0.f;
-    vp = in_p ? zb0[RK * 16] : 0.f;
+    vm = in_m ? zbg[-16] : 0.f;
+    vp = in_p ? zbg[32] : 0.f;
   };
   // y = S^ x: y_t = -x_t + L^_t x_{t-1} + L^_{t+1}^T x_{t+1}; xm / xp: the
-  // elements of knots 4lg - 1 and 4lg + 4 (the next group's L^ from LDS)
-  const float *Lown = sh.L + g * L_SLOT;
+  // elements of knots 4lg - 1 and 4lg + 4.  Lanes 14, 15 of every block row
+  // are zero, so y (and r, p, lambda^) stay zero there.
   auto smv = [&](const float (&x)[RK], float xm, float xp) {
-    float a0, a1, a2, a3;
+    float a0, a1, a2, a3, e3;
     {
       float L0[16];
-      read_lrow(L0, Lown, i);
+      read_slot(L0, Rg, i);
       rot_mv2_minus(L0, xm, x[0], a0, Lr[0], x[0], x[1], a1);
     }
-    rot_mv2_minus(Lr[1], x[1], x[2], a2, Lr[2], x[2], x[3], a3);
-    float c[4];
     {
-      float Ln[16];
-      read_lrow(Ln, Lown + L_SLOT, i);
-      rot_mtv4(Lr[0], x[1], Lr[1], x[2], Lr[2], x[3], Ln, xp, c);
+      float Kn[16];
+      read_slot(Kn, Kg, i);
+      a2 = fmaf(Lr[1][0], x[1], -x[2]);
+      a3 = fmaf(Lr[2][0], x[2], -x[3]);
+      e3 = Kn[0] * xp;
+      rot_fma15x3(a2, x[1], Lr[1], a3, x[2], Lr[2], e3, xp, Kn);
     }
+    float c[3];
+    rot_mtv3(Lr[0], x[1], Lr[1], x[2], Lr[2], x[3], c);
     y[0] = a0 + c[0];
     y[1] = a1 + c[1];
     y[2] = a2 + c[2];
-    y[3] = a3 + c[3];
+    y[3] = a3 + e3;
   };
-  // dot products: a row total per group to LDS, then, after the barrier,
-  // the instance's group totals (lanes 0 .. ng-1 of every wave) summed
+  // dot products: a row total per group to LDS (red[k][instance][group]),
+  // then, after the barrier, the instance's 64 entries (zero past its
+  // groups) summed by every row: four per lane, then the row total
+  float *redg = &sh.red[0][mine ? slot : 0][lg & (NGRP - 1)];
   auto post = [&](int k, float part) {
     const float v = row_total(part);
-    if (i == 0) sh.red[k][g] = v;
+    if (i == 0) redg[k * MAXI * NGRP] = v;
   };
+  const float4 *red4 = reinterpret_cast<const float4 *>(&sh.red[0][mine ? slot : 0][0]) + i;
   auto total = [&](int k) {
-    const uint32_t l = tid & 63u;
-    return wave_sum(l < ng ? sh.red[k][g0 + l] : 0.f);
+    const float4 v = red4[k * MAXI * NGRP / 4];
+    return row_total((v.x + v.y) + (v.z + v.w));
   };
   stage(lam);
   __syncthreads();
@@ -368,8 +430,7 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
 #pragma unroll
       for (int t = 0; t < RK; ++t) {
         lam[t] += alpha * p[t];
-        // lanes 14, 15: y holds products with the next LDS slot's rows (finite, unused)
-        r[t] = lane_ok ? r[t] - alpha * y[t] : 0.f;
+        r[t] -= alpha * y[t];
         rz -= r[t] * r[t];
       }
       stage_neg(r);

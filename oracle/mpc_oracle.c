@@ -630,8 +630,11 @@ static void F(orc_precond)(int N, const real *S, const real *Pinv, const real *r
   F(orc_bd_mv)(N, Pinv, t, z);
 }
 
-void F(orc_pcg_ex)(int N, const real *S, const real *Pinv, const real *gamma, real *lambda,
-                   int max_iter, real tol, int stair, int *iters, int *max_iter_exit) {
+/* orc_pcg_trace: as orc_pcg_ex, and when eta_trace is not NULL it receives
+ * |r^T z| before the first iteration and after each one (max_iter + 1
+ * values; entries past the exit are left as they were). */
+void F(orc_pcg_trace)(int N, const real *S, const real *Pinv, const real *gamma, real *lambda,
+                      int max_iter, real tol, int stair, int *iters, int *max_iter_exit, real *eta_trace) {
   int n = NX * N;
   real *r = (real *)malloc(sizeof(real) * n), *z = (real *)malloc(sizeof(real) * n);
   real *p = (real *)malloc(sizeof(real) * n), *y = (real *)malloc(sizeof(real) * n);
@@ -642,6 +645,7 @@ void F(orc_pcg_ex)(int N, const real *S, const real *Pinv, const real *gamma, re
   real eta = 0;
   for (int i = 0; i < n; ++i) { p[i] = z[i]; eta += r[i] * z[i]; }
   int it = 0, conv = 0;
+  if (eta_trace) eta_trace[0] = (real)fabs(eta);
   if ((real)fabs(eta) < tol) conv = 1;
   while (!conv && it < max_iter) {
     F(orc_bd_mv)(N, S, p, y);
@@ -653,6 +657,7 @@ void F(orc_pcg_ex)(int N, const real *S, const real *Pinv, const real *gamma, re
     real eta_new = 0;
     for (int i = 0; i < n; ++i) eta_new += r[i] * z[i];
     ++it;
+    if (eta_trace) eta_trace[it] = (real)fabs(eta_new);
     if ((real)fabs(eta_new) < tol) { conv = 1; break; }
     real beta = eta_new / eta;
     eta = eta_new;
@@ -661,6 +666,11 @@ void F(orc_pcg_ex)(int N, const real *S, const real *Pinv, const real *gamma, re
   *iters = it;
   *max_iter_exit = !conv;
   free(r); free(z); free(p); free(y); free(t);
+}
+
+void F(orc_pcg_ex)(int N, const real *S, const real *Pinv, const real *gamma, real *lambda,
+                   int max_iter, real tol, int stair, int *iters, int *max_iter_exit) {
+  F(orc_pcg_trace)(N, S, Pinv, gamma, lambda, max_iter, tol, stair, iters, max_iter_exit, NULL);
 }
 
 void F(orc_pcg)(int N, const real *S, const real *Pinv, const real *gamma, real *lambda,

@@ -191,13 +191,19 @@ def bd_mv(N, S, x, dtype=np.float64):
     return y
 
 
-def pcg(N, S, P, gamma, lam0, max_iter, tol, dtype=np.float64, stair=False):
+def pcg(N, S, P, gamma, lam0, max_iter, tol, dtype=np.float64, stair=False, trace=False):
+    """The reference's PCG (see orc_pcg_ex): (lambda, iterations, capped);
+    with trace=True also |r^T z| before the first iteration and after each
+    one (iterations + 1 values)."""
     S, P, gamma = (_arr(a, dtype) for a in (S, P, gamma))
     lam = _arr(lam0, dtype).copy()
     it, mx = C.c_int(), C.c_int()
     tolc = (C.c_double if dtype == np.float64 else C.c_float)(tol)
-    _fn("orc_pcg_ex", dtype)(N, _ptr(S), _ptr(P), _ptr(gamma), _ptr(lam), max_iter, tolc, int(stair),
-                             C.byref(it), C.byref(mx))
+    etas = np.zeros(max_iter + 1, dtype)
+    _fn("orc_pcg_trace", dtype)(N, _ptr(S), _ptr(P), _ptr(gamma), _ptr(lam), max_iter, tolc, int(stair),
+                                C.byref(it), C.byref(mx), _ptr(etas) if trace else None)
+    if trace:
+        return lam, it.value, mx.value, etas[:it.value + 1]
     return lam, it.value, mx.value
 
 

@@ -119,3 +119,23 @@ def pcg_fp32_floor(orc, N, S, P, gamma, lam0, cap, tol, ref, samples=3, stair=Fa
         lt, _, _ = orc.pcg(N, Sk, P, gamma, lam0, cap, tol, dtype=np.float32, stair=stair)
         worst = max(worst, rel_err(lt, ref))
     return worst
+
+
+def pcg_exit_ok(it_gpu, etas64, tol, cap, band=0.25):
+    """Whether an fp32 PCG may exit after it_gpu iterations, given the fp64
+    oracle's |r^T z| after each iteration (etas64: before the first and
+    after each of `cap` iterations, run with tol = 0).  Near the exit the
+    fp32 runs' |r^T z| sit up to ~15 % from the fp64 values (measured on
+    this workload for the oracle's own fp32 PCG), and |r^T z| is not
+    monotone in CG, so a crossing within that band can move by several
+    iterations.  Accepted: an exit no later than the first iteration where
+    the fp64 value is clearly below tol (< (1 - band) tol), at an iteration
+    where it is below (1 + band) tol, or at the cap when there is no clear
+    crossing before it."""
+    e = np.abs(np.asarray(etas64, np.float64))
+    assert len(e) == cap + 1
+    below = np.flatnonzero(e < (1.0 - band) * tol)
+    clear = int(below[0]) if len(below) else cap
+    if it_gpu > clear:
+        return False
+    return it_gpu == cap or e[it_gpu] < (1.0 + band) * tol

@@ -13,11 +13,12 @@ bench.py, 3 SQP iterations, PCG tol 1e-5, cap 173, replayed as a HIP graph):
   in 3-4 % of this workload's instances for every fp32 PCG here, the
   oracle's own fp32 twin included; at most 2 of the 16 may take steps none
   of the four oracle runs took, and are only checked finite); for the
-  others, the first PCG count within 2 %, later
+  others, the first PCG count where the fp64 oracle's |r^T z| crosses the
+  tolerance (within the band of fp32 rounding, helpers.pcg_exit_ok), later
   ones within the oracle runs' range +- 10 % (after the first SQP iteration
   the systems differ by the rounding of the earlier steps, and where the
   oracle's own fp32 runs spread by more than 1 % in xu the later counts are
-  not compared here; on the GPU's own systems the counts agree within 2 %,
+  not compared here; on the GPU's own systems the counts are checked
   below); final xu within
   1e-3 or 4 x the spread of the fp32 runs on the same steps (a PCG capped at
   173 iterations leaves lambda at the fp32 scatter level, and dz amplifies
@@ -28,8 +29,10 @@ bench.py, 3 SQP iterations, PCG tol 1e-5, cap 173, replayed as a HIP graph):
   points at the batch's iterate and rho, warm started from the batch's
   previous multipliers) within 1e-4 relative, or 4 x the fp32 floor of that
   solve (the largest deviation of the oracle's fp32 PCG on S and on three
-  one-ulp perturbations of S) where it exceeds 2.5e-5; iteration counts
-  within 2 %.
+  one-ulp perturbations of S) where it exceeds 2.5e-5; the iteration
+  count where the fp64 oracle's |r^T z| crosses the tolerance, within the
+  band of fp32 rounding (helpers.pcg_exit_ok: near the exit, rounding moves
+  a crossing by a few iterations).
 
 Stress workload (4096 instances, N ~ U{128, 256, 512}, Q/R = 1e6):
 every instance finite with only ACTIVE / MAXITER (/ PCG-fault free) bits
@@ -175,13 +178,18 @@ def test_batched_sampled_instances_vs_oracle_sqp(batched, orc):
               f"steps={ref['ls_steps']} ({'fp64' if ref is r64 else 'fp32'} run) rel(xu)={err:.2e} "
               f"floor={floor:.2e} rho={rho[i]:.3e}")
         assert ref["sqp_iters"] == 3
-        # PCG counts: the first solve (the same system) within 2 %; the later
+        # PCG counts: the first solve (the same system) where the fp64
+        # oracle's |r^T z| on it crosses the tolerance (helpers.pcg_exit_ok); the later
         # ones (warm started after a capped solve) within the range of the
         # oracle runs on the same steps, +- 10 %, unless the fp32 runs
         # themselves spread by more than 1 % in xu (the later systems then
         # differ by that much; the every-iteration test below compares the
         # counts on the GPU's own systems)
-        assert abs(int(pit[i][0]) - ref["pcg_iters"][0]) <= max(2, ref["pcg_iters"][0] // 50), i
+        xu0_i = batched.xu0[batched.xu_slice(i)].astype(np.float64)
+        G, Cm, g, c = orc.form_kkt(cfg, N, xu0_i, batched.ee[batched.ee_slice(i)].astype(np.float64), xu0_i[:14])
+        S0, P0, gam0, _ = orc.form_schur(N, G, Cm, g, c, 1e-3)
+        _, _, _, e0 = orc.pcg(N, S0, P0, gam0, np.zeros(14 * N), CAP, 0.0, trace=True)
+        assert helpers.pcg_exit_ok(int(pit[i][0]), e0, 1e-5, CAP), i
         for j in (1, 2) if floor < 1e-2 else ():
             lo = min(r["pcg_iters"][j] for r in match)
             hi = max(r["pcg_iters"][j] for r in match)
@@ -223,13 +231,14 @@ def test_batched_every_iteration_lambda_vs_oracle_pcg_on_gpu_schur(batched, orc,
             warm = lam_k[batched.lam_slice(i)].astype(np.float64)
             lo, it_o, fl_o = orc.pcg(N, S64, P64, g64, warm, CAP, 1e-5)
             floor = helpers.pcg_fp32_floor(orc, N, S64, P64, g64, warm, CAP, 1e-5, lo)
+            _, _, _, e64 = orc.pcg(N, S64, P64, g64, warm, CAP, 0.0, trace=True)
             lam_gpu = states[k + 1][1][batched.lam_slice(i)]
             err = helpers.rel_err(lam_gpu, lo)
             worst = max(worst, err)
             it_g = int(pit3[i, k])
             print(f"inst {i} N={N} SQP iteration {k}: PCG gpu={it_g} oracle={it_o} (cap {CAP}) rel(lambda)={err:.2e} "
-                  f"fp32 floor={floor:.2e}")
-            assert abs(it_g - it_o) <= max(2, it_o // 50), (i, k)
+                  f"fp32 floor={floor:.2e} fp64 |eta| at gpu exit={e64[min(it_g, CAP)]:.3e}")
+            assert helpers.pcg_exit_ok(it_g, e64, 1e-5, CAP), (i, k, it_g, it_o)
             assert err < max(1e-4, 4 * floor), (i, k)
     print(f"worst rel(lambda) {worst:.2e}")
 

@@ -11,8 +11,9 @@ toplevel.cuh:143-157; restated in tests/test_split_form.py).  Here:
   iterate and rho, warm started from the batch's previous multipliers)
   within 1e-4, or 4 x the fp32 floor of that solve (the largest deviation
   of the oracle's fp32 PCG on S and on three one-ulp perturbations of S);
-  iteration counts within 2 %; the exit state (flag, |r^T Pinv r|) as the
-  exit rule defines it;
+  iteration counts where the fp64 oracle's |r^T z| crosses the tolerance,
+  within the band of fp32 rounding (helpers.pcg_exit_ok); the exit state
+  (flag, |r^T Pinv r|) as the exit rule defines it;
 - every packing of the same layout (instances per bin 8, 4, 2): bitwise
   the same solve (an instance's arithmetic does not depend on its bin);
 - an instance with a non-finite reference beside healthy ones: only its
@@ -130,12 +131,13 @@ def test_split_layout_vs_oracle_pcg(split_solver, pkg, orc, name):
             warm = lam_k[B.ls(i)].astype(np.float64)
             lo, it_o, _ = orc.pcg(N, S64, P64, g64, warm, CAP, 1e-5)
             floor = helpers.pcg_fp32_floor(orc, N, S64, P64, g64, warm, CAP, 1e-5, lo)
+            _, _, _, e64 = orc.pcg(N, S64, P64, g64, warm, CAP, 0.0, trace=True)
             err = helpers.rel_err(states[k + 1][1][B.ls(i)], lo)
             it_g = int(pits[1][i, k])
             worst = max(worst, err)
             print(f"inst {i} N={N} SQP iteration {k}: PCG gpu={it_g} oracle={it_o} rel(lambda)={err:.2e} "
-                  f"fp32 floor={floor:.2e}")
-            assert abs(it_g - it_o) <= max(2, it_o // 50), (i, k)
+                  f"fp32 floor={floor:.2e} fp64 |eta| at gpu exit={e64[min(it_g, CAP)]:.3e}")
+            assert helpers.pcg_exit_ok(it_g, e64, 1e-5, CAP), (i, k, it_g, it_o)
             assert err < max(1e-4, 4 * floor), (i, k)
     print(f"worst rel(lambda) {worst:.2e}")
 

@@ -147,6 +147,33 @@ def test_pcg_converges_and_exit_semantics(orc, inst):
     assert abs(r @ (orc.dense_S(N, P) @ r)) < 1e-5 * (1 + 1e-6)
 
 
+def test_pcg_trace_and_exit_band(orc, inst):
+    """The oracle PCG's |r^T Pinv r| trace: entry k is that of the iterate
+    after k iterations (recomputed from lambda), the tolerance exit is the
+    first entry below tol, and the exit band of helpers.pcg_exit_ok accepts
+    that count and rejects exits far from the crossing."""
+    N, xu, ee, xs = inst
+    cfg = orc.make_cfg()
+    G, Cm, g, c = orc.form_kkt(cfg, N, xu, ee, xs)
+    S, P, gam, _ = orc.form_schur(N, G, Cm, g, c, 1e-3)
+    Sd, Pd = orc.dense_S(N, S), orc.dense_S(N, P)
+    cap = 60
+    _, itc, mxc, e = orc.pcg(N, S, P, gam, np.zeros(14 * N), cap, 0.0, trace=True)
+    assert (itc, mxc, len(e)) == (cap, 1, cap + 1)
+    for k in (0, 1, 5, 17):
+        lam_k, _, _ = orc.pcg(N, S, P, gam, np.zeros(14 * N), k, 0.0)
+        r = gam - Sd @ lam_k
+        assert e[k] == pytest.approx(abs(r @ (Pd @ r)), rel=1e-6, abs=1e-300)
+    tol = float(e[12]) * 1.0001  # a tolerance the trace crosses
+    _, it, mx = orc.pcg(N, S, P, gam, np.zeros(14 * N), cap, tol)
+    assert mx == 0 and it == int(np.flatnonzero(e < tol)[0])
+    assert helpers.pcg_exit_ok(it, e, tol, cap)
+    assert not helpers.pcg_exit_ok(0, e, tol, cap)
+    clear = int(np.flatnonzero(e < 0.75 * tol)[0])
+    assert not helpers.pcg_exit_ok(clear + 1, e, tol, cap)
+    assert not helpers.pcg_exit_ok(cap, e, tol, cap)
+
+
 @pytest.mark.parametrize("N", [2, 5, 12])
 def test_ldl_reconstructs_schur_matrix(orc, N):
     x0, ee, xu = helpers.perturbed_instance(orc, 90 + N, N)
