@@ -899,7 +899,7 @@ def main():
                              "roof is the LDS rate; traffic = HBM bytes per launch (PMC), the one-time load; "
                              "k_pcg_split: block-Jacobi PCG as CG on W^T S W (one 14x14 block per knot on chip, "
                              "two block products per iteration: fewer bytes and flops than the algorithmic count), "
-                             "one launch over bins of 1024-lane workgroups; k_pcg_pool / k_pcg_team: the reference "
+                             "one launch over bins of 512-lane workgroups; k_pcg_pool / k_pcg_team: the reference "
                              "form (S, Pinv on chip) on teams and bins",
                      "hbm": None if traffic is None else
                      {"achieved": round(traffic / (t_launch * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -208,9 +208,9 @@ bool pcg_pool_ok(uint32_t n_teams, uint32_t max_iter);
 // Block-Jacobi PCG in split-preconditioned form (k_pcg_split.hip), N <= 256:
 // launch_split_prep turns S (left and diagonal blocks) and gamma into the
 // split system (rows of Ps: L^_b at 0, W_b at NXX; ghat = W^T gamma), then
-// launch_pcg_split solves bins of instances (one 1024-lane workgroup per
-// bin; items / bin_off as for the resident kernel) and writes lambda = W
-// lambda^.  Same outputs as launch_pcg_res.
+// launch_pcg_split solves bins of instances (one 512-lane workgroup per
+// bin; items / bin_off as for the resident kernel, groups of
+// pcg_split_groups(N)) and writes lambda = W lambda^.  Same outputs as launch_pcg_res.
 hipError_t launch_split_prep(const Batch &b, const float *S, const float *gamma, const uint32_t *status, float *Ps,
                              float *ghat, hipStream_t st);
 hipError_t launch_pcg_split(const Batch &b, const uint4 *items, const uint32_t *bin_off, uint32_t n_bins,
@@ -219,6 +219,7 @@ hipError_t launch_pcg_split(const Batch &b, const uint4 *items, const uint32_t *
 uint32_t pcg_split_max_knots();
 uint32_t pcg_split_bin_groups();
 uint32_t pcg_split_max_items();
+uint32_t pcg_split_groups(uint32_t N);  // 16-lane groups of an instance in a split-form bin
 uint32_t pcg_res_groups(uint32_t N);
 uint32_t pcg_res_bin_groups(uint32_t nt);  // 16-lane groups per workgroup of nt lanes
 uint32_t pcg_res_max_knots(uint32_t nt);

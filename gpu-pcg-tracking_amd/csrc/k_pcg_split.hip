@@ -1,5 +1,5 @@
 // k_pcg_split.hip -- block-Jacobi PCG in split-preconditioned form for
-// gfx950: one 1024-lane workgroup per bin of instances (N <= 256 each).
+// gfx950: one 512-lane workgroup per bin of instances (N <= 256 each).
 //
 // Block-Jacobi PCG on S lambda = gamma (preconditioner Pinv = S(b,b)^-1) is,
 // in exact arithmetic, plain CG on the transformed system
@@ -21,14 +21,11 @@
 // row b, W_b at offset NXX (both 14 x 14, column-major), gamma^ in its own
 // buffer.
 //
-// Workgroup: 64 groups of 16 lanes (16 waves), group g serving knots
-// 4g .. 4g+3 of its instance, lane i element i; up to 256 knots, bins of
-// up to 8 instances, each on whole waves (as in pcg_res_body.h).  Per
-// group: L^ rows of its last three knots in registers (pre-rotated for the
-// DPP products of pcg_onchip.h); in two LDS slots of its own the rows of
-// L^ of its first knot and the rows of L^^T of the knot after its last
-// (the S^(b,b+1) term of its last knot, applied as a row product like
-// L^ x instead of the longer transposed form).
+// Workgroup: 32 groups of 16 lanes (8 waves), group g serving knots
+// 8g .. 8g+7 of its instance, lane i element i; up to 256 knots, bins of
+// up to 8 instances, each on whole waves (as in pcg_res_body.h).  The
+// blocks are held pre-rotated for the DPP products of pcg_onchip.h, in
+// registers and LDS slots of the group's own (below).
 #include "common.h"
 #include "gj16.h"
 #include "pcg_onchip.h"
@@ -120,9 +117,21 @@ __global__ void __launch_bounds__(PWG) k_split_prep(Batch b, const float *__rest
 }
 
 // --------------------------------------------------------------- solve ----
-constexpr int NT = 1024;
-constexpr int NGRP = NT / 16;  // 64 groups
-constexpr int RK = 4;
+// One 512-lane workgroup (8 waves, two per SIMD, up to 256 VGPRs each) per
+// bin: 32 groups of 16 lanes, group g serving knots 8g .. 8g+7 of its
+// instance.  A block L^_b enters two products per iteration, L^_b x_{b-1}
+// and L^_b^T x_b.  Held as rotated rows (Mr[n] = L(i, src(i, n))) the first
+// is 16 multiply-adds with the vector rotated by DPP; held as rotated rows of
+// its transpose (Kr[n] = L(src(i, n), i)) so is the second.  From rows
+// alone the transposed product needs the products rotated instead (Horner
+// form: 16 products and 15 rotated additions).  Per group, of the nine
+// blocks L^_{8g} .. L^_{8g+8} (the last is the next group's first):
+//   registers: rows of L^_1 .. L^_7, transposed rows of L^_3 .. L^_5;
+//   LDS slots: rows of L^_0, transposed rows of L^_1, L^_2 and L^_8;
+// so only L^_6 and L^_7 take the Horner form.
+constexpr int NT = 512;
+constexpr int NGRP = NT / 16;     // 32 groups
+constexpr int RK = 8;             // knots per group
 constexpr int KNOTS = NGRP * RK;  // 256
 constexpr int MAXI = 8;
 
@@ -132,11 +141,13 @@ constexpr int MAXI = 8;
 constexpr int SLOT = 256;
 
 struct SplitShared {
-  float R[NGRP * SLOT];       // rows of L^ of each group's first knot
-  float K[NGRP * SLOT];       // rows of L^^T of the knot after each group's last
-  float zb[NGRP * 2 * 16];    // staged vector element of each group's first and last knot
-  float red[2][MAXI][NGRP];   // dot-product partials: one per group, by instance and group
-  uint32_t nlive;             // instances of the bin still iterating
+  float R0[NGRP * SLOT];     // rows of L^_{8g}
+  float K1[NGRP * SLOT];     // rows of L^_{8g+1}^T
+  float K2[NGRP * SLOT];     // rows of L^_{8g+2}^T
+  float K8[NGRP * SLOT];     // rows of L^_{8g+8}^T
+  float zb[NGRP * 2 * 16];   // staged vector element of each group's first and last knot
+  float red[2][MAXI][NGRP];  // dot-product partials: one per group, by instance and group
+  uint32_t nlive;            // instances of the bin still iterating
 };
 constexpr uint32_t SPLIT_CLEAR4 = (uint32_t)(offsetof(SplitShared, nlive) / 16);
 static_assert(offsetof(SplitShared, nlive) % 16 == 0, "slots, zb and red are cleared as one float4 range");
@@ -169,26 +180,24 @@ __device__ __forceinline__ float row_total(float v) {
 
 __device__ __forceinline__ float ror1(float v) { return dpp_f<0x121>(v); }
 
-// (M_k^T x_k)_i for three blocks with rotated rows M_k: sum_n ror_n(m_n),
+// (M_k^T x_k)_i for two blocks with rotated rows M_k: sum_n ror_n(m_n),
 // m_n = M[(16 - n) & 15] x, in Horner form m_0 + ror_1(m_1 + ror_1(m_2 + ...
 // + ror_1(m_15))): 16 products and 15 row_ror:1 additions per block, the
-// three chains interleaved
-__device__ __forceinline__ void rot_mtv3(const float (&M0)[16], float x0, const float (&M1)[16], float x1,
-                                         const float (&M2)[16], float x2, float (&c)[3]) {
+// two chains interleaved
+__device__ __forceinline__ void rot_mtv2(const float (&M0)[16], float x0, const float (&M1)[16], float x1,
+                                         float &c0, float &c1) {
   // separate products: a fused multiply-add would take the rotated sum as
   // its addend, which DPP cannot supply (it rotates the first source only)
 #pragma clang fp contract(off)
-  float a0 = M0[1] * x0, a1 = M1[1] * x1, a2 = M2[1] * x2;
+  float a0 = M0[1] * x0, a1 = M1[1] * x1;
 #pragma unroll
   for (int n = 14; n >= 0; --n) {
     const int k = (16 - n) & 15;
     a0 = ror1(a0) + M0[k] * x0;
     a1 = ror1(a1) + M1[k] * x1;
-    a2 = ror1(a2) + M2[k] * x2;
   }
-  c[0] = a0;
-  c[1] = a1;
-  c[2] = a2;
+  c0 = a0;
+  c1 = a1;
 }
 
 // Three interleaved accumulations a_k += sum_{n=1..15} M_k[n] * ror_n(x_k)
@@ -220,12 +229,14 @@ __device__ __forceinline__ void rot_fma15x3(float &a, float xa, const float (&Ma
 #undef MPC_RF3
 }
 
-// (Ma xa)_i - sa and (Mb xb)_i - sb (the -x term of S^'s -I diagonal)
-__device__ __forceinline__ void rot_mv2_minus(const float (&Ma)[16], float xa, float sa, float &ya,
-                                              const float (&Mb)[16], float xb, float sb, float &yb) {
-  ya = fmaf(Ma[0], xa, -sa);
-  yb = fmaf(Mb[0], xb, -sb);
-  rot_fma15x2(ya, xa, Ma, yb, xb, Mb);
+// a_k = M_k x_k (+ the addend s_k), three blocks of rotated rows
+__device__ __forceinline__ void rot_mv3(const float (&Ma)[16], float xa, float sa, float &a, const float (&Mb)[16],
+                                        float xb, float sb, float &b, const float (&Mc)[16], float xc, float sc,
+                                        float &c) {
+  a = fmaf(Ma[0], xa, sa);
+  b = fmaf(Mb[0], xb, sb);
+  c = fmaf(Mc[0], xc, sc);
+  rot_fma15x3(a, xa, Ma, b, xb, Mb, c, xc, Mc);
 }
 
 // rotated rows of the column-major 14 x 14 block Lh (T = true: of its
@@ -241,13 +252,11 @@ __device__ __forceinline__ void load_lrows(float (&Lr)[16], bool has, uint32_t i
   }
 }
 
-__global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restrict__ items,
-                                                  const uint32_t *__restrict__ bin_off,
-                                                  const float *__restrict__ Ps, const float *__restrict__ ghat,
-                                                  float *__restrict__ lambda, uint32_t max_iter, float tol,
-                                                  const uint32_t *__restrict__ status,
-                                                  uint32_t *__restrict__ iters_out, uint32_t *__restrict__ flag_out,
-                                                  float *__restrict__ eta_out) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_pcg_split(Batch b, const uint4 *__restrict__ items, const uint32_t *__restrict__ bin_off,
+            const float *__restrict__ Ps, const float *__restrict__ ghat, float *__restrict__ lambda,
+            uint32_t max_iter, float tol, const uint32_t *__restrict__ status, uint32_t *__restrict__ iters_out,
+            uint32_t *__restrict__ flag_out, float *__restrict__ eta_out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   SplitShared &sh = *reinterpret_cast<SplitShared *>(smem_raw);
   const uint32_t tid = threadIdx.x, bin = blockIdx.x;
@@ -280,11 +289,10 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
   const bool lane_ok = i < (uint32_t)NX;
   __syncthreads();  // LDS cleared
 
-  // ---- load: L^ rows (registers; the first knot's also to LDS slot g),
-  // lambda^ = W^-1 lambda by back substitution (W upper triangular), gamma^
+  // ---- load: lambda^ = W^-1 lambda by back substitution (W upper
+  // triangular) and gamma^, then the blocks (fewer values live at once)
   const uint32_t b0 = lg * RK;
   float lam[RK], r[RK], p[RK], y[RK], gh[RK];  // z = -r is not stored
-  // lambda^ and gamma^ first, then the L^ rows (fewer values live at once)
 #pragma unroll
   for (int t = 0; t < RK; ++t) {
     const uint32_t bk = b0 + (uint32_t)t;
@@ -305,28 +313,41 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
     lam[t] = has ? lh : 0.f;
     gh[t] = has ? ghat[(size_t)NX * (K0 + bk) + i] : 0.f;
   }
-  // Lr[t - 1]: rows of knot t = 1..3 in registers; knot 0's rows to LDS slot
-  // R[g], the transposed rows of knot 4 (the next group's first) to K[g]
-  float Lr[RK - 1][16];
-  float *Rg = sh.R + g * SLOT, *Kg = sh.K + g * SLOT;
+  float *R0g = sh.R0 + g * SLOT, *K1g = sh.K1 + g * SLOT, *K2g = sh.K2 + g * SLOT, *K8g = sh.K8 + g * SLOT;
+  auto blk = [&](uint32_t t, bool &has) {
+    const uint32_t bk = b0 + t;
+    has = live && bk < N && bk > 0;  // L^_0 = 0
+    return Ps + (size_t)S_STRIDE * (K0 + (has ? bk : 0u));
+  };
+  {
+    const uint32_t ts[4] = {0u, 1u, 2u, (uint32_t)RK};
+    float *dst[4] = {R0g, K1g, K2g, K8g};
 #pragma unroll
-  for (int t = 0; t <= RK; t += RK) {
-    const uint32_t bk = b0 + (uint32_t)t;
-    const bool has = live && bk < N && bk > 0;
-    const float *Pr = Ps + (size_t)S_STRIDE * (K0 + (has ? bk : 0u));
-    float Ls[16];
-    if (t == 0)
-      load_lrows(Ls, has, i, d, Pr);
-    else
-      load_lrows<true>(Ls, has, i, d, Pr);
-    if (lane_ok) write_slot(t == 0 ? Rg : Kg, i, Ls);
-    asm volatile("" ::: "memory");  // store now: Ls need not stay live across the other loads
+    for (int u = 0; u < 4; ++u) {
+      bool has;
+      const float *Pr = blk(ts[u], has);
+      float Ls[16];
+      if (u == 0)
+        load_lrows(Ls, has, i, d, Pr);
+      else
+        load_lrows<true>(Ls, has, i, d, Pr);
+      if (lane_ok) write_slot(dst[u], i, Ls);
+      asm volatile("" ::: "memory");  // store now: Ls need not stay live across the other loads
+    }
   }
+  float Lr[RK - 1][16];  // Lr[t - 1]: rows of L^_t, t = 1 .. 7
+  float Kr[3][16];       // Kr[t - 3]: rows of L^_t^T, t = 3 .. 5
 #pragma unroll
   for (int t = 1; t < RK; ++t) {
-    const uint32_t bk = b0 + (uint32_t)t;
-    const bool has = live && bk < N;
-    load_lrows(Lr[t - 1], has, i, d, Ps + (size_t)S_STRIDE * (K0 + (has ? bk : 0u)));
+    bool has;
+    const float *Pr = blk((uint32_t)t, has);
+    load_lrows(Lr[t - 1], has, i, d, Pr);
+  }
+#pragma unroll
+  for (int t = 3; t < 6; ++t) {
+    bool has;
+    const float *Pr = blk((uint32_t)t, has);
+    load_lrows<true>(Kr[t - 3], has, i, d, Pr);
   }
   const bool in_m = lg > 0, in_p = b0 + RK < N;
   // zb: [group][first, last knot][lane]
@@ -344,42 +365,49 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
     vp = in_p ? zbg[32] : 0.f;
   };
   // y = S^ x: y_t = -x_t + L^_t x_{t-1} + L^_{t+1}^T x_{t+1}; xm / xp: the
-  // elements of knots 4lg - 1 and 4lg + 4.  Lanes 14, 15 of every block row
+  // elements of knots 8lg - 1 and 8lg + 8.  Lanes 14, 15 of every block row
   // are zero, so y (and r, p, lambda^) stay zero there.
   auto smv = [&](const float (&x)[RK], float xm, float xp) {
-    float a0, a1, a2, a3, e3;
     {
-      float L0[16];
-      read_slot(L0, Rg, i);
-      rot_mv2_minus(L0, xm, x[0], a0, Lr[0], x[0], x[1], a1);
+      float M[16];
+      read_slot(M, R0g, i);
+      rot_mv3(M, xm, -x[0], y[0], Lr[0], x[0], -x[1], y[1], Lr[1], x[1], -x[2], y[2]);
+    }
+    rot_mv3(Lr[2], x[2], -x[3], y[3], Lr[3], x[3], -x[4], y[4], Lr[4], x[4], -x[5], y[5]);
+    {
+      float M[16];
+      read_slot(M, K1g, i);
+      rot_mv3(Lr[5], x[5], -x[6], y[6], Lr[6], x[6], -x[7], y[7], M, x[1], y[0], y[0]);
     }
     {
-      float Kn[16];
-      read_slot(Kn, Kg, i);
-      a2 = fmaf(Lr[1][0], x[1], -x[2]);
-      a3 = fmaf(Lr[2][0], x[2], -x[3]);
-      e3 = Kn[0] * xp;
-      rot_fma15x3(a2, x[1], Lr[1], a3, x[2], Lr[2], e3, xp, Kn);
+      float M[16];
+      read_slot(M, K2g, i);
+      rot_mv3(M, x[2], y[1], y[1], Kr[0], x[3], y[2], y[2], Kr[1], x[4], y[3], y[3]);
     }
-    float c[3];
-    rot_mtv3(Lr[0], x[1], Lr[1], x[2], Lr[2], x[3], c);
-    y[0] = a0 + c[0];
-    y[1] = a1 + c[1];
-    y[2] = a2 + c[2];
-    y[3] = a3 + e3;
+    {
+      float M[16];
+      read_slot(M, K8g, i);
+      y[4] = fmaf(Kr[2][0], x[5], y[4]);
+      y[7] = fmaf(M[0], xp, y[7]);
+      rot_fma15x2(y[4], x[5], Kr[2], y[7], xp, M);
+    }
+    float c5, c6;
+    rot_mtv2(Lr[5], x[6], Lr[6], x[7], c5, c6);
+    y[5] += c5;
+    y[6] += c6;
   };
   // dot products: a row total per group to LDS (red[k][instance][group]),
-  // then, after the barrier, the instance's 64 entries (zero past its
-  // groups) summed by every row: four per lane, then the row total
+  // then, after the barrier, the instance's 32 entries (zero past its
+  // groups) summed by every row: two per lane, then the row total
   float *redg = &sh.red[0][mine ? slot : 0][lg & (NGRP - 1)];
   auto post = [&](int k, float part) {
     const float v = row_total(part);
     if (i == 0) redg[k * MAXI * NGRP] = v;
   };
-  const float4 *red4 = reinterpret_cast<const float4 *>(&sh.red[0][mine ? slot : 0][0]) + i;
+  const float2 *red2 = reinterpret_cast<const float2 *>(&sh.red[0][mine ? slot : 0][0]) + i;
   auto total = [&](int k) {
-    const float4 v = red4[k * MAXI * NGRP / 4];
-    return row_total((v.x + v.y) + (v.z + v.w));
+    const float2 v = red2[k * MAXI * NGRP / 2];
+    return row_total(v.x + v.y);
   };
   stage(lam);
   __syncthreads();
@@ -489,6 +517,10 @@ __global__ void __launch_bounds__(NT) k_pcg_split(Batch b, const uint4 *__restri
 uint32_t pcg_split_max_knots() { return (uint32_t)KNOTS; }
 uint32_t pcg_split_bin_groups() { return (uint32_t)NGRP; }
 uint32_t pcg_split_max_items() { return (uint32_t)MAXI; }
+uint32_t pcg_split_groups(uint32_t N) {
+  const uint32_t groups = (N + RK - 1) / RK;
+  return (groups + 3u) & ~3u;  // whole waves
+}
 
 hipError_t launch_split_prep(const Batch &b, const float *S, const float *gamma, const uint32_t *status, float *Ps,
                              float *ghat, hipStream_t st) {

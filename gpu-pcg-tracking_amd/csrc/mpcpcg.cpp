@@ -53,7 +53,7 @@ struct mpc_handle {
   uint4 *bin_items = nullptr;
   uint32_t *bin_off = nullptr, *big_list = nullptr;
   uint32_t n_bins = 0, n_bins_small = 0, n_big = 0, max_big_n = 0;
-  // split-form PCG: its bins (1024-lane workgroups) and gamma^ = W^T gamma
+  // split-form PCG: its bins (512-lane workgroups) and gamma^ = W^T gamma
   // (the split system itself lives in P)
   uint4 *split_items = nullptr;
   uint32_t *split_off = nullptr;
@@ -694,11 +694,11 @@ static int build_pcg_plan(mpc_handle *h) {
     uint32_t used = 0;
     std::vector<uint32_t> inst;
   };
-  auto pack = [&](std::vector<uint32_t> &ord, uint32_t cap_groups) {
+  auto pack = [&](std::vector<uint32_t> &ord, uint32_t cap_groups, uint32_t (*groups)(uint32_t)) {
     std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t c) { return horizons[a] > horizons[c]; });
     std::vector<BinB> bins;
     for (uint32_t i : ord) {
-      const uint32_t need = pcg_res_groups(horizons[i]);
+      const uint32_t need = groups(horizons[i]);
       BinB *dst = nullptr;
       for (auto &bb : bins)
         if (bb.used + need <= cap_groups && bb.inst.size() < max_items) {
@@ -714,20 +714,20 @@ static int build_pcg_plan(mpc_handle *h) {
     }
     return bins;
   };
-  // split form: every instance in bins of the 1024-lane kernel
+  // split form: every instance in bins of the 512-lane kernel
   h->split_ok = n_inst > 0;
   for (uint32_t i = 0; i < n_inst; ++i) h->split_ok = h->split_ok && horizons[i] <= pcg_split_max_knots();
   h->n_split_bins = 0;
   if (h->split_ok) {
     std::vector<uint32_t> all(n_inst);
     for (uint32_t i = 0; i < n_inst; ++i) all[i] = i;
-    const std::vector<BinB> sb = pack(all, pcg_split_bin_groups());
+    const std::vector<BinB> sb = pack(all, pcg_split_bin_groups(), pcg_split_groups);
     std::vector<uint4> sitems;
     std::vector<uint32_t> soff(1, 0);
     for (auto &bb : sb) {
       uint32_t g0 = 0;
       for (uint32_t i : bb.inst) {
-        const uint32_t ng = pcg_res_groups(horizons[i]);
+        const uint32_t ng = pcg_split_groups(horizons[i]);
         sitems.push_back(make_uint4(i, g0, ng, 0u));
         g0 += ng;
       }
@@ -737,8 +737,8 @@ static int build_pcg_plan(mpc_handle *h) {
     HIP_TRY(hipMemcpy(h->split_off, soff.data(), sizeof(uint32_t) * soff.size(), hipMemcpyHostToDevice));
     h->n_split_bins = (uint32_t)sb.size();
   }
-  std::vector<BinB> bins = pack(order, pcg_res_bin_groups(512));
-  const std::vector<BinB> bins_s = pack(order_s, pcg_res_bin_groups(256));
+  std::vector<BinB> bins = pack(order, pcg_res_bin_groups(512), pcg_res_groups);
+  const std::vector<BinB> bins_s = pack(order_s, pcg_res_bin_groups(256), pcg_res_groups);
   const uint32_t n512 = (uint32_t)bins.size();
   bins.insert(bins.end(), bins_s.begin(), bins_s.end());
   std::vector<uint4> items;

@@ -313,7 +313,7 @@ int mpc_batch_pcg_launch_map(const mpc_handle *h, uint32_t *launch_of_inst);
  * arithmetic per instance (bins and teams: bitwise), only the packing and
  * the schedule change.  split = 1 (block Jacobi, every N <= 256): all
  * instances instead run on the split-form kernel, bins of up to max_items
- * instances on 1024-lane workgroups (256 knots, one per CU); block-Jacobi
+ * instances on 512-lane workgroups (256 knots, one per CU); block-Jacobi
  * PCG there is CG on W^T S W (W = chol(-S(b,b))^-1 blockwise, formed after
  * the Schur complement), the same iteration in exact arithmetic with its
  * own fp32 rounding, and the other fields do not apply.  Default

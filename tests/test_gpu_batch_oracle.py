@@ -28,7 +28,7 @@ bench.py, 3 SQP iterations, PCG tol 1e-5, cap 173, replayed as a HIP graph):
   fp64 oracle PCG on the GPU's own Schur system (formed by the stage entry
   points at the batch's iterate and rho, warm started from the batch's
   previous multipliers) within 1e-4 relative, or 4 x the fp32 floor of that
-  solve (the largest deviation of the oracle's fp32 PCG on S and on three
+  solve (the largest deviation of the oracle's fp32 PCG on S and on seven
   one-ulp perturbations of S) where it exceeds 2.5e-5; the iteration
   count where the fp64 oracle's |r^T z| crosses the tolerance, within the
   band of fp32 rounding (helpers.pcg_exit_ok: near the exit, rounding moves

@@ -10,7 +10,7 @@ toplevel.cuh:143-157; restated in tests/test_split_form.py).  Here:
   PCG on the GPU's own Schur system (the stage entry points at the batch's
   iterate and rho, warm started from the batch's previous multipliers)
   within 1e-4, or 4 x the fp32 floor of that solve (the largest deviation
-  of the oracle's fp32 PCG on S and on three one-ulp perturbations of S);
+  of the oracle's fp32 PCG on S and on seven one-ulp perturbations of S);
   iteration counts where the fp64 oracle's |r^T z| crosses the tolerance,
   within the band of fp32 rounding (helpers.pcg_exit_ok); the exit state
   (flag, |r^T Pinv r|) as the exit rule defines it;
