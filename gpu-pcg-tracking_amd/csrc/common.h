@@ -132,9 +132,11 @@ __host__ __device__ __forceinline__ uint32_t pcg_cap(uint32_t max_iter, uint32_t
 hipError_t launch_fk(uint32_t n, const float *q, uint32_t q_stride, float *ee, hipStream_t st);
 hipError_t launch_dyn_grad(uint32_t n, const float *qqdu, float *qdd, float *df_du, hipStream_t st);
 hipError_t launch_inverse_dynamics(uint32_t n, const float *qqdqdd, float *tau, hipStream_t st);
+// gq (optional, NQ floats per knot): the tracking-cost gradient g_q of each
+// knot, the factor of the cost Hessian Q = g_q g_q^T + diag(wx)
 hipError_t launch_form_kkt(const Batch &b, const Params &p, const float *xu, const float *ee_ref,
                            const float *xs, const uint32_t *status, float *G, float *C, float *g,
-                           float *c, hipStream_t st);
+                           float *c, hipStream_t st, float *gq = nullptr);
 // mode: 0..7 line-search alphas (-2^-p, with the initial-state term); 8 the
 // current iterate (alpha 0, without it).  Writes out[(p - p0) * total + gk].
 hipError_t launch_merit(const Batch &b, const Params &p, const float *xu, const float *dz,
@@ -150,13 +152,19 @@ hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, con
                              const float *c, const float *rho_dev, float rho, const uint32_t *status,
                              float *S, float *Pinv, float *gamma, float *Ginv, int out_mode,
                              hipStream_t st);
-// The split system (for launch_pcg_split) from the KKT blocks: G~^-1 into
-// Ginv (for launch_compute_dz), then L^_b and W_b into the rows of Ps and
-// gamma^ into ghat with the Schur block rows (S, Pinv and gamma are not
-// formed); each active instance's rho into rho_used (optional).
-hipError_t launch_form_split(const Batch &b, const float *G, const float *C, const float *g, const float *c,
-                             const float *rho_dev, float rho, const uint32_t *status, float *Ginv, float *Ps,
+// The split system (for launch_pcg_split) from the KKT blocks of
+// launch_form_kkt, with Q~ = diag(wx + rho) + v v^T (v = (g_q, 0), gq of
+// launch_form_kkt) and R~ = diag(wu + rho) inverted in closed form: L^_b
+// and W_b into the rows of Ps and gamma^ into ghat with the Schur block
+// rows (S, Pinv, gamma and G~^-1 are not formed); each active instance's
+// rho into rho_used (optional).
+hipError_t launch_form_split(const Batch &b, const Params &p, const float *C, const float *g, const float *c,
+                             const float *gq, const float *rho_dev, float rho, const uint32_t *status, float *Ps,
                              float *ghat, float *rho_used, hipStream_t st);
+// dz as launch_compute_dz, with G~^-1 in the closed form of launch_form_split
+hipError_t launch_compute_dz_lr(const Batch &b, const Params &p, const float *gq, const float *rho_dev,
+                                const float *C, const float *g, const float *lambda, const uint32_t *status,
+                                float *dz, hipStream_t st);
 hipError_t launch_compute_dz(const Batch &b, const float *Ginv, const float *C, const float *g,
                              const float *lambda, const uint32_t *status, float *dz, hipStream_t st);
 // PCG of every instance.  Outputs per instance: iters (S products), flag (0

@@ -221,7 +221,8 @@ __global__ void __launch_bounds__(KKT_WG) k_form_kkt(Batch b, Params p, const fl
                                                      const float *__restrict__ xs,
                                                      const uint32_t *__restrict__ status,
                                                      float *__restrict__ G, float *__restrict__ C,
-                                                     float *__restrict__ g, float *__restrict__ cvec) {
+                                                     float *__restrict__ g, float *__restrict__ cvec,
+                                                     float *__restrict__ gq_out) {
   __shared__ KktLds lds[KKT_GROUPS];
   const int tid = threadIdx.x, grp = tid / GROUP, lane = tid % GROUP, w = tid >> 6, l64 = tid & 63;
   const uint32_t gk_raw = blockIdx.x * KKT_GROUPS + grp;
@@ -370,6 +371,7 @@ __global__ void __launch_bounds__(KKT_WG) k_form_kkt(Batch b, Params p, const fl
   }
   float *gk_ = g + (size_t)NXU * knot;
   if (lane < NX) gk_[lane] = (lane < NQ ? L.gq[lane] : 0.f) + p.wx[lane] * L.xc[lane];
+  if (gq_out && lane < NQ) gq_out[(size_t)NQ * knot + lane] = L.gq[lane];  // Q = gq gq^T + diag(wx)
   if (kr.k == 0 && lane < NX) cvec[(size_t)NX * kr.K0 + lane] = L.x[lane] - xs[(size_t)NX * kr.inst + lane];
   if (!dyn) return;
   for (int i = lane; i < 49; i += GROUP) Gk[NXX + i] = (i % NU == i / NU) ? p.wu[i % NU] : 0.f;
@@ -508,10 +510,10 @@ hipError_t launch_dyn_grad(uint32_t n, const float *qqdu, float *qdd, float *df_
 
 hipError_t launch_form_kkt(const Batch &b, const Params &p, const float *xu, const float *ee_ref,
                            const float *xs, const uint32_t *status, float *G, float *C, float *g,
-                           float *c, hipStream_t st) {
+                           float *c, hipStream_t st, float *gq) {
   if (b.total_knots == 0) return hipSuccess;
   hipLaunchKernelGGL(k_form_kkt, dim3((b.total_knots + KKT_GROUPS - 1) / KKT_GROUPS), dim3(KKT_WG), 0, st, b, p,
-                     xu, ee_ref, xs, status, G, C, g, c);
+                     xu, ee_ref, xs, status, G, C, g, c, gq);
   return hipGetLastError();
 }
 

@@ -260,10 +260,20 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
 }
 
 // ---- the split system for the split-form PCG (k_pcg_split.hip), formed
-// with the block rows: per block row b, theta_b as in k_form_schur (from
-// G~^-1 of k_invert_g), W_b = chol(theta_b)^-1 (chol_inv_rows), gamma^_b =
-// W_b^T gamma_b and L^_b = W_b^T S(b,b-1) W_{b-1}, S(b,b-1) = A Q~_{b-1}^-1.
-// S, gamma and Pinv are not written.  Workgroups of SG groups overlap by
+// with the block rows: per block row b, theta_b as in k_form_schur, W_b =
+// chol(theta_b)^-1 (chol_inv_rows), gamma^_b = W_b^T gamma_b and L^_b =
+// W_b^T S(b,b-1) W_{b-1}, S(b,b-1) = A Q~_{b-1}^-1.  S, gamma, Pinv and
+// G~^-1 are not written.  The inverses are taken in closed form from the
+// structure of the cost Hessian (k_form_kkt; iiwa_eepos_plant.cuh:357-375):
+//   Q~ = D + v v^T, D = diag(wx + rho), v = (g_q, 0):
+//   Q~^-1 = D^-1 - u u^T / s, u = D^-1 v, s = 1 + v^T D^-1 v (Sherman-Morrison),
+//   R~^-1 = diag(1 / (wu + rho)),
+// so A Q~^-1 = A D^-1 - (A u) u^T / s.  Along v the two terms are of order
+// 1 / rho and cancel to 1 / (rho + |v|^2), so D^-1 is rounded to fp32 once
+// and u, s, A u and the cancelling sums are taken in fp64 (products of fp32
+// values are exact there): the result is, to fp32 rounding of its own
+// entries, the inverse of Q~ with its diagonal moved by at most one ulp.
+// Workgroups of SG groups overlap by
 // one row: group g of workgroup w forms row (SG - 1) w + g - 1, and groups
 // g >= 1 take W_{b-1} from group g - 1 through LDS and write their row
 // (each row from exactly one group).
@@ -280,11 +290,34 @@ struct alignas(16) SplitRowLds {
 };  // 2416 B: four 16-group workgroups fit one CU's LDS
 static_assert(offsetof(SplitRowLds, BR) == offsetof(SplitRowLds, B) + 98 * sizeof(float), "B, BR adjacent");
 
-__global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k_form_split(Batch b, const float *__restrict__ C,
+// u = D^-1 v and 1 / s of Q~ = D + v v^T for the g_q of one knot, in fp64
+// (every lane the same values, in the same order)
+__device__ __forceinline__ void q_lowrank(const float *__restrict__ vq, const float (&dinv)[NQ], bool zero,
+                                          double (&u)[NQ], double &inv_s) {
+  double s = 1.0;
+#pragma unroll
+  for (int l = 0; l < NQ; ++l) {
+    const double v = zero ? 0.0 : (double)vq[l];
+    u[l] = v * (double)dinv[l];
+    s = fma(u[l], v, s);
+  }
+  inv_s = 1.0 / s;
+}
+
+__device__ __forceinline__ double bcast_dyn16_f64(double v, int p) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const float lo = bcast_dyn16(__builtin_bit_cast(float, (uint32_t)b), p);
+  const float hi = bcast_dyn16(__builtin_bit_cast(float, (uint32_t)(b >> 32)), p);
+  return __builtin_bit_cast(double, ((unsigned long long)__builtin_bit_cast(uint32_t, hi) << 32) |
+                                        __builtin_bit_cast(uint32_t, lo));
+}
+
+__global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k_form_split(Batch b, Params p,
+                                                    const float *__restrict__ C,
                                                     const float *__restrict__ g, const float *__restrict__ cvec,
+                                                    const float *__restrict__ gq,
                                                     const float *__restrict__ rho_dev, float rho_s,
-                                                    const uint32_t *__restrict__ status,
-                                                    const float *__restrict__ Ginv, float *__restrict__ Ps,
+                                                    const uint32_t *__restrict__ status, float *__restrict__ Ps,
                                                     float *__restrict__ ghat, float *__restrict__ rho_used) {
   __shared__ SplitRowLds lds[SG];
   const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
@@ -302,22 +335,30 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   const int j = lane < NX ? lane : NX - 1;
   // q_{b-1}, r_{b-1}, q_b: read by every lane of the group (cached)
   const float *qk = g + (size_t)NXU * kprev, *rk = qk + NX, *q1v = g + (size_t)NXU * row;
-  const float *Qi = Ginv + (size_t)G_STRIDE * kprev;  // Q~_{b-1}^-1
-  const float *Q1i = Ginv + (size_t)G_STRIDE * row;   // Q~_b^-1
-  __syncthreads();
-  // AQ = A Q~_{b-1}^-1 (column j), BR = B R~_{b-1}^-1 (column j < 7); zero
-  // inverses for row 0
-  {
-    float qi[NX];
+  const float rho = rho_dev ? rho_dev[kr.inst] : rho_s;
+  // D^-1 in fp32 (the same for both knots: one instance, one rho); u and 1/s
+  // of Q~_{b-1} (zero for row 0) and of Q~_b in fp64
+  float dq[NQ];
 #pragma unroll
-    for (int l = 0; l < NX; ++l) qi[l] = row0 ? 0.f : Qi[l + NX * j];
+  for (int l = 0; l < NQ; ++l) dq[l] = 1.f / (p.wx[l] + rho);
+  const float dj = 1.f / (p.wx[j] + rho);  // D^-1(j, j)
+  const float dpj = row0 ? 0.f : dj;       // of Q~_{b-1}^-1 (zero for row 0)
+  __syncthreads();
+  // AQ = A Q~_{b-1}^-1 (column j) = A(:, j) D^-1(j, j) - (A u)(u_j / s), the
+  // vector A u formed one element per lane and broadcast; BR = B R~^-1
+  // (column j < 7)
+  {
+    double up[NQ], isp;
+    q_lowrank(gq + (size_t)NQ * kprev, dq, row0, up, isp);
+    double au = 0.0;
+#pragma unroll
+    for (int l = 0; l < NQ; ++l) au = fma((double)L.A[j + NX * l], up[l], au);
+    const double upj = j < NQ && !row0 ? (double)gq[(size_t)NQ * kprev + j] * (double)dj : 0.0;  // u_{b-1}(j)
+    const double cj = upj * isp;
     float acc[NX];
 #pragma unroll
-    for (int r = 0; r < NX; ++r) acc[r] = 0.f;
-#pragma unroll
-    for (int l = 0; l < NX; ++l)
-#pragma unroll
-      for (int r = 0; r < NX; ++r) acc[r] += L.A[r + NX * l] * qi[l];
+    for (int r = 0; r < NX; ++r)
+      acc[r] = (float)fma(-bcast_dyn16_f64(au, r), cj, (double)L.A[r + NX * j] * (double)dpj);
     pin(acc);
     if (lane < NX) {
 #pragma unroll
@@ -326,27 +367,24 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   }
   {
     const int ju = lane < NU ? lane : NU - 1;
-    float ri[NU];
-#pragma unroll
-    for (int l = 0; l < NU; ++l) ri[l] = row0 ? 0.f : Qi[NXX + l + NU * ju];
-    float bacc[NX];
-#pragma unroll
-    for (int r = 0; r < NX; ++r) bacc[r] = 0.f;
-#pragma unroll
-    for (int l = 0; l < NU; ++l)
-#pragma unroll
-      for (int r = 0; r < NX; ++r) bacc[r] += L.B[r + NX * l] * ri[l];
-    pin(bacc);
+    const float wr = row0 ? 0.f : 1.f / (p.wu[ju] + rho);
     if (lane < NU) {
 #pragma unroll
-      for (int r = 0; r < NX; ++r) L.BR[r + NX * lane] = bacc[r];
+      for (int r = 0; r < NX; ++r) L.BR[r + NX * lane] = L.B[r + NX * lane] * wr;
     }
   }
   __syncthreads();
   // theta(:, j) = sum_l AQ(:, l) A(j, l) + sum_l BR(:, l) B(j, l) + Q~_b^-1(:, j)
   float th[NX];
+  double isb;
+  const double ubj = j < NQ ? (double)gq[(size_t)NQ * row + j] * (double)dj : 0.0;  // u_b(j)
+  {
+    double ub[NQ];
+    q_lowrank(gq + (size_t)NQ * row, dq, false, ub, isb);
+    const double cb = ubj * isb;
 #pragma unroll
-  for (int r = 0; r < NX; ++r) th[r] = Q1i[r + NX * j];
+    for (int r = 0; r < NX; ++r) th[r] = (float)((r == j ? (double)dj : 0.0) - (r < NQ ? ub[r] * cb : 0.0));
+  }
 #pragma unroll
   for (int l = 0; l < NX; ++l) {
     const float ajl = L.A[j + NX * l];
@@ -362,12 +400,13 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   pin(th);
   // gamma = c - (Q~_b^-1 q_b - A Q~^-1 q_{b-1} - B R~^-1 r_{b-1}) (row 0: -Q~_0^-1 q_0)
   if (lane < NX) {
-    float a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    float a2 = 0.f, a3 = 0.f;
+    double uq = 0.0;
 #pragma unroll
-    for (int l = 0; l < NX; ++l) {
-      a1 += Q1i[lane + NX * l] * q1v[l];  // row lane, as k_form_schur
-      a2 += L.AQ[lane + NX * l] * qk[l];
-    }
+    for (int l = 0; l < NQ; ++l) uq = fma((double)gq[(size_t)NQ * row + l] * (double)dq[l], (double)q1v[l], uq);
+    const float a1 = (float)fma(-ubj, uq * isb, (double)q1v[lane] * (double)dj);
+#pragma unroll
+    for (int l = 0; l < NX; ++l) a2 += L.AQ[lane + NX * l] * qk[l];
 #pragma unroll
     for (int l = 0; l < NU; ++l) a3 += L.BR[lane + NX * l] * rk[l];
     const float cb = (row0 || !cvec) ? 0.f : cvec[(size_t)NX * row + lane];  // gamma_0 has no c term
@@ -391,7 +430,7 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
       for (int k = 0; k < NX; ++k) acc += L.A[k + NX * lane] * L.gm[k];
       ghat[(size_t)NX * row + lane] = acc;
     }
-    if (row0 && lane == 0 && rho_used) rho_used[kr.inst] = rho_dev ? rho_dev[kr.inst] : rho_s;
+    if (row0 && lane == 0 && rho_used) rho_used[kr.inst] = rho;
     if (!row0) {
       const float *Wp = lds[grp - 1].A;  // W_{b-1}
       float m[NX];
@@ -479,6 +518,64 @@ __global__ void __launch_bounds__(WG) k_compute_dz(Batch b, const float *__restr
   }
 }
 
+// dz of k_compute_dz with Q~_k^-1 and R~_k^-1 in the closed form of
+// k_form_split (no G~^-1)
+__global__ void __launch_bounds__(WG) k_compute_dz_lr(Batch b, Params p, const float *__restrict__ gq,
+                                                      const float *__restrict__ rho_dev,
+                                                      const float *__restrict__ C,
+                                                      const float *__restrict__ g,
+                                                      const float *__restrict__ lambda,
+                                                      const uint32_t *__restrict__ status,
+                                                      float *__restrict__ dz) {
+  __shared__ DzLds lds[GROUPS_PER_WG];
+  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
+  const uint32_t gk_raw = blockIdx.x * GROUPS_PER_WG + grp;
+  bool valid = gk_raw < b.total_knots;
+  const uint32_t gk = valid ? gk_raw : b.total_knots - 1;
+  const KnotRef kr = locate(b, gk);
+  if (status && !(status[kr.inst] & ST_ACTIVE)) valid = false;
+  DzLds &L = lds[grp];
+  const size_t knot = (size_t)kr.K0 + kr.k;
+  const bool has_next = kr.k + 1 < kr.N;
+  const float *Ck = C + (size_t)C_STRIDE * knot;
+  if (lane < NX) L.lam1[lane] = has_next ? lambda[(size_t)NX * (knot + 1) + lane] : 0.f;
+  __syncthreads();
+  if (lane < NX) {
+    // t_j = q_j - lambda_k,j + (A^T lambda_{k+1})_j, A = -C_A
+    float s = 0.f;
+    if (has_next) {
+#pragma unroll
+      for (int l = 0; l < NX; ++l) s -= Ck[l + NX * lane] * L.lam1[l];
+    }
+    L.t[lane] = g[(size_t)NXU * knot + lane] - lambda[(size_t)NX * knot + lane] + s;
+  }
+  if (has_next && lane < NU) {
+    float s = 0.f;
+#pragma unroll
+    for (int l = 0; l < NX; ++l) s -= Ck[NXX + l + NX * lane] * L.lam1[l];
+    L.tu[lane] = g[(size_t)NXU * knot + NX + lane] + s;
+  }
+  __syncthreads();
+  if (!valid) return;
+  const float rho = rho_dev[kr.inst];
+  float *dzi = dz + (size_t)NXU * kr.K0 - (size_t)NU * kr.inst + (size_t)NXU * kr.k;
+  if (lane < NX) {
+    // Q~^-1 t = D^-1 t - u (u^T t) / s, the cancelling terms in fp64 (as
+    // k_form_split)
+    float dq[NQ];
+    double u[NQ], inv_s, ut = 0.0;
+#pragma unroll
+    for (int l = 0; l < NQ; ++l) dq[l] = 1.f / (p.wx[l] + rho);
+    q_lowrank(gq + (size_t)NQ * knot, dq, false, u, inv_s);
+#pragma unroll
+    for (int l = 0; l < NQ; ++l) ut = fma(u[l], (double)L.t[l], ut);
+    const float dl = 1.f / (p.wx[lane] + rho);
+    const double ul = lane < NQ ? (double)gq[(size_t)NQ * knot + lane] * (double)dl : 0.0;
+    dzi[lane] = (float)fma(-ul, ut * inv_s, (double)L.t[lane] * (double)dl);
+  }
+  if (has_next && lane < NU) dzi[NX + lane] = L.tu[lane] / (p.wu[lane] + rho);
+}
+
 }  // namespace
 
 hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, const float *g,
@@ -495,16 +592,21 @@ hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, con
   return hipGetLastError();
 }
 
-hipError_t launch_form_split(const Batch &b, const float *G, const float *C, const float *g, const float *c,
-                             const float *rho_dev, float rho, const uint32_t *status, float *Ginv, float *Ps,
+hipError_t launch_form_split(const Batch &b, const Params &p, const float *C, const float *g, const float *c,
+                             const float *gq, const float *rho_dev, float rho, const uint32_t *status, float *Ps,
                              float *ghat, float *rho_used, hipStream_t st) {
   if (b.total_knots == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_invert_g, dim3((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG), 0, st, b, G,
-                     rho_dev, rho, status, Ginv);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_form_split, dim3((b.total_knots + SSTEP - 1) / SSTEP), dim3(SWG), 0, st, b, C, g, c, rho_dev,
-                     rho, status, (const float *)Ginv, Ps, ghat, rho_used);
+  hipLaunchKernelGGL(k_form_split, dim3((b.total_knots + SSTEP - 1) / SSTEP), dim3(SWG), 0, st, b, p, C, g, c, gq,
+                     rho_dev, rho, status, Ps, ghat, rho_used);
+  return hipGetLastError();
+}
+
+hipError_t launch_compute_dz_lr(const Batch &b, const Params &p, const float *gq, const float *rho_dev,
+                                const float *C, const float *g, const float *lambda, const uint32_t *status,
+                                float *dz, hipStream_t st) {
+  if (b.total_knots == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_compute_dz_lr, dim3((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG), 0, st, b,
+                     p, gq, rho_dev, C, g, lambda, status, dz);
   return hipGetLastError();
 }
 

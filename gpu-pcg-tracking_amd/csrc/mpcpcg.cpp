@@ -38,6 +38,7 @@ struct mpc_handle {
   int device = 0;
   // workspace, sized for cfg.max_total_knots (T) and cfg.max_instances (B)
   float *G = nullptr, *Ginv = nullptr, *C = nullptr, *g = nullptr, *c = nullptr;
+  float *gq = nullptr;  // [NQ T] tracking-cost gradient per knot (Q = gq gq^T + diag(wx))
   float *S = nullptr, *P = nullptr, *gamma = nullptr, *dz = nullptr;
   float *knot_merits = nullptr;  // [9 T]
   float *merits = nullptr;       // [9 B]
@@ -275,7 +276,7 @@ const char *mpc_status_string(int s) {
 }
 
 static void free_all(mpc_handle *h) {
-  void *ptrs[] = {h->G, h->Ginv, h->C, h->g, h->c, h->S, h->P, h->gamma, h->dz, h->knot_merits,
+  void *ptrs[] = {h->G, h->Ginv, h->gq, h->C, h->g, h->c, h->S, h->P, h->gamma, h->dz, h->knot_merits,
                   h->merits, h->xs, h->drho, h->merit0, h->rho1, h->status_ws, h->sqp_done,
                   h->pcg_iters, h->pcg_flag, h->pcg_eta, h->t0, h->knot_inst, h->inst_k0, h->bin_items,
                   h->bin_off, h->big_list, h->team_list, h->team_one, h->team_err, h->mbox, h->dir_order,
@@ -319,6 +320,7 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
   auto alloc = [&](void **p, size_t bytes) { return hipMalloc(p, bytes) == hipSuccess; };
   bool ok = alloc((void **)&h->G, sizeof(float) * G_STRIDE * T) &&
             alloc((void **)&h->Ginv, sizeof(float) * G_STRIDE * T) &&
+            alloc((void **)&h->gq, sizeof(float) * NQ * T) &&
             alloc((void **)&h->C, sizeof(float) * C_STRIDE * T) &&
             alloc((void **)&h->g, sizeof(float) * NXU * T) &&
             alloc((void **)&h->c, sizeof(float) * NX * T) &&
@@ -1133,14 +1135,15 @@ static hipError_t batch_sqp_run(mpc_handle *h, float *d_xu, float *d_lambda, con
   MPC_RUN(launch_batch_init(b, d_xu, h->knot_merits, s, st));
   for (uint32_t it = 0; it < sqp_iters; ++it) {
     if (tm) MPC_RUN(hipEventRecord(h->ev[0], st));
-    MPC_RUN(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, d_status, h->G, h->C, h->g, h->c, st));
+    MPC_RUN(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, d_status, h->G, h->C, h->g, h->c, st, h->gq));
     if (tm) MPC_RUN(hipEventRecord(h->ev[1], st));
-    // split form: the split system with the Schur block rows (no S)
+    // split form: the split system with the Schur block rows (no S, no
+    // G~^-1: Q~^-1 and R~^-1 in closed form from gq)
     const bool split = split_active(h);
     h->schur_formed = !split;
     if (split)
-      MPC_RUN(launch_form_split(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->Ginv, h->P, h->ghat,
-                                h->rho_used, st));
+      MPC_RUN(launch_form_split(b, prm, h->C, h->g, h->c, h->gq, d_rho, 0.f, d_status, h->P, h->ghat, h->rho_used,
+                                st));
     else
       MPC_RUN(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma, h->Ginv, 0,
                                 st));
@@ -1148,7 +1151,10 @@ static hipError_t batch_sqp_run(mpc_handle *h, float *d_xu, float *d_lambda, con
     MPC_RUN(pcg_batch(h, d_lambda, d_status, sqp_pcg_cap(h->cfg), h->cfg.pcg_exit_tol, tm ? h->ev[3] : nullptr, st,
                       split));
     if (tm) MPC_RUN(hipEventRecord(h->ev[4], st));
-    MPC_RUN(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
+    if (split)
+      MPC_RUN(launch_compute_dz_lr(b, prm, h->gq, d_rho, h->C, h->g, d_lambda, d_status, h->dz, st));
+    else
+      MPC_RUN(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
     if (h->cfg.flags & FLAG_DIRECT_SOLVE)
       MPC_RUN(refine_kkt(h, d_rho, h->g, h->c, h->dz, d_lambda, h->cfg.direct_kkt_refine, 0.f, st));
     if (tm) MPC_RUN(hipEventRecord(h->ev[5], st));

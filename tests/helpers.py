@@ -106,20 +106,82 @@ def ulp_perturbed(a, seed):
     return a * (1.0 + 2.0 ** -24 * rng.choice([-1.0, 1.0], a.shape))
 
 
-def pcg_fp32_floor(orc, N, S, P, gamma, lam0, cap, tol, ref, samples=7, stair=False):
-    """The fp32 rounding floor of a PCG solve: the largest relative distance
-    from the fp64 result ref of the oracle's fp32 PCG, run on S as given and
-    on `samples` one-ulp perturbations of S.  A single fp32 run is one draw
-    of this floor; after a capped solve (173 iterations) two draws can differ
-    by an order of magnitude, so a parity bound built on one draw rejects
-    correct fp32 implementations at random.  The draws are heavy-tailed
-    (measured on a capped N = 64 solve: 16 draws from 1e-4 to 2.5e-3,
-    median 4.6e-4), hence eight of them."""
+def split_system(N, S, gamma, dtype=np.float64):
+    """The split-preconditioned form of block-Jacobi PCG (k_pcg_split.hip):
+    with theta_b = -S(b,b) = U_b^T U_b (Cholesky) and W_b = U_b^-1,
+    L^_b = W_b^T S(b,b-1) W_{b-1} and gamma^_b = W_b^T gamma_b.  Returns
+    L^, W, U, gamma^ (blocks row-major), computed in dtype."""
+    f = np.dtype(dtype).type
+    B = np.asarray(S, np.float64).reshape(N, 3, 14, 14).astype(f)  # column-major blocks: (r, c) at r + 14 c
+    L, D = np.transpose(B[:, 0], (0, 2, 1)), np.transpose(B[:, 1], (0, 2, 1))
+    U = np.transpose(np.linalg.cholesky(-D), (0, 2, 1)).astype(f)  # theta = U^T U, U upper
+    W = np.linalg.inv(U).astype(f)
+    Lh = np.zeros_like(L)
+    for b in range(1, N):
+        Lh[b] = (W[b].T @ L[b] @ W[b - 1]).astype(f)
+    gh = np.einsum("bkc,bk->bc", W, np.asarray(gamma).reshape(N, 14).astype(f)).astype(f)
+    return Lh, W, U, gh
+
+
+def split_cg(N, S, gamma, lam0, cap, tol, dtype=np.float64):
+    """Block-Jacobi PCG on S lambda = gamma as CG on W^T S W lambda^ =
+    W^T gamma (lambda = W lambda^), in dtype: (lambda, iterations).  The
+    diagonal blocks of W^T S W are -I, the preconditioner is -I and
+    r^T Pinv r = r^ . (-r^) (pinned against the oracle's PCG in
+    test_split_form.py)."""
+    f = np.dtype(dtype).type
+    Lh, W, U, gh = split_system(N, S, gamma, dtype)
+
+    def mv(x):
+        y = -x.copy()
+        y[1:] += np.einsum("brc,bc->br", Lh[1:], x[:-1]).astype(f)
+        y[:-1] += np.einsum("bcr,bc->br", Lh[1:], x[1:]).astype(f)
+        return y.astype(f)
+
+    lh = np.einsum("brc,bc->br", U, np.asarray(lam0).reshape(N, 14).astype(f)).astype(f)  # lambda^ = U lambda
+    r = (gh - mv(lh)).astype(f)
+    p = -r
+    eta = f(-np.sum(r * r, dtype=f))
+    it = 0
+    if abs(eta) >= tol:
+        while it < cap:
+            y = mv(p)
+            a = f(eta / f(np.sum(p * y, dtype=f)))
+            lh = (lh + a * p).astype(f)
+            r = (r - a * y).astype(f)
+            en = f(-np.sum(r * r, dtype=f))
+            it += 1
+            if abs(en) < tol:
+                break
+            p = (-r + f(en / eta) * p).astype(f)
+            eta = en
+    return np.einsum("brc,bc->br", W, lh).ravel().astype(np.float64), it
+
+
+def schur_pcg_fp32_floor(orc, N, G, Cm, g, c, rho, lam0, cap, tol, ref, samples=7, split=False):
+    """The fp32 floor of a Schur formation and PCG solve from the KKT blocks
+    (G, Cm, g, c): the largest relative distance from the fp64 result ref of
+    the oracle's fp32 pipeline (form_schur, then PCG), run on G as given and
+    on `samples` one-ulp perturbations of G.  It covers the rounding of the
+    formation as well: the block inverses of the regularised cost Hessian,
+    whose condition number grows as 1 / rho, and the products with them.
+    A single fp32 run is one draw of this floor; after a capped solve (173
+    iterations) the draws are heavy-tailed (measured on a capped N = 64
+    solve: 16 draws from 1e-4 to 2.5e-3, median 4.6e-4), so a bound built
+    on one draw rejects correct fp32 implementations at random.  split=True
+    adds draws of the split form (split_cg in fp32 on the same fp32 S): the
+    same iteration in exact arithmetic, with its own rounding, which on
+    ill-conditioned capped solves (rho = 1e-3) scatters several times wider
+    than the reference form's."""
     worst = 0.0
     for k in range(samples + 1):
-        Sk = S if k == 0 else ulp_perturbed(S, k)
-        lt, _, _ = orc.pcg(N, Sk, P, gamma, lam0, cap, tol, dtype=np.float32, stair=stair)
+        Gk = G if k == 0 else ulp_perturbed(G, k)
+        S, P, gam, _ = orc.form_schur(N, Gk, Cm, g, c, rho, dtype=np.float32)
+        lt, _, _ = orc.pcg(N, S, P, gam, lam0, cap, tol, dtype=np.float32)
         worst = max(worst, rel_err(lt, ref))
+        if split:
+            ls, _ = split_cg(N, S, gam, lam0, cap, tol, dtype=np.float32)
+            worst = max(worst, rel_err(ls, ref))
     return worst
 
 

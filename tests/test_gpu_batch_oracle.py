@@ -25,11 +25,12 @@ bench.py, 3 SQP iterations, PCG tol 1e-5, cap 173, replayed as a HIP graph):
   that by about 1 / rho);
 - every SQP iteration of the sampled instances: the batched PCG's
   multipliers (block Jacobi in split form, k_pcg_split.hip) against the
-  fp64 oracle PCG on the GPU's own Schur system (formed by the stage entry
-  points at the batch's iterate and rho, warm started from the batch's
-  previous multipliers) within 1e-4 relative, or 4 x the fp32 floor of that
-  solve (the largest deviation of the oracle's fp32 PCG on S and on seven
-  one-ulp perturbations of S) where it exceeds 2.5e-5; the iteration
+  fp64 oracle (Schur formation and PCG) on the GPU's own KKT blocks (the
+  stage entry point at the batch's iterate, the batch's rho, warm started
+  from the batch's previous multipliers) within 1e-4 relative, or 4 x the
+  fp32 floor of that solve (the largest deviation of the oracle's fp32
+  formation and PCG on G and on seven one-ulp perturbations of G, the PCG
+  in the reference form and in the split form) where it exceeds 2.5e-5; the iteration
   count where the fp64 oracle's |r^T z| crosses the tolerance, within the
   band of fp32 rounding (helpers.pcg_exit_ok: near the exit, rounding moves
   a crossing by a few iterations).
@@ -218,19 +219,17 @@ def test_batched_every_iteration_lambda_vs_oracle_pcg_on_gpu_schur(batched, orc,
         Cm = torch.zeros(294 * (N - 1), device="cuda")
         g = torch.zeros(21 * N - 7, device="cuda")
         c = torch.zeros(14 * N, device="cuda")
-        S = torch.zeros(588 * N, device="cuda")
-        P = torch.zeros(588 * N, device="cuda")
-        gam = torch.zeros(14 * N, device="cuda")
         xs = torch.from_numpy(batched.xu0[batched.xu_slice(i)][:14].copy()).cuda()
         ee = batched.ee_dev[6 * int(batched.K0[i]):6 * int(batched.K0[i + 1])]
         for k in range(3):
             xu_k, lam_k, rho_k = states[k]
             solver.form_kkt(N, torch.from_numpy(xu_k[batched.xu_slice(i)].copy()).cuda(), ee, xs, G, Cm, g, c)
-            solver.form_schur(N, G, Cm, g, c, S, P, gam, float(rho_k[i]))
-            S64, P64, g64 = (host(t).astype(np.float64) for t in (S, P, gam))
+            kkt = [host(t).astype(np.float64) for t in (G, Cm, g, c)]
+            rho_i = float(rho_k[i])
+            S64, P64, g64, _ = orc.form_schur(N, *kkt, rho_i)
             warm = lam_k[batched.lam_slice(i)].astype(np.float64)
             lo, it_o, fl_o = orc.pcg(N, S64, P64, g64, warm, CAP, 1e-5)
-            floor = helpers.pcg_fp32_floor(orc, N, S64, P64, g64, warm, CAP, 1e-5, lo)
+            floor = helpers.schur_pcg_fp32_floor(orc, N, *kkt, rho_i, warm, CAP, 1e-5, lo, split=True)
             _, _, _, e64 = orc.pcg(N, S64, P64, g64, warm, CAP, 0.0, trace=True)
             lam_gpu = states[k + 1][1][batched.lam_slice(i)]
             err = helpers.rel_err(lam_gpu, lo)

@@ -7,10 +7,12 @@ count are those of the reference's PCG (GPU-PCG pcg<T,14,N>, call site
 toplevel.cuh:143-157; restated in tests/test_split_form.py).  Here:
 - ragged layouts (N = 2 .. 256, up to 8 instances in a bin): after each of
   two SQP iterations, every instance's multipliers against the fp64 oracle
-  PCG on the GPU's own Schur system (the stage entry points at the batch's
-  iterate and rho, warm started from the batch's previous multipliers)
-  within 1e-4, or 4 x the fp32 floor of that solve (the largest deviation
-  of the oracle's fp32 PCG on S and on seven one-ulp perturbations of S);
+  (Schur formation and PCG) on the GPU's own KKT blocks (the stage entry
+  point at the batch's iterate, the batch's rho, warm started from the
+  batch's previous multipliers) within 1e-4, or 4 x the fp32 floor of that
+  solve (the largest deviation of the oracle's fp32 formation and PCG on G
+  and on seven one-ulp perturbations of G, the PCG in the reference form and
+  in the split form);
   iteration counts where the fp64 oracle's |r^T z| crosses the tolerance,
   within the band of fp32 rounding (helpers.pcg_exit_ok); the exit state
   (flag, |r^T Pinv r|) as the exit rule defines it;
@@ -118,19 +120,17 @@ def test_split_layout_vs_oracle_pcg(split_solver, pkg, orc, name):
         Cm = torch.zeros(294 * (N - 1), device="cuda")
         g = torch.zeros(21 * N - 7, device="cuda")
         c = torch.zeros(14 * N, device="cuda")
-        S = torch.zeros(588 * N, device="cuda")
-        P = torch.zeros(588 * N, device="cuda")
-        gam = torch.zeros(14 * N, device="cuda")
         ee = dev(B.ee[6 * B.K0[i]:6 * B.K0[i + 1]])
         xs = dev(B.xu0[B.xs(i)][:14])
         for k in range(2):
             xu_k, lam_k, rho_k = states[k]
             s.form_kkt(N, dev(xu_k[B.xs(i)]), ee, xs, G, Cm, g, c)
-            s.form_schur(N, G, Cm, g, c, S, P, gam, float(rho_k[i]))
-            S64, P64, g64 = (host(t).astype(np.float64) for t in (S, P, gam))
+            kkt = [host(t).astype(np.float64) for t in (G, Cm, g, c)]
+            rho_i = float(rho_k[i])
+            S64, P64, g64, _ = orc.form_schur(N, *kkt, rho_i)
             warm = lam_k[B.ls(i)].astype(np.float64)
             lo, it_o, _ = orc.pcg(N, S64, P64, g64, warm, CAP, 1e-5)
-            floor = helpers.pcg_fp32_floor(orc, N, S64, P64, g64, warm, CAP, 1e-5, lo)
+            floor = helpers.schur_pcg_fp32_floor(orc, N, *kkt, rho_i, warm, CAP, 1e-5, lo, split=True)
             _, _, _, e64 = orc.pcg(N, S64, P64, g64, warm, CAP, 0.0, trace=True)
             err = helpers.rel_err(states[k + 1][1][B.ls(i)], lo)
             it_g = int(pits[1][i, k])

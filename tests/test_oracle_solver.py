@@ -147,6 +147,40 @@ def test_pcg_converges_and_exit_semantics(orc, inst):
     assert abs(r @ (orc.dense_S(N, P) @ r)) < 1e-5 * (1 + 1e-6)
 
 
+@pytest.mark.parametrize("wx_q", [0.0, 0.3])
+def test_cost_hessian_closed_form_inverse(orc, inst, wx_q):
+    """The batched split path inverts the regularised cost blocks in closed
+    form (k_schur.hip k_form_split, k_compute_dz_lr): the oracle's Q block
+    is g_q g_q^T + diag(wx) (iiwa_eepos_plant.cuh:357-375), so
+    (Q + rho I)^-1 = D^-1 - u u^T / s (D = diag(wx + rho), u = D^-1 v,
+    s = 1 + v^T u, v = (g_q, 0)), and R + rho I is diagonal.  Checked
+    against the oracle's own block inverses (form_schur's G~^-1), with the
+    default weights and with a nonzero position weight."""
+    N, xu, ee, xs = inst
+    wx = np.array([wx_q] * 7 + [1e-2] * 7)
+    cfg = orc.make_cfg(wx=wx)
+    G, Cm, g, c = orc.form_kkt(cfg, N, xu, ee, xs)
+    rho = 1e-3
+    _, _, _, Ginv = orc.form_schur(N, G, Cm, g, c, rho)
+    wu = np.array([1e-4] * 7)
+    for k in range(N):
+        Q = G[245 * k:245 * k + 196].reshape(14, 14).T
+        # Q - diag(wx) = v v^T: one nonzero eigenvalue, in the position block
+        ev, V = np.linalg.eigh(Q - np.diag(wx))
+        v = np.sqrt(max(ev[-1], 0.0)) * V[:, -1]
+        assert np.abs(ev[:-1]).max() <= 1e-12 * max(ev[-1], 1e-30) + 1e-15
+        assert np.abs(v[7:]).max() <= 1e-12 * np.abs(v).max() + 1e-15
+        assert np.allclose(Q, np.outer(v, v) + np.diag(wx), rtol=1e-12, atol=1e-15)
+        d = 1.0 / (wx + rho)
+        u = d * v
+        Qi = np.diag(d) - np.outer(u, u) / (1.0 + v @ u)
+        Qi_orc = Ginv[245 * k:245 * k + 196].reshape(14, 14).T
+        assert np.abs(Qi - Qi_orc).max() <= 1e-9 * np.abs(Qi_orc).max()
+        if k + 1 < N:
+            Ri_orc = Ginv[245 * k + 196:245 * k + 245].reshape(7, 7).T
+            assert np.allclose(Ri_orc, np.diag(1.0 / (wu + rho)), rtol=1e-12, atol=0)
+
+
 def test_pcg_trace_and_exit_band(orc, inst):
     """The oracle PCG's |r^T Pinv r| trace: entry k is that of the iterate
     after k iterations (recomputed from lambda), the tolerance exit is the

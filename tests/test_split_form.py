@@ -1,6 +1,7 @@
 """CPU: the split-preconditioned form of block-Jacobi PCG that the batched
-GPU path runs (k_pcg_split.hip), restated in numpy fp64 and checked against
-the oracle's PCG (the reference's pcg<T,14,N>, toplevel.cuh:143-157).
+GPU path runs (k_pcg_split.hip), restated in numpy (helpers.split_cg, in
+fp64 here) and checked against the oracle's PCG (the reference's
+pcg<T,14,N>, toplevel.cuh:143-157).
 
 With theta_b = -S(b,b) = U_b^T U_b (Cholesky) and W_b = U_b^-1, block-Jacobi
 PCG on S lambda = gamma is CG on W^T S W lambda^ = W^T gamma with
@@ -21,47 +22,6 @@ def _blocks(N, S):
     return np.transpose(B[:, 0], (0, 2, 1)), np.transpose(B[:, 1], (0, 2, 1))
 
 
-def _split_system(N, S, gamma):
-    """L^_b = W_b^T S(b,b-1) W_{b-1}, W_b, gamma^_b = W_b^T gamma_b."""
-    L, D = _blocks(N, S)
-    U = np.transpose(np.linalg.cholesky(-D), (0, 2, 1))  # theta = U^T U, U upper
-    W = np.linalg.inv(U)
-    Lh = np.zeros_like(L)
-    for b in range(1, N):
-        Lh[b] = W[b].T @ L[b] @ W[b - 1]
-    gh = np.einsum("bkc,bk->bc", W, gamma.reshape(N, 14))
-    return Lh, W, U, gh
-
-
-def _split_cg(N, S, gamma, lam0, cap, tol):
-    Lh, W, U, gh = _split_system(N, S, gamma)
-
-    def mv(x):
-        y = -x.copy()
-        y[1:] += np.einsum("brc,bc->br", Lh[1:], x[:-1])
-        y[:-1] += np.einsum("bcr,bc->br", Lh[1:], x[1:])
-        return y
-
-    lh = np.einsum("brc,bc->br", U, lam0.reshape(N, 14))  # lambda^ = U lambda
-    r = gh - mv(lh)
-    p = -r
-    eta = -np.sum(r * r)
-    it = 0
-    if abs(eta) >= tol:
-        while it < cap:
-            y = mv(p)
-            a = eta / np.sum(p * y)
-            lh = lh + a * p
-            r = r - a * y
-            en = -np.sum(r * r)
-            it += 1
-            if abs(en) < tol:
-                break
-            p = -r + (en / eta) * p
-            eta = en
-    return np.einsum("brc,bc->br", W, lh).ravel(), it
-
-
 def _system(orc, N, seed):
     cfg = orc.make_cfg()
     x0, ee, xu = helpers.perturbed_instance(orc, seed, N)
@@ -73,7 +33,7 @@ def _system(orc, N, seed):
 @pytest.mark.parametrize("N,seed", [(2, 1), (9, 2), (32, 3), (128, 4)])
 def test_split_cg_is_block_jacobi_pcg(orc, N, seed):
     S, P, gam = _system(orc, N, seed)
-    Lh, W, U, gh = _split_system(N, S, gam)
+    Lh, W, U, gh = helpers.split_system(N, S, gam)
     _, D = _blocks(N, S)
     # the transform: W^T theta W = I, and gamma^ . gamma^ = gamma^T (-Pinv) gamma
     assert np.allclose(np.einsum("bki,bkl,blj->bij", W, -D, W), np.eye(14)[None], atol=1e-9)
@@ -84,7 +44,7 @@ def test_split_cg_is_block_jacobi_pcg(orc, N, seed):
     for lam0 in (np.zeros(14 * N), 1e-2 * rng.standard_normal(14 * N)):
         for cap, tol in ((173, 1e-5), (40, 1e-12), (5000, 1e-9)):
             lo, it_o, _ = orc.pcg(N, S, P, gam, lam0, cap, tol)
-            ls, it_s = _split_cg(N, S, gam, lam0, cap, tol)
+            ls, it_s = helpers.split_cg(N, S, gam, lam0, cap, tol)
             assert it_s == it_o, (cap, tol)
             assert helpers.rel_err(ls, lo) < 1e-7, (cap, tol)  # fp64 rounding over up to 5000 iterations
 
@@ -100,9 +60,9 @@ def test_split_catches_a_transposed_neighbour_block(orc):
     Lbad = np.zeros_like(L)
     for b in range(1, N):
         Lbad[b] = W[b - 1].T @ L[b] @ W[b]
-    Lgood, _, _, _ = _split_system(N, S, gam)
+    Lgood, _, _, _ = helpers.split_system(N, S, gam)
     assert np.abs(Lbad - Lgood).max() > 1e-3 * np.abs(Lgood).max()
-    ls, _ = _split_cg(N, S, gam, np.zeros(14 * N), 5000, 1e-12)
+    ls, _ = helpers.split_cg(N, S, gam, np.zeros(14 * N), 5000, 1e-12)
     assert helpers.rel_err(ls, lo) < 1e-7
     # solving the system built from the slipped blocks gives another lambda
     Lh = Lbad
