@@ -64,24 +64,30 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batched_1024.json")
 # loop body from the kernel's own ISA); see the .md beside it
 COUNTERS_FILE = os.path.join(ROOT, "profiles", "counters_batched_1024.json")
 # algorithmic HBM bytes and flops per knot of the other phases of an SQP
-# iteration (fp32):
+# iteration (fp32), as the batched split path runs them:
 # linearize: reads x_k, u_k, x_{k+1}, ref_k (21 + 14 + 6 floats), writes G
-#   (245), C (294), g (21), c (14); ~2e4 flops (dynamics, its gradient, FK)
-# schur: reads A, B, Q, R, q, r, c (574 floats), writes S (301 symmetric),
-#   Pinv (105), gamma (14); 3.24e4 flops (Gauss-Jordan inverses and GEMMs)
-# dz: reads G^-1, C, g, lambda (609 floats), writes dz (21); 1.1e3 flops
+#   (245), C (294), g (21), c (14), g_q (7); ~2e4 flops (dynamics, its
+#   gradient, FK)
+# schur (split form, k_form_split): reads C of the knot before (294), g_q
+#   of both knots (14), q, r of the knot before and q of its own (35), c
+#   (14); writes L^ and W (392) and gamma^ (14); ~2.5e4 flops (theta, its
+#   whitening factor, the two block products of L^)
+# dz (k_compute_dz_lr): reads C (294), g_q (7), g (21), lambda of the knot
+#   and the next (28), writes dz (21); 1.1e3 flops
 # merit: reads xu, dz, ref (21 + 21 + 6 floats) per knot for all 8 step
 #   lengths; flops not counted (priced by bytes only)
-# pcg: per solve the S row, Jacobi block, gamma, lambda in and out
-#   (588 + 196 + 14 + 28 floats) from HBM; per iteration the
-#   PCG_BYTES_PER_KNOT_ITER / PCG_FLOPS_PER_KNOT_ITER above, on chip (LDS)
+# pcg: per solve L^ and W (392), gamma^ (14), lambda in and out (28) from
+#   HBM; per iteration the PCG_BYTES_PER_KNOT_ITER / PCG_FLOPS_PER_KNOT_ITER
+#   above, on chip (LDS)
+# The flops of linearize, schur, dz and merit are the counted ones (fp32
+# VALU instructions x 64 lanes) when the counter file has them.
 PHASE_MODEL = {
-    "linearize": (4 * (21 + 14 + 6 + 245 + 294 + 21 + 14), 2.0e4),
-    "schur": (4 * (574 + 420), 3.24e4),
-    "dz": (4 * (609 + 21), 1.1e3),
+    "linearize": (4 * (21 + 14 + 6 + 245 + 294 + 21 + 14 + 7), 2.0e4),
+    "schur": (4 * (294 + 14 + 35 + 14 + 392 + 14), 2.5e4),
+    "dz": (4 * (294 + 7 + 21 + 28 + 21), 1.1e3),
     "merit": (4 * (21 + 21 + 6), 0.0),
 }
-PCG_HBM_BYTES_PER_KNOT = 4 * (588 + 196 + 14 + 28)
+PCG_HBM_BYTES_PER_KNOT = 4 * (392 + 14 + 28)
 
 
 def load_package():
@@ -594,19 +600,20 @@ def measure_bandwidth(pkg, torch):
 def phase_roofline(phases, knots, pcg_units, bw, counters=None):
     """Per phase of one SQP iteration: roofline time = max(B_LDS / BW_LDS,
     B_HBM / BW_HBM, F / fp32 peak) over the measured bandwidths, and the
-    fraction roofline time / measured time.  The linearisation and merit
-    flops per knot are counted from their kernels when the counter file
-    has them (fp32 VALU instructions x 64 lanes), else the estimate."""
+    fraction roofline time / measured time.  The flops per knot of the
+    linearisation, the split formation, dz and the merits are counted from
+    their kernels when the counter file has them (fp32 VALU instructions x
+    64 lanes), else the estimate."""
     rows = {}
     model = dict(PHASE_MODEL)
     flop_source = {}
-    for name in ("linearize", "merit"):
+    for name in ("linearize", "schur", "dz", "merit"):
         per = (counters or {}).get("flops_per_knot", {}).get(name)
         if per:
             model[name] = (model[name][0], float(per))
             flop_source[name] = "counted"
         else:
-            flop_source[name] = "estimate" if name == "linearize" else "not counted"
+            flop_source[name] = "not counted" if name == "merit" else "estimate"
 
     def row(name, us, b_hbm, b_lds, flops):
         t = {"hbm": b_hbm / (bw["hbm_gbs"] * 1e9), "lds": b_lds / (bw["lds_gbs"] * 1e9),
@@ -626,11 +633,12 @@ def phase_roofline(phases, knots, pcg_units, bw, counters=None):
         row(name, phases[name], b * knots, 0.0, f * knots)
     return {"bandwidth": bw, "fp32_peak_tflops": FP32_PEAK_TFLOPS, "knots_per_sqp_iter": int(knots),
             "pcg_knot_iterations_per_sqp_iter": int(pcg_units), "phases": rows, "flops_source": flop_source,
-            "note": "per SQP iteration of the headline workload; bytes are the algorithmic ones per knot "
-                    "(PCG: per knot and iteration, priced at the LDS rate, its S kept on chip); flops: Schur, dz "
-                    "and PCG from the operation counts of their algorithms, linearisation and merit counted from "
-                    "their kernels' fp32 VALU instructions x 64 lanes (an upper bound: masked lanes count); fp32 "
-                    "peak is the published packed rate (this build issues unpacked fp32: half of it)"}
+            "note": "per SQP iteration of the headline workload; bytes are the algorithmic ones per knot of the "
+                    "split path as it runs (PCG: per knot and iteration the reference algorithm's S, Pinv and "
+                    "vectors, priced at the LDS rate); flops: PCG from the operation count of its algorithm, the "
+                    "other phases counted from their kernels' fp32 VALU instructions x 64 lanes (an upper bound: "
+                    "masked lanes count); fp32 peak is the published packed rate (this build issues unpacked "
+                    "fp32: half of it)"}
 
 
 def other_modes(pkg, solver, torch, timed, step, K):

@@ -126,9 +126,12 @@ __global__ void __launch_bounds__(PWG) k_split_prep(Batch b, const float *__rest
 // alone the transposed product needs the products rotated instead (Horner
 // form: 16 products and 15 rotated additions).  Per group, of the nine
 // blocks L^_{8g} .. L^_{8g+8} (the last is the next group's first):
-//   registers: rows of L^_1 .. L^_7, transposed rows of L^_3 .. L^_5;
+//   registers: rows of L^_1 .. L^_7, transposed rows of L^_3 and L^_4;
 //   LDS slots: rows of L^_0, transposed rows of L^_1, L^_2 and L^_8;
-// so only L^_6 and L^_7 take the Horner form.
+// so L^_5 .. L^_7 take the Horner form.  Two 16-register buffers take the
+// slot reads, each issued one product group ahead of its use, and the
+// products run as four or five interleaved chains: with two waves per SIMD
+// a wave's own independent instructions hide the latency of the chains.
 constexpr int NT = 512;
 constexpr int NGRP = NT / 16;     // 32 groups
 constexpr int RK = 8;             // knots per group
@@ -180,64 +183,71 @@ __device__ __forceinline__ float row_total(float v) {
 
 __device__ __forceinline__ float ror1(float v) { return dpp_f<0x121>(v); }
 
-// (M_k^T x_k)_i for two blocks with rotated rows M_k: sum_n ror_n(m_n),
+// (M_k^T x_k)_i for three blocks with rotated rows M_k: sum_n ror_n(m_n),
 // m_n = M[(16 - n) & 15] x, in Horner form m_0 + ror_1(m_1 + ror_1(m_2 + ...
 // + ror_1(m_15))): 16 products and 15 row_ror:1 additions per block, the
-// two chains interleaved
-__device__ __forceinline__ void rot_mtv2(const float (&M0)[16], float x0, const float (&M1)[16], float x1,
-                                         float &c0, float &c1) {
+// three chains interleaved
+__device__ __forceinline__ void rot_mtv3(const float (&M0)[16], float x0, const float (&M1)[16], float x1,
+                                         const float (&M2)[16], float x2, float &c0, float &c1, float &c2) {
   // separate products: a fused multiply-add would take the rotated sum as
   // its addend, which DPP cannot supply (it rotates the first source only)
 #pragma clang fp contract(off)
-  float a0 = M0[1] * x0, a1 = M1[1] * x1;
+  float a0 = M0[1] * x0, a1 = M1[1] * x1, a2 = M2[1] * x2;
 #pragma unroll
   for (int n = 14; n >= 0; --n) {
     const int k = (16 - n) & 15;
     a0 = ror1(a0) + M0[k] * x0;
     a1 = ror1(a1) + M1[k] * x1;
+    a2 = ror1(a2) + M2[k] * x2;
   }
   c0 = a0;
   c1 = a1;
+  c2 = a2;
 }
 
-// Three interleaved accumulations a_k += sum_{n=1..15} M_k[n] * ror_n(x_k)
-// (as rot_fma15x2 of pcg_onchip.h, one more chain)
-__device__ __forceinline__ void rot_fma15x3(float &a, float xa, const float (&Ma)[16], float &b, float xb,
-                                            const float (&Mb)[16], float &c, float xc, const float (&Mc)[16]) {
-#define MPC_RF3(n, oa, ob, oc)                                                                    \
-  "v_fmac_f32_dpp %0, %3, %" #oa " row_ror:" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
-  "v_fmac_f32_dpp %1, %4, %" #ob " row_ror:" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
-  "v_fmac_f32_dpp %2, %5, %" #oc " row_ror:" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
-  asm("s_nop 4\n" MPC_RF3(1, 6, 11, 16) MPC_RF3(2, 7, 12, 17) MPC_RF3(3, 8, 13, 18) MPC_RF3(4, 9, 14, 19)
-          MPC_RF3(5, 10, 15, 20)
-      : "+v"(a), "+v"(b), "+v"(c)
-      : "v"(xa), "v"(xb), "v"(xc), "v"(Ma[1]), "v"(Ma[2]), "v"(Ma[3]), "v"(Ma[4]), "v"(Ma[5]), "v"(Mb[1]),
-        "v"(Mb[2]), "v"(Mb[3]), "v"(Mb[4]), "v"(Mb[5]), "v"(Mc[1]), "v"(Mc[2]), "v"(Mc[3]), "v"(Mc[4]),
-        "v"(Mc[5]));
-  asm("s_nop 4\n" MPC_RF3(6, 6, 11, 16) MPC_RF3(7, 7, 12, 17) MPC_RF3(8, 8, 13, 18) MPC_RF3(9, 9, 14, 19)
-          MPC_RF3(10, 10, 15, 20)
-      : "+v"(a), "+v"(b), "+v"(c)
-      : "v"(xa), "v"(xb), "v"(xc), "v"(Ma[6]), "v"(Ma[7]), "v"(Ma[8]), "v"(Ma[9]), "v"(Ma[10]), "v"(Mb[6]),
-        "v"(Mb[7]), "v"(Mb[8]), "v"(Mb[9]), "v"(Mb[10]), "v"(Mc[6]), "v"(Mc[7]), "v"(Mc[8]), "v"(Mc[9]),
-        "v"(Mc[10]));
-  asm("s_nop 4\n" MPC_RF3(11, 6, 11, 16) MPC_RF3(12, 7, 12, 17) MPC_RF3(13, 8, 13, 18) MPC_RF3(14, 9, 14, 19)
-          MPC_RF3(15, 10, 15, 20)
-      : "+v"(a), "+v"(b), "+v"(c)
-      : "v"(xa), "v"(xb), "v"(xc), "v"(Ma[11]), "v"(Ma[12]), "v"(Ma[13]), "v"(Ma[14]), "v"(Ma[15]), "v"(Mb[11]),
-        "v"(Mb[12]), "v"(Mb[13]), "v"(Mb[14]), "v"(Mb[15]), "v"(Mc[11]), "v"(Mc[12]), "v"(Mc[13]), "v"(Mc[14]),
-        "v"(Mc[15]));
-#undef MPC_RF3
+// Four / five interleaved accumulations a_k += sum_{n=1..15} M_k[n] * ror_n(x_k),
+// three terms per asm statement; one s_nop 4 ahead of the first (the wait
+// states between a VALU write of x_k and its DPP read)
+#define MPC_TERM(n, acc, x, m) \
+  "v_fmac_f32_dpp %" #acc ", %" #x ", %" #m " row_ror:" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+#define MPC_T4(n, m0, m1, m2, m3) MPC_TERM(n, 0, 4, m0) MPC_TERM(n, 1, 5, m1) MPC_TERM(n, 2, 6, m2) MPC_TERM(n, 3, 7, m3)
+#define MPC_B4(pre, n0, n1, n2)                                                                                 \
+  asm(pre MPC_T4(n0, 8, 9, 10, 11) MPC_T4(n1, 12, 13, 14, 15) MPC_T4(n2, 16, 17, 18, 19)                        \
+      : "+v"(a), "+v"(b), "+v"(c), "+v"(e)                                                                     \
+      : "v"(xa), "v"(xb), "v"(xc), "v"(xe), "v"(Ma[n0]), "v"(Mb[n0]), "v"(Mc[n0]), "v"(Me[n0]), "v"(Ma[n1]),     \
+        "v"(Mb[n1]), "v"(Mc[n1]), "v"(Me[n1]), "v"(Ma[n2]), "v"(Mb[n2]), "v"(Mc[n2]), "v"(Me[n2]))
+__device__ __forceinline__ void rot_fma15x4(float &a, float xa, const float (&Ma)[16], float &b, float xb,
+                                            const float (&Mb)[16], float &c, float xc, const float (&Mc)[16],
+                                            float &e, float xe, const float (&Me)[16]) {
+  MPC_B4("s_nop 4\n", 1, 2, 3);
+  MPC_B4("", 4, 5, 6);
+  MPC_B4("", 7, 8, 9);
+  MPC_B4("", 10, 11, 12);
+  MPC_B4("", 13, 14, 15);
 }
-
-// a_k = M_k x_k (+ the addend s_k), three blocks of rotated rows
-__device__ __forceinline__ void rot_mv3(const float (&Ma)[16], float xa, float sa, float &a, const float (&Mb)[16],
-                                        float xb, float sb, float &b, const float (&Mc)[16], float xc, float sc,
-                                        float &c) {
-  a = fmaf(Ma[0], xa, sa);
-  b = fmaf(Mb[0], xb, sb);
-  c = fmaf(Mc[0], xc, sc);
-  rot_fma15x3(a, xa, Ma, b, xb, Mb, c, xc, Mc);
+#undef MPC_B4
+#undef MPC_T4
+#define MPC_T5(n, m0, m1, m2, m3, m4) \
+  MPC_TERM(n, 0, 5, m0) MPC_TERM(n, 1, 6, m1) MPC_TERM(n, 2, 7, m2) MPC_TERM(n, 3, 8, m3) MPC_TERM(n, 4, 9, m4)
+#define MPC_B5(pre, n0, n1, n2)                                                                                  \
+  asm(pre MPC_T5(n0, 10, 11, 12, 13, 14) MPC_T5(n1, 15, 16, 17, 18, 19) MPC_T5(n2, 20, 21, 22, 23, 24)          \
+      : "+v"(a), "+v"(b), "+v"(c), "+v"(e), "+v"(f)                                                             \
+      : "v"(xa), "v"(xb), "v"(xc), "v"(xe), "v"(xf), "v"(Ma[n0]), "v"(Mb[n0]), "v"(Mc[n0]), "v"(Me[n0]),        \
+        "v"(Mf[n0]), "v"(Ma[n1]), "v"(Mb[n1]), "v"(Mc[n1]), "v"(Me[n1]), "v"(Mf[n1]), "v"(Ma[n2]), "v"(Mb[n2]), \
+        "v"(Mc[n2]), "v"(Me[n2]), "v"(Mf[n2]))
+__device__ __forceinline__ void rot_fma15x5(float &a, float xa, const float (&Ma)[16], float &b, float xb,
+                                            const float (&Mb)[16], float &c, float xc, const float (&Mc)[16],
+                                            float &e, float xe, const float (&Me)[16], float &f, float xf,
+                                            const float (&Mf)[16]) {
+  MPC_B5("s_nop 4\n", 1, 2, 3);
+  MPC_B5("", 4, 5, 6);
+  MPC_B5("", 7, 8, 9);
+  MPC_B5("", 10, 11, 12);
+  MPC_B5("", 13, 14, 15);
 }
+#undef MPC_B5
+#undef MPC_T5
+#undef MPC_TERM
 
 // rotated rows of the column-major 14 x 14 block Lh (T = true: of its
 // transpose); zeros where has is false and in lanes 14, 15
@@ -336,7 +346,7 @@ k_pcg_split(Batch b, const uint4 *__restrict__ items, const uint32_t *__restrict
     }
   }
   float Lr[RK - 1][16];  // Lr[t - 1]: rows of L^_t, t = 1 .. 7
-  float Kr[3][16];       // Kr[t - 3]: rows of L^_t^T, t = 3 .. 5
+  float Kr[2][16];       // Kr[t - 3]: rows of L^_t^T, t = 3, 4
 #pragma unroll
   for (int t = 1; t < RK; ++t) {
     bool has;
@@ -344,7 +354,7 @@ k_pcg_split(Batch b, const uint4 *__restrict__ items, const uint32_t *__restrict
     load_lrows(Lr[t - 1], has, i, d, Pr);
   }
 #pragma unroll
-  for (int t = 3; t < 6; ++t) {
+  for (int t = 3; t < 5; ++t) {
     bool has;
     const float *Pr = blk((uint32_t)t, has);
     load_lrows<true>(Kr[t - 3], has, i, d, Pr);
@@ -367,32 +377,33 @@ k_pcg_split(Batch b, const uint4 *__restrict__ items, const uint32_t *__restrict
   // y = S^ x: y_t = -x_t + L^_t x_{t-1} + L^_{t+1}^T x_{t+1}; xm / xp: the
   // elements of knots 8lg - 1 and 8lg + 8.  Lanes 14, 15 of every block row
   // are zero, so y (and r, p, lambda^) stay zero there.
+  // two slot buffers: each slot read is issued one product group ahead of
+  // its use; A holds the rows of L^_0 between calls
+  float A[16], B[16];
+  read_slot(A, R0g, i);
   auto smv = [&](const float (&x)[RK], float xm, float xp) {
-    {
-      float M[16];
-      read_slot(M, R0g, i);
-      rot_mv3(M, xm, -x[0], y[0], Lr[0], x[0], -x[1], y[1], Lr[1], x[1], -x[2], y[2]);
-    }
-    rot_mv3(Lr[2], x[2], -x[3], y[3], Lr[3], x[3], -x[4], y[4], Lr[4], x[4], -x[5], y[5]);
-    {
-      float M[16];
-      read_slot(M, K1g, i);
-      rot_mv3(Lr[5], x[5], -x[6], y[6], Lr[6], x[6], -x[7], y[7], M, x[1], y[0], y[0]);
-    }
-    {
-      float M[16];
-      read_slot(M, K2g, i);
-      rot_mv3(M, x[2], y[1], y[1], Kr[0], x[3], y[2], y[2], Kr[1], x[4], y[3], y[3]);
-    }
-    {
-      float M[16];
-      read_slot(M, K8g, i);
-      y[4] = fmaf(Kr[2][0], x[5], y[4]);
-      y[7] = fmaf(M[0], xp, y[7]);
-      rot_fma15x2(y[4], x[5], Kr[2], y[7], xp, M);
-    }
-    float c5, c6;
-    rot_mtv2(Lr[5], x[6], Lr[6], x[7], c5, c6);
+    read_slot(B, K1g, i);
+    // first chain of every knot: rows of L^_t times x_{t-1}
+    y[0] = fmaf(A[0], xm, -x[0]);
+#pragma unroll
+    for (int t = 1; t < 5; ++t) y[t] = fmaf(Lr[t - 1][0], x[t - 1], -x[t]);
+    rot_fma15x5(y[0], xm, A, y[1], x[0], Lr[0], y[2], x[1], Lr[1], y[3], x[2], Lr[2], y[4], x[3], Lr[3]);
+    read_slot(A, K2g, i);
+#pragma unroll
+    for (int t = 5; t < 8; ++t) y[t] = fmaf(Lr[t - 1][0], x[t - 1], -x[t]);
+    y[0] = fmaf(B[0], x[1], y[0]);
+    rot_fma15x4(y[5], x[4], Lr[4], y[6], x[5], Lr[5], y[7], x[6], Lr[6], y[0], x[1], B);
+    read_slot(B, K8g, i);
+    float c4, c5, c6;
+    rot_mtv3(Lr[4], x[5], Lr[5], x[6], Lr[6], x[7], c4, c5, c6);
+    // second chains: rows of L^_{t+1}^T times x_{t+1}
+    y[1] = fmaf(A[0], x[2], y[1]);
+    y[2] = fmaf(Kr[0][0], x[3], y[2]);
+    y[3] = fmaf(Kr[1][0], x[4], y[3]);
+    y[7] = fmaf(B[0], xp, y[7]);
+    rot_fma15x4(y[1], x[2], A, y[2], x[3], Kr[0], y[3], x[4], Kr[1], y[7], xp, B);
+    read_slot(A, R0g, i);
+    y[4] += c4;
     y[5] += c5;
     y[6] += c6;
   };

@@ -27,9 +27,9 @@ def test_phase_roofline_bounds_and_fractions(bench):
     out = bench.phase_roofline(phases, knots, pcg_units, bw)
     rows = out["phases"]
     assert set(rows) == {"linearize", "schur", "pcg", "dz", "merit"}
-    # schur: 3976 B / knot over 5 TB/s against 3.24e4 flops / knot over 157.3 TFLOP/s
-    t_hbm = 3976 * knots / 5000e9
-    t_f = 3.24e4 * knots / 157.3e12
+    # schur (split form): 3052 B / knot over 5 TB/s against 2.5e4 flops / knot over 157.3 TFLOP/s
+    t_hbm = 3052 * knots / 5000e9
+    t_f = 2.5e4 * knots / 157.3e12
     assert rows["schur"]["bound"] == ("hbm" if t_hbm > t_f else "fp32")
     assert rows["schur"]["roofline_us"] == pytest.approx(max(t_hbm, t_f) * 1e6, abs=0.01)
     assert rows["schur"]["frac"] == pytest.approx(max(t_hbm, t_f) * 1e6 / 400.0, abs=1e-4)  # 4 decimals
@@ -37,7 +37,7 @@ def test_phase_roofline_bounds_and_fractions(bench):
     assert rows["pcg"]["us"] == pytest.approx(2000.0)
     t_lds = 2296 * pcg_units / 100000e9
     t_pf = 1708 * pcg_units / 157.3e12
-    t_ph = 4 * (588 + 196 + 14 + 28) * knots / 5000e9
+    t_ph = 4 * (392 + 14 + 28) * knots / 5000e9
     assert rows["pcg"]["roofline_us"] == pytest.approx(max(t_lds, t_pf, t_ph) * 1e6, abs=0.01)
     assert rows["pcg"]["lds_bytes"] == int(2296 * pcg_units)
     # merit has no flops: priced by bytes
@@ -62,7 +62,7 @@ def test_phase_roofline_uses_counted_flops(bench):
     out = bench.phase_roofline(phases, 1.0e5, 1.0e6, bw, counters)
     rows = out["phases"]
     assert rows["linearize"]["flops"] == int(6.0e4 * 1.0e5) and rows["merit"]["flops"] == int(2.0e4 * 1.0e5)
-    assert out["flops_source"] == {"linearize": "counted", "merit": "counted"}
+    assert out["flops_source"] == {"linearize": "counted", "schur": "estimate", "dz": "estimate", "merit": "counted"}
     assert bench.phase_roofline(phases, 1.0e5, 1.0e6, bw)["flops_source"]["linearize"] == "estimate"
 
 
