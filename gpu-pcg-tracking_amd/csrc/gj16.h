@@ -95,6 +95,10 @@ __device__ __forceinline__ void gj_inverse(int lane, float (&a)[NX], float (&x)[
 // holds row j of W = U^-1 in x, where M = U^T U (Cholesky, U upper
 // triangular; the scaled rows of the left half are the rows of U), so
 // W^T M W = I; a is destroyed.  Every lane of the group must be active.
+// The pivot's reciprocal and inverse square root are the hardware's
+// (v_rcp_f32, v_rsq_f32: 1 ulp), not the correctly rounded sequences: W is
+// this build's own factor (no reference operation), and a one-ulp change
+// of a multiplier is a rounding-level change of M.
 template <int n>
 __device__ __forceinline__ void chol_inv_rows(int lane, float (&a)[NX], float (&x)[NX]) {
   const int j = lane < n ? lane : n - 1;
@@ -105,8 +109,8 @@ __device__ __forceinline__ void chol_inv_rows(int lane, float (&a)[NX], float (&
     float pc[NX];
 #pragma unroll
     for (int r = p; r < n; ++r) pc[r] = bcast_dyn16(a[r], p);
-    const float inv = 1.f / pc[p];
-    const float s = 1.f / sqrtf(pc[p]);
+    const float inv = __builtin_amdgcn_rcpf(pc[p]);
+    const float s = __builtin_amdgcn_rsqf(pc[p]);
     const float ap = a[p], xp = x[p];
 #pragma unroll
     for (int r = p + 1; r < n; ++r) {

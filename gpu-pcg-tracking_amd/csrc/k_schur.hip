@@ -269,10 +269,11 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
 //   Q~^-1 = D^-1 - u u^T / s, u = D^-1 v, s = 1 + v^T D^-1 v (Sherman-Morrison),
 //   R~^-1 = diag(1 / (wu + rho)),
 // so A Q~^-1 = A D^-1 - (A u) u^T / s.  Along v the two terms are of order
-// 1 / rho and cancel to 1 / (rho + |v|^2), so D^-1 is rounded to fp32 once
-// and u, s, A u and the cancelling sums are taken in fp64 (products of fp32
-// values are exact there): the result is, to fp32 rounding of its own
-// entries, the inverse of Q~ with its diagonal moved by at most one ulp.
+// 1 / rho and cancel to 1 / (rho + |v|^2), so D^-1 is taken in fp32 once
+// (v_rcp_f32, 1 ulp) and u, s, A u and the cancelling sums in fp64
+// (products of fp32 values are exact there): the result is, to fp32
+// rounding of its own entries, the inverse of Q~ with its diagonal moved by
+// a few ulps.
 // Workgroups of SG groups overlap by
 // one row: group g of workgroup w forms row (SG - 1) w + g - 1, and groups
 // g >= 1 take W_{b-1} from group g - 1 through LDS and write their row
@@ -340,8 +341,8 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   // of Q~_{b-1} (zero for row 0) and of Q~_b in fp64
   float dq[NQ];
 #pragma unroll
-  for (int l = 0; l < NQ; ++l) dq[l] = 1.f / (p.wx[l] + rho);
-  const float dj = 1.f / (p.wx[j] + rho);  // D^-1(j, j)
+  for (int l = 0; l < NQ; ++l) dq[l] = __builtin_amdgcn_rcpf(p.wx[l] + rho);
+  const float dj = __builtin_amdgcn_rcpf(p.wx[j] + rho);  // D^-1(j, j)
   const float dpj = row0 ? 0.f : dj;       // of Q~_{b-1}^-1 (zero for row 0)
   __syncthreads();
   // AQ = A Q~_{b-1}^-1 (column j) = A(:, j) D^-1(j, j) - (A u)(u_j / s), the
@@ -367,7 +368,7 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   }
   {
     const int ju = lane < NU ? lane : NU - 1;
-    const float wr = row0 ? 0.f : 1.f / (p.wu[ju] + rho);
+    const float wr = row0 ? 0.f : __builtin_amdgcn_rcpf(p.wu[ju] + rho);
     if (lane < NU) {
 #pragma unroll
       for (int r = 0; r < NX; ++r) L.BR[r + NX * lane] = L.B[r + NX * lane] * wr;
@@ -565,15 +566,15 @@ __global__ void __launch_bounds__(WG) k_compute_dz_lr(Batch b, Params p, const f
     float dq[NQ];
     double u[NQ], inv_s, ut = 0.0;
 #pragma unroll
-    for (int l = 0; l < NQ; ++l) dq[l] = 1.f / (p.wx[l] + rho);
+    for (int l = 0; l < NQ; ++l) dq[l] = __builtin_amdgcn_rcpf(p.wx[l] + rho);
     q_lowrank(gq + (size_t)NQ * knot, dq, false, u, inv_s);
 #pragma unroll
     for (int l = 0; l < NQ; ++l) ut = fma(u[l], (double)L.t[l], ut);
-    const float dl = 1.f / (p.wx[lane] + rho);
+    const float dl = __builtin_amdgcn_rcpf(p.wx[lane] + rho);
     const double ul = lane < NQ ? (double)gq[(size_t)NQ * knot + lane] * (double)dl : 0.0;
     dzi[lane] = (float)fma(-ul, ut * inv_s, (double)L.t[lane] * (double)dl);
   }
-  if (has_next && lane < NU) dzi[NX + lane] = L.tu[lane] / (p.wu[lane] + rho);
+  if (has_next && lane < NU) dzi[NX + lane] = L.tu[lane] * __builtin_amdgcn_rcpf(p.wu[lane] + rho);
 }
 
 }  // namespace
