@@ -314,9 +314,11 @@ int mpc_batch_pcg_launch_map(const mpc_handle *h, uint32_t *launch_of_inst);
  * the schedule change.  split = 1 (block Jacobi, every N <= 256): all
  * instances instead run on the split-form kernel, bins of up to max_items
  * instances on 512-lane workgroups (256 knots, one per CU); block-Jacobi
- * PCG there is CG on W^T S W (W = chol(-S(b,b))^-1 blockwise, formed after
- * the Schur complement), the same iteration in exact arithmetic with its
- * own fp32 rounding, and the other fields do not apply.  Default
+ * PCG there is CG on W^T S W (W = chol(-S(b,b))^-1 blockwise, formed with
+ * the Schur block rows; in mpc_batch_sqp the regularised cost blocks are
+ * inverted in closed form, Q~ being diagonal plus rank one), the same
+ * iteration in exact arithmetic with its own fp32 rounding, and the other
+ * fields do not apply.  Default
  * {512, 8, 1, 1, split 1}.  No counterpart in the reference (one instance
  * per solve). */
 typedef struct mpc_pcg_plan {
