@@ -162,6 +162,7 @@ hipError_t launch_form_split(const Batch &b, const Params &p, const float *C, co
                              const float *gq, const float *rho_dev, float rho, const uint32_t *status, float *Ps,
                              float *ghat, float *rho_used, hipStream_t st);
 // dz as launch_compute_dz, with G~^-1 in the closed form of launch_form_split
+// (rho_dev: one rho per instance, required, as gq)
 hipError_t launch_compute_dz_lr(const Batch &b, const Params &p, const float *gq, const float *rho_dev,
                                 const float *C, const float *g, const float *lambda, const uint32_t *status,
                                 float *dz, hipStream_t st);

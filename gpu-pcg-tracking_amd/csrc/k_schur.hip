@@ -597,6 +597,7 @@ hipError_t launch_form_split(const Batch &b, const Params &p, const float *C, co
                              const float *gq, const float *rho_dev, float rho, const uint32_t *status, float *Ps,
                              float *ghat, float *rho_used, hipStream_t st) {
   if (b.total_knots == 0) return hipSuccess;
+  if (!gq) return hipErrorInvalidValue;  // g_q of launch_form_kkt is read by every block row
   hipLaunchKernelGGL(k_form_split, dim3((b.total_knots + SSTEP - 1) / SSTEP), dim3(SWG), 0, st, b, p, C, g, c, gq,
                      rho_dev, rho, status, Ps, ghat, rho_used);
   return hipGetLastError();
@@ -606,6 +607,7 @@ hipError_t launch_compute_dz_lr(const Batch &b, const Params &p, const float *gq
                                 const float *C, const float *g, const float *lambda, const uint32_t *status,
                                 float *dz, hipStream_t st) {
   if (b.total_knots == 0) return hipSuccess;
+  if (!rho_dev || !gq) return hipErrorInvalidValue;  // per-instance rho and g_q are read by every knot
   hipLaunchKernelGGL(k_compute_dz_lr, dim3((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG), 0, st, b,
                      p, gq, rho_dev, C, g, lambda, status, dz);
   return hipGetLastError();
