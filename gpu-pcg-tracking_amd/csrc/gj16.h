@@ -1,8 +1,9 @@
 // gj16.h -- 14x14 block algebra inside a 16-lane group (gfx950): lane j
 // holds column j of a block in registers; a pivot column is broadcast to
-// the group by ds_swizzle, so a Gauss-Jordan inverse needs no LDS round
-// trip and no barrier.  Used by the Schur formation (k_schur.hip) and the
-// direct block-tridiagonal solve (k_direct.hip).
+// the group by ds_swizzle (Gauss-Jordan) or a DPP row broadcast (the
+// whitening factor), so neither needs an LDS buffer or a barrier.  Used by
+// the Schur formation (k_schur.hip), the split PCG (k_pcg_split.hip) and
+// the direct block-tridiagonal solve (k_direct.hip).
 #pragma once
 #include "common.h"
 
@@ -15,27 +16,29 @@ __device__ __forceinline__ float bcast16(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x10 | (P << 5)));
 }
 
-// bcast16 with the lane p given as a value that is a constant after loop
-// unrolling (ds_swizzle takes its pattern as an immediate)
+// The same broadcast by DPP row_newbcast:P (dpp_ctrl 0x150 + P, gfx90a and
+// later): lane P of each 16-lane row to the whole row, one VALU instruction
+// with no trip through the LDS pipeline.  Lane P must be active.  Used on
+// the dependent pivot chain of chol_inv_rows, where the swizzle's LDS
+// latency sits on the critical path of every step.
+template <int P>
+__device__ __forceinline__ float bcast16_dpp(float v) {
+  static_assert(P >= 0 && P < 16, "row lane");
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + P, 0xf, 0xf, true));
+}
+
+// bcast16 (Dpp: bcast16_dpp) with the lane p given as a value that is a
+// constant after loop unrolling (both take the lane as an immediate)
+template <bool Dpp = false>
 __device__ __forceinline__ float bcast_dyn16(float v, int p) {
+#define MPC_BCASE(P) \
+  case P: return Dpp ? bcast16_dpp<P>(v) : bcast16<P>(v);
   switch (p) {
-    case 0: return bcast16<0>(v);
-    case 1: return bcast16<1>(v);
-    case 2: return bcast16<2>(v);
-    case 3: return bcast16<3>(v);
-    case 4: return bcast16<4>(v);
-    case 5: return bcast16<5>(v);
-    case 6: return bcast16<6>(v);
-    case 7: return bcast16<7>(v);
-    case 8: return bcast16<8>(v);
-    case 9: return bcast16<9>(v);
-    case 10: return bcast16<10>(v);
-    case 11: return bcast16<11>(v);
-    case 12: return bcast16<12>(v);
-    case 13: return bcast16<13>(v);
-    case 14: return bcast16<14>(v);
-    default: return bcast16<15>(v);
+    MPC_BCASE(0) MPC_BCASE(1) MPC_BCASE(2) MPC_BCASE(3) MPC_BCASE(4) MPC_BCASE(5) MPC_BCASE(6) MPC_BCASE(7)
+    MPC_BCASE(8) MPC_BCASE(9) MPC_BCASE(10) MPC_BCASE(11) MPC_BCASE(12) MPC_BCASE(13) MPC_BCASE(14)
+    default: return Dpp ? bcast16_dpp<15>(v) : bcast16<15>(v);
   }
+#undef MPC_BCASE
 }
 
 // Materialise values at this point of the program: keeps the compiler from
@@ -108,7 +111,7 @@ __device__ __forceinline__ void chol_inv_rows(int lane, float (&a)[NX], float (&
   for (int p = 0; p < n; ++p) {
     float pc[NX];
 #pragma unroll
-    for (int r = p; r < n; ++r) pc[r] = bcast_dyn16(a[r], p);
+    for (int r = p; r < n; ++r) pc[r] = bcast_dyn16<true>(a[r], p);
     const float inv = __builtin_amdgcn_rcpf(pc[p]);
     const float s = __builtin_amdgcn_rsqf(pc[p]);
     const float ap = a[p], xp = x[p];

@@ -300,7 +300,8 @@ k_pcg_split(Batch b, const uint4 *__restrict__ items, const uint32_t *__restrict
   __syncthreads();  // LDS cleared
 
   // ---- load: lambda^ = W^-1 lambda by back substitution (W upper
-  // triangular) and gamma^, then the blocks (fewer values live at once)
+  // triangular; each solved element broadcast to the group by DPP, every
+  // lane active) and gamma^, then the blocks (fewer values live at once)
   const uint32_t b0 = lg * RK;
   float lam[RK], r[RK], p[RK], y[RK], gh[RK];  // z = -r is not stored
 #pragma unroll
@@ -315,7 +316,7 @@ k_pcg_split(Batch b, const uint4 *__restrict__ items, const uint32_t *__restrict
     float lh = 0.f;
 #pragma unroll
     for (int k = NX - 1; k >= 0; --k) {
-      const float xk = bcast_dyn16(acc / wrow[k], k);
+      const float xk = bcast_dyn16<true>(acc / wrow[k], k);
       lh = (int)i == k ? xk : lh;
       asm volatile("" : "+v"(lh));  // select now, not after the last step
       acc -= wrow[k] * xk;

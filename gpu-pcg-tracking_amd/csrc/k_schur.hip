@@ -307,8 +307,8 @@ __device__ __forceinline__ void q_lowrank(const float *__restrict__ vq, const fl
 
 __device__ __forceinline__ double bcast_dyn16_f64(double v, int p) {
   const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  const float lo = bcast_dyn16(__builtin_bit_cast(float, (uint32_t)b), p);
-  const float hi = bcast_dyn16(__builtin_bit_cast(float, (uint32_t)(b >> 32)), p);
+  const float lo = bcast_dyn16<true>(__builtin_bit_cast(float, (uint32_t)b), p);
+  const float hi = bcast_dyn16<true>(__builtin_bit_cast(float, (uint32_t)(b >> 32)), p);
   return __builtin_bit_cast(double, ((unsigned long long)__builtin_bit_cast(uint32_t, hi) << 32) |
                                         __builtin_bit_cast(uint32_t, lo));
 }
