@@ -48,6 +48,16 @@ __device__ __forceinline__ KnotRef locate(const Batch &b, uint32_t gk) {
   return r;
 }
 
+// LDS ordering among the lanes of one wave: a wave's LDS accesses execute
+// in order, so only compiler reordering must be prevented; unlike
+// __syncthreads() this neither waits for the other waves nor for
+// outstanding global loads
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ uint32_t inst_first(const Batch &b, uint32_t inst) {
   return b.knot_inst == nullptr ? 0u : b.inst_k0[inst];
 }

@@ -49,15 +49,9 @@ __device__ __forceinline__ float *slot_E(float *work, size_t row) { return work 
 __device__ __forceinline__ float *slot_Dinv(float *work, size_t row) { return work + (size_t)S_STRIDE * row + NXX; }
 __device__ __forceinline__ float *slot_z(float *work, size_t row) { return work + (size_t)S_STRIDE * row + 2 * NXX; }
 
-// LDS ordering among the lanes of the (single-wave) workgroup: the wave's
-// LDS accesses execute in order, so only compiler reordering must be
-// prevented; unlike __syncthreads() this does not wait for outstanding
-// global loads (the next knot's prefetch)
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// LDS ordering among the lanes of the (single-wave) workgroup: wave_sync
+// (common.h), which does not wait for outstanding global loads (the next
+// knot's prefetch)
 
 // sum_m w[m] v_m, v_m the value v of lane m of the group (m < 14), in order
 template <int... M>

@@ -137,12 +137,6 @@ __device__ __forceinline__ float mb_wait(const unsigned long long *src, bool wan
   return want ? __uint_as_float((uint32_t)v) : 0.f;
 }
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Part `part` of the PCG of team `team` (instance team_inst[team]) on a
 // 512-lane workgroup with the TeamShared block at smem, exchanging through
 // mailbox mb, buffer set `set`, epochs e0 + 1, e0 + 2, ...  Returns false

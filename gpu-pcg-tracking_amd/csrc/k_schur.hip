@@ -23,6 +23,7 @@
 // dz (oldschur.cuh:966-1105):
 //   dx_k = Q~_k^-1 (q_k - lambda_k + A_k^T lambda_{k+1}),  du_k = R~_k^-1 (r_k + B_k^T lambda_{k+1})
 #include <cstddef>
+#include <utility>
 
 #include "common.h"
 #include "gj16.h"
@@ -274,10 +275,14 @@ __global__ void __launch_bounds__(WG) k_form_schur(Batch b, const float *__restr
 // (products of fp32 values are exact there): the result is, to fp32
 // rounding of its own entries, the inverse of Q~ with its diagonal moved by
 // a few ulps.
-// Workgroups of SG groups overlap by
-// one row: group g of workgroup w forms row (SG - 1) w + g - 1, and groups
-// g >= 1 take W_{b-1} from group g - 1 through LDS and write their row
-// (each row from exactly one group).
+// The block products (theta, S(b,b-1) W_{b-1} and W_b^T of that) take
+// their left factor's columns from the group's lanes by DPP broadcasts
+// fused into the FMAs (nb_product), so the factors stay in registers.
+// Workgroups of SG groups overlap by one row: group g of workgroup w forms
+// row (SG - 1) w + g - 1, and groups g >= 1 take W_{b-1} from group g - 1
+// through LDS and write their row (each row from exactly one group).  That
+// hand-over is the one workgroup barrier; every other LDS exchange stays
+// inside a group, so inside one wave (wave_sync).
 constexpr int SG = 16;
 constexpr int SWG = GROUP * SG;
 constexpr uint32_t SSTEP = SG - 1;
@@ -313,6 +318,52 @@ __device__ __forceinline__ double bcast_dyn16_f64(double v, int p) {
                                         __builtin_bit_cast(uint32_t, lo));
 }
 
+// d[r] += s_l[r] * w (r < 14), s_l the s of lane l of the 16-lane group:
+// v_fmac_f32_dpp with DPP row_newbcast:l, so a block product whose left
+// factor is held by columns (lane l: column l) needs no LDS and no
+// separate broadcast instruction.  The s_nop ahead of each seven covers the
+// wait states between a VALU write of an operand (or of EXEC, Nop4) and its
+// DPP read.  The group's lanes must all be active.
+#define MPC_NBF(k) "v_fmac_f32_dpp %" #k ", %" MPC_NBS##k ", %14 row_newbcast:%15 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+#define MPC_NBS0 "7"
+#define MPC_NBS1 "8"
+#define MPC_NBS2 "9"
+#define MPC_NBS3 "10"
+#define MPC_NBS4 "11"
+#define MPC_NBS5 "12"
+#define MPC_NBS6 "13"
+#define MPC_NB7(nop)                                                                                          \
+  asm(nop MPC_NBF(0) MPC_NBF(1) MPC_NBF(2) MPC_NBF(3) MPC_NBF(4) MPC_NBF(5) MPC_NBF(6)                          \
+      : "+v"(d[o]), "+v"(d[o + 1]), "+v"(d[o + 2]), "+v"(d[o + 3]), "+v"(d[o + 4]), "+v"(d[o + 5]), "+v"(d[o + 6]) \
+      : "v"(s[o]), "v"(s[o + 1]), "v"(s[o + 2]), "v"(s[o + 3]), "v"(s[o + 4]), "v"(s[o + 5]), "v"(s[o + 6]),       \
+        "v"(w), "i"(l))
+template <int l, bool Nop4, int o>
+__device__ __forceinline__ void nb_fma7(float (&d)[NX], const float (&s)[NX], float w) {
+  static_assert(l >= 0 && l < 16 && o + 7 <= NX, "row lane, accumulator range");
+  if constexpr (Nop4)
+    MPC_NB7("s_nop 4\n");
+  else
+    MPC_NB7("s_nop 1\n");
+}
+#undef MPC_NB7
+#undef MPC_NBF
+#undef MPC_NBS0
+#undef MPC_NBS1
+#undef MPC_NBS2
+#undef MPC_NBS3
+#undef MPC_NBS4
+#undef MPC_NBS5
+#undef MPC_NBS6
+
+// d[r] += sum_{l in Ls, in order} s_l[r] * w[l]: a block product from the
+// left factor's columns (one per lane) and the calling lane's row of the
+// right factor's transpose
+template <bool Nop4, int... Ls>
+__device__ __forceinline__ void nb_product(float (&d)[NX], const float (&s)[NX], const float *w,
+                                           std::integer_sequence<int, Ls...>) {
+  ((nb_fma7<Ls, Nop4 && Ls == 0, 0>(d, s, w[Ls]), nb_fma7<Ls, false, 7>(d, s, w[Ls])), ...);
+}
+
 __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k_form_split(Batch b, Params p,
                                                     const float *__restrict__ C,
                                                     const float *__restrict__ g, const float *__restrict__ cvec,
@@ -344,10 +395,11 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   for (int l = 0; l < NQ; ++l) dq[l] = __builtin_amdgcn_rcpf(p.wx[l] + rho);
   const float dj = __builtin_amdgcn_rcpf(p.wx[j] + rho);  // D^-1(j, j)
   const float dpj = row0 ? 0.f : dj;       // of Q~_{b-1}^-1 (zero for row 0)
-  __syncthreads();
+  wave_sync();  // the group's staged A, B (a group is inside one wave)
   // AQ = A Q~_{b-1}^-1 (column j) = A(:, j) D^-1(j, j) - (A u)(u_j / s), the
   // vector A u formed one element per lane and broadcast; BR = B R~^-1
   // (column j < 7)
+  float acc[NX];  // column j of AQ (lanes 14, 15: column 13)
   {
     double up[NQ], isp;
     q_lowrank(gq + (size_t)NQ * kprev, dq, row0, up, isp);
@@ -356,7 +408,6 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
     for (int l = 0; l < NQ; ++l) au = fma((double)L.A[j + NX * l], up[l], au);
     const double upj = j < NQ && !row0 ? (double)gq[(size_t)NQ * kprev + j] * (double)dj : 0.0;  // u_{b-1}(j)
     const double cj = upj * isp;
-    float acc[NX];
 #pragma unroll
     for (int r = 0; r < NX; ++r)
       acc[r] = (float)fma(-bcast_dyn16_f64(au, r), cj, (double)L.A[r + NX * j] * (double)dpj);
@@ -366,15 +417,24 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
       for (int r = 0; r < NX; ++r) L.AQ[r + NX * lane] = acc[r];
     }
   }
+  float brc[NX];  // column ju of BR
   {
     const int ju = lane < NU ? lane : NU - 1;
     const float wr = row0 ? 0.f : __builtin_amdgcn_rcpf(p.wu[ju] + rho);
+#pragma unroll
+    for (int r = 0; r < NX; ++r) brc[r] = L.B[r + NX * ju] * wr;
     if (lane < NU) {
 #pragma unroll
-      for (int r = 0; r < NX; ++r) L.BR[r + NX * lane] = L.B[r + NX * lane] * wr;
+      for (int r = 0; r < NX; ++r) L.BR[r + NX * lane] = brc[r];
     }
   }
-  __syncthreads();
+  // row j of A and of B (the right factors of theta, transposed)
+  float arow[NX], brow[NU];
+#pragma unroll
+  for (int l = 0; l < NX; ++l) arow[l] = L.A[j + NX * l];
+#pragma unroll
+  for (int l = 0; l < NU; ++l) brow[l] = L.B[j + NX * l];
+  wave_sync();  // the group's AQ and BR (for gamma)
   // theta(:, j) = sum_l AQ(:, l) A(j, l) + sum_l BR(:, l) B(j, l) + Q~_b^-1(:, j)
   float th[NX];
   double isb;
@@ -386,18 +446,10 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
     for (int r = 0; r < NX; ++r) th[r] = (float)((r == j ? (double)dj : 0.0) - (r < NQ ? ub[r] * cb : 0.0));
   }
-#pragma unroll
-  for (int l = 0; l < NX; ++l) {
-    const float ajl = L.A[j + NX * l];
-#pragma unroll
-    for (int r = 0; r < NX; ++r) th[r] += L.AQ[r + NX * l] * ajl;
-  }
-#pragma unroll
-  for (int l = 0; l < NU; ++l) {
-    const float bjl = L.B[j + NX * l];
-#pragma unroll
-    for (int r = 0; r < NX; ++r) th[r] += L.BR[r + NX * l] * bjl;
-  }
+  // the columns of AQ and BR broadcast from their lanes (DPP), in the
+  // order of the sums above
+  nb_product<true>(th, acc, arow, std::make_integer_sequence<int, NX>{});
+  nb_product<true>(th, brc, brow, std::make_integer_sequence<int, NU>{});
   pin(th);
   // gamma = c - (Q~_b^-1 q_b - A Q~^-1 q_{b-1} - B R~^-1 r_{b-1}) (row 0: -Q~_0^-1 q_0)
   if (lane < NX) {
@@ -415,12 +467,12 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   }
   float x[NX];
   chol_inv_rows<NX>(lane, th, x);  // row j of W_b
-  __syncthreads();                 // every group is done reading its A
+  wave_sync();                     // the group is done reading its A
   if (lane < NX) {
 #pragma unroll
     for (int c = 0; c < NX; ++c) L.A[lane + NX * c] = x[c];  // W_b, column-major
   }
-  __syncthreads();
+  __syncthreads();  // W_b to the next group (the one barrier across waves)
   // gamma^ = W_b^T gamma_b; L^ = W_b^T (S(b,b-1) W_{b-1}) by columns
   float *Pr = Ps + (size_t)S_STRIDE * row;
   float *Lout = L.B;  // B and BR: one 196-float block
@@ -433,28 +485,27 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     if (row0 && lane == 0 && rho_used) rho_used[kr.inst] = rho;
     if (!row0) {
-      const float *Wp = lds[grp - 1].A;  // W_{b-1}
-      float m[NX];
-#pragma unroll
-      for (int r = 0; r < NX; ++r) m[r] = 0.f;
+      // m = AQ W_{b-1}(:, j) from the AQ columns (acc) of the group's
+      // lanes, then L^(:, j) = W_b^T m from the rows of W_b (x), both by
+      // DPP broadcasts (the whole group is active here: owner and row0 are
+      // the same for all its lanes)
+      const float *Wp = lds[grp - 1].A;  // W_{b-1}, column-major
+      float wp[NX], m[NX], lh[NX];
 #pragma unroll
       for (int k = 0; k < NX; ++k) {
-        const float wkj = Wp[k + NX * j];
-#pragma unroll
-        for (int r = 0; r < NX; ++r) m[r] += L.AQ[r + NX * k] * wkj;
+        wp[k] = Wp[k + NX * j];
+        m[k] = 0.f;
+        lh[k] = 0.f;
       }
+      nb_product<true>(m, acc, wp, std::make_integer_sequence<int, NX>{});
+      nb_product<true>(lh, x, m, std::make_integer_sequence<int, NX>{});
       if (lane < NX) {
 #pragma unroll
-        for (int r = 0; r < NX; ++r) {
-          float acc = 0.f;
-#pragma unroll
-          for (int k = 0; k < NX; ++k) acc += L.A[k + NX * r] * m[k];
-          Lout[r + NX * lane] = acc;
-        }
+        for (int r = 0; r < NX; ++r) Lout[r + NX * lane] = lh[r];
       }
     }
   }
-  __syncthreads();
+  wave_sync();  // the group's L^ and W_b, stored below
   if (owner) {
     store_block(lane, Pr + NXX, [&](int i) { return L.A[i]; });
     if (!row0) store_block(lane, Pr, [&](int i) { return Lout[i]; });
