@@ -376,18 +376,38 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   const int64_t row_raw = (int64_t)blockIdx.x * SSTEP + grp - 1;
   const bool in_range = row_raw >= 0 && row_raw < (int64_t)b.total_knots;
   const uint32_t row = in_range ? (uint32_t)row_raw : (row_raw < 0 ? 0u : b.total_knots - 1);
-  const KnotRef kr = locate(b, row);
-  const bool owner = grp >= 1 && in_range && !(status && !(status[kr.inst] & ST_ACTIVE));
-  const bool row0 = kr.k == 0;
-  SplitRowLds &L = lds[grp];
-  const size_t kprev = row0 ? row : row - 1;  // row 0 stages its own knot, multiplied by zero below
+  // The reads of the block row are issued before the instance lookup: the
+  // previous knot's C, g and g_q (for a first knot another instance's last
+  // knot, whose C and r are not written: replaced by zeros below, never
+  // multiplied) and this knot's g and g_q.  A first knot is one whose
+  // predecessor belongs to another instance (one lookup, no dependent
+  // second read).
+  const size_t kprev = row > 0 ? row - 1 : 0u;
   const float *Ck = C + (size_t)C_STRIDE * kprev;
-  for (int i = lane; i < NXX; i += GROUP) L.A[i] = -Ck[i];
-  for (int i = lane; i < 98; i += GROUP) L.B[i] = -Ck[NXX + i];
+  constexpr int CA = (NXX + GROUP - 1) / GROUP, CB = (98 + GROUP - 1) / GROUP;
+  float ca[CA], cb[CB];
+#pragma unroll
+  for (int t = 0; t < CA; ++t) ca[t] = lane + GROUP * t < NXX ? Ck[lane + GROUP * t] : 0.f;
+#pragma unroll
+  for (int t = 0; t < CB; ++t) cb[t] = lane + GROUP * t < 98 ? Ck[NXX + lane + GROUP * t] : 0.f;
+  uint32_t inst = 0;
+  bool row0 = row == 0;
+  if (b.knot_inst) {
+    inst = b.knot_inst[row];
+    row0 = row0 || b.knot_inst[kprev] != inst;
+  }
+  const bool owner = grp >= 1 && in_range && !(status && !(status[inst] & ST_ACTIVE));
+  SplitRowLds &L = lds[grp];
+#pragma unroll
+  for (int t = 0; t < CA; ++t)
+    if (lane + GROUP * t < NXX) L.A[lane + GROUP * t] = row0 ? 0.f : -ca[t];
+#pragma unroll
+  for (int t = 0; t < CB; ++t)
+    if (lane + GROUP * t < 98) L.B[lane + GROUP * t] = row0 ? 0.f : -cb[t];
   const int j = lane < NX ? lane : NX - 1;
   // q_{b-1}, r_{b-1}, q_b: read by every lane of the group (cached)
   const float *qk = g + (size_t)NXU * kprev, *rk = qk + NX, *q1v = g + (size_t)NXU * row;
-  const float rho = rho_dev ? rho_dev[kr.inst] : rho_s;
+  const float rho = rho_dev ? rho_dev[inst] : rho_s;
   // D^-1 in fp32 (the same for both knots: one instance, one rho); u and 1/s
   // of Q~_{b-1} (zero for row 0) and of Q~_b in fp64
   float dq[NQ];
@@ -461,7 +481,7 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
     for (int l = 0; l < NX; ++l) a2 += L.AQ[lane + NX * l] * qk[l];
 #pragma unroll
-    for (int l = 0; l < NU; ++l) a3 += L.BR[lane + NX * l] * rk[l];
+    for (int l = 0; l < NU; ++l) a3 += L.BR[lane + NX * l] * (row0 ? 0.f : rk[l]);
     const float cb = (row0 || !cvec) ? 0.f : cvec[(size_t)NX * row + lane];  // gamma_0 has no c term
     L.gm[lane] = cb - (a1 - a2 - a3);
   }
@@ -483,7 +503,7 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
       for (int k = 0; k < NX; ++k) acc += L.A[k + NX * lane] * L.gm[k];
       ghat[(size_t)NX * row + lane] = acc;
     }
-    if (row0 && lane == 0 && rho_used) rho_used[kr.inst] = rho;
+    if (row0 && lane == 0 && rho_used) rho_used[inst] = rho;
     if (!row0) {
       // m = AQ W_{b-1}(:, j) from the AQ columns (acc) of the group's
       // lanes, then L^(:, j) = W_b^T m from the rows of W_b (x), both by
