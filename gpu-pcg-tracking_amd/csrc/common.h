@@ -48,6 +48,35 @@ __device__ __forceinline__ KnotRef locate(const Batch &b, uint32_t gk) {
   return r;
 }
 
+// Instance of global knot gk and its place in the instance, from knot_inst
+// alone: three independent reads that share cache lines, so what depends on
+// them waits one round trip instead of locate's two (knot_inst, then
+// inst_k0).  Instances lie back to back, each at least two knots long: a
+// knot starts its instance when its predecessor belongs to another one and
+// has a successor when the next knot belongs to the same one.  Its state
+// x_k sits at NXU * gk - NU * inst of the batch's xu (every earlier
+// instance's last knot has no u).
+struct KnotEnds {
+  uint32_t inst;
+  bool first, has_next;
+};
+
+__device__ __forceinline__ KnotEnds knot_ends(const Batch &b, uint32_t gk) {
+  KnotEnds e;
+  const bool last = gk + 1 >= b.total_knots;
+  if (b.knot_inst == nullptr) {
+    e.inst = 0;
+    e.first = gk == 0;
+    e.has_next = !last;
+  } else {
+    e.inst = b.knot_inst[gk];
+    const uint32_t ip = b.knot_inst[gk > 0 ? gk - 1 : gk], in = b.knot_inst[last ? gk : gk + 1];
+    e.first = gk == 0 || ip != e.inst;
+    e.has_next = !last && in == e.inst;
+  }
+  return e;
+}
+
 // LDS ordering among the lanes of one wave: a wave's LDS accesses execute
 // in order, so only compiler reordering must be prevented; unlike
 // __syncthreads() this neither waits for the other waves nor for

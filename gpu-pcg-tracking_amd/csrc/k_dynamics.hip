@@ -228,20 +228,21 @@ __global__ void __launch_bounds__(KKT_WG) k_form_kkt(Batch b, Params p, const fl
   const uint32_t gk_raw = blockIdx.x * KKT_GROUPS + grp;
   bool valid = gk_raw < b.total_knots;
   const uint32_t gk = valid ? gk_raw : b.total_knots - 1;
-  const KnotRef kr = locate(b, gk);
-  if (status && !(status[kr.inst] & ST_ACTIVE)) valid = false;
+  const KnotEnds ke = knot_ends(b, gk);
+  if (status && !(status[ke.inst] & ST_ACTIVE)) valid = false;
   KktLds &L = lds[grp];
   DynLds &D = L.dyn;
-  const float *xi = xu + (size_t)NXU * kr.K0 - (size_t)NU * kr.inst;  // instance trajectory
-  const bool dyn = kr.k + 1 < kr.N;
-  const uint32_t kc = dyn ? kr.k : ((p.flags & FLAG_LAST_KNOT_OWN_STATE) ? kr.k : kr.k - 1);
+  const float *xk = xu + (size_t)NXU * gk - (size_t)NU * ke.inst;  // x_k, u_k, x_{k+1}
+  const bool dyn = ke.has_next;
+  // the cost state: x_k, or at the last knot x_{k-1} unless its own state is asked for
+  const float *xc = (dyn || (p.flags & FLAG_LAST_KNOT_OWN_STATE)) ? xk : xk - NXU;
   for (int i = lane; i < NX; i += GROUP) {
-    L.x[i] = xi[(size_t)NXU * kr.k + i];
-    L.xc[i] = xi[(size_t)NXU * kc + i];
-    L.xn[i] = dyn ? xi[(size_t)NXU * (kr.k + 1) + i] : 0.f;
+    L.x[i] = xk[i];
+    L.xc[i] = xc[i];
+    L.xn[i] = dyn ? xk[NXU + i] : 0.f;
   }
-  for (int i = lane; i < NU; i += GROUP) L.u[i] = dyn ? xi[(size_t)NXU * kr.k + NX + i] : 0.f;
-  if (lane < 3) L.ref[lane] = ee_ref[(size_t)6 * (kr.K0 + kr.k) + lane];
+  for (int i = lane; i < NU; i += GROUP) L.u[i] = dyn ? xk[NX + i] : 0.f;
+  if (lane < 3) L.ref[lane] = ee_ref[(size_t)6 * gk + lane];
   __syncthreads();
 
   // dynamics linearisation (every group runs it so barriers line up; the
@@ -362,7 +363,7 @@ __global__ void __launch_bounds__(KKT_WG) k_form_kkt(Batch b, Params p, const fl
   __syncthreads();
   if (!valid) return;
 
-  const size_t knot = (size_t)kr.K0 + kr.k;
+  const size_t knot = gk;
   float *Gk = G + (size_t)G_STRIDE * knot;
   for (int i = lane; i < NXX; i += GROUP) {
     const int r = i % NX, col = i / NX;
@@ -372,7 +373,7 @@ __global__ void __launch_bounds__(KKT_WG) k_form_kkt(Batch b, Params p, const fl
   float *gk_ = g + (size_t)NXU * knot;
   if (lane < NX) gk_[lane] = (lane < NQ ? L.gq[lane] : 0.f) + p.wx[lane] * L.xc[lane];
   if (gq_out && lane < NQ) gq_out[(size_t)NQ * knot + lane] = L.gq[lane];  // Q = gq gq^T + diag(wx)
-  if (kr.k == 0 && lane < NX) cvec[(size_t)NX * kr.K0 + lane] = L.x[lane] - xs[(size_t)NX * kr.inst + lane];
+  if (ke.first && lane < NX) cvec[(size_t)NX * gk + lane] = L.x[lane] - xs[(size_t)NX * ke.inst + lane];
   if (!dyn) return;
   for (int i = lane; i < 49; i += GROUP) Gk[NXX + i] = (i % NU == i / NU) ? p.wu[i % NU] : 0.f;
   if (lane < NU) gk_[NX + lane] = p.wu[lane] * L.u[lane];
@@ -425,18 +426,18 @@ __global__ void __launch_bounds__(MERIT_NT) k_merit(Batch b, Params p, const flo
   const int pidx = p0 + (int)slot;
   const float alpha = (pidx >= 8) ? 0.f : -1.f / (float)(1u << pidx);
   const bool with_x0 = pidx < 8;
-  const KnotRef kr = locate(b, gk);
-  const uint32_t o = NXU * (kr.K0 + kr.k) - NU * kr.inst;  // x_k in the batch xu
+  const KnotEnds ke = knot_ends(b, gk);
+  const uint32_t o = NXU * gk - NU * ke.inst;  // x_k in the batch xu
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)o);
   const uint32_t n_xu = NXU * T - NU * b.n_inst;
   const uint32_t n_stage = n_xu - lo < (uint32_t)MERIT_SPAN ? n_xu - lo : (uint32_t)MERIT_SPAN;
   for (uint32_t j = lane; j < n_stage; j += MERIT_NT) Z[j] = dz ? xu[lo + j] + alpha * dz[lo + j] : xu[lo + j];
   __syncthreads();
   if (!valid) return;
-  const bool active = !(status && !(status[kr.inst] & ST_ACTIVE));
+  const bool active = !(status && !(status[ke.inst] & ST_ACTIVE));
   if (!active) return;
   const float *z = Z + (o - lo);
-  const bool dyn = kr.k + 1 < kr.N;
+  const bool dyn = ke.has_next;
   float s[NQ], c[NQ], ee[3];
 #pragma unroll
   for (int i = 0; i < NQ; ++i) sincosf(z[i], &s[i], &c[i]);
@@ -477,8 +478,9 @@ __global__ void __launch_bounds__(MERIT_NT) k_merit(Batch b, Params p, const flo
     }
   } else if (with_x0) {
     // the terminal slot also carries the initial-state constraint of the instance
-    const size_t o0 = (size_t)NXU * kr.K0 - (size_t)NU * kr.inst;
-    const float *x0s = xs + (size_t)NX * kr.inst;
+    const size_t k0 = b.knot_inst ? b.inst_k0[ke.inst] : 0u;  // the instance's first knot
+    const size_t o0 = (size_t)NXU * k0 - (size_t)NU * ke.inst;
+    const float *x0s = xs + (size_t)NX * ke.inst;
     for (int i = 0; i < NX; ++i) {
       const float z0 = dz ? xu[o0 + i] + alpha * dz[o0 + i] : xu[o0 + i];
       ck += fabsf(z0 - x0s[i]);

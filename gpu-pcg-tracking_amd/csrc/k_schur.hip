@@ -379,9 +379,7 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   // The reads of the block row are issued before the instance lookup: the
   // previous knot's C, g and g_q (for a first knot another instance's last
   // knot, whose C and r are not written: replaced by zeros below, never
-  // multiplied) and this knot's g and g_q.  A first knot is one whose
-  // predecessor belongs to another instance (one lookup, no dependent
-  // second read).
+  // multiplied) and this knot's g and g_q.
   const size_t kprev = row > 0 ? row - 1 : 0u;
   const float *Ck = C + (size_t)C_STRIDE * kprev;
   constexpr int CA = (NXX + GROUP - 1) / GROUP, CB = (98 + GROUP - 1) / GROUP;
@@ -390,12 +388,9 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   for (int t = 0; t < CA; ++t) ca[t] = lane + GROUP * t < NXX ? Ck[lane + GROUP * t] : 0.f;
 #pragma unroll
   for (int t = 0; t < CB; ++t) cb[t] = lane + GROUP * t < 98 ? Ck[NXX + lane + GROUP * t] : 0.f;
-  uint32_t inst = 0;
-  bool row0 = row == 0;
-  if (b.knot_inst) {
-    inst = b.knot_inst[row];
-    row0 = row0 || b.knot_inst[kprev] != inst;
-  }
+  const KnotEnds ke = knot_ends(b, row);
+  const uint32_t inst = ke.inst;
+  const bool row0 = ke.first;
   const bool owner = grp >= 1 && in_range && !(status && !(status[inst] & ST_ACTIVE));
   SplitRowLds &L = lds[grp];
 #pragma unroll
@@ -549,11 +544,11 @@ __global__ void __launch_bounds__(WG) k_compute_dz(Batch b, const float *__restr
   const uint32_t gk_raw = blockIdx.x * GROUPS_PER_WG + grp;
   bool valid = gk_raw < b.total_knots;
   const uint32_t gk = valid ? gk_raw : b.total_knots - 1;
-  const KnotRef kr = locate(b, gk);
-  if (status && !(status[kr.inst] & ST_ACTIVE)) valid = false;
+  const KnotEnds ke = knot_ends(b, gk);
+  if (status && !(status[ke.inst] & ST_ACTIVE)) valid = false;
   DzLds &L = lds[grp];
-  const size_t knot = (size_t)kr.K0 + kr.k;
-  const bool has_next = kr.k + 1 < kr.N;
+  const size_t knot = gk;
+  const bool has_next = ke.has_next;
   const float *Ck = C + (size_t)C_STRIDE * knot;
   if (lane < NX) L.lam1[lane] = has_next ? lambda[(size_t)NX * (knot + 1) + lane] : 0.f;
   __syncthreads();
@@ -574,7 +569,7 @@ __global__ void __launch_bounds__(WG) k_compute_dz(Batch b, const float *__restr
   }
   __syncthreads();
   if (!valid) return;
-  float *dzi = dz + (size_t)NXU * kr.K0 - (size_t)NU * kr.inst + (size_t)NXU * kr.k;
+  float *dzi = dz + (size_t)NXU * gk - (size_t)NU * ke.inst;
   const float *Gi = Ginv + (size_t)G_STRIDE * knot;
   if (lane < NX) {
     float acc = 0.f;
@@ -604,11 +599,11 @@ __global__ void __launch_bounds__(WG) k_compute_dz_lr(Batch b, Params p, const f
   const uint32_t gk_raw = blockIdx.x * GROUPS_PER_WG + grp;
   bool valid = gk_raw < b.total_knots;
   const uint32_t gk = valid ? gk_raw : b.total_knots - 1;
-  const KnotRef kr = locate(b, gk);
-  if (status && !(status[kr.inst] & ST_ACTIVE)) valid = false;
+  const KnotEnds ke = knot_ends(b, gk);
+  if (status && !(status[ke.inst] & ST_ACTIVE)) valid = false;
   DzLds &L = lds[grp];
-  const size_t knot = (size_t)kr.K0 + kr.k;
-  const bool has_next = kr.k + 1 < kr.N;
+  const size_t knot = gk;
+  const bool has_next = ke.has_next;
   const float *Ck = C + (size_t)C_STRIDE * knot;
   if (lane < NX) L.lam1[lane] = has_next ? lambda[(size_t)NX * (knot + 1) + lane] : 0.f;
   __syncthreads();
@@ -629,8 +624,8 @@ __global__ void __launch_bounds__(WG) k_compute_dz_lr(Batch b, Params p, const f
   }
   __syncthreads();
   if (!valid) return;
-  const float rho = rho_dev[kr.inst];
-  float *dzi = dz + (size_t)NXU * kr.K0 - (size_t)NU * kr.inst + (size_t)NXU * kr.k;
+  const float rho = rho_dev[ke.inst];
+  float *dzi = dz + (size_t)NXU * gk - (size_t)NU * ke.inst;
   if (lane < NX) {
     // Q~^-1 t = D^-1 t - u (u^T t) / s, the cancelling terms in fp64 (as
     // k_form_split)
