@@ -246,40 +246,54 @@ __global__ void __launch_bounds__(KKT_WG) k_form_kkt(Batch b, Params p, const fl
   __syncthreads();
 
   // dynamics linearisation (every group runs it so barriers line up; the
-  // terminal knot's results are discarded), and sin / cos of the cost state
+  // terminal knot's results are discarded), and sin / cos of the cost state.
+  // The phases that need at most eight lanes per knot run two knots to a
+  // group: groups 0..7 (waves 0 and 1) serve knot 2g + h on lanes 8h ..
+  // 8h + 7 while waves 2 and 3 wait, half the instructions of one knot per
+  // group.
   const float *q = L.x, *qd = L.x + NQ;
-  if (lane < NQ) {
+  const bool packed = grp < KKT_GROUPS / 2;
+  const int pl = lane & 7;
+  KktLds &P = lds[packed ? 2 * grp + (lane >> 3) : grp];
+  if (packed && pl < NQ) {
     float sv, cv;
-    sincosf(q[lane], &sv, &cv);
-    D.sn[lane] = sv;
-    D.cs[lane] = cv;
+    sincosf(P.x[pl], &sv, &cv);
+    P.dyn.sn[pl] = sv;
+    P.dyn.cs[pl] = cv;
+  }
+  // the cost state is x_k except at an instance's last knot (x_{k-1} there
+  // unless FLAG_LAST_KNOT_OWN_STATE): its sin / cos are copied below
+  const bool cost_own = dyn || (p.flags & FLAG_LAST_KNOT_OWN_STATE);
+  if (!cost_own && lane < NQ) {
+    float sv, cv;
     sincosf(L.xc[lane], &sv, &cv);
     L.csn[lane] = sv;
     L.ccs[lane] = cv;
   }
   __syncthreads();
-  float s[NQ], c[NQ];
-#pragma unroll
-  for (int i = 0; i < NQ; ++i) {
-    s[i] = D.sn[i];
-    c[i] = D.cs[i];
+  if (cost_own && lane < NQ) {
+    L.csn[lane] = D.sn[lane];
+    L.ccs[lane] = D.cs[lane];
   }
-  // one RNEA on lanes 0..7 with lane-dependent inputs: lane j < 7: (qd, qdd)
-  // = (0, e_j) gives column j of M; lane 7: (qd, 0) the bias forces c(q, qd)
-  if (lane <= NQ) {
-    float qdv[NQ], qddv[NQ], tau[NQ];
+  // one RNEA on lanes 0..7 of a knot with lane-dependent inputs: lane j < 7:
+  // (qd, qdd) = (0, e_j) gives column j of M; lane 7: (qd, 0) the bias
+  // forces c(q, qd)
+  if (packed) {
+    float ps[NQ], pc[NQ], qdv[NQ], qddv[NQ], tau[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
-      qdv[i] = lane == NQ ? qd[i] : 0.f;
-      qddv[i] = i == lane ? 1.f : 0.f;
+      ps[i] = P.dyn.sn[i];
+      pc[i] = P.dyn.cs[i];
+      qdv[i] = pl == NQ ? P.x[NQ + i] : 0.f;
+      qddv[i] = i == pl ? 1.f : 0.f;
     }
-    rnea<false>(s, c, qdv, qddv, tau, nullptr, nullptr, nullptr, nullptr, nullptr);
-    if (lane < NQ) {
+    rnea<false>(ps, pc, qdv, qddv, tau, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (pl < NQ) {
 #pragma unroll
-      for (int r = 0; r < NQ; ++r) D.M[r + 7 * lane] = tau[r];
+      for (int r = 0; r < NQ; ++r) P.dyn.M[r + 7 * pl] = tau[r];
     } else {
 #pragma unroll
-      for (int i = 0; i < NQ; ++i) D.bias[i] = tau[i];
+      for (int i = 0; i < NQ; ++i) P.dyn.bias[i] = tau[i];
     }
   }
   __syncthreads();
@@ -299,16 +313,17 @@ __global__ void __launch_bounds__(KKT_WG) k_form_kkt(Batch b, Params p, const fl
     K.ee[2] = K.org[20];
   }
   __syncthreads();
-  if (lane < NQ) {
+  // M^-1 by columns, two knots to a group as above
+  if (packed && pl < NQ) {
     float Lf[49];
 #pragma unroll
-    for (int i = 0; i < 49; ++i) Lf[i] = D.Lc[i];
+    for (int i = 0; i < 49; ++i) Lf[i] = P.dyn.Lc[i];
     float e[NQ], x[NQ];
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) e[i] = (i == lane) ? 1.f : 0.f;
+    for (int i = 0; i < NQ; ++i) e[i] = (i == pl) ? 1.f : 0.f;
     chol_solve7(Lf, e, x);
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) D.Minv[i + 7 * lane] = x[i];
+    for (int i = 0; i < NQ; ++i) P.dyn.Minv[i + 7 * pl] = x[i];
   }
   __syncthreads();
   if (lane < NQ) {
@@ -334,9 +349,13 @@ __global__ void __launch_bounds__(KKT_WG) k_form_kkt(Batch b, Params p, const fl
   __syncthreads();
   float dtau[NQ];
   if (lane < NX) {
-    float qdv[NQ];
+    float s[NQ], c[NQ], qdv[NQ];
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) qdv[i] = qd[i];
+    for (int i = 0; i < NQ; ++i) {
+      s[i] = D.sn[i];
+      c[i] = D.cs[i];
+      qdv[i] = qd[i];
+    }
     rnea_dir(lane, s, c, qdv, D.bv, D.bf, D.bxv, D.bxa, dtau);
   }
   __syncthreads();  // every lane is done reading the base point
