@@ -172,10 +172,31 @@ hipError_t launch_fk(uint32_t n, const float *q, uint32_t q_stride, float *ee, h
 hipError_t launch_dyn_grad(uint32_t n, const float *qqdu, float *qdd, float *df_du, hipStream_t st);
 hipError_t launch_inverse_dynamics(uint32_t n, const float *qqdqdd, float *tau, hipStream_t st);
 // gq (optional, NQ floats per knot): the tracking-cost gradient g_q of each
-// knot, the factor of the cost Hessian Q = g_q g_q^T + diag(wx)
+// knot, the factor of the cost Hessian Q = g_q g_q^T + diag(wx).
+// Cl (optional, with gq): the compact form instead of G and C (neither is
+// written): per knot CL_STRIDE floats, the lower block rows (rows NQ..NX-1)
+// of C's A part column-major (CL_A), then of its B part (CL_B).  The upper
+// rows are fixed by the Euler step: C_A(r, c) = -(delta_rc + dt delta_{c,r+NQ})
+// and C_B(r, c) = 0 for r < NQ (cl_upper_a); G follows from gq and the
+// weights.  launch_expand_gc writes G and C back from it.
+constexpr int CL_A = NQ * NX;          // 98
+constexpr int CL_B = NQ * NU;          // 49
+constexpr int CL_STRIDE = 148;         // CL_A + CL_B, padded to 16 bytes
+static_assert(CL_A + CL_B <= CL_STRIDE && CL_STRIDE % 4 == 0, "compact C rows");
+// C_A(r, c) for r < NQ: the negated Euler-step rows, exactly as
+// launch_form_kkt computes them (v = delta + dt, stored as -v)
+__host__ __device__ __forceinline__ float cl_upper_a(int r, int c, float dt) {
+  float v = (r == c) ? 1.f : 0.f;
+  if (c == r + NQ) v += dt;
+  return -v;
+}
 hipError_t launch_form_kkt(const Batch &b, const Params &p, const float *xu, const float *ee_ref,
                            const float *xs, const uint32_t *status, float *G, float *C, float *g,
-                           float *c, hipStream_t st, float *gq = nullptr);
+                           float *c, hipStream_t st, float *gq = nullptr, float *Cl = nullptr);
+// G (cost blocks) and C (dynamics blocks) of every knot from the compact
+// form of launch_form_kkt (gq, Cl), bitwise as launch_form_kkt writes them
+hipError_t launch_expand_gc(const Batch &b, const Params &p, const float *gq, const float *Cl, float *G, float *C,
+                            hipStream_t st);
 // mode: 0..7 line-search alphas (-2^-p, with the initial-state term); 8 the
 // current iterate (alpha 0, without it).  Writes out[(p - p0) * total + gk].
 hipError_t launch_merit(const Batch &b, const Params &p, const float *xu, const float *dz,
@@ -197,13 +218,15 @@ hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, con
 // and W_b into the rows of Ps and gamma^ into ghat with the Schur block
 // rows (S, Pinv, gamma and G~^-1 are not formed); each active instance's
 // rho into rho_used (optional).
-hipError_t launch_form_split(const Batch &b, const Params &p, const float *C, const float *g, const float *c,
+// The dynamics blocks come in the compact form Cl of launch_form_kkt.
+hipError_t launch_form_split(const Batch &b, const Params &p, const float *Cl, const float *g, const float *c,
                              const float *gq, const float *rho_dev, float rho, const uint32_t *status, float *Ps,
                              float *ghat, float *rho_used, hipStream_t st);
 // dz as launch_compute_dz, with G~^-1 in the closed form of launch_form_split
-// (rho_dev: one rho per instance, required, as gq)
+// (rho_dev: one rho per instance, required, as gq) and the dynamics blocks
+// in the compact form Cl of launch_form_kkt
 hipError_t launch_compute_dz_lr(const Batch &b, const Params &p, const float *gq, const float *rho_dev,
-                                const float *C, const float *g, const float *lambda, const uint32_t *status,
+                                const float *Cl, const float *g, const float *lambda, const uint32_t *status,
                                 float *dz, hipStream_t st);
 hipError_t launch_compute_dz(const Batch &b, const float *Ginv, const float *C, const float *g,
                              const float *lambda, const uint32_t *status, float *dz, hipStream_t st);

@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(KKT_WG) k_form_kkt(Batch b, Params p, const fl
                                                      const uint32_t *__restrict__ status,
                                                      float *__restrict__ G, float *__restrict__ C,
                                                      float *__restrict__ g, float *__restrict__ cvec,
-                                                     float *__restrict__ gq_out) {
+                                                     float *__restrict__ gq_out, float *__restrict__ Cl) {
   __shared__ KktLds lds[KKT_GROUPS];
   const int tid = threadIdx.x, grp = tid / GROUP, lane = tid % GROUP, w = tid >> 6, l64 = tid & 63;
   const uint32_t gk_raw = blockIdx.x * KKT_GROUPS + grp;
@@ -383,31 +383,46 @@ __global__ void __launch_bounds__(KKT_WG) k_form_kkt(Batch b, Params p, const fl
   if (!valid) return;
 
   const size_t knot = gk;
+  const bool compact = Cl != nullptr;  // G and C in the compact form (gq, Cl)
   float *Gk = G + (size_t)G_STRIDE * knot;
-  for (int i = lane; i < NXX; i += GROUP) {
-    const int r = i % NX, col = i / NX;
-    const float w = r == col ? p.wx[r] : 0.f;
-    Gk[i] = (r < NQ && col < NQ) ? L.gq[r] * L.gq[col] + w : w;
-  }
+  if (!compact)
+    for (int i = lane; i < NXX; i += GROUP) {
+      const int r = i % NX, col = i / NX;
+      const float w = r == col ? p.wx[r] : 0.f;
+      Gk[i] = (r < NQ && col < NQ) ? L.gq[r] * L.gq[col] + w : w;
+    }
   float *gk_ = g + (size_t)NXU * knot;
   if (lane < NX) gk_[lane] = (lane < NQ ? L.gq[lane] : 0.f) + p.wx[lane] * L.xc[lane];
   if (gq_out && lane < NQ) gq_out[(size_t)NQ * knot + lane] = L.gq[lane];  // Q = gq gq^T + diag(wx)
   if (ke.first && lane < NX) cvec[(size_t)NX * gk + lane] = L.x[lane] - xs[(size_t)NX * ke.inst + lane];
   if (!dyn) return;
-  for (int i = lane; i < 49; i += GROUP) Gk[NXX + i] = (i % NU == i / NU) ? p.wu[i % NU] : 0.f;
+  if (!compact)
+    for (int i = lane; i < 49; i += GROUP) Gk[NXX + i] = (i % NU == i / NU) ? p.wu[i % NU] : 0.f;
   if (lane < NU) gk_[NX + lane] = p.wu[lane] * L.u[lane];
-  float *Ck = C + (size_t)C_STRIDE * knot;
   const float dt = p.dt;
-  for (int i = lane; i < NXX; i += GROUP) {
-    const int r = i % NX, col = i / NX;
-    float v = (r == col) ? 1.f : 0.f;
-    if (r < NQ && col == r + NQ) v += dt;
-    if (r >= NQ) v += dt * L.dyn.df[(r - NQ) + NQ * col];
-    Ck[i] = -v;
-  }
-  for (int i = lane; i < NX * NU; i += GROUP) {
-    const int r = i % NX, col = i / NX;
-    Ck[NXX + i] = r >= NQ ? -dt * L.dyn.Minv[(r - NQ) + NQ * col] : 0.f;
+  if (compact) {
+    // the lower block rows only (the upper ones are cl_upper_a and zero)
+    float *Clk = Cl + (size_t)CL_STRIDE * knot;
+    for (int i = lane; i < CL_A; i += GROUP) {
+      const int r = NQ + i % NQ, col = i / NQ;
+      float v = (r == col) ? 1.f : 0.f;
+      v += dt * L.dyn.df[(r - NQ) + NQ * col];
+      Clk[i] = -v;
+    }
+    for (int i = lane; i < CL_B; i += GROUP) Clk[CL_A + i] = -dt * L.dyn.Minv[i];
+  } else {
+    float *Ck = C + (size_t)C_STRIDE * knot;
+    for (int i = lane; i < NXX; i += GROUP) {
+      const int r = i % NX, col = i / NX;
+      float v = (r == col) ? 1.f : 0.f;
+      if (r < NQ && col == r + NQ) v += dt;
+      if (r >= NQ) v += dt * L.dyn.df[(r - NQ) + NQ * col];
+      Ck[i] = -v;
+    }
+    for (int i = lane; i < NX * NU; i += GROUP) {
+      const int r = i % NX, col = i / NX;
+      Ck[NXX + i] = r >= NQ ? -dt * L.dyn.Minv[(r - NQ) + NQ * col] : 0.f;
+    }
   }
   if (lane < NX) {
     float v;
@@ -531,10 +546,51 @@ hipError_t launch_dyn_grad(uint32_t n, const float *qqdu, float *qdd, float *df_
 
 hipError_t launch_form_kkt(const Batch &b, const Params &p, const float *xu, const float *ee_ref,
                            const float *xs, const uint32_t *status, float *G, float *C, float *g,
-                           float *c, hipStream_t st, float *gq) {
+                           float *c, hipStream_t st, float *gq, float *Cl) {
   if (b.total_knots == 0) return hipSuccess;
+  if (Cl && !gq) return hipErrorInvalidValue;  // the compact form needs gq for G
   hipLaunchKernelGGL(k_form_kkt, dim3((b.total_knots + KKT_GROUPS - 1) / KKT_GROUPS), dim3(KKT_WG), 0, st, b, p,
-                     xu, ee_ref, xs, status, G, C, g, c, gq);
+                     xu, ee_ref, xs, status, G, C, g, c, gq, Cl);
+  return hipGetLastError();
+}
+
+// G and C of every knot from gq and the compact dynamics rows, with the
+// expressions of k_form_kkt; one 16-lane group per knot
+__global__ void __launch_bounds__(WG) k_expand_gc(Batch b, Params p, const float *__restrict__ gq,
+                                                  const float *__restrict__ Cl, float *__restrict__ G,
+                                                  float *__restrict__ C) {
+  const int grp = threadIdx.x / GROUP, lane = threadIdx.x % GROUP;
+  const uint32_t gk = blockIdx.x * GROUPS_PER_WG + grp;
+  if (gk >= b.total_knots) return;
+  const KnotEnds ke = knot_ends(b, gk);
+  const size_t knot = gk;
+  const float *gqk = gq + (size_t)NQ * knot;
+  float *Gk = G + (size_t)G_STRIDE * knot;
+  for (int i = lane; i < NXX; i += GROUP) {
+    const int r = i % NX, col = i / NX;
+    const float w = r == col ? p.wx[r] : 0.f;
+    Gk[i] = (r < NQ && col < NQ) ? gqk[r] * gqk[col] + w : w;
+  }
+  if (!ke.has_next) return;
+  for (int i = lane; i < 49; i += GROUP) Gk[NXX + i] = (i % NU == i / NU) ? p.wu[i % NU] : 0.f;
+  const float *Clk = Cl + (size_t)CL_STRIDE * knot;
+  float *Ck = C + (size_t)C_STRIDE * knot;
+  for (int i = lane; i < NXX; i += GROUP) {
+    const int r = i % NX, col = i / NX;
+    Ck[i] = r < NQ ? cl_upper_a(r, col, p.dt) : Clk[(r - NQ) + NQ * col];
+  }
+  for (int i = lane; i < NX * NU; i += GROUP) {
+    const int r = i % NX, col = i / NX;
+    Ck[NXX + i] = r < NQ ? 0.f : Clk[CL_A + (r - NQ) + NQ * col];
+  }
+}
+
+hipError_t launch_expand_gc(const Batch &b, const Params &p, const float *gq, const float *Cl, float *G, float *C,
+                            hipStream_t st) {
+  if (b.total_knots == 0) return hipSuccess;
+  if (!gq || !Cl || !G || !C) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_expand_gc, dim3((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG), 0, st, b, p,
+                     gq, Cl, G, C);
   return hipGetLastError();
 }
 

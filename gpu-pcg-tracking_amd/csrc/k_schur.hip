@@ -365,7 +365,7 @@ __device__ __forceinline__ void nb_product(float (&d)[NX], const float (&s)[NX],
 }
 
 __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k_form_split(Batch b, Params p,
-                                                    const float *__restrict__ C,
+                                                    const float *__restrict__ Cl,
                                                     const float *__restrict__ g, const float *__restrict__ cvec,
                                                     const float *__restrict__ gq,
                                                     const float *__restrict__ rho_dev, float rho_s,
@@ -381,24 +381,28 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   // knot, whose C and r are not written: replaced by zeros below, never
   // multiplied) and this knot's g and g_q.
   const size_t kprev = row > 0 ? row - 1 : 0u;
-  const float *Ck = C + (size_t)C_STRIDE * kprev;
-  constexpr int CA = (NXX + GROUP - 1) / GROUP, CB = (98 + GROUP - 1) / GROUP;
-  float ca[CA], cb[CB];
+  // the previous knot's lower block rows (compact form); its upper rows are
+  // the Euler step's (cl_upper_a, and zero in B)
+  const float *Clk = Cl + (size_t)CL_STRIDE * kprev;
+  constexpr int CN = (CL_A + CL_B + GROUP - 1) / GROUP;
+  float cl[CN];
 #pragma unroll
-  for (int t = 0; t < CA; ++t) ca[t] = lane + GROUP * t < NXX ? Ck[lane + GROUP * t] : 0.f;
-#pragma unroll
-  for (int t = 0; t < CB; ++t) cb[t] = lane + GROUP * t < 98 ? Ck[NXX + lane + GROUP * t] : 0.f;
+  for (int t = 0; t < CN; ++t) cl[t] = lane + GROUP * t < CL_A + CL_B ? Clk[lane + GROUP * t] : 0.f;
   const KnotEnds ke = knot_ends(b, row);
   const uint32_t inst = ke.inst;
   const bool row0 = ke.first;
   const bool owner = grp >= 1 && in_range && !(status && !(status[inst] & ST_ACTIVE));
   SplitRowLds &L = lds[grp];
+  // A = -C_A and B = -C_B staged column-major, as negations of the values
+  // launch_form_kkt stores (zeros for a first knot)
 #pragma unroll
-  for (int t = 0; t < CA; ++t)
-    if (lane + GROUP * t < NXX) L.A[lane + GROUP * t] = row0 ? 0.f : -ca[t];
-#pragma unroll
-  for (int t = 0; t < CB; ++t)
-    if (lane + GROUP * t < 98) L.B[lane + GROUP * t] = row0 ? 0.f : -cb[t];
+  for (int t = 0; t < CN; ++t) {
+    const int c = lane + GROUP * t;
+    if (c < CL_A) L.A[NQ + c % NQ + NX * (c / NQ)] = row0 ? 0.f : -cl[t];
+    else if (c < CL_A + CL_B) L.B[NQ + (c - CL_A) % NQ + NX * ((c - CL_A) / NQ)] = row0 ? 0.f : -cl[t];
+  }
+  for (int c = lane; c < NQ * NX; c += GROUP) L.A[c % NQ + NX * (c / NQ)] = row0 ? 0.f : -cl_upper_a(c % NQ, c / NQ, p.dt);
+  for (int c = lane; c < NQ * NU; c += GROUP) L.B[c % NQ + NX * (c / NQ)] = row0 ? 0.f : -0.f;
   const int j = lane < NX ? lane : NX - 1;
   // q_{b-1}, r_{b-1}, q_b: read by every lane of the group (cached)
   const float *qk = g + (size_t)NXU * kprev, *rk = qk + NX, *q1v = g + (size_t)NXU * row;
@@ -589,7 +593,7 @@ __global__ void __launch_bounds__(WG) k_compute_dz(Batch b, const float *__restr
 // k_form_split (no G~^-1)
 __global__ void __launch_bounds__(WG) k_compute_dz_lr(Batch b, Params p, const float *__restrict__ gq,
                                                       const float *__restrict__ rho_dev,
-                                                      const float *__restrict__ C,
+                                                      const float *__restrict__ Cl,
                                                       const float *__restrict__ g,
                                                       const float *__restrict__ lambda,
                                                       const uint32_t *__restrict__ status,
@@ -604,7 +608,9 @@ __global__ void __launch_bounds__(WG) k_compute_dz_lr(Batch b, Params p, const f
   DzLds &L = lds[grp];
   const size_t knot = gk;
   const bool has_next = ke.has_next;
-  const float *Ck = C + (size_t)C_STRIDE * knot;
+  // the lower block rows of C (compact form); the upper ones are the Euler
+  // step's, cl_upper_a in A and zero in B
+  const float *Clk = Cl + (size_t)CL_STRIDE * knot;
   if (lane < NX) L.lam1[lane] = has_next ? lambda[(size_t)NX * (knot + 1) + lane] : 0.f;
   __syncthreads();
   if (lane < NX) {
@@ -612,14 +618,15 @@ __global__ void __launch_bounds__(WG) k_compute_dz_lr(Batch b, Params p, const f
     float s = 0.f;
     if (has_next) {
 #pragma unroll
-      for (int l = 0; l < NX; ++l) s -= Ck[l + NX * lane] * L.lam1[l];
+      for (int l = 0; l < NX; ++l)
+        s -= (l < NQ ? cl_upper_a(l, lane, p.dt) : Clk[(l - NQ) + NQ * lane]) * L.lam1[l];
     }
     L.t[lane] = g[(size_t)NXU * knot + lane] - lambda[(size_t)NX * knot + lane] + s;
   }
   if (has_next && lane < NU) {
     float s = 0.f;
 #pragma unroll
-    for (int l = 0; l < NX; ++l) s -= Ck[NXX + l + NX * lane] * L.lam1[l];
+    for (int l = 0; l < NX; ++l) s -= (l < NQ ? 0.f : Clk[CL_A + (l - NQ) + NQ * lane]) * L.lam1[l];
     L.tu[lane] = g[(size_t)NXU * knot + NX + lane] + s;
   }
   __syncthreads();
@@ -659,23 +666,23 @@ hipError_t launch_form_schur(const Batch &b, const float *G, const float *C, con
   return hipGetLastError();
 }
 
-hipError_t launch_form_split(const Batch &b, const Params &p, const float *C, const float *g, const float *c,
+hipError_t launch_form_split(const Batch &b, const Params &p, const float *Cl, const float *g, const float *c,
                              const float *gq, const float *rho_dev, float rho, const uint32_t *status, float *Ps,
                              float *ghat, float *rho_used, hipStream_t st) {
   if (b.total_knots == 0) return hipSuccess;
-  if (!gq) return hipErrorInvalidValue;  // g_q of launch_form_kkt is read by every block row
-  hipLaunchKernelGGL(k_form_split, dim3((b.total_knots + SSTEP - 1) / SSTEP), dim3(SWG), 0, st, b, p, C, g, c, gq,
+  if (!gq || !Cl) return hipErrorInvalidValue;  // g_q and the compact rows of launch_form_kkt
+  hipLaunchKernelGGL(k_form_split, dim3((b.total_knots + SSTEP - 1) / SSTEP), dim3(SWG), 0, st, b, p, Cl, g, c, gq,
                      rho_dev, rho, status, Ps, ghat, rho_used);
   return hipGetLastError();
 }
 
 hipError_t launch_compute_dz_lr(const Batch &b, const Params &p, const float *gq, const float *rho_dev,
-                                const float *C, const float *g, const float *lambda, const uint32_t *status,
+                                const float *Cl, const float *g, const float *lambda, const uint32_t *status,
                                 float *dz, hipStream_t st) {
   if (b.total_knots == 0) return hipSuccess;
-  if (!rho_dev || !gq) return hipErrorInvalidValue;  // per-instance rho and g_q are read by every knot
+  if (!rho_dev || !gq || !Cl) return hipErrorInvalidValue;  // per-instance rho, g_q and the compact rows
   hipLaunchKernelGGL(k_compute_dz_lr, dim3((b.total_knots + GROUPS_PER_WG - 1) / GROUPS_PER_WG), dim3(WG), 0, st, b,
-                     p, gq, rho_dev, C, g, lambda, status, dz);
+                     p, gq, rho_dev, Cl, g, lambda, status, dz);
   return hipGetLastError();
 }
 

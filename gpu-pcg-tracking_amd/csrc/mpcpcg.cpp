@@ -39,6 +39,7 @@ struct mpc_handle {
   // workspace, sized for cfg.max_total_knots (T) and cfg.max_instances (B)
   float *G = nullptr, *Ginv = nullptr, *C = nullptr, *g = nullptr, *c = nullptr;
   float *gq = nullptr;  // [NQ T] tracking-cost gradient per knot (Q = gq gq^T + diag(wx))
+  float *Cl = nullptr;  // [CL_STRIDE T] compact dynamics rows of the split path (G, C not written)
   float *S = nullptr, *P = nullptr, *gamma = nullptr, *dz = nullptr;
   float *knot_merits = nullptr;  // [9 T]
   float *merits = nullptr;       // [9 B]
@@ -65,6 +66,7 @@ struct mpc_handle {
   // formation is kept so that mpc_batch_direct_resolve can form S again
   float *rho_used = nullptr;
   bool schur_formed = true;  // the last batched solve left S and gamma in the workspace
+  bool gc_compact = false;   // its linearisation left G and C in the compact form (gq, Cl)
   // PCG teams: two workgroups for 128 < N <= 256, four for 256 < N <= 512
   uint32_t *team_list = nullptr, *team_one = nullptr, *team_err = nullptr;
   uint32_t *pcg_done = nullptr;       // [B] parts that finished each team solve
@@ -276,7 +278,7 @@ const char *mpc_status_string(int s) {
 }
 
 static void free_all(mpc_handle *h) {
-  void *ptrs[] = {h->G, h->Ginv, h->gq, h->C, h->g, h->c, h->S, h->P, h->gamma, h->dz, h->knot_merits,
+  void *ptrs[] = {h->G, h->Ginv, h->gq, h->Cl, h->C, h->g, h->c, h->S, h->P, h->gamma, h->dz, h->knot_merits,
                   h->merits, h->xs, h->drho, h->merit0, h->rho1, h->status_ws, h->sqp_done,
                   h->pcg_iters, h->pcg_flag, h->pcg_eta, h->t0, h->knot_inst, h->inst_k0, h->bin_items,
                   h->bin_off, h->big_list, h->team_list, h->team_one, h->team_err, h->mbox, h->dir_order,
@@ -321,6 +323,7 @@ int mpc_create(const mpc_config *cfg, mpc_handle **out) {
   bool ok = alloc((void **)&h->G, sizeof(float) * G_STRIDE * T) &&
             alloc((void **)&h->Ginv, sizeof(float) * G_STRIDE * T) &&
             alloc((void **)&h->gq, sizeof(float) * NQ * T) &&
+            alloc((void **)&h->Cl, sizeof(float) * CL_STRIDE * T) &&
             alloc((void **)&h->C, sizeof(float) * C_STRIDE * T) &&
             alloc((void **)&h->g, sizeof(float) * NXU * T) &&
             alloc((void **)&h->c, sizeof(float) * NX * T) &&
@@ -592,6 +595,7 @@ int mpc_sqp_solve(mpc_handle *h, uint32_t N, const float *d_ee_ref, float *d_lam
   bool time_exit = false;
   for (uint32_t it = 0; it < cfg.sqp_max_iter; ++it) {
     HIP_TRY(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, nullptr, h->G, h->C, h->g, h->c, st));
+    h->gc_compact = false;  // G and C written in full
     if ((time_exit = over_budget())) break;  // toplevel.cuh:271
     HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, h->c, nullptr, *rho, nullptr, h->S, h->P, h->gamma,
                               h->Ginv, 0, st));
@@ -846,6 +850,7 @@ int mpc_batch_qp_layer(mpc_handle *h, const float *d_xu, const float *d_ee_ref, 
   const float tol = pcg_tol > 0.f ? pcg_tol : h->cfg.pcg_exit_tol;
   // c_0 (x_0 - x_s) is not used by the Schur step, so x_s is left as it is
   HIP_TRY(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, nullptr, h->G, h->C, h->g, h->c, st));
+  h->gc_compact = false;  // G and C written in full
   HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, nullptr, h->S, h->P, h->gamma, h->Ginv, 0, st));
   HIP_TRY(pcg_batch(h, d_lambda, nullptr, h->cfg.pcg_max_iter, tol, nullptr, st));
   HIP_TRY(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, nullptr, d_dz, st));
@@ -879,6 +884,7 @@ int mpc_batch_qp_adjoint(mpc_handle *h, const float *d_xu, const float *d_ee_ref
   // recompute the linearisation at xu (nothing of the forward pass is kept
   // but dz), then the Schur system with g := -a and c := 0
   HIP_TRY(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, nullptr, h->G, h->C, h->g, h->c, st));
+  h->gc_compact = false;  // G and C written in full
   HIP_TRY(launch_pack_neg(b, d_a, h->g, st));
   HIP_TRY(launch_form_schur(b, h->G, h->C, h->g, nullptr, d_rho, 0.f, nullptr, h->S, h->P, h->gamma, h->Ginv, 0,
                             st));
@@ -1041,6 +1047,8 @@ int mpc_batch_pcg_exit(mpc_handle *h, uint32_t *d_iters, uint32_t *d_flags, floa
 int mpc_batch_direct_resolve(mpc_handle *h, float *d_lambda_out, uint32_t refine_steps, mpc_stream_t stream) {
   if (!h || !d_lambda_out || refine_steps > 16) return MPC_ERR_ARG;
   if (!h->batch_ready) return MPC_ERR_NOT_READY;
+  if (h->gc_compact)  // G and C of the last linearisation from its compact form
+    HIP_TRY(launch_expand_gc(batch_of(h), make_params(h), h->gq, h->Cl, h->G, h->C, (hipStream_t)stream));
   if (!h->schur_formed)  // the last solve was in split form: S and gamma at its last iterate and rho
     HIP_TRY(launch_form_schur(batch_of(h), h->G, h->C, h->g, h->c, h->rho_used, 0.f, nullptr, h->S, h->P, h->gamma,
                               h->Ginv, 0, (hipStream_t)stream));
@@ -1135,14 +1143,17 @@ static hipError_t batch_sqp_run(mpc_handle *h, float *d_xu, float *d_lambda, con
   MPC_RUN(launch_batch_init(b, d_xu, h->knot_merits, s, st));
   for (uint32_t it = 0; it < sqp_iters; ++it) {
     if (tm) MPC_RUN(hipEventRecord(h->ev[0], st));
-    MPC_RUN(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, d_status, h->G, h->C, h->g, h->c, st, h->gq));
-    if (tm) MPC_RUN(hipEventRecord(h->ev[1], st));
-    // split form: the split system with the Schur block rows (no S, no
-    // G~^-1: Q~^-1 and R~^-1 in closed form from gq)
+    // split form: G and C in the compact form (gq, Cl), the split system
+    // with the Schur block rows (no S, no G~^-1: Q~^-1 and R~^-1 in closed
+    // form from gq)
     const bool split = split_active(h);
     h->schur_formed = !split;
+    h->gc_compact = split;
+    MPC_RUN(launch_form_kkt(b, prm, d_xu, d_ee_ref, h->xs, d_status, h->G, h->C, h->g, h->c, st, h->gq,
+                            split ? h->Cl : nullptr));
+    if (tm) MPC_RUN(hipEventRecord(h->ev[1], st));
     if (split)
-      MPC_RUN(launch_form_split(b, prm, h->C, h->g, h->c, h->gq, d_rho, 0.f, d_status, h->P, h->ghat, h->rho_used,
+      MPC_RUN(launch_form_split(b, prm, h->Cl, h->g, h->c, h->gq, d_rho, 0.f, d_status, h->P, h->ghat, h->rho_used,
                                 st));
     else
       MPC_RUN(launch_form_schur(b, h->G, h->C, h->g, h->c, d_rho, 0.f, d_status, h->S, h->P, h->gamma, h->Ginv, 0,
@@ -1152,7 +1163,7 @@ static hipError_t batch_sqp_run(mpc_handle *h, float *d_xu, float *d_lambda, con
                       split));
     if (tm) MPC_RUN(hipEventRecord(h->ev[4], st));
     if (split)
-      MPC_RUN(launch_compute_dz_lr(b, prm, h->gq, d_rho, h->C, h->g, d_lambda, d_status, h->dz, st));
+      MPC_RUN(launch_compute_dz_lr(b, prm, h->gq, d_rho, h->Cl, h->g, d_lambda, d_status, h->dz, st));
     else
       MPC_RUN(launch_compute_dz(b, h->Ginv, h->C, h->g, d_lambda, d_status, h->dz, st));
     if (h->cfg.flags & FLAG_DIRECT_SOLVE)

@@ -233,3 +233,24 @@ def test_direct_resolve_is_the_direct_mode_solve(restore, orc, pkg):
     solver.set_config(flags=pkg.FLAG_DIRECT_SOLVE, direct_refine=2, direct_kkt_refine=0)
     _, lam_dir, _, _, _ = _run_batch(solver, horizons, xus, ees, iters=1)
     assert np.array_equal(host(lam_re), lam_dir)
+
+
+def test_direct_resolve_after_split_solve(restore, orc, pkg):
+    """A split-form solve leaves G and C in the compact form (g_q and the
+    lower dynamics rows); the resolve expands them and must give, bitwise,
+    the direct mode's solve of the same first Schur systems."""
+    solver = restore
+    horizons = [32, 200, 64, 256, 128]  # every horizon in reach of the split form
+    xus, ees = _batch(orc, pkg, horizons, seed0=3500)
+    T = sum(horizons)
+    solver.set_config(sqp_max_iter=1)
+    _run_batch(solver, horizons, xus, ees, iters=1)
+    assert solver.batch_get_plan()["split"] == 1
+    lam_re = torch.zeros(14 * T, device="cuda")
+    solver.batch_direct_resolve(lam_re, refine=2)
+    lam_re2 = torch.zeros(14 * T, device="cuda")
+    solver.batch_direct_resolve(lam_re2, refine=2)  # the compact form is still there
+    solver.set_config(flags=pkg.FLAG_DIRECT_SOLVE, direct_refine=2, direct_kkt_refine=0)
+    _, lam_dir, _, _, _ = _run_batch(solver, horizons, xus, ees, iters=1)
+    assert np.array_equal(host(lam_re), lam_dir)
+    assert np.array_equal(host(lam_re2), lam_dir)
