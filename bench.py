@@ -65,15 +65,16 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batched_1024.json")
 COUNTERS_FILE = os.path.join(ROOT, "profiles", "counters_batched_1024.json")
 # algorithmic HBM bytes and flops per knot of the other phases of an SQP
 # iteration (fp32), as the batched split path runs them:
-# linearize: reads x_k, u_k, x_{k+1}, ref_k (21 + 14 + 6 floats), writes G
-#   (245), C (294), g (21), c (14), g_q (7); ~2e4 flops (dynamics, its
-#   gradient, FK)
-# schur (split form, k_form_split): reads C of the knot before (294), g_q
-#   of both knots (14), q, r of the knot before and q of its own (35), c
-#   (14); writes L^ and W (392) and gamma^ (14); ~2.5e4 flops (theta, its
-#   whitening factor, the two block products of L^)
-# dz (k_compute_dz_lr): reads C (294), g_q (7), g (21), lambda of the knot
-#   and the next (28), writes dz (21); 1.1e3 flops
+# linearize: reads x_k, u_k, x_{k+1}, ref_k (21 + 14 + 6 floats), writes
+#   the lower block rows of C (147: the upper ones are fixed by the Euler
+#   step, G follows from g_q and the weights), g (21), c (14), g_q (7);
+#   ~2e4 flops (dynamics, its gradient, FK)
+# schur (split form, k_form_split): reads the lower rows of C of the knot
+#   before (147), g_q of both knots (14), q, r of the knot before and q of
+#   its own (35), c (14); writes L^ and W (392) and gamma^ (14); ~2.5e4
+#   flops (theta, its whitening factor, the two block products of L^)
+# dz (k_compute_dz_lr): reads the lower rows of C (147), g_q (7), g (21),
+#   lambda of the knot and the next (28), writes dz (21); 1.1e3 flops
 # merit: reads xu, dz, ref (21 + 21 + 6 floats) per knot for all 8 step
 #   lengths; flops not counted (priced by bytes only)
 # pcg: per solve L^ and W (392), gamma^ (14), lambda in and out (28) from
@@ -82,9 +83,9 @@ COUNTERS_FILE = os.path.join(ROOT, "profiles", "counters_batched_1024.json")
 # The flops of linearize, schur, dz and merit are the counted ones (fp32
 # VALU instructions x 64 lanes) when the counter file has them.
 PHASE_MODEL = {
-    "linearize": (4 * (21 + 14 + 6 + 245 + 294 + 21 + 14 + 7), 2.0e4),
-    "schur": (4 * (294 + 14 + 35 + 14 + 392 + 14), 2.5e4),
-    "dz": (4 * (294 + 7 + 21 + 28 + 21), 1.1e3),
+    "linearize": (4 * (21 + 14 + 6 + 147 + 21 + 14 + 7), 2.0e4),
+    "schur": (4 * (147 + 14 + 35 + 14 + 392 + 14), 2.5e4),
+    "dz": (4 * (147 + 7 + 21 + 28 + 21), 1.1e3),
     "merit": (4 * (21 + 21 + 6), 0.0),
 }
 PCG_HBM_BYTES_PER_KNOT = 4 * (392 + 14 + 28)

@@ -27,8 +27,10 @@ def test_phase_roofline_bounds_and_fractions(bench):
     out = bench.phase_roofline(phases, knots, pcg_units, bw)
     rows = out["phases"]
     assert set(rows) == {"linearize", "schur", "pcg", "dz", "merit"}
-    # schur (split form): 3052 B / knot over 5 TB/s against 2.5e4 flops / knot over 157.3 TFLOP/s
-    t_hbm = 3052 * knots / 5000e9
+    # schur (split form): 2464 B / knot (the lower rows of C of the knot
+    # before, g_q, q, r, c in; L^, W and gamma^ out) over 5 TB/s against
+    # 2.5e4 flops / knot over 157.3 TFLOP/s
+    t_hbm = 4 * (147 + 14 + 35 + 14 + 392 + 14) * knots / 5000e9
     t_f = 2.5e4 * knots / 157.3e12
     assert rows["schur"]["bound"] == ("hbm" if t_hbm > t_f else "fp32")
     assert rows["schur"]["roofline_us"] == pytest.approx(max(t_hbm, t_f) * 1e6, abs=0.01)
