@@ -381,13 +381,17 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   // knot, whose C and r are not written: replaced by zeros below, never
   // multiplied) and this knot's g and g_q.
   const size_t kprev = row > 0 ? row - 1 : 0u;
-  // the previous knot's lower block rows (compact form); its upper rows are
-  // the Euler step's (cl_upper_a, and zero in B)
+  // the previous knot's lower block rows (compact form), lane l its column
+  // l: rows NQ..NX-1 of A (l < 14) and of B (l < 7); the upper rows are the
+  // Euler step's (cl_upper_a, and zero in B)
   const float *Clk = Cl + (size_t)CL_STRIDE * kprev;
-  constexpr int CN = (CL_A + CL_B + GROUP - 1) / GROUP;
-  float cl[CN];
+  const int la = lane < NX ? lane : NX - 1, lb = lane < NU ? lane : NU - 1;
+  float ca[NQ], cb[NQ];
 #pragma unroll
-  for (int t = 0; t < CN; ++t) cl[t] = lane + GROUP * t < CL_A + CL_B ? Clk[lane + GROUP * t] : 0.f;
+  for (int r = 0; r < NQ; ++r) {
+    ca[r] = Clk[r + NQ * la];
+    cb[r] = Clk[CL_A + r + NQ * lb];
+  }
   const KnotEnds ke = knot_ends(b, row);
   const uint32_t inst = ke.inst;
   const bool row0 = ke.first;
@@ -395,14 +399,20 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(4))) k
   SplitRowLds &L = lds[grp];
   // A = -C_A and B = -C_B staged column-major, as negations of the values
   // launch_form_kkt stores (zeros for a first knot)
+  if (lane < NX) {
 #pragma unroll
-  for (int t = 0; t < CN; ++t) {
-    const int c = lane + GROUP * t;
-    if (c < CL_A) L.A[NQ + c % NQ + NX * (c / NQ)] = row0 ? 0.f : -cl[t];
-    else if (c < CL_A + CL_B) L.B[NQ + (c - CL_A) % NQ + NX * ((c - CL_A) / NQ)] = row0 ? 0.f : -cl[t];
+    for (int r = 0; r < NQ; ++r) {
+      L.A[r + NX * lane] = row0 ? 0.f : -cl_upper_a(r, lane, p.dt);
+      L.A[NQ + r + NX * lane] = row0 ? 0.f : -ca[r];
+    }
   }
-  for (int c = lane; c < NQ * NX; c += GROUP) L.A[c % NQ + NX * (c / NQ)] = row0 ? 0.f : -cl_upper_a(c % NQ, c / NQ, p.dt);
-  for (int c = lane; c < NQ * NU; c += GROUP) L.B[c % NQ + NX * (c / NQ)] = row0 ? 0.f : -0.f;
+  if (lane < NU) {
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) {
+      L.B[r + NX * lane] = row0 ? 0.f : -0.f;
+      L.B[NQ + r + NX * lane] = row0 ? 0.f : -cb[r];
+    }
+  }
   const int j = lane < NX ? lane : NX - 1;
   // q_{b-1}, r_{b-1}, q_b: read by every lane of the group (cached)
   const float *qk = g + (size_t)NXU * kprev, *rk = qk + NX, *q1v = g + (size_t)NXU * row;
