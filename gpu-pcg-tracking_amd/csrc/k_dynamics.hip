@@ -401,15 +401,21 @@ __global__ void __launch_bounds__(KKT_WG) k_form_kkt(Batch b, Params p, const fl
   if (lane < NU) gk_[NX + lane] = p.wu[lane] * L.u[lane];
   const float dt = p.dt;
   if (compact) {
-    // the lower block rows only (the upper ones are cl_upper_a and zero)
+    // the lower block rows only (the upper ones are cl_upper_a and zero),
+    // lane l its column l: 7 contiguous floats of A's (l < 14) and of B's
     float *Clk = Cl + (size_t)CL_STRIDE * knot;
-    for (int i = lane; i < CL_A; i += GROUP) {
-      const int r = NQ + i % NQ, col = i / NQ;
-      float v = (r == col) ? 1.f : 0.f;
-      v += dt * L.dyn.df[(r - NQ) + NQ * col];
-      Clk[i] = -v;
+    if (lane < NX) {
+#pragma unroll
+      for (int r = 0; r < NQ; ++r) {
+        float v = (NQ + r == lane) ? 1.f : 0.f;
+        v += dt * L.dyn.df[r + NQ * lane];
+        Clk[r + NQ * lane] = -v;
+      }
     }
-    for (int i = lane; i < CL_B; i += GROUP) Clk[CL_A + i] = -dt * L.dyn.Minv[i];
+    if (lane < NU) {
+#pragma unroll
+      for (int r = 0; r < NQ; ++r) Clk[CL_A + r + NQ * lane] = -dt * L.dyn.Minv[r + NQ * lane];
+    }
   } else {
     float *Ck = C + (size_t)C_STRIDE * knot;
     for (int i = lane; i < NXX; i += GROUP) {
