@@ -359,7 +359,9 @@ int mpc_batch_pcg_exit(mpc_handle *h, uint32_t *d_iters, uint32_t *d_flags, floa
  * that solve returned (the reference's QDLDL solve of the same system,
  * toplevel.cuh:314-335).  After a split-form solve (mpc_pcg_plan.split),
  * which forms no S, S and gamma are first formed from the last iteration's
- * KKT blocks and rho.  Overwrites the preconditioner workspace, so it is for
+ * KKT blocks and rho (its cost and dynamics blocks expanded from the
+ * compact form that path keeps: g_q and the lower dynamics rows).
+ * Overwrites the preconditioner workspace, so it is for
  * use after a solve, before the next one; instances that were no longer
  * active in that iteration get the solve of a stale system. */
 int mpc_batch_direct_resolve(mpc_handle *h, float *d_lambda_out, uint32_t refine_steps, mpc_stream_t stream);
