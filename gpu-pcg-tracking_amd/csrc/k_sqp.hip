@@ -97,14 +97,27 @@ __global__ void __launch_bounds__(NT) k_ls_update(Batch b, Params p, const float
   __shared__ float red[8 * (NT / 64)];
   __shared__ float merits[8];
   __shared__ float alpha_s[1];
+  __shared__ uint32_t status_s[1];
   const uint32_t inst = blockIdx.x;
-  if (!(s.status[inst] & ST_ACTIVE)) return;  // uniform over the workgroup
+  // the instance's values, read up front: independent addresses, one round
+  // trip (this workgroup is their only writer below)
+  const uint32_t st0 = s.status[inst];
   const uint32_t K0 = inst_first(b, inst), N = inst_len(b, inst);
+  uint32_t pit = 0u, pf = 0u, done = 0u;
+  float m0 = 0.f, rho0 = 0.f, drho0 = 0.f;
+  if (threadIdx.x == 0) {
+    pit = pcg_iters_now ? pcg_iters_now[inst] : 0u;
+    pf = pcg_flag_now ? pcg_flag_now[inst] : 0u;
+    m0 = s.merit0[inst];
+    rho0 = s.rho[inst];
+    drho0 = s.drho[inst];
+    done = s.sqp_done[inst];
+  }
+  if (!(st0 & ST_ACTIVE)) return;  // uniform over the workgroup
   reduce_slots<8>(km, b.total_knots, K0, N, red, merits);
   if (threadIdx.x == 0) {
-    uint32_t st = s.status[inst];
-    if (pcg_log) pcg_log[(size_t)inst * log_stride + it] = pcg_iters_now ? pcg_iters_now[inst] : 0u;
-    const uint32_t pf = pcg_flag_now ? pcg_flag_now[inst] : 0u;
+    uint32_t st = st0;
+    if (pcg_log) pcg_log[(size_t)inst * log_stride + it] = pit;
     if (pf == PCG_FLAG_FAULT) st |= ST_PCG_FAULT;
     else if (pf) st |= ST_PCG_MAXITER;
     // the time budget, checked where the reference checks it before its
@@ -113,12 +126,12 @@ __global__ void __launch_bounds__(NT) k_ls_update(Batch b, Params p, const float
     if (p.budget_ticks && s.t0 && (unsigned long long)wall_clock64() - *s.t0 > p.budget_ticks)
       st = (st | ST_TIME_BUDGET) & ~ST_ACTIVE;
     s.status[inst] = st;
+    status_s[0] = st;
   }
   __syncthreads();
-  if (!(s.status[inst] & ST_ACTIVE)) return;  // uniform: written by thread 0 before the barrier
+  if (!(status_s[0] & ST_ACTIVE)) return;  // uniform: written by thread 0 before the barrier
   if (threadIdx.x == 0) {
-    uint32_t st = s.status[inst];
-    const float m0 = s.merit0[inst];
+    uint32_t st = status_s[0];
     float minm = m0;
     int step = -1;
     for (int i = 0; i < 8; ++i)
@@ -126,7 +139,7 @@ __global__ void __launch_bounds__(NT) k_ls_update(Batch b, Params p, const float
         minm = merits[i];
         step = i;
       }
-    float rho = s.rho[inst], drho = s.drho[inst];
+    float rho = rho0, drho = drho0;
     float alpha = 0.f;
     if (step < 0) {
       drho = fmaxf(drho * p.rho_factor, p.rho_factor);
@@ -149,7 +162,7 @@ __global__ void __launch_bounds__(NT) k_ls_update(Batch b, Params p, const float
     }
     s.rho[inst] = rho;
     s.drho[inst] = drho;
-    s.sqp_done[inst] += 1;
+    s.sqp_done[inst] = done + 1u;
     s.status[inst] = st;
     alpha_s[0] = alpha;
   }
