@@ -613,35 +613,49 @@ __global__ void __launch_bounds__(WG) k_compute_dz_lr(Batch b, Params p, const f
   const uint32_t gk_raw = blockIdx.x * GROUPS_PER_WG + grp;
   bool valid = gk_raw < b.total_knots;
   const uint32_t gk = valid ? gk_raw : b.total_knots - 1;
+  const size_t knot = gk;
+  // every read of the knot is issued at entry, before the instance lookup:
+  // lambda of the knot and of the next one (a clamped address, selected away
+  // at an instance's last knot), the lower block rows of C (compact form:
+  // that knot's are never written, so their sums are selected away, not
+  // multiplied by zero), g and g_q
+  const int la = lane < NX ? lane : NX - 1, lu = lane < NU ? lane : NU - 1;
+  const size_t nk = knot + 1 < b.total_knots ? knot + 1 : knot;
+  const float lam_next = lambda[(size_t)NX * nk + la], lam_k = lambda[(size_t)NX * knot + la];
+  const float *Clk = Cl + (size_t)CL_STRIDE * knot;
+  float ca[NQ], cb[NQ], gqv[NQ];
+#pragma unroll
+  for (int r = 0; r < NQ; ++r) {
+    ca[r] = Clk[r + NQ * la];
+    cb[r] = Clk[CL_A + r + NQ * lu];
+    gqv[r] = gq[(size_t)NQ * knot + r];
+  }
+  const float gx = g[(size_t)NXU * knot + la], gu = g[(size_t)NXU * knot + NX + lu];
+  const float gql = gq[(size_t)NQ * knot + (lane < NQ ? lane : 0)];
   const KnotEnds ke = knot_ends(b, gk);
   if (status && !(status[ke.inst] & ST_ACTIVE)) valid = false;
+  const float rho = rho_dev[ke.inst];
   DzLds &L = lds[grp];
-  const size_t knot = gk;
   const bool has_next = ke.has_next;
-  // the lower block rows of C (compact form); the upper ones are the Euler
-  // step's, cl_upper_a in A and zero in B
-  const float *Clk = Cl + (size_t)CL_STRIDE * knot;
-  if (lane < NX) L.lam1[lane] = has_next ? lambda[(size_t)NX * (knot + 1) + lane] : 0.f;
+  if (lane < NX) L.lam1[lane] = has_next ? lam_next : 0.f;
   __syncthreads();
   if (lane < NX) {
-    // t_j = q_j - lambda_k,j + (A^T lambda_{k+1})_j, A = -C_A
+    // t_j = q_j - lambda_k,j + (A^T lambda_{k+1})_j, A = -C_A; the upper
+    // rows of C are the Euler step's (cl_upper_a)
     float s = 0.f;
-    if (has_next) {
 #pragma unroll
-      for (int l = 0; l < NX; ++l)
-        s -= (l < NQ ? cl_upper_a(l, lane, p.dt) : Clk[(l - NQ) + NQ * lane]) * L.lam1[l];
-    }
-    L.t[lane] = g[(size_t)NXU * knot + lane] - lambda[(size_t)NX * knot + lane] + s;
+    for (int l = 0; l < NX; ++l) s -= (l < NQ ? cl_upper_a(l, lane, p.dt) : ca[l - NQ]) * L.lam1[l];
+    s = has_next ? s : 0.f;
+    L.t[lane] = gx - lam_k + s;
   }
   if (has_next && lane < NU) {
     float s = 0.f;
 #pragma unroll
-    for (int l = 0; l < NX; ++l) s -= (l < NQ ? 0.f : Clk[CL_A + (l - NQ) + NQ * lane]) * L.lam1[l];
-    L.tu[lane] = g[(size_t)NXU * knot + NX + lane] + s;
+    for (int l = 0; l < NX; ++l) s -= (l < NQ ? 0.f : cb[l - NQ]) * L.lam1[l];
+    L.tu[lane] = gu + s;
   }
   __syncthreads();
   if (!valid) return;
-  const float rho = rho_dev[ke.inst];
   float *dzi = dz + (size_t)NXU * gk - (size_t)NU * ke.inst;
   if (lane < NX) {
     // Q~^-1 t = D^-1 t - u (u^T t) / s, the cancelling terms in fp64 (as
@@ -650,11 +664,11 @@ __global__ void __launch_bounds__(WG) k_compute_dz_lr(Batch b, Params p, const f
     double u[NQ], inv_s, ut = 0.0;
 #pragma unroll
     for (int l = 0; l < NQ; ++l) dq[l] = __builtin_amdgcn_rcpf(p.wx[l] + rho);
-    q_lowrank(gq + (size_t)NQ * knot, dq, false, u, inv_s);
+    q_lowrank(gqv, dq, false, u, inv_s);
 #pragma unroll
     for (int l = 0; l < NQ; ++l) ut = fma(u[l], (double)L.t[l], ut);
     const float dl = __builtin_amdgcn_rcpf(p.wx[lane] + rho);
-    const double ul = lane < NQ ? (double)gq[(size_t)NQ * knot + lane] * (double)dl : 0.0;
+    const double ul = lane < NQ ? (double)gql * (double)dl : 0.0;
     dzi[lane] = (float)fma(-ul, ut * inv_s, (double)L.t[lane] * (double)dl);
   }
   if (has_next && lane < NU) dzi[NX + lane] = L.tu[lane] * __builtin_amdgcn_rcpf(p.wu[lane] + rho);
